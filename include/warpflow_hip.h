@@ -1,0 +1,253 @@
+/* warpflow_hip.h -- C ABI of the MI355X (gfx950) per-step hot path of the
+ * warpflow-les lid-driven-cavity LES solver.
+ *
+ * The reference (dr-chunshui/warpflow-les) launches NVIDIA Warp kernels with
+ *     wp.launch(kernel, dim=(d0, d1, d2), inputs=[arrays..., scalars...])
+ * over C-ordered float64 wp.array3d fields.  Each entry below replaces one
+ * such kernel (or one host method that issues a fixed launch sequence); the
+ * citation names the reference file:line it stands in for.
+ *
+ * Conventions
+ *  - Every entry returns an int: 0 on success, else a hipError_t code
+ *    (wf_error_string() turns it into text).  Arguments are checked on the
+ *    host before any launch; a bad shape returns WF_EINVAL (10001) and launches
+ *    nothing.
+ *  - Arrays are device pointers.  Fields use wf_grid: real cells ni, nj, nk,
+ *    npl ghost layers, element strides sj (of j) and si (of i); k is
+ *    unit-stride.  The launch box of a kernel is given as (d0, d1, d2) and
+ *    starts at full index (npl, npl, npl), exactly like the reference's
+ *    `i += npl` after wp.tid().
+ *  - `stream` is a hipStream_t (NULL = the null stream).  Work is enqueued
+ *    asynchronously; reductions write device scalars that the caller reads
+ *    back (the reference's 1-element wp.array + .numpy()).
+ *  - No entry point allocates device memory except wf_malloc and the
+ *    multigrid set-up helpers that say so.
+ */
+#ifndef WARPFLOW_HIP_H
+#define WARPFLOW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_EINVAL 10001
+
+typedef struct wf_grid {
+  int32_t ni, nj, nk; /* real cells per axis */
+  int32_t npl;        /* ghost layers per side */
+  int64_t sj, si;     /* element strides of j and i */
+} wf_grid;
+
+/* One multigrid level: arrays of shape (n0+2, n1+2, n2+2) with a one-cell
+ * ring (warpflow/multigrid.py:207-220).  prec = 8 (float64) or 4 (float32).
+ * h0..h2 are the level spacings dx*2^l, dy*2^l, dz*2^l (multigrid.py:253).
+ * Coefficients are matrix-free: 1/h^2 per face, 0 across the six physical
+ * faces of the single cavity domain, ap = -(aw+ae+as+an+ab+at)
+ * (multigrid.py:23-67). */
+typedef struct wf_mg_level {
+  void* phi;
+  void* rhs;
+  int32_t n0, n1, n2;
+  int32_t prec;
+  double h0, h1, h2;
+} wf_mg_level;
+
+/* ------------------------------------------------------------------------
+ * Runtime and device arrays.  Replace wp.zeros / wp.empty (solver.py:47,
+ * multigrid.py:212-220), wp.copy (solver.py:223-225), array.numpy()
+ * (solver.py:119), array.assign() (tests/test_mg.py:45), array.zero_()
+ * (multigrid.py:312) and array.fill_() (solver.py:77, :127).
+ * ---------------------------------------------------------------------- */
+int wf_device_count(int* n);
+int wf_set_device(int device);
+int wf_get_device_name(char* buf, int len);
+int wf_get_device_attr(int* n_cu, int* max_lds_per_block, int* warp_size);
+int wf_malloc(void** ptr, size_t bytes); /* zero-filled */
+int wf_free(void* ptr);
+int wf_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int wf_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int wf_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int wf_memset(void* ptr, int value, size_t bytes, void* stream);
+int wf_fill_f64(double* ptr, double value, size_t n, void* stream);
+int wf_fill_f32(float* ptr, float value, size_t n, void* stream);
+int wf_synchronize(void);
+int wf_stream_create(void** stream);
+int wf_stream_destroy(void* stream);
+int wf_stream_synchronize(void* stream);
+int wf_event_create(void** event);
+int wf_event_destroy(void* event);
+int wf_event_record(void* event, void* stream);
+int wf_event_elapsed_ms(float* ms, void* start, void* stop);
+const char* wf_error_string(int code);
+
+/* ------------------------------------------------------------------------
+ * Momentum, projection and SGS kernels (warpflow/kernels.py).
+ * comp: 0 = u, 1 = v, 2 = w.
+ * ---------------------------------------------------------------------- */
+
+/* conv4th_u / conv4th_v / conv4th_w, kernels.py:53-89, :92-129, :132-168:
+ * cstar = coo - dt*alpha*N(c), 4th-order staggered convection. */
+int wf_conv4th(int comp, const double* u, const double* v, const double* w,
+               const double* coo, double* cstar, wf_grid g, int d0, int d1,
+               int d2, double dx, double dy, double dz, double dt, double alpha,
+               void* stream);
+
+/* diff4th_u / _v / _w, kernels.py:189-225: cstar += dt*alpha*vis_s*L4(c),
+ * vis sampled at (i,j,k) for u, (i,j+1,k) for v, (i,j,k+1) for w. */
+int wf_diff4th(int comp, const double* c, const double* vis, double* cstar,
+               wf_grid g, int d0, int d1, int d2, double dx, double dy,
+               double dz, double dt, double alpha, void* stream);
+
+/* The conv4th_c + diff4th_c pair of one RK stage (solver.py:254-265) in one
+ * pass: the same two roundings (cstar = coo - dt*alpha*N, then
+ * cstar += dt*alpha*vis*L4), reading c, the other two components, coo and
+ * vis once.  Equal to the pair because diff4th reads only face ghosts of c,
+ * which the intervening fill_bc(c) leaves unchanged. */
+int wf_convdiff4th(int comp, const double* u, const double* v, const double* w,
+                   const double* coo, const double* vis, double* cstar,
+                   wf_grid g, int d0, int d1, int d2, double dx, double dy,
+                   double dz, double dt, double alpha, void* stream);
+
+/* calvel_u / _v / _w, kernels.py:232-264: c = cstar - dt*alpha*(p[+1]-p)/h
+ * along the component's axis. */
+int wf_calvel(int comp, const double* cstar, const double* p, double* c,
+              wf_grid g, int d0, int d1, int d2, double h, double dt,
+              double alpha, void* stream);
+
+/* FlowSolver._calvel, solver.py:155-166: the three calvel launches over
+ * their operator ranges (ni-1,nj,nk), (ni,nj-1,nk), (ni,nj,nk-1)
+ * (solver.py:99-110) in one pass. */
+int wf_calvel3(const double* ustar, const double* vstar, const double* wstar,
+               const double* p, double* u, double* v, double* w, wf_grid g,
+               double dx, double dy, double dz, double dt, double alpha,
+               void* stream);
+
+/* calmas, kernels.py:271-289: su = div/dt with the 4th-order divergence;
+ * *rmax_dev = max(*rmax_dev, max|div*dx*dy*dz|) (caller zeroes it first,
+ * solver.py:169). */
+int wf_calmas(const double* u, const double* v, const double* w, double* su,
+              double* rmax_dev, wf_grid g, int d0, int d1, int d2, double dx,
+              double dy, double dz, double dt, void* stream);
+
+/* wale_viscosity, kernels.py:297-367: vis = rrey + nut (WALE, cw), incl. the
+ * reference's g2zx/g2zy terms (kernels.py:336, :338).  delta is
+ * (dx*dy*dz)^(1/3) (kernels.py:360), computed once by the caller. */
+int wf_wale(const double* u, const double* v, const double* w, double* vis,
+            wf_grid g, int d0, int d1, int d2, double dx, double dy, double dz,
+            double rrey, double cw, double delta, void* stream);
+
+/* maxabs_reduce, kernels.py:374-381: *out = max(*out, max|f|). */
+int wf_maxabs(const double* f, double* out_dev, wf_grid g, int d0, int d1,
+              int d2, void* stream);
+
+/* dtvisc_reduce, kernels.py:384-402: *out = min(*out, min dt-limit). */
+int wf_dtvisc(const double* u, const double* v, const double* w,
+              const double* vis, double* out_dev, wf_grid g, int d0, int d1,
+              int d2, double dx, double dy, double dz, void* stream);
+
+/* FlowSolver.compute_dt reductions, solver.py:115-134: out4 = {max|u|,
+ * max|v|, max|w|, min dtvisc} over the real cells in one pass (the entry
+ * initialises out4 to {0, 0, 0, 1e10} itself). */
+int wf_checkdt(const double* u, const double* v, const double* w,
+               const double* vis, double* out4_dev, wf_grid g, double dx,
+               double dy, double dz, void* stream);
+
+/* add_inplace, kernels.py:405-414, launched over ncells+2 (solver.py:198-201):
+ * dst += src over the real cells plus one ghost ring. */
+int wf_add_ring(double* dst, const double* src, wf_grid g, void* stream);
+
+/* fill_const, kernels.py:442-449: f = val over the launch box. */
+int wf_fill_interior(double* f, double val, wf_grid g, int d0, int d1, int d2,
+                     void* stream);
+
+/* init_noise, kernels.py:452-471: uint32 hash of the full-array indices and
+ * seed; u = v = amp*r, w = -amp*r. */
+int wf_init_noise(double* u, double* v, double* w, double amp, int32_t seed,
+                  wf_grid g, int d0, int d1, int d2, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Ghost fills (warpflow/bcs.py).
+ * ---------------------------------------------------------------------- */
+
+/* _mk_lin(axis, high), bcs.py:42-106: fld[ghost m] = sign*fld[source m],
+ * m < nlay; low side ghost g0-m / source s0+m, high side g0+m / s0-m; the
+ * perpendicular axes (ascending) run over [a0, a0+na) x [b0, b0+nb). */
+int wf_bc_lin(double* fld, wf_grid g, int axis, int high, int g0, int s0,
+              double sign, int a0, int na, int b0, int nb, int nlay,
+              void* stream);
+
+/* _mk_const(axis, high), bcs.py:109-163: fld[ghost m] = val, m < nlay. */
+int wf_bc_const(double* fld, wf_grid g, int axis, int high, int g0, double val,
+                int a0, int na, int b0, int nb, int nlay, void* stream);
+
+/* BoundaryKernels.apply_* (bcs.py:201-311) for the cavity boundary set
+ * (W/E/B wall, S/N slip, T moving lid).  which: 0 = apply_u (param = lid
+ * velocity), 1 = apply_v, 2 = apply_w, 3 = apply_vis (param = 1/Re),
+ * 4 = apply_pbound, 5 = apply_p_mirror (param = number of layers).  The six
+ * side launches of the reference run as three launches, one per axis: a
+ * thread applies the low-side then the high-side operations of its
+ * perpendicular column in the reference order, which is the same sequence
+ * per cell (each side operation only touches its own column). */
+int wf_bc_fill(double* fld, wf_grid g, int which, double param, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Multigrid (warpflow/multigrid.py).  Level arrays follow wf_mg_level.
+ * ---------------------------------------------------------------------- */
+
+/* _mgcoeff, multigrid.py:23-67: write the seven coefficient arrays of a level
+ * (shape n+2 per axis, real cells only) -- for inspection; the solver itself
+ * is matrix-free. */
+int wf_mg_coeff(const wf_mg_level* lev, void* aw, void* ae, void* as,
+                void* an, void* ab, void* at, void* ap, void* stream);
+
+/* _rbgs, multigrid.py:70-90: one half-sweep on cells with
+ * (i+j+k+color) % 2 == 1 (0-based real indices), in place. */
+int wf_mg_rbgs(const wf_mg_level* lev, int color, void* stream);
+
+/* Multigrid._refill, multigrid.py:264-287 with bcs._mk_neumann
+ * (bcs.py:326-360): one-layer Neumann mirror of the six faces of the ring
+ * (faces only; edge and corner ring cells are not written). */
+int wf_mg_refill(const wf_mg_level* lev, void* stream);
+
+/* Multigrid._relax, multigrid.py:289-299: nsweeps x (colour 0, colour 1,
+ * refill).  Levels whose arrays fit the workgroup's LDS run all sweeps in one
+ * single-workgroup launch. */
+int wf_mg_relax(const wf_mg_level* lev, int nsweeps, void* stream);
+
+/* _restrict, multigrid.py:93-119, followed by the coarse phi zeroing of
+ * multigrid.py:312 (whole coarse array incl. the ring faces). */
+int wf_mg_restrict(const wf_mg_level* fine, const wf_mg_level* coarse,
+                   void* stream);
+
+/* _prolong, multigrid.py:122-146, followed by the fine-level refill of
+ * multigrid.py:321. */
+int wf_mg_prolong(const wf_mg_level* coarse, const wf_mg_level* fine,
+                  void* stream);
+
+/* _copy_in, multigrid.py:149-154: field su (wf_grid g) -> level rhs. */
+int wf_mg_copy_in(const double* src, wf_grid g, const wf_mg_level* lev,
+                  void* stream);
+
+/* _copy_out_ring, multigrid.py:157-163: level phi incl. ring -> field at
+ * offset npl-1. */
+int wf_mg_copy_out_ring(const wf_mg_level* lev, double* dst, wf_grid g,
+                        void* stream);
+
+/* Multigrid._vcycle, multigrid.py:301-322, over levels[0..nlev). */
+int wf_mg_vcycle(const wf_mg_level* levels, int nlev, int n_pre, int n_post,
+                 int n_coarse, void* stream);
+
+/* Multigrid.solve, multigrid.py:324-337: copy-in, zero pp and phi0,
+ * n_vcycles V-cycles, copy-out incl. ring.  All levels float64. */
+int wf_mg_solve(const wf_mg_level* levels, int nlev, const double* su,
+                double* pp, wf_grid g, int n_pre, int n_post, int n_coarse,
+                int n_vcycles, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WARPFLOW_HIP_H */

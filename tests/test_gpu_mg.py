@@ -1,0 +1,154 @@
+"""GPU parity of the multigrid pieces (matrix-free HIP) against the oracle's
+stored-coefficient restatement of warpflow/multigrid.py, including the ring
+state the reference leaves behind (faces mirrored, edges/corners zero)."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from oracle import np_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+class _Cfg:
+    """Minimal CaseConfig stand-in for Multigrid(cfg, ...)."""
+
+    def __init__(self, levels):
+        from warpflow.config import CaseConfig
+        c = CaseConfig()
+        c.numerics.mg_levels = levels
+        self.__dict__.update(c.__dict__)
+
+
+def build(n3, levels):
+    from warpflow.multigrid import Multigrid
+    from warpflow.topology import Topology, SerialComm
+    topo = Topology(0, 1, SerialComm(), (1, 1, 1), (0, 0, 0),
+                    {"x": (-1, -1), "y": (-1, -1), "z": (-1, -1)}, n3, (0, 0, 0))
+    h = tuple(1.0 / n for n in n3)
+    mg = Multigrid(_Cfg(levels), topo, None, *h)
+    ref = O.Multigrid(n3, levels, *h)
+    return mg, ref
+
+
+def load(mg, ref, seed=0):
+    rng = np.random.default_rng(seed)
+    for lev, rlev in zip(mg.levels, ref.levels):
+        phi = rng.standard_normal(rlev.shape)
+        rhs = rng.standard_normal(rlev.shape)
+        rlev.phi[...] = phi
+        rlev.rhs[...] = rhs
+        lev.phi.assign(phi)
+        lev.rhs.assign(rhs)
+
+
+@pytest.mark.parametrize("n3", [(16, 16, 16), (8, 12, 20)])
+def test_coefficients_bitwise(gpu, n3):
+    mg, ref = build(n3, 2)
+    for lev, rlev in zip(mg.levels, ref.levels):
+        for name in ("aw", "ae", "as_", "an", "ab", "at", "ap"):
+            assert np.array_equal(getattr(lev, name).numpy(), getattr(rlev, name)), name
+
+
+@pytest.mark.parametrize("n3", [(16, 16, 16), (8, 12, 20)])
+def test_rbgs_refill_restrict_prolong(gpu, n3):
+    mg, ref = build(n3, 2)
+    load(mg, ref)
+    f, c = mg.levels
+    rf, rc = ref.levels
+    from warpflow import _hip
+    import ctypes as C
+    for color in (0, 1):
+        O.rbgs(rf, color)
+        d = f.desc()
+        _hip.call("wf_mg_rbgs", C.byref(d), color, None)
+        assert rel_err(f.phi.numpy(), rf.phi) < TOL
+    ref._refill(rf)
+    mg._refill(f)
+    assert rel_err(f.phi.numpy(), rf.phi) < TOL
+    # restriction (+ coarse zeroing)
+    O.restrict(rf, rc)
+    rc.phi[...] = 0.0
+    df, dc = f.desc(), c.desc()
+    _hip.call("wf_mg_restrict", C.byref(df), C.byref(dc), None)
+    assert rel_err(c.rhs.numpy()[1:-1, 1:-1, 1:-1], rc.rhs[1:-1, 1:-1, 1:-1]) < TOL
+    cphi = c.phi.numpy()
+    assert not cphi[1:-1, 1:-1, 1:-1].any()
+    # prolongation reads the coarse ring: faces mirrored, edges zero
+    rng = np.random.default_rng(3)
+    cint = rng.standard_normal(rc.n3)
+    rc.phi[...] = 0.0
+    rc.phi[1:-1, 1:-1, 1:-1] = cint
+    ref._refill(rc)
+    c.phi.assign(rc.phi)
+    O.prolong(rc, rf)
+    ref._refill(rf)
+    _hip.call("wf_mg_prolong", C.byref(dc), C.byref(df), None)
+    assert rel_err(f.phi.numpy(), rf.phi) < TOL
+
+
+@pytest.mark.parametrize("n3,nsweeps", [((8, 8, 8), 300), ((20, 20, 20), 6),
+                                        ((40, 40, 40), 3), ((16, 8, 24), 5)])
+def test_relax_paths(gpu, n3, nsweeps):
+    """_relax: small levels run in one LDS-resident workgroup, large ones as
+    colour half-sweep launches; both must match the oracle incl. the ring."""
+    mg, ref = build(n3, 1)
+    load(mg, ref, seed=4)
+    ref._relax(0, nsweeps)
+    mg._relax(0, nsweeps)
+    got = mg.levels[0].phi.numpy()
+    assert rel_err(got, ref.levels[0].phi) < 1e-11
+
+
+@pytest.mark.parametrize("n3,levels", [((32, 32, 32), 3), ((16, 24, 32), 3), ((64, 64, 64), 4)])
+def test_vcycle_and_solve(gpu, n3, levels):
+    from warpflow.device import array, zeros
+    mg, ref = build(n3, levels)
+    rng = np.random.default_rng(5)
+    shape = tuple(n + 6 for n in n3)
+    su = rng.standard_normal(shape)
+    pp_ref = np.zeros(shape)
+    ref.solve(su, pp_ref, 3)
+    pp = zeros(shape)
+    mg.solve(array(su), pp, 3)
+    got = pp.numpy()
+    assert rel_err(got, pp_ref) < 1e-11
+    ring = np.zeros(shape, bool)
+    ring[2:-2, 2:-2, 2:-2] = True
+    assert not got[~ring].any()  # outside interior+ring untouched (zeroed)
+    # one more V-cycle from the current state through the API
+    ref._vcycle()
+    mg._vcycle()
+    assert rel_err(mg.levels[0].phi.numpy(), ref.levels[0].phi) < 1e-11
+
+
+def test_reference_mg_script_flow(gpu):
+    """tests/test_mg.py:48-66 driven through the public API: random Neumann
+    rhs, phi = 0, refill, V-cycles; the residual (computed from the
+    materialised coefficient arrays) drops >= 1 order per cycle."""
+    mg, _ = build((32, 32, 32), 3)
+    lev0 = mg.levels[0]
+    rng = np.random.default_rng(0)
+    h = rng.standard_normal(lev0.shape)
+    h[0], h[-1], h[:, 0], h[:, -1], h[:, :, 0], h[:, :, -1] = 0, 0, 0, 0, 0, 0
+    h[1:-1, 1:-1, 1:-1] -= h[1:-1, 1:-1, 1:-1].mean()
+    lev0.rhs.assign(h)
+    lev0.phi.zero_()
+    mg._refill(lev0)
+
+    def residual(lev):
+        phi, rhs = lev.phi.numpy(), lev.rhs.numpy()
+        A = [getattr(lev, k).numpy() for k in ("aw", "ae", "as_", "an", "ab", "at", "ap")]
+        n = lev.n3[0]
+        s = slice(1, n + 1)
+        r = rhs[s, s, s] - (A[6][s, s, s] * phi[s, s, s] + A[0][s, s, s] * phi[0:n, s, s]
+                            + A[1][s, s, s] * phi[2:n + 2, s, s] + A[2][s, s, s] * phi[s, 0:n, s]
+                            + A[3][s, s, s] * phi[s, 2:n + 2, s] + A[4][s, s, s] * phi[s, s, 0:n]
+                            + A[5][s, s, s] * phi[s, s, 2:n + 2])
+        return np.abs(r).max()
+    r = [residual(lev0)]
+    for _ in range(4):
+        mg._vcycle()
+        r.append(residual(lev0))
+    assert all(b < 0.1 * a for a, b in zip(r, r[1:])), r

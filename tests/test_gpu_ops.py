@@ -1,0 +1,189 @@
+"""GPU parity of every momentum / projection / SGS operator and every ghost
+fill against the CPU oracle, on seeded random fields (ghosts included).
+
+Bar: integer and copy work (noise hash, fills, add, fill) bit-exact; floating
+point operators within 1e-12 relative (for the ``star = oo - dt*alpha*N``
+operators the increment ``star - oo`` is compared, relative to its max-abs,
+so the tolerance is not hidden behind ``oo``)."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from oracle import np_oracle as O
+
+pytestmark = pytest.mark.gpu
+N = O.NPL
+TOL = 1e-12
+SHAPES = [(32, 32, 32), (20, 24, 28)]
+
+
+def rand_fields(n3, names, seed=0, scale=1.0):
+    rng = np.random.default_rng(seed)
+    shape = tuple(n + 2 * N for n in n3)
+    return {nm: scale * rng.standard_normal(shape) for nm in names}
+
+
+def up(a):
+    from warpflow.device import array
+    return array(a, np.float64)
+
+
+@pytest.mark.parametrize("n3", SHAPES)
+@pytest.mark.parametrize("comp", ["u", "v", "w"])
+def test_conv_diff_convdiff(gpu, n3, comp):
+    from warpflow import kernels as K
+    h = tuple(1.0 / n for n in n3)
+    dt, alpha = 3e-4, 0.5
+    f = rand_fields(n3, ["u", "v", "w", "coo", "vis", "star0"], seed=1)
+    f["vis"] = 0.0025 + 0.001 * np.abs(f["vis"])
+    dims = {"u": (n3[0] - 1, n3[1], n3[2]), "v": (n3[0], n3[1] - 1, n3[2]),
+            "w": (n3[0], n3[1], n3[2] - 1)}[comp]
+    d = {k: up(v) for k, v in f.items()}
+    # conv
+    star_ref = f["star0"].copy()
+    getattr(O, "conv4th_" + comp)(f["u"], f["v"], f["w"], f["coo"], star_ref, *h, dt, alpha, N, dims)
+    star = up(f["star0"])
+    getattr(K, "conv4th_" + comp)(dims, d["u"], d["v"], d["w"], d["coo"], star, *h, dt, alpha, N)
+    got = star.numpy()
+    assert rel_err(got - f["coo"], star_ref - f["coo"]) < TOL
+    ghost = np.ones(got.shape, bool)
+    ghost[tuple(slice(N, N + x) for x in dims)] = False
+    assert np.array_equal(got[ghost], f["star0"][ghost])  # writes only its box
+    # diff on top of conv
+    c = {"u": f["u"], "v": f["v"], "w": f["w"]}[comp]
+    diff_ref = star_ref.copy()
+    getattr(O, "diff4th_" + comp)(c, f["vis"], diff_ref, *h, dt, alpha, N, dims)
+    getattr(K, "diff4th_" + comp)(dims, d[comp], d["vis"], star, *h, dt, alpha, N)
+    got2 = star.numpy()
+    assert rel_err(got2 - star_ref, diff_ref - star_ref) < TOL
+    # fused pair == conv then diff
+    fused = up(f["star0"])
+    getattr(K, "convdiff4th_" + comp)(dims, d["u"], d["v"], d["w"], d["coo"], d["vis"], fused,
+                                      *h, dt, alpha, N)
+    assert rel_err(fused.numpy() - f["coo"], got2 - f["coo"]) < 1e-14
+
+
+@pytest.mark.parametrize("n3", SHAPES)
+def test_calvel_calvel3_calmas(gpu, n3):
+    from warpflow import kernels as K
+    h = tuple(1.0 / n for n in n3)
+    dt = 1e-3
+    f = rand_fields(n3, ["us", "vs", "ws", "p", "u", "v", "w", "su"], seed=2)
+    d = {k: up(v) for k, v in f.items()}
+    ref = {c: f[c].copy() for c in "uvw"}
+    ops = {"u": (n3[0] - 1, n3[1], n3[2]), "v": (n3[0], n3[1] - 1, n3[2]),
+           "w": (n3[0], n3[1], n3[2] - 1)}
+    for c, a in (("u", 0), ("v", 1), ("w", 2)):
+        getattr(O, "calvel_" + c)(f[c + "s"], f["p"], ref[c], h[a], dt, 0.5, N, ops[c])
+    K.calvel3(n3, d["us"], d["vs"], d["ws"], d["p"], d["u"], d["v"], d["w"], *h, dt, 0.5, N)
+    for c in "uvw":
+        assert rel_err(d[c].numpy() - f[c + "s"], ref[c] - f[c + "s"]) < TOL, c
+        single = up(f[c])
+        getattr(K, "calvel_" + c)(ops[c], d[c + "s"], d["p"], single, h["uvw".index(c)], dt,
+                                  0.5, N)
+        assert np.array_equal(single.numpy(), d[c].numpy()), c
+    # calmas on the calvel output
+    su_ref = f["su"].copy()
+    rmax_ref = O.calmas(ref["u"], ref["v"], ref["w"], su_ref, *h, dt, N, n3)
+    from warpflow.device import zeros
+    rbuf = zeros(1)
+    K.calmas(n3, d["u"], d["v"], d["w"], d["su"], rbuf, *h, dt, N)
+    assert rel_err(d["su"].numpy(), su_ref) < TOL
+    assert abs(rbuf.numpy()[0] - rmax_ref) <= TOL * rmax_ref
+
+
+@pytest.mark.parametrize("n3", SHAPES)
+def test_wale_and_checkdt(gpu, n3):
+    from warpflow import kernels as K
+    from warpflow.device import zeros
+    h = tuple(1.0 / n for n in n3)
+    f = rand_fields(n3, ["u", "v", "w", "vis"], seed=3, scale=0.3)
+    f["vis"] = 0.0025 + np.abs(f["vis"]) * 0.01
+    d = {k: up(v) for k, v in f.items()}
+    vis_ref = f["vis"].copy()
+    O.wale_viscosity(f["u"], f["v"], f["w"], vis_ref, *h, 1 / 400, 0.46, N, n3)
+    K.wale_viscosity(n3, d["u"], d["v"], d["w"], d["vis"], *h, 1 / 400, 0.46, N)
+    assert rel_err(d["vis"].numpy(), vis_ref) < TOL
+    # reductions: separate and fused
+    out4 = zeros(4)
+    K.checkdt(n3, d["u"], d["v"], d["w"], d["vis"], out4, *h, N)
+    got = out4.numpy()
+    exp = [O.maxabs_reduce(f[c], N, n3) for c in "uvw"]
+    exp.append(O.dtvisc_reduce(f["u"], f["v"], f["w"], vis_ref, *h, N, n3))
+    assert np.array_equal(got[:3], exp[:3])
+    assert abs(got[3] - exp[3]) <= 1e-14 * exp[3]
+    b = zeros(1)
+    K.maxabs_reduce(n3, d["w"], b, N)
+    assert b.numpy()[0] == exp[2]
+    b.fill_(1e10)
+    K.dtvisc_reduce(n3, d["u"], d["v"], d["w"], d["vis"], b, *h, N)
+    assert abs(b.numpy()[0] - exp[3]) <= 1e-14 * exp[3]
+
+
+@pytest.mark.parametrize("n3", SHAPES)
+def test_noise_add_fill_bitwise(gpu, n3):
+    from warpflow import kernels as K
+    from warpflow.device import zeros
+    shape = tuple(n + 2 * N for n in n3)
+    ref = [np.zeros(shape) for _ in range(3)]
+    O.init_noise(*ref, 0.01, 7, N, n3)
+    dev = [zeros(shape) for _ in range(3)]
+    K.init_noise(n3, *dev, 0.01, 7, N)
+    for a, b in zip(dev, ref):
+        assert np.array_equal(a.numpy(), b)
+    f = rand_fields(n3, ["p", "pp"], seed=4)
+    p_ref = f["p"].copy()
+    O.add_inplace(p_ref, f["pp"], N, tuple(n + 2 for n in n3))
+    p, pp = up(f["p"]), up(f["pp"])
+    K.add_inplace(tuple(n + 2 for n in n3), p, pp, N)
+    assert np.array_equal(p.numpy(), p_ref)
+    O.fill_const(p_ref, 0.125, N, n3)
+    K.fill_const(n3, p, 0.125, N)
+    assert np.array_equal(p.numpy(), p_ref)
+
+
+@pytest.mark.parametrize("n3", SHAPES)
+def test_ghost_fills_bitwise(gpu, n3):
+    from warpflow.bcs import BoundaryKernels, bc_const, bc_lin
+    bnd = {"west": "wall", "east": "wall", "south": "slip", "north": "slip",
+           "bottom": "wall", "top": "moving_lid"}
+    B = BoundaryKernels(bnd)
+    f = rand_fields(n3, ["u", "v", "w", "vis", "p", "q"], seed=5)
+    d = {k: up(v) for k, v in f.items()}
+    O.apply_u(f["u"], n3, 1.0)
+    B.apply_u(d["u"], n3[0], 1.0)
+    O.apply_v(f["v"], n3)
+    B.apply_v(d["v"], n3[0])
+    O.apply_w(f["w"], n3)
+    B.apply_w(d["w"], n3[0])
+    O.apply_vis(f["vis"], n3, 0.0025)
+    B.apply_vis(d["vis"], n3[0], 0.0025)
+    O.apply_pbound(f["p"], n3)
+    B.apply_pbound(d["p"], n3[0])
+    for k in ("u", "v", "w", "vis", "p"):
+        assert np.array_equal(d[k].numpy(), f[k]), k
+    O.apply_p_mirror(f["q"], n3, 3)
+    B.apply_p_mirror(d["q"], n3[0], 3)
+    assert np.array_equal(d["q"].numpy(), f["q"])
+    # single _mk_lin / _mk_const launches
+    g = rand_fields(n3, ["a"], seed=6)["a"]
+    da = up(g)
+    O.bc_lin(g, 1, True, N + n3[1], N + n3[1] - 1, -1.0, 1, n3[0] + 3, 2, n3[2] + 1, 3)
+    bc_lin(da, 1, True, N + n3[1], N + n3[1] - 1, -1.0, 1, n3[0] + 3, 2, n3[2] + 1, 3)
+    O.bc_const(g, 2, False, N - 1, 0.7, 0, n3[0] + 6, 0, n3[1] + 6, 2)
+    bc_const(da, 2, False, N - 1, 0.7, 0, n3[0] + 6, 0, n3[1] + 6, 2)
+    assert np.array_equal(da.numpy(), g)
+
+
+def test_host_checks_reject_bad_boxes(gpu):
+    from warpflow import _hip
+    from warpflow import kernels as K
+    from warpflow.device import zeros
+    n3 = (8, 8, 8)
+    a = [zeros((14, 14, 14)) for _ in range(5)]
+    with pytest.raises(_hip.HipError, match="invalid argument"):
+        K.conv4th_u((9, 8, 8), *a, 0.1, 0.1, 0.1, 1e-3, 0.5, 3)  # box + reach outside
+    with pytest.raises(_hip.HipError, match="invalid argument"):
+        from warpflow.bcs import bc_lin
+        bc_lin(a[0], 0, False, 2, 3, 1.0, 0, 15, 0, 14, 1)  # perpendicular overrun
+    K.conv4th_u(n3, *a, 0.1, 0.1, 0.1, 1e-3, 0.5, 3)  # the full box is fine

@@ -1,0 +1,117 @@
+"""Whole-step parity of FlowSolver (HIP) against the oracle's step, the
+reference's published step-1 residual, and the tests/test_cavity.py
+invariants on the GPU."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from oracle import np_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_cfg(n3=(24, 24, 24), re=400.0, sgs=True, levels=3, variable=True, dt=3e-4,
+             noise=0.01, precision="fp64"):
+    from warpflow.config import CaseConfig, validate
+    c = CaseConfig()
+    c.grid.nx, c.grid.ny, c.grid.nz = n3
+    c.physics.reynolds = re
+    c.physics.sgs_enabled = sgs
+    c.numerics.mg_levels = levels
+    c.numerics.mg_precision = precision
+    c.time.variable_dt = variable
+    c.time.dt_init = dt
+    c.initial.noise = noise
+    validate(c)
+    return c
+
+
+def gpu_solver(cfg, fused=True):
+    from warpflow import FlowSolver, build_topology
+    return FlowSolver(cfg, build_topology(cfg), "hip:0", fused=fused)
+
+
+def compare_fields(sol, ref, names=("u", "v", "w", "vis"), tol=1e-10):
+    for nm in names:
+        got = sol.f[nm].numpy()
+        e = rel_err(got, ref.f[nm])
+        assert e < tol, (nm, e)
+
+
+def test_initial_state_bitwise(gpu):
+    cfg = make_cfg()
+    sol = gpu_solver(cfg)
+    ref = O.Solver(O.Case.from_config(cfg))
+    for nm in ("u", "v", "w", "p", "vis"):
+        assert np.array_equal(sol.f[nm].numpy(), ref.f[nm]), nm
+
+
+@pytest.mark.parametrize("n3,sgs,variable", [((24, 24, 24), True, True),
+                                             ((32, 32, 32), False, False),
+                                             ((16, 24, 32), True, True)])
+def test_steps_match_oracle(gpu, n3, sgs, variable):
+    cfg = make_cfg(n3=n3, sgs=sgs, variable=variable, dt=3e-4 if variable else 1e-3,
+                   re=400.0 if sgs else 100.0)
+    sol = gpu_solver(cfg)
+    ref = O.Solver(O.Case.from_config(cfg))
+    for it in range(5):
+        a, b = sol.step(), ref.step()
+        assert a["iters"] == b["iters"], it
+        assert a["dt"] == pytest.approx(b["dt"], rel=1e-13)
+        assert a["rmax"] == pytest.approx(b["rmax"], rel=1e-8)
+        assert a["cfl"] == pytest.approx(b["cfl"], rel=1e-12)
+    compare_fields(sol, ref)
+    # pressure: the MG drift constant is reproduced too (no mean removal)
+    assert rel_err(sol.f["p"].numpy(), ref.f["p"]) < 1e-9
+    # ghosts, incl. the history-dependent edges, match the oracle's fills
+    for nm in ("u", "v", "w"):
+        assert rel_err(sol.f[nm].numpy(), ref.f[nm]) < 1e-10
+
+
+def test_fused_equals_reference_launch_order(gpu):
+    cfg = make_cfg()
+    a, b = gpu_solver(cfg, fused=True), gpu_solver(cfg, fused=False)
+    for _ in range(4):
+        ia, ib = a.step(), b.step()
+        assert ia["iters"] == ib["iters"]
+    for nm in ("u", "v", "w", "p", "vis", "ustar", "vstar", "wstar"):
+        assert rel_err(a.f[nm].numpy(), b.f[nm].numpy()) < 1e-13, nm
+
+
+def test_step1_rmax_matches_report(gpu):
+    """REPORT.md:51,62: 80^3 Re 400, WALE, 1% noise, fixed dt 3e-4:
+    step-1 rmax = 1.69e-7."""
+    cfg = make_cfg(n3=(80, 80, 80), variable=False, dt=3e-4)
+    sol = gpu_solver(cfg)
+    info = sol.step()
+    assert info["rmax"] == pytest.approx(1.69e-7, abs=0.006e-7)
+    assert info["iters"] == 3
+
+
+@pytest.fixture(scope="module")
+def cavity24_gpu(gpu):
+    """tests/test_cavity.py:56-65 on the GPU: 24^3, 30 steps."""
+    cfg = make_cfg()
+    sol = gpu_solver(cfg)
+    infos = [sol.step() for _ in range(30)]
+    return sol, infos
+
+
+def test_cavity_invariants(cavity24_gpu):
+    sol, infos = cavity24_gpu
+    for info in infos:
+        assert info["rmax"] < 1.0e-6 and info["iters"] <= 6
+    N = 3
+    n = sol.ncells[0]
+    s = slice(N, N + n)
+    g = {c: sol.f[c].numpy() for c in ("u", "v", "w", "p", "vis")}
+    for c in g:
+        assert np.isfinite(g[c]).all(), c
+    assert np.allclose(g["u"][N:N + n, N:N + n, N + n], sol.ulid)
+    assert np.allclose(g["u"][N - 1, s, s], 0) and np.allclose(g["u"][N + n, s, s], 0)
+    assert np.allclose(g["v"][s, N - 1, s], 0) and np.allclose(g["v"][s, N + n, s], 0)
+    assert np.allclose(g["w"][s, s, N - 1], 0) and np.allclose(g["w"][s, s, N + n], 0)
+    p = g["p"]
+    assert np.allclose(p[N - 1, s, s], p[N, s, s]) and np.allclose(p[N + n, s, s], p[N + n - 1, s, s])
+    assert np.allclose(p[s, N - 1, s], p[s, N, s]) and np.allclose(p[s, N + n, s], p[s, N + n - 1, s])
+    assert np.allclose(p[s, s, N - 1], p[s, s, N]) and np.allclose(p[s, s, N + n], p[s, s, N + n - 1])

@@ -1,0 +1,131 @@
+// Runtime side of the C ABI: device selection, device memory, copies,
+// fills, streams and events.  Fills use a kernel rather than
+// hipMemsetAsync so that the same calls are safe inside captured graphs.
+#include <string.h>
+
+#include "wf_common.h"
+
+namespace wf {
+
+template <class T>
+__global__ void __launch_bounds__(256) fill_kernel(T* __restrict__ p, T v, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = t; q < n; q += stride) p[q] = v;
+}
+
+template <class T>
+int fill(T* p, T v, size_t n, hipStream_t s) {
+  if (n == 0) return 0;
+  if (!p) return WF_EINVAL;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(fill_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+  WF_LAUNCHED();
+  return 0;
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+extern "C" {
+
+int wf_device_count(int* n) { return (int)hipGetDeviceCount(n); }
+
+int wf_set_device(int device) { return (int)hipSetDevice(device); }
+
+int wf_get_device_name(char* buf, int len) {
+  int dev = 0;
+  WF_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  WF_TRY(hipGetDeviceProperties(&p, dev));
+  if (len > 0) {
+    strncpy(buf, p.gcnArchName, (size_t)len - 1);
+    buf[len - 1] = 0;
+  }
+  return 0;
+}
+
+int wf_get_device_attr(int* n_cu, int* max_lds_per_block, int* warp_size) {
+  int dev = 0;
+  WF_TRY(hipGetDevice(&dev));
+  WF_TRY(hipDeviceGetAttribute(n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  WF_TRY(hipDeviceGetAttribute(max_lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                               dev));
+  WF_TRY(hipDeviceGetAttribute(warp_size, hipDeviceAttributeWarpSize, dev));
+  return 0;
+}
+
+int wf_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return WF_EINVAL;
+  *ptr = nullptr;
+  if (bytes == 0) bytes = 256;
+  WF_TRY(hipMalloc(ptr, bytes));
+  const hipError_t e = hipMemset(*ptr, 0, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(*ptr);
+    *ptr = nullptr;
+    return (int)e;
+  }
+  return 0;
+}
+
+int wf_free(void* ptr) { return ptr ? (int)hipFree(ptr) : 0; }
+
+int wf_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  WF_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return (int)hipStreamSynchronize((hipStream_t)stream);
+}
+
+int wf_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  WF_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return (int)hipStreamSynchronize((hipStream_t)stream);
+}
+
+int wf_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+}
+
+int wf_memset(void* ptr, int value, size_t bytes, void* stream) {
+  return (int)hipMemsetAsync(ptr, value, bytes, (hipStream_t)stream);
+}
+
+int wf_fill_f64(double* ptr, double value, size_t n, void* stream) {
+  return fill<double>(ptr, value, n, (hipStream_t)stream);
+}
+
+int wf_fill_f32(float* ptr, float value, size_t n, void* stream) {
+  return fill<float>(ptr, value, n, (hipStream_t)stream);
+}
+
+int wf_synchronize(void) { return (int)hipDeviceSynchronize(); }
+
+int wf_stream_create(void** stream) {
+  return (int)hipStreamCreateWithFlags((hipStream_t*)stream, hipStreamNonBlocking);
+}
+
+int wf_stream_destroy(void* stream) { return (int)hipStreamDestroy((hipStream_t)stream); }
+
+int wf_stream_synchronize(void* stream) {
+  return (int)hipStreamSynchronize((hipStream_t)stream);
+}
+
+int wf_event_create(void** event) { return (int)hipEventCreate((hipEvent_t*)event); }
+
+int wf_event_destroy(void* event) { return (int)hipEventDestroy((hipEvent_t)event); }
+
+int wf_event_record(void* event, void* stream) {
+  return (int)hipEventRecord((hipEvent_t)event, (hipStream_t)stream);
+}
+
+int wf_event_elapsed_ms(float* ms, void* start, void* stop) {
+  WF_TRY(hipEventSynchronize((hipEvent_t)stop));
+  return (int)hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop);
+}
+
+const char* wf_error_string(int code) {
+  if (code == WF_EINVAL) return "warpflow: invalid argument (shape/range check failed)";
+  return hipGetErrorString((hipError_t)code);
+}
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+// Ghost-layer fills of the cavity boundary set (the reference's
+// warpflow/bcs.py): W/E/B no-slip walls, S/N slip planes, T moving lid.
+//
+// The reference issues one 2-D launch per side in the order W, E, S, N, B,
+// T.  A side operation reads and writes only the line of cells along its
+// axis through one perpendicular position (a, b), so the two sides of one
+// axis can share a launch: a thread runs the low-side then the high-side
+// operations of its (a, b) line, which is exactly the reference's per-cell
+// order.  Different axes stay in separate launches because the y fill reads
+// ghosts written by the x fill at other (a, b) positions, and so on.
+#include "wf_common.h"
+
+namespace wf {
+
+struct BcOp {
+  int32_t kind;  // 0: ghost = val; 1: ghost = val * source
+  int32_t high;
+  int32_t g0, s0;
+  int32_t nlay;
+  double val;
+};
+
+struct BcOps {
+  BcOp op[4];
+  int32_t n;
+};
+
+__global__ void __launch_bounds__(256)
+bc_axis_kernel(double* __restrict__ f, wf_grid g, int axis, int a0, int na, int b0, int nb,
+               BcOps ops) {
+  const int bb = blockIdx.x * blockDim.x + threadIdx.x;
+  const int aa = blockIdx.y * blockDim.y + threadIdx.y;
+  if (aa >= na || bb >= nb) return;
+  const int a = a0 + aa, b = b0 + bb;
+  int64_t base, s;
+  if (axis == 0) {
+    base = at(g, 0, a, b);
+    s = g.si;
+  } else if (axis == 1) {
+    base = at(g, a, 0, b);
+    s = g.sj;
+  } else {
+    base = at(g, a, b, 0);
+    s = 1;
+  }
+  for (int q = 0; q < ops.n; ++q) {
+    const BcOp o = ops.op[q];
+    for (int m = 0; m < o.nlay; ++m) {
+      const int gi = o.high ? o.g0 + m : o.g0 - m;
+      if (o.kind == 0) {
+        f[base + gi * s] = o.val;
+      } else {
+        const int si = o.high ? o.s0 - m : o.s0 + m;
+        f[base + gi * s] = o.val * f[base + si * s];
+      }
+    }
+  }
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+static int axis_extent(const wf_grid& g, int axis) {
+  return (axis == 0 ? g.ni : (axis == 1 ? g.nj : g.nk)) + 2 * g.npl;
+}
+
+// Check every index an op list touches against the array before launching.
+static bool ops_ok(const wf_grid& g, int axis, int a0, int na, int b0, int nb,
+                   const BcOps& ops) {
+  if (g.ni <= 0 || g.nj <= 0 || g.nk <= 0 || axis < 0 || axis > 2) return false;
+  if (g.sj < (int64_t)(g.nk + 2 * g.npl) || g.si < g.sj * (int64_t)(g.nj + 2 * g.npl))
+    return false;
+  const int pa = axis == 0 ? 1 : 0, pb = axis == 2 ? 1 : 2;
+  if (na < 0 || nb < 0 || a0 < 0 || b0 < 0) return false;
+  if (na > 0 && a0 + na > axis_extent(g, pa)) return false;
+  if (nb > 0 && b0 + nb > axis_extent(g, pb)) return false;
+  const int n = axis_extent(g, axis);
+  for (int q = 0; q < ops.n; ++q) {
+    const BcOp& o = ops.op[q];
+    if (o.nlay < 0) return false;
+    for (int m = 0; m < o.nlay; ++m) {
+      const int gi = o.high ? o.g0 + m : o.g0 - m;
+      if (gi < 0 || gi >= n) return false;
+      if (o.kind == 1) {
+        const int si = o.high ? o.s0 - m : o.s0 + m;
+        if (si < 0 || si >= n) return false;
+      }
+    }
+  }
+  return true;
+}
+
+static int launch_axis(double* f, const wf_grid& g, int axis, int a0, int na, int b0, int nb,
+                       const BcOps& ops, hipStream_t s) {
+  if (!ops_ok(g, axis, a0, na, b0, nb, ops)) return WF_EINVAL;
+  if (na == 0 || nb == 0 || ops.n == 0) return 0;
+  dim3 block(64, 4), grid(div_up(nb, 64), div_up(na, 4));
+  hipLaunchKernelGGL(bc_axis_kernel, grid, block, 0, s, f, g, axis, a0, na, b0, nb, ops);
+  WF_LAUNCHED();
+  return 0;
+}
+
+static BcOp C_(int high, int g0, double val, int nlay) {
+  return BcOp{0, high, g0, 0, nlay, val};
+}
+static BcOp L_(int high, int g0, int s0, double sign, int nlay) {
+  return BcOp{1, high, g0, s0, nlay, sign};
+}
+static BcOps ops2(BcOp a, BcOp b) {
+  BcOps o{};
+  o.op[0] = a;
+  o.op[1] = b;
+  o.n = 2;
+  return o;
+}
+
+extern "C" {
+
+int wf_bc_lin(double* fld, wf_grid g, int axis, int high, int g0, int s0, double sign,
+              int a0, int na, int b0, int nb, int nlay, void* stream) {
+  BcOps o{};
+  o.op[0] = L_(high != 0, g0, s0, sign, nlay);
+  o.n = 1;
+  return launch_axis(fld, g, axis, a0, na, b0, nb, o, (hipStream_t)stream);
+}
+
+int wf_bc_const(double* fld, wf_grid g, int axis, int high, int g0, double val, int a0,
+                int na, int b0, int nb, int nlay, void* stream) {
+  BcOps o{};
+  o.op[0] = C_(high != 0, g0, val, nlay);
+  o.n = 1;
+  return launch_axis(fld, g, axis, a0, na, b0, nb, o, (hipStream_t)stream);
+}
+
+// Per-axis restatement of BoundaryKernels.apply_u/v/w/vis/pbound/p_mirror
+// (bcs.py:201-311).  The reference uses one n for every axis; here each
+// range uses its own axis' count, identical for a cubic grid.
+int wf_bc_fill(double* f, wf_grid g, int which, double param, void* stream) {
+  const hipStream_t s = (hipStream_t)stream;
+  const int N = g.npl, ni = g.ni, nj = g.nj, nk = g.nk;
+  if (N < 1) return WF_EINVAL;
+  switch (which) {
+    case 0: {  // apply_u, bcs.py:201-223; param = lid velocity
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 0, N - 1, nj + 2, N - 1, nk + 2,
+          ops2(C_(0, N - 1, 0.0, 3), C_(1, N + ni - 1, 0.0, 3)), s));
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 1, N - 1, ni + 1, N - 1, nk + 2,
+          ops2(L_(0, N - 1, N, 1.0, 3), L_(1, N + nj, N + nj - 1, 1.0, 3)), s));
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 2, N - 1, ni + 1, N - 1, nj + 2,
+          ops2(L_(0, N - 1, N, -1.0, 3), C_(1, N + nk, param, 3)), s));
+      return 0;
+    }
+    case 1: {  // apply_v, bcs.py:225-244
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 0, N - 1, nj + 1, N - 1, nk + 2,
+          ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + ni, N + ni - 1, -1.0, 3)), s));
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 1, N - 1, ni + 2, N - 1, nk + 2,
+          ops2(C_(0, N - 1, 0.0, 3), C_(1, N + nj - 1, 0.0, 3)), s));
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 2, N - 1, ni + 2, N - 1, nj + 1,
+          ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + nk, N + nk - 1, 1.0, 3)), s));
+      return 0;
+    }
+    case 2: {  // apply_w, bcs.py:246-265
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 0, N - 1, nj + 2, N - 1, nk + 1,
+          ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + ni, N + ni - 1, -1.0, 3)), s));
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 1, N - 1, ni + 2, N - 1, nk + 1,
+          ops2(L_(0, N - 1, N, 1.0, 3), L_(1, N + nj, N + nj - 1, 1.0, 3)), s));
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 2, N - 1, ni + 2, N - 1, nj + 2,
+          ops2(C_(0, N - 1, 0.0, 3), C_(1, N + nk - 1, 0.0, 3)), s));
+      return 0;
+    }
+    case 3: {  // apply_vis, bcs.py:267-293; param = 1/Re
+      BcOps x{};
+      x.op[0] = C_(0, N, param, 1);
+      x.op[1] = L_(0, N - 1, N, 1.0, 1);
+      x.op[2] = C_(1, N + ni - 1, param, 1);
+      x.op[3] = L_(1, N + ni, N + ni - 1, 1.0, 1);
+      x.n = 4;
+      WF_TRY((hipError_t)launch_axis(f, g, 0, N - 1, nj + 2, N - 1, nk + 2, x, s));
+      WF_TRY((hipError_t)launch_axis(
+          f, g, 1, N - 1, ni + 2, N - 1, nk + 2,
+          ops2(L_(0, N - 1, N, 1.0, 1), L_(1, N + nj, N + nj - 1, 1.0, 1)), s));
+      BcOps z{};
+      z.op[0] = C_(0, N, param, 1);
+      z.op[1] = L_(0, N - 1, N, 1.0, 1);
+      z.op[2] = L_(1, N + nk, N + nk - 1, 1.0, 1);
+      z.n = 3;
+      WF_TRY((hipError_t)launch_axis(f, g, 2, N - 1, ni + 2, N - 1, nj + 2, z, s));
+      return 0;
+    }
+    case 4: {  // apply_pbound, bcs.py:295-302
+      const int n3[3] = {ni, nj, nk};
+      for (int ax = 0; ax < 3; ++ax) {
+        const int pa = ax == 0 ? nj : ni, pb = ax == 2 ? nj : nk;
+        WF_TRY((hipError_t)launch_axis(
+            f, g, ax, N - 1, pa + 3, N - 1, pb + 3,
+            ops2(L_(0, N - 1, N, 1.0, 1), L_(1, N + n3[ax], N + n3[ax] - 1, 1.0, 1)), s));
+      }
+      return 0;
+    }
+    case 5: {  // apply_p_mirror, bcs.py:304-311; param = layers
+      const int nl = (int)param;
+      const int n3[3] = {ni, nj, nk};
+      for (int ax = 0; ax < 3; ++ax) {
+        const int pa = ax == 0 ? nj : ni, pb = ax == 2 ? nj : nk;
+        WF_TRY((hipError_t)launch_axis(
+            f, g, ax, 0, pa + 2 * N, 0, pb + 2 * N,
+            ops2(L_(0, N - 1, N, 1.0, nl), L_(1, N + n3[ax], N + n3[ax] - 1, 1.0, nl)), s));
+      }
+      return 0;
+    }
+    default:
+      return WF_EINVAL;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,578 @@
+// Geometric multigrid for the cell-centred 7-point Neumann Poisson problem
+// (the reference's warpflow/multigrid.py), matrix-free, templated on the
+// level precision T (double for the plain FP64 form, float for the V-cycle
+// of the mixed form).
+//
+// Level arrays are (n0+2, n1+2, n2+2) with a one-cell ring.  Face
+// coefficients are generated from the cell index: 1/h^2, or 0 across a
+// physical face (all six faces of the single cavity domain), and
+// ap = -(aw+ae+as+an+ab+at) -- the values _mgcoeff stores
+// (multigrid.py:23-67), without the seven coefficient arrays.
+//
+// Ring state.  The reference refreshes the ring faces (Neumann mirror,
+// faces only) after every full sweep and after every prolongation, zeroes the
+// whole coarse array after restriction, and never writes edge/corner ring
+// cells.  Smoothing and restriction multiply ring values by zero
+// coefficients; only prolongation and the final copy-out read them.  Here
+// the kernel that last updates a boundary cell writes its face-ring images
+// (relax, prolong), restriction zeroes them, and edges/corners stay at the
+// zero they were allocated with -- the same ring state at every point the
+// reference could observe it.
+#include <algorithm>
+
+#include "wf_common.h"
+
+namespace wf {
+
+struct MgDev {
+  int n0, n1, n2;
+  int64_t s0, s1;  // strides of i and j in the (n+2)^3 array
+};
+
+inline MgDev mg_dev(const wf_mg_level& L) {
+  MgDev d;
+  d.n0 = L.n0;
+  d.n1 = L.n1;
+  d.n2 = L.n2;
+  d.s1 = L.n2 + 2;
+  d.s0 = (int64_t)(L.n1 + 2) * d.s1;
+  return d;
+}
+
+// Array index of real cell (i, j, k), 0-based.
+__host__ __device__ inline int64_t mg_at(const MgDev& d, int i, int j, int k) {
+  return (int64_t)(i + 1) * d.s0 + (int64_t)(j + 1) * d.s1 + (k + 1);
+}
+
+template <class T>
+struct Coef {
+  T aw, ae, as, an, ab, at, ap;
+};
+
+// _mgcoeff at real cell (i, j, k); c0..c2 = 1/h^2 in precision T.
+template <class T>
+__device__ inline Coef<T> mg_coef(const MgDev& d, int i, int j, int k, T c0, T c1, T c2) {
+  Coef<T> c;
+  c.aw = i == 0 ? T(0) : c0;
+  c.ae = i == d.n0 - 1 ? T(0) : c0;
+  c.as = j == 0 ? T(0) : c1;
+  c.an = j == d.n1 - 1 ? T(0) : c1;
+  c.ab = k == 0 ? T(0) : c2;
+  c.at = k == d.n2 - 1 ? T(0) : c2;
+  c.ap = -(c.aw + c.ae + c.as + c.an + c.ab + c.at);
+  return c;
+}
+
+// aw*W + ae*E + as*S + an*N + ab*B + at*T (multigrid.py:85-90)
+template <class T>
+__device__ inline T nb_sum(const T* __restrict__ phi, int64_t x, const MgDev& d,
+                           const Coef<T>& c) {
+  return c.aw * phi[x - d.s0] + c.ae * phi[x + d.s0] + c.as * phi[x - d.s1] +
+         c.an * phi[x + d.s1] + c.ab * phi[x - 1] + c.at * phi[x + 1];
+}
+
+// Write the face-ring images of real cell (i, j, k) holding value v.
+template <class T>
+__device__ inline void ring_images(T* __restrict__ phi, const MgDev& d, int i, int j, int k,
+                                   int64_t x, T v) {
+  if (i == 0) phi[x - d.s0] = v;
+  if (i == d.n0 - 1) phi[x + d.s0] = v;
+  if (j == 0) phi[x - d.s1] = v;
+  if (j == d.n1 - 1) phi[x + d.s1] = v;
+  if (k == 0) phi[x - 1] = v;
+  if (k == d.n2 - 1) phi[x + 1] = v;
+}
+
+// Thread -> (i, j, k) of colour `color` cells: k = 2m + ((i+j+color+1) & 1),
+// i.e. (i+j+k+color) odd (multigrid.py:80).
+__device__ inline bool colour_cell(const MgDev& d, int color, int& i, int& j, int& k) {
+  const int m = blockIdx.x * 64 + threadIdx.x;
+  j = blockIdx.y * 4 + threadIdx.y;
+  i = blockIdx.z;
+  if (j >= d.n1 || i >= d.n0) return false;
+  k = 2 * m + ((i + j + color + 1) & 1);
+  return k < d.n2;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+rbgs_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, T c1, T c2,
+            int color, int write_ring) {
+  int i, j, k;
+  if (!colour_cell(d, color, i, j, k)) return;
+  const int64_t x = mg_at(d, i, j, k);
+  const Coef<T> c = mg_coef(d, i, j, k, c0, c1, c2);
+  const T v = (rhs[x] - nb_sum(phi, x, d, c)) / c.ap;
+  phi[x] = v;
+  if (write_ring) ring_images(phi, d, i, j, k, x, v);
+}
+
+// Faces-only Neumann mirror of all six sides (bcs.py:326-360 via
+// multigrid.py:264-287); one thread per (face, a, b).
+template <class T>
+__global__ void __launch_bounds__(256)
+refill_kernel(T* __restrict__ phi, MgDev d) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int fx = d.n1 * d.n2, fy = d.n0 * d.n2, fz = d.n0 * d.n1;
+  int q = t;
+  if (q < 2 * fx) {
+    const int hi = q >= fx;
+    q -= hi * fx;
+    const int j = q / d.n2, k = q % d.n2;
+    const int i = hi ? d.n0 - 1 : 0;
+    const int64_t x = mg_at(d, i, j, k);
+    phi[x + (hi ? d.s0 : -d.s0)] = phi[x];
+    return;
+  }
+  q -= 2 * fx;
+  if (q < 2 * fy) {
+    const int hi = q >= fy;
+    q -= hi * fy;
+    const int i = q / d.n2, k = q % d.n2;
+    const int j = hi ? d.n1 - 1 : 0;
+    const int64_t x = mg_at(d, i, j, k);
+    phi[x + (hi ? d.s1 : -d.s1)] = phi[x];
+    return;
+  }
+  q -= 2 * fy;
+  if (q < 2 * fz) {
+    const int hi = q >= fz;
+    q -= hi * fz;
+    const int i = q / d.n1, j = q % d.n1;
+    const int k = hi ? d.n2 - 1 : 0;
+    const int64_t x = mg_at(d, i, j, k);
+    phi[x + (hi ? 1 : -1)] = phi[x];
+  }
+}
+
+// Residual rhs - A phi at a fine real cell, ap*phi added last
+// (multigrid.py:110-117).
+template <class T>
+__device__ inline T residual_at(const T* __restrict__ phi, const T* __restrict__ rhs,
+                                const MgDev& d, int i, int j, int k, T c0, T c1, T c2) {
+  const int64_t x = mg_at(d, i, j, k);
+  const Coef<T> c = mg_coef(d, i, j, k, c0, c1, c2);
+  return rhs[x] - (nb_sum(phi, x, d, c) + c.ap * phi[x]);
+}
+
+// _restrict (multigrid.py:93-119) + coarse phi = 0 (multigrid.py:312).
+template <class T>
+__global__ void __launch_bounds__(256)
+restrict_kernel(const T* __restrict__ phif, const T* __restrict__ rhsf, MgDev f, T c0, T c1,
+                T c2, T* __restrict__ phic, T* __restrict__ rhsc, MgDev c) {
+  const int ck = blockIdx.x * 64 + threadIdx.x;
+  const int cj = blockIdx.y * 4 + threadIdx.y;
+  const int ci = blockIdx.z;
+  if (ck >= c.n2 || cj >= c.n1 || ci >= c.n0) return;
+  T acc = T(0);
+  for (int di = 0; di < 2; ++di)
+    for (int dj = 0; dj < 2; ++dj)
+      for (int dk = 0; dk < 2; ++dk)
+        acc = acc + residual_at(phif, rhsf, f, 2 * ci + di, 2 * cj + dj, 2 * ck + dk, c0, c1,
+                                c2);
+  const int64_t x = mg_at(c, ci, cj, ck);
+  rhsc[x] = acc * T(0.125);
+  phic[x] = T(0);
+  ring_images(phic, c, ci, cj, ck, x, T(0));
+}
+
+// _prolong (multigrid.py:122-146) + fine refill (multigrid.py:321).
+template <class T>
+__global__ void __launch_bounds__(256)
+prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev f) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= f.n2 || j >= f.n1 || i >= f.n0) return;
+  const int c0x = (i % 2 == 0) ? i / 2 - 1 : i / 2;
+  const int c0y = (j % 2 == 0) ? j / 2 - 1 : j / 2;
+  const int c0z = (k % 2 == 0) ? k / 2 - 1 : k / 2;
+  const T wx0 = (i % 2 == 0) ? T(0.25) : T(0.75);
+  const T wy0 = (j % 2 == 0) ? T(0.25) : T(0.75);
+  const T wz0 = (k % 2 == 0) ? T(0.25) : T(0.75);
+  T acc = T(0);
+  for (int di = 0; di < 2; ++di) {
+    const T wxi = di == 0 ? wx0 : T(1) - wx0;
+    for (int dj = 0; dj < 2; ++dj) {
+      const T wyj = dj == 0 ? wy0 : T(1) - wy0;
+      for (int dk = 0; dk < 2; ++dk) {
+        const T wzk = dk == 0 ? wz0 : T(1) - wz0;
+        // coarse array index of real cell (c0+d) is (c0+d+1): the ring at -1
+        const int64_t xc = (int64_t)(c0x + di + 1) * c.s0 + (int64_t)(c0y + dj + 1) * c.s1 +
+                           (c0z + dk + 1);
+        acc = acc + wxi * wyj * wzk * phic[xc];
+      }
+    }
+  }
+  const int64_t x = mg_at(f, i, j, k);
+  const T v = phif[x] + acc;
+  phif[x] = v;
+  ring_images(phif, f, i, j, k, x, v);
+}
+
+// n sweeps of RBGS on one small level held in LDS by one workgroup
+// (multigrid.py:289-299 with the refill folded in).  The real cells of phi
+// live in LDS; ring terms carry zero coefficients and are skipped.  rhs is
+// in LDS too when RHS_LDS, else read from global memory (L2-resident).
+template <class T, bool RHS_LDS>
+__global__ void __launch_bounds__(1024)
+relax_lds_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, T c1, T c2,
+                 int nsweeps) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sp = reinterpret_cast<T*>(smem_raw);
+  const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
+  const int ncell = n0 * n1 * n2;
+  T* sr = sp + ncell;
+  const int nt = blockDim.x, tid = threadIdx.x;
+  for (int l = tid; l < ncell; l += nt) {
+    const int k = l % n2, j = (l / n2) % n1, i = l / (n1 * n2);
+    const int64_t x = mg_at(d, i, j, k);
+    sp[l] = phi[x];
+    if (RHS_LDS) sr[l] = rhs[x];
+  }
+  __syncthreads();
+  const int np = (n2 + 1) / 2;
+  const int nrow = n0 * n1;
+  const int S0 = n1 * n2, S1 = n2;
+  for (int s = 0; s < nsweeps; ++s) {
+    for (int color = 0; color < 2; ++color) {
+      for (int q = tid; q < nrow * np; q += nt) {
+        const int m = q % np, r = q / np;
+        const int j = r % n1, i = r / n1;
+        const int k = 2 * m + ((i + j + color + 1) & 1);
+        if (k >= n2) continue;
+        const int l = (i * n1 + j) * n2 + k;
+        const Coef<T> c = mg_coef(d, i, j, k, c0, c1, c2);
+        const T W = i > 0 ? sp[l - S0] : T(0);
+        const T E = i < n0 - 1 ? sp[l + S0] : T(0);
+        const T S = j > 0 ? sp[l - S1] : T(0);
+        const T N = j < n1 - 1 ? sp[l + S1] : T(0);
+        const T B = k > 0 ? sp[l - 1] : T(0);
+        const T Tt = k < n2 - 1 ? sp[l + 1] : T(0);
+        const T rv = RHS_LDS ? sr[l] : rhs[mg_at(d, i, j, k)];
+        const T sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt;
+        sp[l] = (rv - sum) / c.ap;
+      }
+      __syncthreads();
+    }
+  }
+  for (int l = tid; l < ncell; l += nt) {
+    const int k = l % n2, j = (l / n2) % n1, i = l / (n1 * n2);
+    const int64_t x = mg_at(d, i, j, k);
+    const T v = sp[l];
+    phi[x] = v;
+    if (nsweeps > 0) ring_images(phi, d, i, j, k, x, v);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+copy_in_kernel(const double* __restrict__ su, wf_grid g, double* __restrict__ rhs, MgDev d) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= d.n2 || j >= d.n1 || i >= d.n0) return;
+  rhs[mg_at(d, i, j, k)] = su[at(g, i + g.npl, j + g.npl, k + g.npl)];
+}
+
+__global__ void __launch_bounds__(256)
+copy_out_kernel(const double* __restrict__ phi, MgDev d, double* __restrict__ dst, wf_grid g) {
+  // over the whole (n+2)^3 level array, ring included (multigrid.py:157-163)
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= d.n2 + 2 || j >= d.n1 + 2 || i >= d.n0 + 2) return;
+  dst[at(g, i + g.npl - 1, j + g.npl - 1, k + g.npl - 1)] =
+      phi[(int64_t)i * d.s0 + (int64_t)j * d.s1 + k];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) zero_kernel(T* __restrict__ p, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = T(0);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+coeff_kernel(MgDev d, T c0, T c1, T c2, T* aw, T* ae, T* as, T* an, T* ab, T* at, T* ap) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= d.n2 || j >= d.n1 || i >= d.n0) return;
+  const Coef<T> c = mg_coef(d, i, j, k, c0, c1, c2);
+  const int64_t x = mg_at(d, i, j, k);
+  aw[x] = c.aw;
+  ae[x] = c.ae;
+  as[x] = c.as;
+  an[x] = c.an;
+  ab[x] = c.ab;
+  at[x] = c.at;
+  ap[x] = c.ap;
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+namespace {
+
+bool level_ok(const wf_mg_level* L) {
+  return L && L->phi && L->rhs && L->n0 > 0 && L->n1 > 0 && L->n2 > 0 &&
+         (L->prec == 8 || L->prec == 4) && L->h0 > 0 && L->h1 > 0 && L->h2 > 0;
+}
+
+int64_t level_elems(const wf_mg_level& L) {
+  return (int64_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
+}
+
+// 1/h^2 exactly as _mgcoeff forms it: 1.0 / (h*h) in float64
+// (multigrid.py:34-38); the float32 form rounds that value.
+template <class T>
+void inv_h2(const wf_mg_level& L, T& c0, T& c1, T& c2) {
+  c0 = (T)(1.0 / (L.h0 * L.h0));
+  c1 = (T)(1.0 / (L.h1 * L.h1));
+  c2 = (T)(1.0 / (L.h2 * L.h2));
+}
+
+dim3 cell_grid(int n0, int n1, int n2) { return dim3(div_up(n2, 64), div_up(n1, 4), n0); }
+
+int g_max_lds = -1;
+
+int max_lds() {
+  if (g_max_lds < 0) {
+    int dev = 0, v = 65536;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+            hipSuccess)
+      g_max_lds = v;
+    else
+      g_max_lds = 65536;
+  }
+  return g_max_lds;
+}
+
+template <class T>
+int launch_rbgs(const wf_mg_level& L, int color, int write_ring, hipStream_t s) {
+  const MgDev d = mg_dev(L);
+  T c0, c1, c2;
+  inv_h2(L, c0, c1, c2);
+  hipLaunchKernelGGL(rbgs_kernel<T>, cell_grid(d.n0, d.n1, (d.n2 + 1) / 2), dim3(64, 4), 0, s,
+                     (T*)L.phi, (const T*)L.rhs, d, c0, c1, c2, color, write_ring);
+  WF_LAUNCHED();
+  return 0;
+}
+
+template <class T, bool RHS_LDS>
+int launch_relax_lds(const wf_mg_level& L, int nsweeps, size_t bytes, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    WF_TRY(hipFuncSetAttribute((const void*)relax_lds_kernel<T, RHS_LDS>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds()));
+    attr_set = true;
+  }
+  const MgDev d = mg_dev(L);
+  T c0, c1, c2;
+  inv_h2(L, c0, c1, c2);
+  const int ncell = d.n0 * d.n1 * d.n2;
+  const int nt = ncell >= 4096 ? 1024 : (ncell >= 1024 ? 512 : 256);
+  hipLaunchKernelGGL((relax_lds_kernel<T, RHS_LDS>), dim3(1), dim3(nt), bytes, s, (T*)L.phi,
+                     (const T*)L.rhs, d, c0, c1, c2, nsweeps);
+  WF_LAUNCHED();
+  return 0;
+}
+
+template <class T>
+int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s) {
+  if (nsweeps <= 0) return 0;
+  const size_t cells = (size_t)L.n0 * L.n1 * L.n2;
+  const size_t lds = (size_t)max_lds();
+  if (2 * cells * sizeof(T) <= lds) return launch_relax_lds<T, true>(L, nsweeps, 2 * cells * sizeof(T), s);
+  if (cells * sizeof(T) <= lds) return launch_relax_lds<T, false>(L, nsweeps, cells * sizeof(T), s);
+  for (int q = 0; q < nsweeps; ++q) {
+    WF_TRY((hipError_t)launch_rbgs<T>(L, 0, 1, s));
+    WF_TRY((hipError_t)launch_rbgs<T>(L, 1, 1, s));
+  }
+  return 0;
+}
+
+template <class T>
+int restrict_t(const wf_mg_level& F, const wf_mg_level& C, hipStream_t s) {
+  const MgDev f = mg_dev(F), c = mg_dev(C);
+  T c0, c1, c2;
+  inv_h2(F, c0, c1, c2);
+  hipLaunchKernelGGL(restrict_kernel<T>, cell_grid(c.n0, c.n1, c.n2), dim3(64, 4), 0, s,
+                     (const T*)F.phi, (const T*)F.rhs, f, c0, c1, c2, (T*)C.phi, (T*)C.rhs, c);
+  WF_LAUNCHED();
+  return 0;
+}
+
+template <class T>
+int prolong_t(const wf_mg_level& C, const wf_mg_level& F, hipStream_t s) {
+  const MgDev f = mg_dev(F), c = mg_dev(C);
+  hipLaunchKernelGGL(prolong_kernel<T>, cell_grid(f.n0, f.n1, f.n2), dim3(64, 4), 0, s,
+                     (const T*)C.phi, c, (T*)F.phi, f);
+  WF_LAUNCHED();
+  return 0;
+}
+
+template <class T>
+int zero_t(void* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(zero_kernel<T>, dim3(div_up(n, 256)), dim3(256), 0, s, (T*)p, n);
+  WF_LAUNCHED();
+  return 0;
+}
+
+bool pair_ok(const wf_mg_level* F, const wf_mg_level* C) {
+  return level_ok(F) && level_ok(C) && F->prec == C->prec && F->n0 == 2 * C->n0 &&
+         F->n1 == 2 * C->n1 && F->n2 == 2 * C->n2;
+}
+
+int relax_any(const wf_mg_level& L, int n, hipStream_t s) {
+  return L.prec == 8 ? relax_t<double>(L, n, s) : relax_t<float>(L, n, s);
+}
+
+int vcycle_any(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse,
+               hipStream_t s) {
+  for (int l = 0; l < nlev; ++l) {
+    if (l > 0) {
+      const int r = lv[l].prec == 8 ? restrict_t<double>(lv[l - 1], lv[l], s)
+                                    : restrict_t<float>(lv[l - 1], lv[l], s);
+      if (r) return r;
+    }
+    const int r = relax_any(lv[l], l < nlev - 1 ? npre : ncoarse, s);
+    if (r) return r;
+  }
+  for (int l = nlev - 2; l >= 0; --l) {
+    const int r = lv[l].prec == 8 ? prolong_t<double>(lv[l + 1], lv[l], s)
+                                  : prolong_t<float>(lv[l + 1], lv[l], s);
+    if (r) return r;
+    const int r2 = relax_any(lv[l], npost, s);
+    if (r2) return r2;
+  }
+  return 0;
+}
+
+bool hier_ok(const wf_mg_level* lv, int nlev) {
+  if (!lv || nlev < 1 || nlev > 16) return false;
+  for (int l = 0; l < nlev; ++l) {
+    if (!level_ok(&lv[l])) return false;
+    if (l > 0 && !pair_ok(&lv[l - 1], &lv[l])) return false;
+  }
+  return true;
+}
+
+bool field_grid_ok(const wf_grid& g) {
+  return g.ni > 0 && g.nj > 0 && g.nk > 0 && g.npl >= 1 &&
+         g.sj >= (int64_t)(g.nk + 2 * g.npl) && g.si >= g.sj * (int64_t)(g.nj + 2 * g.npl);
+}
+
+}  // namespace
+
+#define S_(s) ((hipStream_t)(s))
+
+extern "C" {
+
+int wf_mg_coeff(const wf_mg_level* L, void* aw, void* ae, void* as, void* an, void* ab,
+                void* at, void* ap, void* stream) {
+  if (!level_ok(L)) return WF_EINVAL;
+  const MgDev d = mg_dev(*L);
+  if (L->prec == 8) {
+    double c0, c1, c2;
+    inv_h2(*L, c0, c1, c2);
+    hipLaunchKernelGGL(coeff_kernel<double>, cell_grid(d.n0, d.n1, d.n2), dim3(64, 4), 0,
+                       S_(stream), d, c0, c1, c2, (double*)aw, (double*)ae, (double*)as,
+                       (double*)an, (double*)ab, (double*)at, (double*)ap);
+  } else {
+    float c0, c1, c2;
+    inv_h2(*L, c0, c1, c2);
+    hipLaunchKernelGGL(coeff_kernel<float>, cell_grid(d.n0, d.n1, d.n2), dim3(64, 4), 0,
+                       S_(stream), d, c0, c1, c2, (float*)aw, (float*)ae, (float*)as,
+                       (float*)an, (float*)ab, (float*)at, (float*)ap);
+  }
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_mg_rbgs(const wf_mg_level* L, int color, void* stream) {
+  if (!level_ok(L) || (color != 0 && color != 1)) return WF_EINVAL;
+  return L->prec == 8 ? launch_rbgs<double>(*L, color, 0, S_(stream))
+                      : launch_rbgs<float>(*L, color, 0, S_(stream));
+}
+
+int wf_mg_refill(const wf_mg_level* L, void* stream) {
+  if (!level_ok(L)) return WF_EINVAL;
+  const MgDev d = mg_dev(*L);
+  const int nthreads = 2 * (d.n1 * d.n2 + d.n0 * d.n2 + d.n0 * d.n1);
+  if (L->prec == 8)
+    hipLaunchKernelGGL(refill_kernel<double>, dim3(div_up(nthreads, 256)), dim3(256), 0,
+                       S_(stream), (double*)L->phi, d);
+  else
+    hipLaunchKernelGGL(refill_kernel<float>, dim3(div_up(nthreads, 256)), dim3(256), 0,
+                       S_(stream), (float*)L->phi, d);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_mg_relax(const wf_mg_level* L, int nsweeps, void* stream) {
+  if (!level_ok(L) || nsweeps < 0) return WF_EINVAL;
+  return relax_any(*L, nsweeps, S_(stream));
+}
+
+int wf_mg_restrict(const wf_mg_level* F, const wf_mg_level* C, void* stream) {
+  if (!pair_ok(F, C)) return WF_EINVAL;
+  return F->prec == 8 ? restrict_t<double>(*F, *C, S_(stream))
+                      : restrict_t<float>(*F, *C, S_(stream));
+}
+
+int wf_mg_prolong(const wf_mg_level* C, const wf_mg_level* F, void* stream) {
+  if (!pair_ok(F, C)) return WF_EINVAL;
+  return F->prec == 8 ? prolong_t<double>(*C, *F, S_(stream))
+                      : prolong_t<float>(*C, *F, S_(stream));
+}
+
+int wf_mg_copy_in(const double* src, wf_grid g, const wf_mg_level* L, void* stream) {
+  if (!level_ok(L) || L->prec != 8 || !field_grid_ok(g) || g.ni != L->n0 || g.nj != L->n1 ||
+      g.nk != L->n2)
+    return WF_EINVAL;
+  const MgDev d = mg_dev(*L);
+  hipLaunchKernelGGL(copy_in_kernel, cell_grid(d.n0, d.n1, d.n2), dim3(64, 4), 0, S_(stream),
+                     src, g, (double*)L->rhs, d);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_mg_copy_out_ring(const wf_mg_level* L, double* dst, wf_grid g, void* stream) {
+  if (!level_ok(L) || L->prec != 8 || !field_grid_ok(g) || g.ni != L->n0 ||
+      g.nj != L->n1 || g.nk != L->n2)
+    return WF_EINVAL;
+  const MgDev d = mg_dev(*L);
+  hipLaunchKernelGGL(copy_out_kernel, cell_grid(d.n0 + 2, d.n1 + 2, d.n2 + 2), dim3(64, 4), 0,
+                     S_(stream), (const double*)L->phi, d, dst, g);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_mg_vcycle(const wf_mg_level* levels, int nlev, int n_pre, int n_post, int n_coarse,
+                 void* stream) {
+  if (!hier_ok(levels, nlev) || n_pre < 0 || n_post < 0 || n_coarse < 0) return WF_EINVAL;
+  return vcycle_any(levels, nlev, n_pre, n_post, n_coarse, S_(stream));
+}
+
+int wf_mg_solve(const wf_mg_level* levels, int nlev, const double* su, double* pp, wf_grid g,
+                int n_pre, int n_post, int n_coarse, int n_vcycles, void* stream) {
+  if (!hier_ok(levels, nlev) || levels[0].prec != 8 || n_vcycles < 0) return WF_EINVAL;
+  const hipStream_t s = S_(stream);
+  int r = wf_mg_copy_in(su, g, &levels[0], stream);
+  if (r) return r;
+  r = zero_t<double>(pp, (int64_t)(g.ni + 2 * g.npl) * g.si, s);
+  if (r) return r;
+  r = zero_t<double>(levels[0].phi, level_elems(levels[0]), s);
+  if (r) return r;
+  for (int c = 0; c < n_vcycles; ++c) {
+    r = vcycle_any(levels, nlev, n_pre, n_post, n_coarse, s);
+    if (r) return r;
+  }
+  return wf_mg_copy_out_ring(&levels[0], pp, g, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,571 @@
+// Momentum, projection and SGS kernels of one RK2 fractional step
+// (the reference's warpflow/kernels.py), FP64, for gfx950.
+//
+// Every kernel maps one thread to one cell of its launch box: 64 lanes along
+// the unit-stride k axis, 4 j-rows per workgroup, one i-plane per grid.z, so
+// each wavefront reads and writes whole 512-byte k-rows.  Expressions keep
+// the reference's evaluation order (left to right) so results agree with
+// the FP64 oracle to a few ulps.
+#include "wf_common.h"
+
+namespace wf {
+
+// 4th-order midpoint interpolant along stride s at c (kernels.py:31-46):
+// (-f[c+s] + 9 f[c] + 9 f[c-s] - f[c-2s]) / 16
+__device__ inline double I4(const double* __restrict__ f, int64_t c, int64_t s) {
+  return (-f[c + s] + 9.0 * f[c] + 9.0 * f[c - s] - f[c - 2 * s]) / 16.0;
+}
+
+// Staggered 4-point derivative (kernels.py:69, :80, :281-286):
+// (-s[+] + 27 s[0] - 27 s[-] + s[--]) / (24 h)
+__device__ inline double D4(double sp, double sn, double sm, double smm, double h) {
+  return (-sp + 27.0 * sn - 27.0 * sm + smm) / (24.0 * h);
+}
+
+// d(c c)/dx_a for the component c living on axis a (kernels.py:64-69,
+// :103-108, :143-148).
+__device__ inline double conv_self(const double* __restrict__ c, int64_t x, int64_t sa,
+                                   double ha) {
+  const double p2 = I4(c, x + 2 * sa, sa), p1 = I4(c, x + sa, sa);
+  const double p0 = I4(c, x, sa), m1 = I4(c, x - sa, sa);
+  return D4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, ha);
+}
+
+// d(c q)/dx_b for component c (axis a) and the component q along the cross
+// axis b: c is interpolated along b at b+2..b-1, q along a at a+1 and
+// b+1..b-2, so the derivative lands on c's face (kernels.py:71-87,
+// :110-127, :150-166).
+__device__ inline double conv_cross(const double* __restrict__ c,
+                                    const double* __restrict__ q, int64_t x,
+                                    int64_t sa, int64_t sb, double hb) {
+  const double sp1 = I4(c, x + 2 * sb, sb) * I4(q, x + sa + sb, sa);
+  const double sn = I4(c, x + sb, sb) * I4(q, x + sa, sa);
+  const double sm1 = I4(c, x, sb) * I4(q, x + sa - sb, sa);
+  const double sm2 = I4(c, x - sb, sb) * I4(q, x + sa - 2 * sb, sa);
+  return D4(sp1, sn, sm1, sm2, hb);
+}
+
+// N(c) = x-term + y-term + z-term, the reference's summation order for all
+// three components (kernels.py:89, :129, :168).
+template <int C>
+__device__ inline double conv_sum(const double* __restrict__ u,
+                                  const double* __restrict__ v,
+                                  const double* __restrict__ w, int64_t x,
+                                  const wf_grid& g, double dx, double dy, double dz) {
+  const int64_t sx = g.si, sy = g.sj, sz = 1;
+  if (C == 0) {
+    return conv_self(u, x, sx, dx) + conv_cross(u, v, x, sx, sy, dy) +
+           conv_cross(u, w, x, sx, sz, dz);
+  } else if (C == 1) {
+    return conv_cross(v, u, x, sy, sx, dx) + conv_self(v, x, sy, dy) +
+           conv_cross(v, w, x, sy, sz, dz);
+  } else {
+    return conv_cross(w, u, x, sz, sx, dx) + conv_cross(w, v, x, sz, sy, dy) +
+           conv_self(w, x, sz, dz);
+  }
+}
+
+// 4th-order Laplacian (kernels.py:177-186); r12xx = 12*dx*dx etc.
+__device__ inline double lap4(const double* __restrict__ f, int64_t x, const wf_grid& g,
+                              double r12xx, double r12yy, double r12zz) {
+  const int64_t sx = g.si, sy = g.sj;
+  const double f0 = f[x];
+  const double lx = (-f[x + 2 * sx] + 16.0 * f[x + sx] - 30.0 * f0 + 16.0 * f[x - sx] -
+                     f[x - 2 * sx]) / r12xx;
+  const double ly = (-f[x + 2 * sy] + 16.0 * f[x + sy] - 30.0 * f0 + 16.0 * f[x - sy] -
+                     f[x - 2 * sy]) / r12yy;
+  const double lz =
+      (-f[x + 2] + 16.0 * f[x + 1] - 30.0 * f0 + 16.0 * f[x - 1] - f[x - 2]) / r12zz;
+  return lx + ly + lz;
+}
+
+template <int C>
+__device__ inline int64_t vis_offset(const wf_grid& g) {
+  return C == 0 ? 0 : (C == 1 ? g.sj : 1);  // kernels.py:198, :211, :224
+}
+
+template <int C>
+__global__ void __launch_bounds__(256)
+conv4th_kernel(const double* __restrict__ u, const double* __restrict__ v,
+               const double* __restrict__ w, const double* __restrict__ coo,
+               double* __restrict__ cstar, wf_grid g, Box3 b, double dx, double dy,
+               double dz, double dta) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  const int64_t x = at(g, i, j, k);
+  cstar[x] = coo[x] - dta * conv_sum<C>(u, v, w, x, g, dx, dy, dz);
+}
+
+template <int C>
+__global__ void __launch_bounds__(256)
+diff4th_kernel(const double* __restrict__ c, const double* __restrict__ vis,
+               double* __restrict__ cstar, wf_grid g, Box3 b, double r12xx,
+               double r12yy, double r12zz, double dta) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  const int64_t x = at(g, i, j, k);
+  const double diff = vis[x + vis_offset<C>(g)] * lap4(c, x, g, r12xx, r12yy, r12zz);
+  cstar[x] = cstar[x] + dta * diff;
+}
+
+template <int C>
+__global__ void __launch_bounds__(256)
+convdiff4th_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                   const double* __restrict__ w, const double* __restrict__ coo,
+                   const double* __restrict__ vis, double* __restrict__ cstar, wf_grid g,
+                   Box3 b, double dx, double dy, double dz, double r12xx, double r12yy,
+                   double r12zz, double dta) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  const int64_t x = at(g, i, j, k);
+  const double* __restrict__ c = C == 0 ? u : (C == 1 ? v : w);
+  double t = coo[x] - dta * conv_sum<C>(u, v, w, x, g, dx, dy, dz);
+  const double diff = vis[x + vis_offset<C>(g)] * lap4(c, x, g, r12xx, r12yy, r12zz);
+  t = t + dta * diff;
+  cstar[x] = t;
+}
+
+__global__ void __launch_bounds__(256)
+calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
+              double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double h, double dta) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  const int64_t x = at(g, i, j, k);
+  c[x] = cstar[x] - dta * (p[x + s] - p[x]) / h;
+}
+
+// All three calvel launches over the real cells; component c is written
+// only inside its operator range (solver.py:99-110).
+__global__ void __launch_bounds__(256)
+calvel3_kernel(const double* __restrict__ us, const double* __restrict__ vs,
+               const double* __restrict__ ws, const double* __restrict__ p,
+               double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+               wf_grid g, Box3 b, double dx, double dy, double dz, double dta) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  const int64_t x = at(g, i, j, k);
+  const double p0 = p[x];
+  if (i < g.npl + g.ni - 1) u[x] = us[x] - dta * (p[x + g.si] - p0) / dx;
+  if (j < g.npl + g.nj - 1) v[x] = vs[x] - dta * (p[x + g.sj] - p0) / dy;
+  if (k < g.npl + g.nk - 1) w[x] = ws[x] - dta * (p[x + 1] - p0) / dz;
+}
+
+// Block-wide reductions for 256-thread (4-wave) workgroups; every thread of
+// the block must call them.
+__device__ inline double block_max(double v) {
+  __shared__ double part[4];
+  v = wave_max(v);
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  if ((t & 63) == 0) part[t >> 6] = v;
+  __syncthreads();
+  double r = part[0];
+  for (int q = 1; q < 4; ++q) r = nan_max(r, part[q]);
+  return r;
+}
+__device__ inline double block_min(double v) {
+  __shared__ double part[4];
+  v = wave_min(v);
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  if ((t & 63) == 0) part[t >> 6] = v;
+  __syncthreads();
+  double r = part[0];
+  for (int q = 1; q < 4; ++q) r = nan_min(r, part[q]);
+  return r;
+}
+__device__ inline bool block_leader() { return threadIdx.x == 0 && threadIdx.y == 0; }
+
+__global__ void __launch_bounds__(256)
+calmas_kernel(const double* __restrict__ u, const double* __restrict__ v,
+              const double* __restrict__ w, double* __restrict__ su,
+              double* __restrict__ rmax, wf_grid g, Box3 b, double dx, double dy,
+              double dz, double dt) {
+  int i, j, k;
+  double val = 0.0;
+  if (box_thread(b, i, j, k)) {
+    const int64_t x = at(g, i, j, k);
+    const int64_t sx = g.si, sy = g.sj;
+    const double dudx = D4(u[x + sx], u[x], u[x - sx], u[x - 2 * sx], dx);
+    const double dvdy = D4(v[x + sy], v[x], v[x - sy], v[x - 2 * sy], dy);
+    const double dwdz = D4(w[x + 1], w[x], w[x - 1], w[x - 2], dz);
+    const double fact = dudx + dvdy + dwdz;
+    su[x] = fact / dt;
+    val = fabs(fact * dx * dy * dz);
+  }
+  const double r = block_max(val);
+  if (block_leader()) atomic_max_nonneg(rmax, r);
+}
+
+__global__ void __launch_bounds__(256)
+wale_kernel(const double* __restrict__ u, const double* __restrict__ v,
+            const double* __restrict__ w, double* __restrict__ vis, wf_grid g, Box3 b,
+            double dx, double dy, double dz, double rrey, double ls) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  const int64_t x = at(g, i, j, k);
+  const int64_t X = g.si, Y = g.sj, Z = 1;
+  // kernels.py:311-327
+  const double dudx = (u[x] - u[x - X]) / dx;
+  const double dudy = (0.25 * (u[x + Y] + u[x - X + Y] - u[x - Y] - u[x - X - Y])) / dy;
+  const double dudz = (0.25 * (u[x + Z] + u[x - X + Z] - u[x - Z] - u[x - X - Z])) / dz;
+  const double dvdx = (0.25 * (v[x + X] + v[x + X - Y] - v[x - X] - v[x - X - Y])) / dx;
+  const double dvdy = (v[x] - v[x - Y]) / dy;
+  const double dvdz = (0.25 * (v[x + Z] + v[x - Y + Z] - v[x - Z] - v[x - Y - Z])) / dz;
+  const double dwdx = (0.25 * (w[x + X] + w[x + X - Z] - w[x - X] - w[x - X - Z])) / dx;
+  const double dwdy = (0.25 * (w[x + Y] + w[x + Y - Z] - w[x - Y] - w[x - Y - Z])) / dy;
+  const double dwdz = (w[x] - w[x - Z]) / dz;
+  // kernels.py:330-338, g2zx and g2zy with the reference's last terms
+  const double g2xx = dudx * dudx + dudy * dvdx + dudz * dwdx;
+  const double g2yy = dvdx * dudy + dvdy * dvdy + dvdz * dwdy;
+  const double g2zz = dwdx * dudz + dwdy * dvdz + dwdz * dwdz;
+  const double g2xy = dudx * dudy + dudy * dvdy + dudz * dwdy;
+  const double g2yx = dvdx * dudx + dvdy * dvdx + dvdz * dwdx;
+  const double g2xz = dudx * dudz + dudy * dvdz + dudz * dwdz;
+  const double g2zx = dwdx * dudx + dwdy * dvdx + dwdz * dudz;
+  const double g2yz = dvdx * dudz + dvdy * dvdz + dvdz * dwdz;
+  const double g2zy = dwdx * dudy + dwdy * dvdy + dwdz * dudy;
+  // kernels.py:340-358
+  const double tr = (g2xx + g2yy + g2zz) / 3.0;
+  const double sdxx = g2xx - tr, sdyy = g2yy - tr, sdzz = g2zz - tr;
+  const double sdxy = 0.5 * (g2xy + g2yx);
+  const double sdxz = 0.5 * (g2xz + g2zx);
+  const double sdyz = 0.5 * (g2yz + g2zy);
+  const double sdsd = sdxx * sdxx + sdyy * sdyy + sdzz * sdzz +
+                      2.0 * (sdxy * sdxy + sdxz * sdxz + sdyz * sdyz);
+  const double sxy = 0.5 * (dudy + dvdx);
+  const double sxz = 0.5 * (dudz + dwdx);
+  const double syz = 0.5 * (dvdz + dwdy);
+  const double ss = dudx * dudx + dvdy * dvdy + dwdz * dwdz +
+                    2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+  // kernels.py:360-367 with x^1.5 = x*sqrt(x), |s|^2.5 = s*s*sqrt(|s|),
+  // x^1.25 = x*sqrt(sqrt(x)) (exact identities; a few ulps from pow)
+  double nut = 0.0;
+  if (sdsd > 1.0e-8) {
+    const double rs = sqrt(sdsd);
+    const double sdsd15 = sdsd * rs;
+    const double ass = fabs(ss);
+    const double denom = ass * ass * sqrt(ass) + sdsd * sqrt(rs);
+    nut = ls * ls * sdsd15 / denom;
+  }
+  vis[x] = rrey + nut;
+}
+
+__global__ void __launch_bounds__(256)
+maxabs_kernel(const double* __restrict__ f, double* __restrict__ out, wf_grid g, Box3 b) {
+  int i, j, k;
+  double val = 0.0;
+  if (box_thread(b, i, j, k)) val = fabs(f[at(g, i, j, k)]);
+  const double r = block_max(val);
+  if (block_leader()) atomic_max_nonneg(out, r);
+}
+
+// checkdt.for dt limit at one cell (kernels.py:394-401); rsum = 1/dx^2 +
+// 1/dy^2 + 1/dz^2 computed once by the host in the reference's order.
+__device__ inline double dtvisc_cell(const double* __restrict__ u,
+                                     const double* __restrict__ v,
+                                     const double* __restrict__ w,
+                                     const double* __restrict__ vis, int64_t x,
+                                     const wf_grid& g, double dx, double dy, double dz,
+                                     double rsum) {
+  const double uc = 0.5 * (u[x] + u[x - g.si]);
+  const double vc = 0.5 * (v[x] + v[x - g.sj]);
+  const double wc = 0.5 * (w[x] + w[x - 1]);
+  return 1.0 / (fabs(uc) / dx + fabs(vc) / dy + fabs(wc) / dz + 2.0 * vis[x] * rsum +
+                1.0e-30);
+}
+
+__global__ void __launch_bounds__(256)
+dtvisc_kernel(const double* __restrict__ u, const double* __restrict__ v,
+              const double* __restrict__ w, const double* __restrict__ vis,
+              double* __restrict__ out, wf_grid g, Box3 b, double dx, double dy,
+              double dz, double rsum) {
+  int i, j, k;
+  double val = 1.0e10;
+  if (box_thread(b, i, j, k)) val = dtvisc_cell(u, v, w, vis, at(g, i, j, k), g, dx, dy, dz, rsum);
+  const double r = block_min(val);
+  if (block_leader()) atomic_min_nonneg(out, r);
+}
+
+__global__ void init4_kernel(double* out) {
+  out[0] = 0.0;
+  out[1] = 0.0;
+  out[2] = 0.0;
+  out[3] = 1.0e10;
+}
+
+__global__ void __launch_bounds__(256)
+checkdt_kernel(const double* __restrict__ u, const double* __restrict__ v,
+               const double* __restrict__ w, const double* __restrict__ vis,
+               double* __restrict__ out, wf_grid g, Box3 b, double dx, double dy,
+               double dz, double rsum) {
+  int i, j, k;
+  double mu = 0.0, mv = 0.0, mw = 0.0, dtv = 1.0e10;
+  if (box_thread(b, i, j, k)) {
+    const int64_t x = at(g, i, j, k);
+    mu = fabs(u[x]);
+    mv = fabs(v[x]);
+    mw = fabs(w[x]);
+    dtv = dtvisc_cell(u, v, w, vis, x, g, dx, dy, dz, rsum);
+  }
+  mu = block_max(mu);
+  __syncthreads();
+  mv = block_max(mv);
+  __syncthreads();
+  mw = block_max(mw);
+  __syncthreads();
+  dtv = block_min(dtv);
+  if (block_leader()) {
+    atomic_max_nonneg(out + 0, mu);
+    atomic_max_nonneg(out + 1, mv);
+    atomic_max_nonneg(out + 2, mw);
+    atomic_min_nonneg(out + 3, dtv);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+add_kernel(double* __restrict__ dst, const double* __restrict__ src, wf_grid g, Box3 b) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  const int64_t x = at(g, i, j, k);
+  dst[x] = dst[x] + src[x];
+}
+
+__global__ void __launch_bounds__(256)
+fill_box_kernel(double* __restrict__ f, double val, wf_grid g, Box3 b) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  f[at(g, i, j, k)] = val;
+}
+
+// kernels.py:460-471; uint32 wrap-around arithmetic on the full-array indices.
+__global__ void __launch_bounds__(256)
+noise_kernel(double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+             double amp, uint32_t seed, wf_grid g, Box3 b) {
+  int i, j, k;
+  if (!box_thread(b, i, j, k)) return;
+  uint32_t h = ((uint32_t)i * 73856093u) ^ ((uint32_t)j * 19349663u) ^
+               ((uint32_t)k * 83492791u) ^ seed;
+  h = h ^ (h >> 13);
+  h = h * 1274126177u;
+  h = h ^ (h >> 16);
+  const double r = ((double)h / 4294967295.0) * 2.0 - 1.0;
+  const int64_t x = at(g, i, j, k);
+  u[x] = amp * r;
+  v[x] = amp * r;
+  w[x] = -amp * r;
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+// ---------------------------------------------------------------------------
+// Host-side argument checks: the launch box plus the stencil reach must stay
+// inside the allocated (ni+2npl, nj+2npl, nk+2npl) array.
+// ---------------------------------------------------------------------------
+static bool grid_ok(const wf_grid& g) {
+  return g.ni > 0 && g.nj > 0 && g.nk > 0 && g.npl >= 0 &&
+         g.sj >= (int64_t)(g.nk + 2 * g.npl) &&
+         g.si >= g.sj * (int64_t)(g.nj + 2 * g.npl);
+}
+// Box [lo, lo+d) widened by `reach` must lie in [0, n+2npl) on every axis.
+static bool box_ok(const wf_grid& g, const Box3& b, int reach_lo, int reach_hi) {
+  if (!grid_ok(g) || b.d0 < 0 || b.d1 < 0 || b.d2 < 0) return false;
+  const int lo[3] = {b.lo0, b.lo1, b.lo2};
+  const int d[3] = {b.d0, b.d1, b.d2};
+  const int n[3] = {g.ni, g.nj, g.nk};
+  for (int a = 0; a < 3; ++a) {
+    if (d[a] == 0) continue;
+    if (lo[a] - reach_lo < 0) return false;
+    if (lo[a] + d[a] - 1 + reach_hi > n[a] + 2 * g.npl - 1) return false;
+  }
+  return true;
+}
+static Box3 npl_box(const wf_grid& g, int d0, int d1, int d2) {
+  return Box3{g.npl, g.npl, g.npl, d0, d1, d2};
+}
+static bool empty(const Box3& b) { return b.d0 == 0 || b.d1 == 0 || b.d2 == 0; }
+#define S_(s) ((hipStream_t)(s))
+
+extern "C" {
+
+int wf_conv4th(int comp, const double* u, const double* v, const double* w,
+               const double* coo, double* cstar, wf_grid g, int d0, int d1, int d2,
+               double dx, double dy, double dz, double dt, double alpha, void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (comp < 0 || comp > 2 || !box_ok(g, b, 3, 3)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  const double dta = dt * alpha;
+  if (comp == 0)
+    hipLaunchKernelGGL(conv4th_kernel<0>, box_grid(b), box_block(), 0, S_(stream), u, v, w,
+                       coo, cstar, g, b, dx, dy, dz, dta);
+  else if (comp == 1)
+    hipLaunchKernelGGL(conv4th_kernel<1>, box_grid(b), box_block(), 0, S_(stream), u, v, w,
+                       coo, cstar, g, b, dx, dy, dz, dta);
+  else
+    hipLaunchKernelGGL(conv4th_kernel<2>, box_grid(b), box_block(), 0, S_(stream), u, v, w,
+                       coo, cstar, g, b, dx, dy, dz, dta);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_diff4th(int comp, const double* c, const double* vis, double* cstar, wf_grid g,
+               int d0, int d1, int d2, double dx, double dy, double dz, double dt,
+               double alpha, void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (comp < 0 || comp > 2 || !box_ok(g, b, 2, 2)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  const double dta = dt * alpha;
+  const double rx = 12.0 * dx * dx, ry = 12.0 * dy * dy, rz = 12.0 * dz * dz;
+  if (comp == 0)
+    hipLaunchKernelGGL(diff4th_kernel<0>, box_grid(b), box_block(), 0, S_(stream), c, vis,
+                       cstar, g, b, rx, ry, rz, dta);
+  else if (comp == 1)
+    hipLaunchKernelGGL(diff4th_kernel<1>, box_grid(b), box_block(), 0, S_(stream), c, vis,
+                       cstar, g, b, rx, ry, rz, dta);
+  else
+    hipLaunchKernelGGL(diff4th_kernel<2>, box_grid(b), box_block(), 0, S_(stream), c, vis,
+                       cstar, g, b, rx, ry, rz, dta);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_convdiff4th(int comp, const double* u, const double* v, const double* w,
+                   const double* coo, const double* vis, double* cstar, wf_grid g, int d0,
+                   int d1, int d2, double dx, double dy, double dz, double dt, double alpha,
+                   void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (comp < 0 || comp > 2 || !box_ok(g, b, 3, 3)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  const double dta = dt * alpha;
+  const double rx = 12.0 * dx * dx, ry = 12.0 * dy * dy, rz = 12.0 * dz * dz;
+  if (comp == 0)
+    hipLaunchKernelGGL(convdiff4th_kernel<0>, box_grid(b), box_block(), 0, S_(stream), u, v,
+                       w, coo, vis, cstar, g, b, dx, dy, dz, rx, ry, rz, dta);
+  else if (comp == 1)
+    hipLaunchKernelGGL(convdiff4th_kernel<1>, box_grid(b), box_block(), 0, S_(stream), u, v,
+                       w, coo, vis, cstar, g, b, dx, dy, dz, rx, ry, rz, dta);
+  else
+    hipLaunchKernelGGL(convdiff4th_kernel<2>, box_grid(b), box_block(), 0, S_(stream), u, v,
+                       w, coo, vis, cstar, g, b, dx, dy, dz, rx, ry, rz, dta);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_calvel(int comp, const double* cstar, const double* p, double* c, wf_grid g, int d0,
+              int d1, int d2, double h, double dt, double alpha, void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (comp < 0 || comp > 2 || !box_ok(g, b, 0, 1)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  const int64_t s = comp == 0 ? g.si : (comp == 1 ? g.sj : 1);
+  hipLaunchKernelGGL(calvel_kernel, box_grid(b), box_block(), 0, S_(stream), cstar, p, c, g,
+                     b, s, h, dt * alpha);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_calvel3(const double* ustar, const double* vstar, const double* wstar,
+               const double* p, double* u, double* v, double* w, wf_grid g, double dx,
+               double dy, double dz, double dt, double alpha, void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 0, 1)) return WF_EINVAL;
+  hipLaunchKernelGGL(calvel3_kernel, box_grid(b), box_block(), 0, S_(stream), ustar, vstar,
+                     wstar, p, u, v, w, g, b, dx, dy, dz, dt * alpha);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_calmas(const double* u, const double* v, const double* w, double* su,
+              double* rmax_dev, wf_grid g, int d0, int d1, int d2, double dx, double dy,
+              double dz, double dt, void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (!box_ok(g, b, 2, 1)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  hipLaunchKernelGGL(calmas_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, su,
+                     rmax_dev, g, b, dx, dy, dz, dt);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_wale(const double* u, const double* v, const double* w, double* vis, wf_grid g,
+            int d0, int d1, int d2, double dx, double dy, double dz, double rrey,
+            double cw, double delta, void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (!box_ok(g, b, 1, 1)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  hipLaunchKernelGGL(wale_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, vis, g,
+                     b, dx, dy, dz, rrey, cw * delta);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_maxabs(const double* f, double* out_dev, wf_grid g, int d0, int d1, int d2,
+              void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  hipLaunchKernelGGL(maxabs_kernel, box_grid(b), box_block(), 0, S_(stream), f, out_dev, g,
+                     b);
+  WF_LAUNCHED();
+  return 0;
+}
+
+static double inv_sq_sum(double dx, double dy, double dz) {
+  return 1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz);
+}
+
+int wf_dtvisc(const double* u, const double* v, const double* w, const double* vis,
+              double* out_dev, wf_grid g, int d0, int d1, int d2, double dx, double dy,
+              double dz, void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (!box_ok(g, b, 1, 0)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  hipLaunchKernelGGL(dtvisc_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, vis,
+                     out_dev, g, b, dx, dy, dz, inv_sq_sum(dx, dy, dz));
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_checkdt(const double* u, const double* v, const double* w, const double* vis,
+               double* out4_dev, wf_grid g, double dx, double dy, double dz, void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 1, 0)) return WF_EINVAL;
+  hipLaunchKernelGGL(init4_kernel, dim3(1), dim3(1), 0, S_(stream), out4_dev);
+  WF_LAUNCHED();
+  hipLaunchKernelGGL(checkdt_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, vis,
+                     out4_dev, g, b, dx, dy, dz, inv_sq_sum(dx, dy, dz));
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_add_ring(double* dst, const double* src, wf_grid g, void* stream) {
+  if (g.npl < 1) return WF_EINVAL;
+  const Box3 b{g.npl - 1, g.npl - 1, g.npl - 1, g.ni + 2, g.nj + 2, g.nk + 2};
+  if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
+  hipLaunchKernelGGL(add_kernel, box_grid(b), box_block(), 0, S_(stream), dst, src, g, b);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_fill_interior(double* f, double val, wf_grid g, int d0, int d1, int d2,
+                     void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  hipLaunchKernelGGL(fill_box_kernel, box_grid(b), box_block(), 0, S_(stream), f, val, g,
+                     b);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_init_noise(double* u, double* v, double* w, double amp, int32_t seed, wf_grid g,
+                  int d0, int d1, int d2, void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
+  if (empty(b)) return 0;
+  hipLaunchKernelGGL(noise_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, amp,
+                     (uint32_t)seed, g, b);
+  WF_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"

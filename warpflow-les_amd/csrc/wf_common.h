@@ -1,0 +1,81 @@
+// Shared definitions for the warpflow gfx950 kernels.
+//
+// Layout: every field is a C-ordered [i][j][k] array with k unit-stride;
+// `wf_grid` carries the real-cell counts, the ghost depth and the element
+// strides of j and i, so padded pitches and batched members use the same
+// kernels.  Full-array index (i, j, k) -> i*si + j*sj + k.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/warpflow_hip.h"
+
+#define WF_TRY(expr)                                   \
+  do {                                                 \
+    hipError_t e_ = (expr);                            \
+    if (e_ != hipSuccess) return (int)e_;              \
+  } while (0)
+
+#define WF_LAUNCHED() WF_TRY(hipPeekAtLastError())
+
+namespace wf {
+
+__host__ __device__ inline int64_t at(const wf_grid& g, int i, int j, int k) {
+  return (int64_t)i * g.si + (int64_t)j * g.sj + (int64_t)k;
+}
+
+inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Launch geometry for a 3-D index box of extent (d0, d1, d2): 64 lanes along
+// k (one wavefront per k-row), 4 j-rows per workgroup, one i-plane per
+// grid.z.  Every kernel bounds-checks against the box.
+struct Box3 {
+  int lo0, lo1, lo2;
+  int d0, d1, d2;
+};
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 box_grid(const Box3& b) {
+  return dim3(div_up(b.d2, BX), div_up(b.d1, BY), b.d0);
+}
+inline dim3 box_block() { return dim3(BX, BY, 1); }
+
+__device__ inline bool box_thread(const Box3& b, int& i, int& j, int& k) {
+  k = blockIdx.x * BX + threadIdx.x;
+  j = blockIdx.y * BY + threadIdx.y;
+  i = blockIdx.z;
+  if (k >= b.d2 || j >= b.d1 || i >= b.d0) return false;
+  i += b.lo0;
+  j += b.lo1;
+  k += b.lo2;
+  return true;
+}
+
+// Non-negative doubles order like their bit patterns, so max over them is an
+// integer max; order-independent and therefore deterministic.
+__device__ inline void atomic_max_nonneg(double* p, double v) {
+  atomicMax((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
+}
+__device__ inline void atomic_min_nonneg(double* p, double v) {
+  atomicMin((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
+}
+
+// Wavefront (64-lane) reductions via cross-lane shuffles.  max/min propagate
+// NaN (a blown-up field must not vanish from rmax or the dt limits).
+__device__ inline double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+__device__ inline double nan_min(double a, double b) { return (a < b || a != a) ? a : b; }
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = nan_min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace wf

@@ -1,0 +1,171 @@
+"""Kernel handles of the per-step hot path (the reference's
+warpflow/kernels.py), dispatched to the gfx950 HIP library.
+
+Each handle keeps the reference kernel's name and input list, so the
+reference's call sites read the same::
+
+    launch(K.conv4th_u, dim=sol._op_dim("u"),
+           inputs=[u, v, w, uoo, ustar, dx, dy, dz, dt, alpha, NPL])
+
+``dim`` is the launch box; like the reference kernels, every handle offsets
+the thread index by ``npl`` (the last input).  Arrays are ``DeviceArray``s of
+the same shape.  Three fused handles have no single reference kernel:
+``convdiff4th_{u,v,w}`` (the conv/diff pair of one RK stage), ``calvel3``
+(the three calvel launches of ``FlowSolver._calvel``) and ``checkdt`` (the
+four reductions of ``FlowSolver.compute_dt``).
+"""
+from __future__ import annotations
+
+from . import _hip
+from .device import DeviceArray, current_stream, grid_of
+
+NPL = 3
+
+
+class Kernel:
+    """A launchable kernel: ``kernel(dim, *inputs)`` or ``launch(kernel, ...)``."""
+
+    def __init__(self, name, fn):
+        self.name = name
+        self._fn = fn
+
+    def __call__(self, dim, *inputs):
+        d = tuple(int(x) for x in dim)
+        if len(d) != 3:
+            raise ValueError("%s: dim must have 3 entries, got %s" % (self.name, dim))
+        return self._fn(d, *inputs)
+
+    def __repr__(self):
+        return "<warpflow kernel %s>" % self.name
+
+
+def launch(kernel, dim, inputs, device=None, stream=None):
+    """Stand-in for ``wp.launch(kernel, dim=..., inputs=[...], device=...)``."""
+    return kernel(dim, *inputs)
+
+
+def _same(*arrs):
+    s = arrs[0].shape
+    for a in arrs:
+        if not isinstance(a, DeviceArray) or a.shape != s:
+            raise ValueError("arrays must be DeviceArrays of one shape: %r" % (arrs,))
+
+
+def _conv(comp):
+    def f(dim, u, v, w, coo, cstar, dx, dy, dz, dt, alpha, npl):
+        _same(u, v, w, coo, cstar)
+        _hip.call("wf_conv4th", comp, u.ptr, v.ptr, w.ptr, coo.ptr, cstar.ptr,
+                  grid_of(cstar, npl), *dim, dx, dy, dz, dt, alpha, current_stream())
+    return f
+
+
+def _diff(comp):
+    def f(dim, c, vis, cstar, dx, dy, dz, dt, alpha, npl):
+        _same(c, vis, cstar)
+        _hip.call("wf_diff4th", comp, c.ptr, vis.ptr, cstar.ptr, grid_of(cstar, npl),
+                  *dim, dx, dy, dz, dt, alpha, current_stream())
+    return f
+
+
+def _convdiff(comp):
+    def f(dim, u, v, w, coo, vis, cstar, dx, dy, dz, dt, alpha, npl):
+        _same(u, v, w, coo, vis, cstar)
+        _hip.call("wf_convdiff4th", comp, u.ptr, v.ptr, w.ptr, coo.ptr, vis.ptr,
+                  cstar.ptr, grid_of(cstar, npl), *dim, dx, dy, dz, dt, alpha,
+                  current_stream())
+    return f
+
+
+def _calvel(comp):
+    def f(dim, cstar, p, c, h, dt, alpha, npl):
+        _same(cstar, p, c)
+        _hip.call("wf_calvel", comp, cstar.ptr, p.ptr, c.ptr, grid_of(c, npl), *dim, h,
+                  dt, alpha, current_stream())
+    return f
+
+
+def _calvel3(dim, ustar, vstar, wstar, p, u, v, w, dx, dy, dz, dt, alpha, npl):
+    _same(ustar, vstar, wstar, p, u, v, w)
+    g = grid_of(u, npl)
+    if dim != (g.ni, g.nj, g.nk):
+        raise ValueError("calvel3 runs over the real cells (dim = ncells)")
+    _hip.call("wf_calvel3", ustar.ptr, vstar.ptr, wstar.ptr, p.ptr, u.ptr, v.ptr, w.ptr, g,
+              dx, dy, dz, dt, alpha, current_stream())
+
+
+def _calmas(dim, u, v, w, su, rmax, dx, dy, dz, dt, npl):
+    _same(u, v, w, su)
+    _hip.call("wf_calmas", u.ptr, v.ptr, w.ptr, su.ptr, rmax.ptr, grid_of(su, npl), *dim,
+              dx, dy, dz, dt, current_stream())
+
+
+def wale_delta(dx, dy, dz):
+    """(dx*dy*dz)^(1/3) (kernels.py:360)."""
+    return (dx * dy * dz) ** (1.0 / 3.0)
+
+
+def _wale(dim, u, v, w, vis, dx, dy, dz, rrey, cw, npl):
+    _same(u, v, w, vis)
+    _hip.call("wf_wale", u.ptr, v.ptr, w.ptr, vis.ptr, grid_of(vis, npl), *dim, dx, dy, dz,
+              rrey, cw, wale_delta(dx, dy, dz), current_stream())
+
+
+def _maxabs(dim, f, out, npl):
+    _hip.call("wf_maxabs", f.ptr, out.ptr, grid_of(f, npl), *dim, current_stream())
+
+
+def _dtvisc(dim, u, v, w, vis, out, dx, dy, dz, npl):
+    _same(u, v, w, vis)
+    _hip.call("wf_dtvisc", u.ptr, v.ptr, w.ptr, vis.ptr, out.ptr, grid_of(vis, npl), *dim,
+              dx, dy, dz, current_stream())
+
+
+def _checkdt(dim, u, v, w, vis, out4, dx, dy, dz, npl):
+    _same(u, v, w, vis)
+    g = grid_of(vis, npl)
+    if dim != (g.ni, g.nj, g.nk) or out4.size < 4:
+        raise ValueError("checkdt runs over the real cells into a 4-element buffer")
+    _hip.call("wf_checkdt", u.ptr, v.ptr, w.ptr, vis.ptr, out4.ptr, g, dx, dy, dz,
+              current_stream())
+
+
+def _add_inplace(dim, dst, src, npl):
+    _same(dst, src)
+    g = grid_of(dst, npl)
+    if dim != (g.ni + 2, g.nj + 2, g.nk + 2):
+        raise ValueError("add_inplace covers the real cells plus one ring (dim = n+2)")
+    _hip.call("wf_add_ring", dst.ptr, src.ptr, g, current_stream())
+
+
+def _fill_const(dim, f, val, npl):
+    _hip.call("wf_fill_interior", f.ptr, float(val), grid_of(f, npl), *dim,
+              current_stream())
+
+
+def _init_noise(dim, u, v, w, amp, seed, npl):
+    _same(u, v, w)
+    _hip.call("wf_init_noise", u.ptr, v.ptr, w.ptr, float(amp), int(seed),
+              grid_of(u, npl), *dim, current_stream())
+
+
+conv4th_u = Kernel("conv4th_u", _conv(0))
+conv4th_v = Kernel("conv4th_v", _conv(1))
+conv4th_w = Kernel("conv4th_w", _conv(2))
+diff4th_u = Kernel("diff4th_u", _diff(0))
+diff4th_v = Kernel("diff4th_v", _diff(1))
+diff4th_w = Kernel("diff4th_w", _diff(2))
+convdiff4th_u = Kernel("convdiff4th_u", _convdiff(0))
+convdiff4th_v = Kernel("convdiff4th_v", _convdiff(1))
+convdiff4th_w = Kernel("convdiff4th_w", _convdiff(2))
+calvel_u = Kernel("calvel_u", _calvel(0))
+calvel_v = Kernel("calvel_v", _calvel(1))
+calvel_w = Kernel("calvel_w", _calvel(2))
+calvel3 = Kernel("calvel3", _calvel3)
+calmas = Kernel("calmas", _calmas)
+wale_viscosity = Kernel("wale_viscosity", _wale)
+maxabs_reduce = Kernel("maxabs_reduce", _maxabs)
+dtvisc_reduce = Kernel("dtvisc_reduce", _dtvisc)
+checkdt = Kernel("checkdt", _checkdt)
+add_inplace = Kernel("add_inplace", _add_inplace)
+fill_const = Kernel("fill_const", _fill_const)
+init_noise = Kernel("init_noise", _init_noise)

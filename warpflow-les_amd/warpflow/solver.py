@@ -1,0 +1,264 @@
+"""Time-marching flow solver (the reference's warpflow/solver.py).
+
+One RK2 (Heun) fractional step of the lid-driven-cavity LES, in the
+reference's order:
+
+  1. checkdt: max|u|, max|v|, max|w| and the viscous/advective dt limit
+  2. uoo, voo, woo <- u, v, w
+  3. WALE eddy viscosity + its ghost fill (or vis = 1/Re)
+  4. RK2 stages alpha = 1/2, 1: per component fill_bc + conv4th, then per
+     component fill_bc + diff4th; calvel(1/2) after the first stage
+  5. projection: calvel(1); repeat {calmas; MG Poisson solve; p += pp;
+     pbound; calvel(1)} until rmax < eps (at least one solve); final fills
+
+Every operator runs on the GPU through the HIP library; the host keeps only
+the scalars the reference keeps (dt, ctime, rmax, cfl) and reads back the
+reductions it branches on, as the reference does.  ``fused=True`` (default)
+runs each component's conv4th+diff4th pair and the three calvel launches as
+single passes; ``fused=False`` replays the reference's launches one for one.
+Both give the same fields.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import kernels as K
+from .bcs import BoundaryKernels
+from .device import copy, zeros
+from .halo import HaloExchanger
+from .multigrid import Multigrid
+from .topology import NPL
+
+ALFARK = (0.5, 1.0)  # Heun RK2 stage coefficients (solver.py:26)
+FIELDS = ("u", "v", "w", "p", "ustar", "vstar", "wstar",
+          "uoo", "voo", "woo", "vis", "su", "pp")
+
+
+class FlowSolver:
+    def __init__(self, cfg, topo, device=None, fused=True):
+        self.cfg = cfg
+        self.topo = topo
+        self.device = device
+        self.fused = fused
+
+        ni, nj, nk = topo.local_ncells
+        self.ncells = (ni, nj, nk)
+        self.dx = cfg.domain.lx / cfg.grid.nx
+        self.dy = cfg.domain.ly / cfg.grid.ny
+        self.dz = cfg.domain.lz / cfg.grid.nz
+        self.rey = 1.0 / cfg.physics.reynolds
+
+        shape = (ni + 2 * NPL, nj + 2 * NPL, nk + 2 * NPL)
+        self.f = {name: zeros(shape, np.float64, device) for name in FIELDS}
+
+        self.halo = HaloExchanger(topo, device, NPL)
+        b = cfg.boundaries
+        self.bnd = {"west": b.west, "east": b.east, "south": b.south,
+                    "north": b.north, "bottom": b.bottom, "top": b.top}
+        self.bcs = BoundaryKernels(self.bnd, device)
+        self.ulid = b.lid_velocity
+
+        self.mg = self._make_mg()
+        self._buf = zeros(4, np.float64, device)
+
+        self.dt = cfg.time.dt_init
+        self.ctime = 0.0
+        self.itime = 0
+        self.rmax = 0.0
+        self.cfl = 0.0
+        self.outer_iters = 0
+        self.rmax_history = []
+
+        # initial condition: rest + deterministic hash noise (solver.py:67-78)
+        amp = cfg.initial.noise if cfg.initial.type == "rest" else 0.0
+        if amp > 0.0:
+            K.init_noise(self.ncells, self.f["u"], self.f["v"], self.f["w"], amp,
+                         getattr(cfg.initial, "seed", 1), NPL)
+        self.f["vis"].fill_(self.rey)
+        self.bcs.apply(self.f, self.ncells, self.ulid, NPL)
+
+    def _make_mg(self):
+        if getattr(self.cfg.numerics, "mg_precision", "fp64") == "mixed":
+            from .mixed import MixedMultigrid
+            return MixedMultigrid(self.cfg, self.topo, self.device, self.dx, self.dy, self.dz)
+        return Multigrid(self.cfg, self.topo, self.device, self.dx, self.dy, self.dz)
+
+    # ------------------------------------------------------------------ #
+    # ghost fills                                                        #
+    # ------------------------------------------------------------------ #
+    def fill_bc(self, name, alfabc=1):
+        """solver.py:88-97: one boundu/boundv/boundw call for one field."""
+        self.halo.exchange(self.f[name])
+        n = self.ncells[0]
+        if name == "u":
+            self.bcs.apply_u(self.f["u"], n, self.ulid, alfabc)
+        elif name == "v":
+            self.bcs.apply_v(self.f["v"], n, alfabc)
+        elif name == "w":
+            self.bcs.apply_w(self.f["w"], n, alfabc)
+
+    def _op_dim(self, comp):
+        """solver.py:99-110: u (ni-1,nj,nk), v (ni,nj-1,nk), w (ni,nj,nk-1)."""
+        ni, nj, nk = self.ncells
+        if comp == "u":
+            return (ni - 1, nj, nk)
+        if comp == "v":
+            return (ni, nj - 1, nk)
+        if comp == "w":
+            return (ni, nj, nk - 1)
+        return (ni, nj, nk)
+
+    # ------------------------------------------------------------------ #
+    # time step control (checkdt)                                        #
+    # ------------------------------------------------------------------ #
+    def _maxabs(self, name):
+        """solver.py:115-120"""
+        self._buf.zero_()
+        K.maxabs_reduce(self.ncells, self.f[name], self._buf, NPL)
+        loc = float(self._buf.numpy()[0])
+        return self.topo.comm.allreduce(loc)
+
+    def _reductions(self):
+        if not self.fused:
+            umax, vmax, wmax = (self._maxabs(c) for c in ("u", "v", "w"))
+            self._buf.fill_(1.0e10)
+            K.dtvisc_reduce(self.ncells, self.f["u"], self.f["v"], self.f["w"],
+                            self.f["vis"], self._buf, self.dx, self.dy, self.dz, NPL)
+            return umax, vmax, wmax, float(self._buf.numpy()[0])
+        K.checkdt(self.ncells, self.f["u"], self.f["v"], self.f["w"], self.f["vis"],
+                  self._buf, self.dx, self.dy, self.dz, NPL)
+        r = self._buf.numpy()
+        return float(r[0]), float(r[1]), float(r[2]), float(r[3])
+
+    def compute_dt(self):
+        """solver.py:122-150"""
+        umax, vmax, wmax, dtvisc = self._reductions()
+        dtvisc = self.topo.comm.allreduce(dtvisc)
+        big = 1.0e10
+        dtcfl = big
+        for d, um in zip((self.dx, self.dy, self.dz), (umax, vmax, wmax)):
+            if um > 0.0:
+                dtcfl = min(dtcfl, d / um)
+        dtmax = min(dtvisc, dtcfl)
+        if dtmax <= 0.0 or not (dtmax < big):
+            raise RuntimeError("time step collapsed (dtmax = %.3e)" % dtmax)
+        self.cfl = self.dt / dtmax
+        if self.cfg.time.variable_dt:
+            dt_new = min(self.dt * self.cfg.time.dt_growth,
+                         self.cfg.time.safety_factor * dtmax)
+            self.dt = self.topo.comm.allreduce(dt_new)
+
+    # ------------------------------------------------------------------ #
+    # projection pieces                                                  #
+    # ------------------------------------------------------------------ #
+    def _calvel(self, alpha):
+        """solver.py:155-166: calvel for u, v, w over their operator ranges,
+        then halo exchange only (no BC fill)."""
+        f = self.f
+        if self.fused:
+            K.calvel3(self.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
+                      f["u"], f["v"], f["w"], self.dx, self.dy, self.dz, self.dt,
+                      alpha, NPL)
+        else:
+            for ker, star, comp, d in ((K.calvel_u, "ustar", "u", self.dx),
+                                       (K.calvel_v, "vstar", "v", self.dy),
+                                       (K.calvel_w, "wstar", "w", self.dz)):
+                ker(self._op_dim(comp), f[star], f["p"], f[comp], d, self.dt, alpha, NPL)
+        for name in ("u", "v", "w"):
+            self.halo.exchange(f[name])
+
+    def _calmas(self):
+        """solver.py:168-176"""
+        self._buf.zero_()
+        K.calmas(self.ncells, self.f["u"], self.f["v"], self.f["w"], self.f["su"],
+                 self._buf, self.dx, self.dy, self.dz, self.dt, NPL)
+        loc = float(self._buf.numpy()[0])
+        self.rmax = self.topo.comm.allreduce(loc)
+
+    def _project(self):
+        """solver.py:178-212: outer Poisson/projection iterations."""
+        f = self.f
+        eps = self.cfg.numerics.div_tolerance
+        maxcy = self.cfg.numerics.max_outer_iters
+        self._calvel(1.0)
+        iters = 0
+        self.rmax_history = []
+        for it in range(1, maxcy + 1):
+            iters = it
+            self._calmas()
+            self.rmax_history.append(self.rmax)
+            if self.rmax < eps and it > 1:
+                break
+            if self.rmax > 10.0:
+                raise RuntimeError("projection diverged: rmax = %.6e at itime %d"
+                                   % (self.rmax, self.itime))
+            self.mg.solve(f["su"], f["pp"], NPL)
+            K.add_inplace(tuple(n + 2 for n in self.ncells), f["p"], f["pp"], NPL)
+            self.halo.exchange(f["p"])
+            self.bcs.apply_pbound(f["p"], self.ncells[0])
+            self._calvel(1.0)
+        else:
+            if self.topo.rank == 0:
+                print("warning: projection did not converge, rmax = %.6e after %d "
+                      "iterations" % (self.rmax, maxcy), flush=True)
+        self.outer_iters = iters
+        for name in ("u", "v", "w"):
+            self.fill_bc(name, 0)
+
+    # ------------------------------------------------------------------ #
+    # one time step                                                      #
+    # ------------------------------------------------------------------ #
+    def _sgs(self):
+        """solver.py:227-237"""
+        f = self.f
+        if self.cfg.physics.sgs_enabled:
+            K.wale_viscosity(self.ncells, f["u"], f["v"], f["w"], f["vis"], self.dx,
+                             self.dy, self.dz, self.rey, self.cfg.physics.sgs_cw, NPL)
+            self.bcs.apply_vis(f["vis"], self.ncells[0], self.rey)
+        else:
+            K.fill_const(self.ncells, f["vis"], self.rey, NPL)
+
+    def _rk_stage(self, stage, alpha):
+        """solver.py:249-265"""
+        f = self.f
+        dt = self.dt
+        d3 = (self.dx, self.dy, self.dz)
+        alfabc = 1 if stage == 0 else 0
+        comps = (("u", K.conv4th_u, K.diff4th_u, K.convdiff4th_u),
+                 ("v", K.conv4th_v, K.diff4th_v, K.convdiff4th_v),
+                 ("w", K.conv4th_w, K.diff4th_w, K.convdiff4th_w))
+        if self.fused:
+            # conv+diff of component c right after its first fill; the second
+            # fills still run, in order, for their effect on the ghosts.
+            for c, _, _, kcd in comps:
+                self.fill_bc(c, alfabc)
+                kcd(self._op_dim(c), f["u"], f["v"], f["w"], f[c + "oo"], f["vis"],
+                    f[c + "star"], *d3, dt, alpha, NPL)
+            for c, _, _, _ in comps:
+                self.fill_bc(c, alfabc)
+            return
+        for c, kconv, _, _ in comps:
+            self.fill_bc(c, alfabc)
+            kconv(self._op_dim(c), f["u"], f["v"], f["w"], f[c + "oo"], f[c + "star"],
+                  *d3, dt, alpha, NPL)
+        for c, _, kdiff, _ in comps:
+            self.fill_bc(c, alfabc)
+            kdiff(self._op_dim(c), f[c], f["vis"], f[c + "star"], *d3, dt, alpha, NPL)
+
+    def step(self):
+        """solver.py:217-271"""
+        self.itime += 1
+        self.compute_dt()
+        self.ctime += self.dt
+        f = self.f
+        copy(f["uoo"], f["u"])
+        copy(f["voo"], f["v"])
+        copy(f["woo"], f["w"])
+        self._sgs()
+        for stage, alpha in enumerate(ALFARK):
+            self._rk_stage(stage, alpha)
+            if stage + 1 < len(ALFARK):
+                self._calvel(alpha)
+        self._project()
+        return {"ctime": self.ctime, "dt": self.dt, "rmax": self.rmax,
+                "iters": self.outer_iters, "cfl": self.cfl}
