@@ -1,0 +1,260 @@
+#!/usr/bin/env python
+"""RK2 step throughput of the lid-driven-cavity LES on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--levels 6]
+
+Workload: unit cube, 256^3 cells, Re 1000, WALE (cw 0.46), rest + 1% seeded
+noise, fixed dt 1e-4, projection to rmax < 1e-6 (at most 30 outer
+iterations), RBGS V(6,6) multigrid with 300 coarsest sweeps and 4 V-cycles
+per solve on 6 levels (8^3 coarsest).  Two forms run back to back on the same
+GPU from the same initial state:
+
+  mixed -- FP32 V-cycles inside FP64 defect correction (the headline value)
+  fp64  -- the reference's plain FP64 V-cycles
+
+W untimed steps, then K steps timed between device synchronisations.
+value = mixed-form Mcell/s = nx*ny*nz / (seconds per step) / 1e6.  The JSON
+line also carries the FP64 form's rate, the mixed/FP64 speed-up, the velocity
+rms deviation between the forms, a per-kernel table (HIP-event timings of
+each hot kernel on the solver's own arrays, algorithmic bytes per launch),
+the roofline of the dominant kernel and a CPU baseline (the C oracle, the
+same FP64 schedule, one step of the same 256^3 case on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "warpflow-les_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "RK2 step Mcell/s at 256^3, div<1e-6: FP64 vs FP32-MG+FP64 refine"
+
+
+def parse(argv=None):
+    ap = argparse.ArgumentParser(description="warpflow cavity RK2 step benchmark")
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--reynolds", type=float, default=1000.0)
+    ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--forms", default="mixed,fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the kernel table")
+    ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
+    return ap.parse_args(argv)
+
+
+def make_cfg(a, precision):
+    from warpflow.config import CaseConfig, validate
+    c = CaseConfig(name="cavity%d_re%d" % (a.n, a.reynolds))
+    c.grid.nx = c.grid.ny = c.grid.nz = a.n
+    c.physics.reynolds = a.reynolds
+    c.physics.sgs_enabled = True
+    c.physics.sgs_cw = 0.46
+    c.initial.noise = 0.01
+    c.numerics.mg_levels = a.levels
+    c.numerics.div_tolerance = 1e-6
+    c.numerics.max_outer_iters = 30
+    c.numerics.mg_precision = precision
+    c.time.variable_dt = False
+    c.time.dt_init = a.dt
+    validate(c)
+    return c
+
+
+def run_form(a, precision):
+    from warpflow import FlowSolver, build_topology
+    from warpflow.device import synchronize
+    cfg = make_cfg(a, precision)
+    sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
+    for _ in range(a.warmup):
+        sol.step()
+    synchronize()
+    t0 = time.perf_counter()
+    infos = [sol.step() for _ in range(a.steps)]
+    synchronize()
+    sec = time.perf_counter() - t0
+    return sol, infos, sec
+
+
+class Timer:
+    """HIP events on the stream the kernels are launched on (the null
+    stream): average duration of `reps` back-to-back launches."""
+
+    def __init__(self):
+        from warpflow import _hip
+        self.h = _hip
+        self.e0, self.e1 = C.c_void_p(), C.c_void_p()
+        _hip.call("wf_event_create", C.byref(self.e0))
+        _hip.call("wf_event_create", C.byref(self.e1))
+
+    def avg_us(self, fn, reps):
+        from warpflow.device import current_stream
+        fn()  # warm
+        self.h.call("wf_event_record", self.e0, current_stream())
+        for _ in range(reps):
+            fn()
+        self.h.call("wf_event_record", self.e1, current_stream())
+        ms = C.c_float(0.0)
+        self.h.call("wf_event_elapsed_ms", C.byref(ms), self.e0, self.e1)
+        return 1000.0 * ms.value / reps
+
+
+def kernel_table(a, s64, smx):
+    """Per-kernel HIP-event timings on the solvers' own arrays.  bytes =
+    algorithmic bytes per launch (distinct arrays read + written, RMW twice,
+    element size x launch cells); per_step = launches per step of the
+    schedule with 2 outer iterations (one MG solve of 4 V-cycles)."""
+    from warpflow import _hip
+    from warpflow import kernels as K
+    from warpflow.device import current_stream
+    from warpflow.topology import NPL
+    n = a.n
+    cells = float(n) ** 3
+    f = s64.f
+    h = (s64.dx, s64.dy, s64.dz)
+    dt = s64.dt
+    st = current_stream()
+    T = Timer()
+    rows = []
+
+    def add(name, fn, nbytes, per_step_64, per_step_mx):
+        us = T.avg_us(fn, a.reps)
+        rows.append({"kernel": name, "us": round(us, 2), "bytes": int(nbytes),
+                     "GBps": round(nbytes / us / 1e3, 1),
+                     "hbm_frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 3),
+                     "per_step_fp64": per_step_64, "per_step_mixed": per_step_mx})
+
+    opu = s64._op_dim("u")
+    add("convdiff4th (u)", lambda: K.convdiff4th_u(opu, f["u"], f["v"], f["w"], f["uoo"],
+                                                   f["vis"], f["ustar"], *h, dt, 0.5, NPL),
+        48 * opu[0] * opu[1] * opu[2], 6, 6)
+    add("calvel3", lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
+                                     f["u"], f["v"], f["w"], *h, dt, 1.0, NPL),
+        56 * cells, 3, 3)
+    add("calmas", lambda: K.calmas(s64.ncells, f["u"], f["v"], f["w"], f["su"], s64._buf,
+                                   *h, dt, NPL), 32 * cells, 2, 2)
+    add("checkdt", lambda: K.checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"], s64._buf,
+                                     *h, NPL), 32 * cells, 1, 1)
+    add("wale", lambda: K.wale_viscosity(s64.ncells, f["u"], f["v"], f["w"], f["vis"], *h,
+                                         s64.rey, 0.46, NPL), 32 * cells, 1, 1)
+    add("add_ring", lambda: K.add_inplace(tuple(x + 2 for x in s64.ncells), f["p"], f["pp"],
+                                          NPL), 24 * (n + 2) ** 3, 1, 1)
+    add("bc_fill (u)", lambda: s64.fill_bc("u"), 0, 15, 15)
+    nv = s64.mg.n_vcycles
+    sweeps = s64.mg.n_pre + s64.mg.n_post
+    for tag, mg, eb, ps64, psmx in (("fp64", s64.mg, 8, 2 * sweeps * nv, 0),
+                                    ("fp32", smx.mg, 4, 0, 2 * sweeps * nv)):
+        L0, L1 = mg.levels[0].desc(), mg.levels[1].desc()
+        add("rbgs half-sweep L0 (%s)" % tag,
+            lambda L0=L0: _hip.call("wf_mg_rbgs", C.byref(L0), 0, st), 2 * eb * cells,
+            ps64, psmx)
+        add("restrict L0 (%s)" % tag,
+            lambda L0=L0, L1=L1: _hip.call("wf_mg_restrict", C.byref(L0), C.byref(L1), st),
+            (2 * eb + 2 * eb / 8) * cells, nv if tag == "fp64" else 0,
+            nv if tag == "fp32" else 0)
+        add("prolong L0 (%s)" % tag,
+            lambda L0=L0, L1=L1: _hip.call("wf_mg_prolong", C.byref(L1), C.byref(L0), st),
+            (2 * eb + eb / 8) * cells, nv if tag == "fp64" else 0,
+            nv if tag == "fp32" else 0)
+    return rows
+
+
+def dominant(rows, key):
+    best = max(rows, key=lambda r: r["us"] * r[key])
+    return best
+
+
+def cpu_baseline(a):
+    from oracle import c_oracle as CO
+    from oracle import np_oracle as O
+    case = O.Case(n3=(a.n,) * 3, reynolds=a.reynolds, sgs_enabled=True, cw=0.46, noise=0.01,
+                  mg_levels=a.levels, variable_dt=False, dt_init=a.dt)
+    sol = CO.CSolver(case)
+    sol.step()  # untimed: first touch of every array
+    t0 = time.perf_counter()
+    sol.step()
+    sec = time.perf_counter() - t0
+    return {"value": round(a.n ** 3 / sec / 1e6, 3), "unit": "Mcell/s", "cores": CO.threads(),
+            "kind": "port",
+            "sample": "step 2 of the same %d^3 Re-%d cavity (plain FP64 schedule, %d MG levels) "
+                      "in the C oracle (oracle/c/wf_oracle.c), %.2f s"
+                      % (a.n, a.reynolds, a.levels, sec)}
+
+
+def deviation(sa, sb):
+    out = {}
+    for nm in ("u", "v", "w"):
+        x = sa.f[nm].numpy()[3:-3, 3:-3, 3:-3]
+        y = sb.f[nm].numpy()[3:-3, 3:-3, 3:-3]
+        out[nm] = float(np.sqrt(np.mean((x - y) ** 2)))
+    p = sa.f["p"].numpy()[3:-3, 3:-3, 3:-3]
+    q = sb.f["p"].numpy()[3:-3, 3:-3, 3:-3]
+    out["p_demeaned"] = float(np.sqrt(np.mean(((p - p.mean()) - (q - q.mean())) ** 2)))
+    return out
+
+
+def main(argv=None):
+    a = parse(argv)
+    if a.gpus != 1:
+        print("bench.py: this build runs one cavity on one GPU; --gpus %d is not supported "
+              "(multi-GPU domain decomposition is out of scope)" % a.gpus, file=sys.stderr)
+        return 2
+    forms = [f.strip() for f in a.forms.split(",") if f.strip()]
+    res = {}
+    for form in forms:
+        sol, infos, sec = run_form(a, form)
+        res[form] = {"sol": sol, "infos": infos, "sec": sec,
+                     "mcell": a.n ** 3 * a.steps / sec / 1e6,
+                     "ms": 1000.0 * sec / a.steps}
+        print("# %s: %.3f ms/step, %.1f Mcell/s, iters %s, rmax %.3e" %
+              (form, res[form]["ms"], res[form]["mcell"], [i["iters"] for i in infos],
+               infos[-1]["rmax"]), file=sys.stderr, flush=True)
+    head = "mixed" if "mixed" in res else forms[0]
+    line = {"metric": METRIC, "value": round(res[head]["mcell"], 3), "unit": "Mcell/s",
+            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": round(res[head]["ms"], 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 (MG V-cycle f32)" if head == "mixed" else "f64",
+            "data": "synthetic (seeded hash-noise initial state)",
+            "config": {"workload": "lid-driven cavity %d^3, Re %g, WALE cw 0.46, fixed dt %g, "
+                                   "rmax<1e-6, %s form" % (a.n, a.reynolds, a.dt, head),
+                       "n": a.n, "mg_levels": a.levels, "vcycles_per_solve": 4,
+                       "sweeps": "V(6,6)+300 coarsest"}}
+    if "fp64" in res:
+        line["fp64_value"] = round(res["fp64"]["mcell"], 3)
+        line["fp64_ms_per_step"] = round(res["fp64"]["ms"], 4)
+        if head == "mixed":
+            line["speedup_mixed_vs_fp64"] = round(res["mixed"]["mcell"] / res["fp64"]["mcell"], 4)
+            line["deviation_mixed_vs_fp64"] = deviation(res["mixed"]["sol"], res["fp64"]["sol"])
+            line["rmax_trajectory"] = {f: [float("%.4g" % i["rmax"]) for i in res[f]["infos"]]
+                                       for f in ("mixed", "fp64")}
+    if not a.no_kernels and "fp64" in res and "mixed" in res:
+        rows = kernel_table(a, res["fp64"]["sol"], res["mixed"]["sol"])
+        line["kernels"] = rows
+        dom = dominant(rows, "per_step_mixed" if head == "mixed" else "per_step_fp64")
+        achieved = dom["bytes"] / dom["us"] / 1e3
+        line["roofline"] = {"bound": "hbm", "kernel": dom["kernel"],
+                            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                            "traffic": None}
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(a)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

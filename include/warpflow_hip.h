@@ -246,6 +246,41 @@ int wf_mg_solve(const wf_mg_level* levels, int nlev, const double* su,
                 double* pp, wf_grid g, int n_pre, int n_post, int n_coarse,
                 int n_vcycles, void* stream);
 
+/* max|rhs - A phi| over the real cells of one level (FP64 or FP32 level),
+ * written to *out_dev -- the residual norm of tests/test_mg.py:16-32. */
+int wf_mg_residual_max(const wf_mg_level* lev, double* out_dev, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Mixed-precision pressure solve.  No single counterpart in the reference:
+ * it stands in for Multigrid.solve (multigrid.py:324-337) when the
+ * V-cycle runs in FP32 on the error equation inside FP64 defect correction.
+ * ---------------------------------------------------------------------- */
+
+/* Bytes of device workspace wf_mg_solve_mixed / wf_field_mean need for a
+ * field of grid g (a few scalars plus one partial sum per workgroup). */
+int wf_mg_mixed_workspace_bytes(wf_grid g, size_t* bytes);
+
+/* pp = (approximate) A^-1 su.  levels: an all-FP32 hierarchy whose level 0
+ * has the field's real-cell counts.  Each cycle: FP64 defect r = su - A pp
+ * (matrix-free on the field), optional FP64 mean removal (flags & 1), FP32
+ * V-cycle from zero on r, pp += correction in FP64.  rtol <= 0: exactly
+ * max_cycles cycles, no host synchronisation.  rtol > 0: stop once
+ * max|r| <= rtol*max|su| (one host read per cycle), at most max_cycles.
+ * flags & 2: subtract the FP64 mean of pp at the end (pressure gauge).  The
+ * first ghost layer of pp gets the Neumann face mirror; every other ghost
+ * cell is zero.  *cycles_out / *res_out (nullable) report the V-cycles run
+ * and, in tolerance mode, the last relative residual.  work: device buffer
+ * of wf_mg_mixed_workspace_bytes. */
+int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su,
+                      double* pp, wf_grid g, int n_pre, int n_post,
+                      int n_coarse, int max_cycles, double rtol, int flags,
+                      double* work, size_t work_bytes, int* cycles_out,
+                      double* res_out, void* stream);
+
+/* *out_dev = mean of f over the real cells (deterministic FP64 sum). */
+int wf_field_mean(const double* f, wf_grid g, double* work, size_t work_bytes,
+                  double* out_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

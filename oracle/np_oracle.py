@@ -514,13 +514,16 @@ def mgcoeff(lev, phys, dxl, dyl, dzl):
     across physical faces; ap = -(aw+ae+as+an+ab+at).  The reference tests
     every axis against n3[0]-1; here each axis uses its own extent."""
     n0, n1, n2 = lev.n3
+    dt = lev.phi.dtype
     i, j, k = np.meshgrid(np.arange(n0), np.arange(n1), np.arange(n2),
                           indexing="ij")
-    awv = np.full(lev.n3, 1.0 / (dxl * dxl))
+    # 1/h^2 formed in float64 as the reference does, rounded once to the level
+    # precision; ap is then summed in the level precision
+    awv = np.full(lev.n3, 1.0 / (dxl * dxl), dtype=dt)
     aev = awv.copy()
-    asv = np.full(lev.n3, 1.0 / (dyl * dyl))
+    asv = np.full(lev.n3, 1.0 / (dyl * dyl), dtype=dt)
     anv = asv.copy()
-    abv = np.full(lev.n3, 1.0 / (dzl * dzl))
+    abv = np.full(lev.n3, 1.0 / (dzl * dzl), dtype=dt)
     atv = abv.copy()
     if phys["west"]:
         awv[i == 0] = 0.0
@@ -594,7 +597,6 @@ def prolong(c, f):
     """multigrid.py:122-146  phi_f += sum over the 8 surrounding coarse
     cells of wx*wy*wz*phi_c, per-axis weights 1/4, 3/4; reads the coarse
     ring (faces mirrored by the last refill, edges/corners still 0)."""
-    dt = f.phi.dtype.type
     c0x, wx0 = _prolong_axis(f.n3[0])
     c0y, wy0 = _prolong_axis(f.n3[1])
     c0z, wz0 = _prolong_axis(f.n3[2])
@@ -610,7 +612,6 @@ def prolong(c, f):
                 acc = acc + wgt * pc
     s = (slice(1, -1),) * 3
     f.phi[s] = f.phi[s] + acc
-    del dt
 
 
 def copy_in(src, lev, npl):
@@ -634,6 +635,7 @@ class Multigrid:
     def __init__(self, n3, nlevels, dx, dy, dz, dtype=np.float64,
                  n_pre=6, n_post=6, n_coarse=300, n_vcycles=4):
         self.phys = {s: True for s in SIDES}   # single domain, no periodic side
+        self.h0 = (dx, dy, dz)
         self.levels = []
         for l in range(nlevels):
             lev = Level(tuple(x >> l for x in n3), dtype)
@@ -683,6 +685,55 @@ class Multigrid:
         for _ in range(self.n_vcycles):
             self._vcycle()
         copy_out_ring(lev0, pp3, npl)
+
+
+def field_residual(su3, x3, lev0, npl):
+    """FP64 defect su - A x on the real cells of an npl-ghost field, with the
+    level-0 coefficients (ap*x added last, multigrid.py:110-117)."""
+    B = _Box((npl,) * 3, lev0.n3)
+    C = _Box((1, 1, 1), lev0.n3)
+    nb = (C.at(lev0.aw).astype(np.float64) * B.at(x3, -1, 0, 0)
+          + C.at(lev0.ae).astype(np.float64) * B.at(x3, 1, 0, 0)
+          + C.at(lev0.as_).astype(np.float64) * B.at(x3, 0, -1, 0)
+          + C.at(lev0.an).astype(np.float64) * B.at(x3, 0, 1, 0)
+          + C.at(lev0.ab).astype(np.float64) * B.at(x3, 0, 0, -1)
+          + C.at(lev0.at).astype(np.float64) * B.at(x3, 0, 0, 1))
+    return B.at(su3) - (nb + C.at(lev0.ap).astype(np.float64) * B.at(x3))
+
+
+def solve_mixed(mg32, su3, pp3, npl, ncycles, defect_mean=False, result_mean=True,
+                coef64=None):
+    """FP32 V-cycles inside FP64 defect correction (the mixed form; no
+    reference counterpart).  mg32: a float32 Multigrid.  coef64: a float64
+    Level with the level-0 coefficients for the FP64 defect (built from mg32's
+    level-0 spacing if omitted)."""
+    lev0 = mg32.levels[0]
+    if coef64 is None:
+        coef64 = Level(lev0.n3)
+        h = mg32.h0
+        mgcoeff(coef64, {s: True for s in SIDES}, *h)
+    pp3[...] = 0.0
+    B = _Box((npl,) * 3, lev0.n3)
+    mean = float(B.at(su3).mean()) if defect_mean else 0.0
+    for c in range(ncycles):
+        r = B.at(su3).copy() if c == 0 else field_residual(su3, pp3, coef64, npl)
+        if defect_mean:
+            r = r - mean
+        lev0.rhs[1:-1, 1:-1, 1:-1] = r.astype(np.float32)
+        lev0.phi[...] = 0.0
+        mg32._vcycle()
+        pp3[B.sl()] = B.at(pp3) + lev0.phi[1:-1, 1:-1, 1:-1].astype(np.float64)
+    if result_mean:
+        pp3[B.sl()] = B.at(pp3) - B.at(pp3).mean()
+    # first ghost layer: Neumann face mirror
+    n0, n1, n2 = lev0.n3
+    a, b, c_ = slice(npl, npl + n0), slice(npl, npl + n1), slice(npl, npl + n2)
+    pp3[npl - 1, b, c_] = pp3[npl, b, c_]
+    pp3[npl + n0, b, c_] = pp3[npl + n0 - 1, b, c_]
+    pp3[a, npl - 1, c_] = pp3[a, npl, c_]
+    pp3[a, npl + n1, c_] = pp3[a, npl + n1 - 1, c_]
+    pp3[a, b, npl - 1] = pp3[a, b, npl]
+    pp3[a, b, npl + n2] = pp3[a, b, npl + n2 - 1]
 
 
 # ===========================================================================

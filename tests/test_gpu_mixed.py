@@ -1,0 +1,105 @@
+"""Mixed-precision pressure solve on the GPU: against the oracle's FP32
+restatement, against the plain FP64 solve, to a tight tolerance, and inside
+the flow solver against the plain FP64 form."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from oracle import np_oracle as O
+from test_gpu_mg import _Cfg
+from test_gpu_step import gpu_solver, make_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def topo_for(n3):
+    from warpflow.topology import SerialComm, Topology
+    return Topology(0, 1, SerialComm(), (1, 1, 1), (0, 0, 0),
+                    {"x": (-1, -1), "y": (-1, -1), "z": (-1, -1)}, n3, (0, 0, 0))
+
+
+def demean(a):
+    s = (slice(3, -3),) * 3
+    return a[s] - a[s].mean()
+
+
+@pytest.mark.parametrize("n3,levels", [((32, 32, 32), 3), ((16, 32, 24), 3), ((64, 64, 64), 5)])
+def test_mixed_solve_matches_oracle_and_fp64(gpu, n3, levels):
+    from warpflow.device import array, zeros
+    from warpflow.mixed import MixedMultigrid
+    from warpflow.multigrid import Multigrid
+    h = tuple(1.0 / n for n in n3)
+    shape = tuple(n + 6 for n in n3)
+    su = np.random.default_rng(2).standard_normal(shape)
+    mgm = MixedMultigrid(_Cfg(levels), topo_for(n3), None, *h, result_mean=False)
+    pp = zeros(shape)
+    mgm.solve(array(su), pp, 3)
+    got = pp.numpy()
+    assert mgm.last_cycles == 4
+    ref32 = O.Multigrid(n3, levels, *h, dtype=np.float32)
+    pp_ref = np.zeros(shape)
+    O.solve_mixed(ref32, su, pp_ref, 3, 4, defect_mean=False, result_mean=False)
+    assert rel_err(got, pp_ref) < 1e-5
+    mg64 = Multigrid(_Cfg(levels), topo_for(n3), None, *h)
+    pp64 = zeros(shape)
+    mg64.solve(array(su), pp64, 3)
+    assert rel_err(got, pp64.numpy()) < 1e-5
+    # gauge: with result_mean the interior mean is removed, gradients unchanged
+    mgm.result_mean = True
+    mgm.solve(array(su), pp, 3)
+    g2 = pp.numpy()
+    assert abs(g2[3:-3, 3:-3, 3:-3].mean()) < 1e-12 * np.abs(g2).max()
+    assert rel_err(demean(g2), demean(got)) < 1e-12
+
+
+def test_tolerance_mode_reaches_1e10(gpu):
+    """Standalone Poisson (tests/test_mg.py:35-45 rhs): the mixed solve
+    converges to max|r| <= 1e-10 max|b| in FP64, like plain FP64 V-cycles."""
+    from warpflow.device import array, zeros
+    from warpflow.mixed import MixedMultigrid
+    from warpflow.multigrid import Multigrid
+    n3, levels = (64, 64, 64), 5
+    h = tuple(1.0 / n for n in n3)
+    b = np.random.default_rng(0).standard_normal(tuple(n + 2 for n in n3))
+    b[0], b[-1], b[:, 0], b[:, -1], b[:, :, 0], b[:, :, -1] = 0, 0, 0, 0, 0, 0
+    b[1:-1, 1:-1, 1:-1] -= b[1:-1, 1:-1, 1:-1].mean()
+    su = np.zeros(tuple(n + 6 for n in n3))
+    su[2:-2, 2:-2, 2:-2] = b
+    mgm = MixedMultigrid(_Cfg(levels), topo_for(n3), None, *h, rtol=1e-10, max_cycles=40,
+                         result_mean=True)
+    pp = zeros(su.shape)
+    mgm.solve(array(su), pp, 3)
+    assert mgm.last_residual <= 1e-10 and mgm.last_cycles < 20, (mgm.last_cycles, mgm.last_residual)
+    mg64 = Multigrid(_Cfg(levels), topo_for(n3), None, *h)
+    lev0 = mg64.levels[0]
+    lev0.rhs.assign(b)
+    lev0.phi.zero_()
+    bmax = np.abs(b).max()
+    for cyc in range(40):
+        mg64._vcycle()
+        if mg64.residual_max(0) <= 1e-10 * bmax:
+            break
+    p64 = np.zeros(su.shape)
+    p64[2:-2, 2:-2, 2:-2] = lev0.phi.numpy()
+    assert rel_err(demean(pp.numpy()), demean(p64)) < 1e-8
+    # the independent residual of the mixed result, in FP64 on the host
+    ref = O.Level(n3)
+    O.mgcoeff(ref, {s: True for s in O.SIDES}, *h)
+    ref.rhs[...] = b
+    ref.phi[1:-1, 1:-1, 1:-1] = pp.numpy()[3:-3, 3:-3, 3:-3]
+    assert np.abs(O.residual(ref)).max() <= 1.01e-10 * bmax
+
+
+def test_flow_solver_mixed_tracks_fp64(gpu):
+    """Velocity rms deviation of the mixed form from the plain FP64 form
+    stays far below 1e-6 over 20 steps (32^3, Re 400, WALE)."""
+    a = gpu_solver(make_cfg(n3=(32, 32, 32), precision="mixed"))
+    b = gpu_solver(make_cfg(n3=(32, 32, 32), precision="fp64"))
+    for _ in range(20):
+        ia, ib = a.step(), b.step()
+        assert ia["iters"] == ib["iters"]
+        assert ia["rmax"] == pytest.approx(ib["rmax"], rel=1e-3)
+    for nm in ("u", "v", "w"):
+        x = a.f[nm].numpy()[3:-3, 3:-3, 3:-3]
+        y = b.f[nm].numpy()[3:-3, 3:-3, 3:-3]
+        assert np.sqrt(np.mean((x - y) ** 2)) < 1e-8, nm

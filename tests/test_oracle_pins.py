@@ -468,3 +468,35 @@ def test_cavity_fields_finite_and_ghosts(cavity24):
     assert np.allclose(p[s, N + n, s], p[s, N + n - 1, s])
     assert np.allclose(p[s, s, N - 1], p[s, s, N])
     assert np.allclose(p[s, s, N + n], p[s, s, N + n - 1])
+
+
+def test_c_oracle_matches_numpy_oracle_bitwise():
+    """Two independent transcriptions of the step schedule (NumPy and C)."""
+    from oracle import c_oracle as CO
+    for case in (O.Case(n3=(24, 24, 24)),
+                 O.Case(n3=(16, 24, 32), sgs_enabled=False, variable_dt=False,
+                        dt_init=1e-3, reynolds=100)):
+        a, b = O.Solver(case), CO.CSolver(case)
+        for _ in range(3):
+            ia, ib = a.step(), b.step()
+            assert ia["iters"] == ib["iters"] and ia["dt"] == ib["dt"]
+            assert a.rmax_history == b.rmax_history()
+        for nm in ("u", "v", "w", "p", "ustar", "su", "pp"):
+            assert np.array_equal(a.f[nm], b.field(nm)), nm
+        np.testing.assert_allclose(b.field("vis"), a.f["vis"], rtol=1e-15)
+
+
+def test_mixed_form_reproduces_fp64_vcycles():
+    """FP32 V-cycles inside FP64 defect correction, with the defect mean
+    kept, equal the plain FP64 V-cycles up to FP32 rounding (a V-cycle from
+    zero on the defect is the same stationary iteration)."""
+    n3, h = (32, 32, 32), 1.0 / 32
+    su = np.random.default_rng(1).standard_normal(tuple(n + 6 for n in n3))
+    mg64 = O.Multigrid(n3, 3, h, h, h)
+    pp64 = np.zeros_like(su)
+    mg64.solve(su, pp64, 3)
+    mg32 = O.Multigrid(n3, 3, h, h, h, dtype=np.float32)
+    pp = np.zeros_like(su)
+    O.solve_mixed(mg32, su, pp, 3, 4, defect_mean=False, result_mean=False)
+    s = (slice(3, -3),) * 3
+    assert np.abs(pp[s] - pp64[s]).max() < 1e-5 * np.abs(pp64[s]).max()

@@ -1,0 +1,319 @@
+// Mixed-precision pressure solve: FP32 RBGS V-cycles on the error equation
+// inside FP64 defect correction.
+//
+//   x = 0 (FP64, the caller's pp field)
+//   repeat:  r = b - A x            FP64 defect, matrix-free, on the field
+//            [r -= mean(b)]          FP64 mean removal (optional)
+//            e = V-cycle_32(r)       FP32, from e = 0, same V(pre, post) and
+//                                    coarse sweeps as the plain form
+//            x += e                  FP64 update
+//   [x -= mean(x)]                   FP64 pressure-gauge removal (optional)
+//   ring faces of x = Neumann mirror (the state the plain solve leaves)
+//
+// A V-cycle started from e = 0 on the defect is the plain iteration
+// x <- x + M(b - A x), so with the same cycle count the result equals the
+// FP64 V-cycles up to FP32 rounding of each correction, while every residual,
+// update and norm stays FP64.  In tolerance mode the loop stops once
+// max|b - A x| <= rtol * max|b| (the reference's residual norm,
+// tests/test_mg.py:16-32).
+//
+// Sums are deterministic: fixed-shape block reductions into a partials
+// buffer, then one workgroup folds the partials in a fixed order.
+#include "mg_common.h"
+
+namespace wf {
+
+enum : int { MIX_DEFECT_MEAN = 1, MIX_RESULT_MEAN = 2 };
+
+constexpr int WORK_HEAD = 4;  // work[0] mean, work[1] max|r|, work[2] sum, work[3] spare
+
+inline dim3 field_grid(const wf_grid& g) {
+  return dim3(div_up(g.nk, 64), div_up(g.nj, 4), g.ni);
+}
+inline int64_t field_blocks(const wf_grid& g) {
+  const dim3 b = field_grid(g);
+  return (int64_t)b.x * b.y * b.z;
+}
+
+__device__ inline int64_t block_linear() {
+  return ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+}
+
+// Block sum of 256 values in a fixed order; every thread must call it.
+__device__ inline double block_sum_fixed(double v) {
+  __shared__ double part[4];
+  v = wave_sum(v);
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  if ((t & 63) == 0) part[t >> 6] = v;
+  __syncthreads();
+  return ((part[0] + part[1]) + part[2]) + part[3];
+}
+
+// Partial sums of f over the real cells of a field, one per block.
+__global__ void __launch_bounds__(256)
+field_partials_kernel(const double* __restrict__ f, wf_grid g, double* __restrict__ partials) {
+  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+  double v = 0.0;
+  if (k < g.nk && j < g.nj) v = f[at(g, i + g.npl, j + g.npl, k + g.npl)];
+  const double s = block_sum_fixed(v);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partials[block_linear()] = s;
+}
+
+// out = scale * sum(partials[0..n)), folded by one 1024-thread workgroup in
+// a fixed order.
+__global__ void __launch_bounds__(1024)
+fold_kernel(const double* __restrict__ partials, int64_t n, double scale, double* __restrict__ out) {
+  __shared__ double s[1024];
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  for (int64_t q = t; q < n; q += 1024) acc += partials[q];
+  s[t] = acc;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) s[t] = s[t] + s[t + w];
+    __syncthreads();
+  }
+  if (t == 0) *out = s[0] * scale;
+}
+
+__global__ void set_scalar_kernel(double* p, double v) { *p = v; }
+
+// r = b - A x (FP64; x == nullptr means x = 0), minus *mean if given, cast to
+// FP32 into the level-0 rhs; *rmax = max(*rmax, max|b - A x|).
+__global__ void __launch_bounds__(256)
+defect_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_grid g,
+              float* __restrict__ r32, MgDev d, double c0, double c1, double c2,
+              const double* __restrict__ mean, double* __restrict__ rmax) {
+  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+  double av = 0.0;
+  if (k < g.nk && j < g.nj) {
+    const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
+    double r = b[q];
+    if (x) {
+      const Coef<double> c = mg_coef(d, i, j, k, c0, c1, c2);
+      const double nb = c.aw * x[q - g.si] + c.ae * x[q + g.si] + c.as * x[q - g.sj] +
+                        c.an * x[q + g.sj] + c.ab * x[q - 1] + c.at * x[q + 1];
+      r = r - (nb + c.ap * x[q]);
+    }
+    av = fabs(r);
+    if (mean) r = r - *mean;
+    r32[mg_at(d, i, j, k)] = (float)r;
+  }
+  if (rmax) {
+    __shared__ double part[4];
+    double m = wave_max(av);
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((t & 63) == 0) part[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) atomic_max_nonneg(rmax, nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3])));
+  }
+}
+
+// x += e (FP32 correction widened to FP64) on the real cells.
+__global__ void __launch_bounds__(256)
+update_kernel(double* __restrict__ x, wf_grid g, const float* __restrict__ e, MgDev d) {
+  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+  if (k >= g.nk || j >= g.nj) return;
+  const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
+  x[q] = x[q] + (double)e[mg_at(d, i, j, k)];
+}
+
+__global__ void __launch_bounds__(256)
+shift_kernel(double* __restrict__ x, wf_grid g, const double* __restrict__ mean) {
+  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+  if (k >= g.nk || j >= g.nj) return;
+  const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
+  x[q] = x[q] - *mean;
+}
+
+// First ghost layer of the field: faces = mirror of the adjacent real cell,
+// edges/corners untouched (the ring state of the plain solve's copy-out).
+__global__ void __launch_bounds__(256) field_ring_kernel(double* __restrict__ x, wf_grid g) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n0 = g.ni, n1 = g.nj, n2 = g.nk, P = g.npl;
+  const int fx = n1 * n2, fy = n0 * n2, fz = n0 * n1;
+  int q = t;
+  if (q < 2 * fx) {
+    const int hi = q >= fx;
+    q -= hi * fx;
+    const int j = q / n2, k = q % n2, i = hi ? n0 - 1 : 0;
+    const int64_t p = at(g, i + P, j + P, k + P);
+    x[p + (hi ? g.si : -g.si)] = x[p];
+    return;
+  }
+  q -= 2 * fx;
+  if (q < 2 * fy) {
+    const int hi = q >= fy;
+    q -= hi * fy;
+    const int i = q / n2, k = q % n2, j = hi ? n1 - 1 : 0;
+    const int64_t p = at(g, i + P, j + P, k + P);
+    x[p + (hi ? g.sj : -g.sj)] = x[p];
+    return;
+  }
+  q -= 2 * fy;
+  if (q < 2 * fz) {
+    const int hi = q >= fz;
+    q -= hi * fz;
+    const int i = q / n1, j = q % n1, k = hi ? n2 - 1 : 0;
+    const int64_t p = at(g, i + P, j + P, k + P);
+    x[p + (hi ? 1 : -1)] = x[p];
+  }
+}
+
+// max|rhs - A phi| over the real cells of one level (FP64 or FP32).
+template <class T>
+__global__ void __launch_bounds__(256)
+level_resmax_kernel(const T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, T c1,
+                    T c2, double* __restrict__ out) {
+  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+  double av = 0.0;
+  if (k < d.n2 && j < d.n1) {
+    const int64_t x = mg_at(d, i, j, k);
+    const Coef<T> c = mg_coef(d, i, j, k, c0, c1, c2);
+    av = fabs((double)(rhs[x] - (nb_sum(phi, x, d, c) + c.ap * phi[x])));
+  }
+  __shared__ double part[4];
+  double m = wave_max(av);
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  if ((t & 63) == 0) part[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) atomic_max_nonneg(out, nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3])));
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+namespace {
+
+MgDev dev_of(const wf_mg_level& L) {
+  MgDev d;
+  d.n0 = L.n0;
+  d.n1 = L.n1;
+  d.n2 = L.n2;
+  d.s1 = L.n2 + 2;
+  d.s0 = (int64_t)(L.n1 + 2) * d.s1;
+  return d;
+}
+
+int field_sum(const double* f, const wf_grid& g, double scale, double* work, double* out,
+              hipStream_t s) {
+  double* partials = work + WORK_HEAD;
+  hipLaunchKernelGGL(field_partials_kernel, field_grid(g), dim3(64, 4), 0, s, f, g, partials);
+  WF_LAUNCHED();
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(1024), 0, s, partials, field_blocks(g), scale,
+                     out);
+  WF_LAUNCHED();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_mg_mixed_workspace_bytes(wf_grid g, size_t* bytes) {
+  if (!bytes || !mg_field_grid_ok(g)) return WF_EINVAL;
+  *bytes = (size_t)(WORK_HEAD + field_blocks(g)) * sizeof(double);
+  return 0;
+}
+
+int wf_mg_residual_max(const wf_mg_level* L, double* out_dev, void* stream) {
+  if (!mg_level_ok(L) || !out_dev) return WF_EINVAL;
+  const hipStream_t s = (hipStream_t)stream;
+  const MgDev d = dev_of(*L);
+  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, s, out_dev, 0.0);
+  WF_LAUNCHED();
+  if (L->prec == 8)
+    hipLaunchKernelGGL(level_resmax_kernel<double>, mg_cell_grid(d.n0, d.n1, d.n2), dim3(64, 4),
+                       0, s, (const double*)L->phi, (const double*)L->rhs, d,
+                       1.0 / (L->h0 * L->h0), 1.0 / (L->h1 * L->h1), 1.0 / (L->h2 * L->h2),
+                       out_dev);
+  else
+    hipLaunchKernelGGL(level_resmax_kernel<float>, mg_cell_grid(d.n0, d.n1, d.n2), dim3(64, 4),
+                       0, s, (const float*)L->phi, (const float*)L->rhs, d,
+                       (float)(1.0 / (L->h0 * L->h0)), (float)(1.0 / (L->h1 * L->h1)),
+                       (float)(1.0 / (L->h2 * L->h2)), out_dev);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, double* pp,
+                      wf_grid g, int n_pre, int n_post, int n_coarse, int max_cycles,
+                      double rtol, int flags, double* work, size_t work_bytes, int* cycles_out,
+                      double* res_out, void* stream) {
+  if (!mg_hier_ok(levels, nlev) || !mg_field_grid_ok(g) || levels[0].prec != 4) return WF_EINVAL;
+  if (levels[0].n0 != g.ni || levels[0].n1 != g.nj || levels[0].n2 != g.nk) return WF_EINVAL;
+  if (max_cycles < 0 || n_pre < 0 || n_post < 0 || n_coarse < 0 || !work) return WF_EINVAL;
+  size_t need = 0;
+  wf_mg_mixed_workspace_bytes(g, &need);
+  if (work_bytes < need) return WF_EINVAL;
+  const hipStream_t s = (hipStream_t)stream;
+  const wf_mg_level& L0 = levels[0];
+  const MgDev d0 = dev_of(L0);
+  const double c0 = 1.0 / (L0.h0 * L0.h0), c1 = 1.0 / (L0.h1 * L0.h1),
+               c2 = 1.0 / (L0.h2 * L0.h2);
+  const double ncell = (double)g.ni * g.nj * g.nk;
+  const bool tol_mode = rtol > 0.0;
+  double* mean = work + 0;
+  double* rmax = work + 1;
+  double* xmean = work + 2;
+
+  int r = mg_zero<double>(pp, (int64_t)(g.ni + 2 * g.npl) * g.si, s);
+  if (r) return r;
+  if (flags & MIX_DEFECT_MEAN) {
+    r = field_sum(su, g, 1.0 / ncell, work, mean, s);
+    if (r) return r;
+  }
+  double bmax = 0.0, res = 0.0;
+  int cycles = 0;
+  for (int c = 0; c <= max_cycles; ++c) {
+    const bool last_check = c == max_cycles;
+    if (last_check && !tol_mode) break;
+    if (tol_mode) {
+      hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, s, rmax, 0.0);
+      WF_LAUNCHED();
+    }
+    hipLaunchKernelGGL(defect_kernel, field_grid(g), dim3(64, 4), 0, s, su, c == 0 ? nullptr : pp,
+                       g, (float*)L0.rhs, d0, c0, c1, c2,
+                       (flags & MIX_DEFECT_MEAN) ? mean : nullptr, tol_mode ? rmax : nullptr);
+    WF_LAUNCHED();
+    if (tol_mode) {
+      double h = 0.0;
+      WF_TRY(hipMemcpyAsync(&h, rmax, sizeof(double), hipMemcpyDeviceToHost, s));
+      WF_TRY(hipStreamSynchronize(s));
+      if (c == 0) bmax = h;
+      res = bmax > 0.0 ? h / bmax : 0.0;
+      if (res <= rtol || last_check) break;
+    }
+    r = mg_zero<float>(L0.phi, mg_level_elems(L0), s);
+    if (r) return r;
+    r = mg_vcycle_levels(levels, nlev, n_pre, n_post, n_coarse, s);
+    if (r) return r;
+    hipLaunchKernelGGL(update_kernel, field_grid(g), dim3(64, 4), 0, s, pp, g,
+                       (const float*)L0.phi, d0);
+    WF_LAUNCHED();
+    ++cycles;
+  }
+  if (flags & MIX_RESULT_MEAN) {
+    r = field_sum(pp, g, 1.0 / ncell, work, xmean, s);
+    if (r) return r;
+    hipLaunchKernelGGL(shift_kernel, field_grid(g), dim3(64, 4), 0, s, pp, g, xmean);
+    WF_LAUNCHED();
+  }
+  const int nface = 2 * (g.nj * g.nk + g.ni * g.nk + g.ni * g.nj);
+  hipLaunchKernelGGL(field_ring_kernel, dim3(div_up(nface, 256)), dim3(256), 0, s, pp, g);
+  WF_LAUNCHED();
+  if (cycles_out) *cycles_out = cycles;
+  if (res_out) *res_out = res;
+  return 0;
+}
+
+int wf_field_mean(const double* f, wf_grid g, double* work, size_t work_bytes, double* out_dev,
+                  void* stream) {
+  size_t need = 0;
+  if (!mg_field_grid_ok(g) || wf_mg_mixed_workspace_bytes(g, &need) || work_bytes < need)
+    return WF_EINVAL;
+  return field_sum(f, g, 1.0 / ((double)g.ni * g.nj * g.nk), work, out_dev, (hipStream_t)stream);
+}
+
+}  // extern "C"

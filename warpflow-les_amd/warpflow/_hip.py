@@ -88,6 +88,12 @@ _SIGS = {
     "wf_mg_copy_out_ring": [_L, _P, _G, _P],
     "wf_mg_vcycle": [_L, _I, _I, _I, _I, _P],
     "wf_mg_solve": [_L, _I, _P, _P, _G, _I, _I, _I, _I, _P],
+    "wf_mg_residual_max": [_L, _P, _P],
+    # mixed-precision solve
+    "wf_mg_mixed_workspace_bytes": [_G, C.POINTER(C.c_size_t)],
+    "wf_mg_solve_mixed": [_L, _I, _P, _P, _G, _I, _I, _I, _I, _D, _I, _P, _Z,
+                          C.POINTER(C.c_int), C.POINTER(C.c_double), _P],
+    "wf_field_mean": [_P, _G, _P, _Z, _P, _P],
 }
 
 

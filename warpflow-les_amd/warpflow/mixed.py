@@ -1,0 +1,60 @@
+"""Mixed-precision pressure solve: FP32 V-cycles inside FP64 defect
+correction, a drop-in for ``Multigrid`` (``solve(su, pp, npl)``).
+
+Each cycle forms the FP64 defect ``r = su - A pp`` on the field, runs one
+FP32 RBGS V-cycle (the same V(n_pre, n_post) schedule and n_coarse coarsest
+sweeps as the plain form) on ``A e = r`` from ``e = 0``, and adds ``e`` to
+``pp`` in FP64.  With ``rtol = 0`` it runs exactly ``max_cycles`` cycles
+(default: the plain form's ``n_vcycles``), which reproduces the plain FP64
+V-cycles up to FP32 rounding of each correction; with ``rtol > 0`` it stops
+once ``max|r| <= rtol * max|su|``.
+
+``defect_mean`` removes the FP64 mean of the defect before the FP32 cycle
+(a consistent singular Neumann system); ``result_mean`` subtracts the FP64
+mean of ``pp`` afterwards (a pressure gauge; velocities only see grad p).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _hip
+from .device import DeviceArray, current_stream, grid_of, zeros
+from .multigrid import Multigrid
+
+
+class MixedMultigrid(Multigrid):
+    def __init__(self, cfg, topo, device, dx, dy, dz, rtol=0.0, max_cycles=None,
+                 defect_mean=False, result_mean=True):
+        super().__init__(cfg, topo, device, dx, dy, dz, dtype=np.float32)
+        self.rtol = float(rtol)
+        self.max_cycles = max_cycles
+        self.defect_mean = bool(defect_mean)
+        self.result_mean = bool(result_mean)
+        self._work = None
+        self.last_cycles = 0
+        self.last_residual = 0.0
+
+    def _workspace(self, g):
+        nbytes = C.c_size_t(0)
+        _hip.call("wf_mg_mixed_workspace_bytes", g, C.byref(nbytes))
+        if self._work is None or self._work.nbytes < nbytes.value:
+            self._work = zeros((nbytes.value + 7) // 8, np.float64, self.device)
+        return self._work
+
+    def solve(self, su3: DeviceArray, pp3: DeviceArray, npl: int):
+        g = grid_of(pp3, npl)
+        if su3.shape != pp3.shape:
+            raise ValueError("su and pp must have one shape")
+        work = self._workspace(g)
+        descs = self._descs()
+        cycles = C.c_int(0)
+        res = C.c_double(0.0)
+        flags = (1 if self.defect_mean else 0) | (2 if self.result_mean else 0)
+        ncyc = self.n_vcycles if self.max_cycles is None else int(self.max_cycles)
+        _hip.call("wf_mg_solve_mixed", descs, len(self.levels), su3.ptr, pp3.ptr, g,
+                  self.n_pre, self.n_post, self.n_coarse, ncyc, self.rtol, flags,
+                  work.ptr, work.nbytes, C.byref(cycles), C.byref(res), current_stream())
+        self.last_cycles = cycles.value
+        self.last_residual = res.value

@@ -62,7 +62,7 @@ class Multigrid:
     Neumann Laplacian; fixed n_vcycles V(n_pre, n_post) cycles with n_coarse
     sweeps on the coarsest level, starting from zero, no mean removal."""
 
-    def __init__(self, cfg, topo, device, dx, dy, dz):
+    def __init__(self, cfg, topo, device, dx, dy, dz, dtype=np.float64):
         self.cfg = cfg
         self.topo = topo
         self.device = device
@@ -77,7 +77,7 @@ class Multigrid:
         for l in range(cfg.numerics.mg_levels):
             n3 = tuple(x >> l for x in nb)
             h3 = (dx * 2.0 ** l, dy * 2.0 ** l, dz * 2.0 ** l)
-            self.levels.append(_Level(n3, h3, device))
+            self.levels.append(_Level(n3, h3, device, dtype))
         self.n_pre = 6
         self.n_post = 6
         self.n_coarse = 300
@@ -88,6 +88,14 @@ class Multigrid:
         for i, lev in enumerate(self.levels):
             arr[i] = lev.desc()
         return arr
+
+    def residual_max(self, l=0):
+        """max|rhs - A phi| over the real cells of level l (the reference's
+        residual norm, tests/test_mg.py:16-32)."""
+        d = self.levels[l].desc()
+        out = zeros(1, np.float64, self.device)
+        _hip.call("wf_mg_residual_max", C.byref(d), out.ptr, current_stream())
+        return float(out.numpy()[0])
 
     def _refill(self, lev):
         """multigrid.py:264-287  faces-only Neumann mirror of the ring."""
