@@ -9,8 +9,9 @@ iterations), RBGS V(6,6) multigrid with 300 coarsest sweeps and 4 V-cycles
 per solve on 6 levels (8^3 coarsest).  Two forms run back to back on the same
 GPU from the same initial state:
 
-  mixed -- FP32 V-cycles inside FP64 defect correction (the headline value)
-  fp64  -- the reference's plain FP64 V-cycles
+  mixed -- FP32 V-cycles inside FP64 defect correction, 2 cycles per solve
+           by default (--mixed-cycles) (the headline value)
+  fp64  -- the reference's plain FP64 V-cycles, 4 per solve
 
 W untimed steps, then K steps timed between device synchronisations.
 value = mixed-form Mcell/s = nx*ny*nz / (seconds per step) / 1e6.  The JSON
@@ -53,6 +54,8 @@ def parse(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel table")
     ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
+    ap.add_argument("--mixed-cycles", type=int, default=2,
+                    help="FP64-refined FP32 V-cycles per pressure solve in the mixed form")
     return ap.parse_args(argv)
 
 
@@ -68,6 +71,7 @@ def make_cfg(a, precision):
     c.numerics.div_tolerance = 1e-6
     c.numerics.max_outer_iters = 30
     c.numerics.mg_precision = precision
+    c.numerics.mg_mixed_cycles = a.mixed_cycles
     c.time.variable_dt = False
     c.time.dt_init = a.dt
     validate(c)
@@ -116,7 +120,8 @@ def kernel_table(a, s64, smx):
     """Per-kernel HIP-event timings on the solvers' own arrays.  bytes =
     algorithmic bytes per launch (distinct arrays read + written, RMW twice,
     element size x launch cells); per_step = launches per step of the
-    schedule with 2 outer iterations (one MG solve of 4 V-cycles)."""
+    schedule with 2 outer iterations (one pressure solve: 4 V-cycles in the
+    FP64 form, --mixed-cycles in the mixed form)."""
     from warpflow import _hip
     from warpflow import kernels as K
     from warpflow.device import current_stream
@@ -154,21 +159,24 @@ def kernel_table(a, s64, smx):
                                           NPL), 24 * (n + 2) ** 3, 1, 1)
     add("bc_fill (u)", lambda: s64.fill_bc("u"), 0, 15, 15)
     nv = s64.mg.n_vcycles
+    nm = a.mixed_cycles
     sweeps = s64.mg.n_pre + s64.mg.n_post
-    for tag, mg, eb, ps64, psmx in (("fp64", s64.mg, 8, 2 * sweeps * nv, 0),
-                                    ("fp32", smx.mg, 4, 0, 2 * sweeps * nv)):
+    for tag, mg, eb, cyc64, cycmx in (("fp64", s64.mg, 8, nv, 0), ("fp32", smx.mg, 4, 0, nm)):
         L0, L1 = mg.levels[0].desc(), mg.levels[1].desc()
-        add("rbgs half-sweep L0 (%s)" % tag,
-            lambda L0=L0: _hip.call("wf_mg_rbgs", C.byref(L0), 0, st), 2 * eb * cells,
-            ps64, psmx)
+        # one call = 2 fused full sweeps (phi -> tmp -> phi); bytes per sweep:
+        # read phi and rhs, write the other buffer
+        T2 = T.avg_us(lambda L0=L0: _hip.call("wf_mg_relax", C.byref(L0), 2, st), a.reps) / 2
+        nb = 3 * eb * cells
+        rows.append({"kernel": "rbgs full sweep L0 (%s)" % tag, "us": round(T2, 2),
+                     "bytes": int(nb), "GBps": round(nb / T2 / 1e3, 1),
+                     "hbm_frac": round(nb / T2 / 1e3 / HBM_PEAK_GBS, 3),
+                     "per_step_fp64": sweeps * cyc64, "per_step_mixed": sweeps * cycmx})
         add("restrict L0 (%s)" % tag,
             lambda L0=L0, L1=L1: _hip.call("wf_mg_restrict", C.byref(L0), C.byref(L1), st),
-            (2 * eb + 2 * eb / 8) * cells, nv if tag == "fp64" else 0,
-            nv if tag == "fp32" else 0)
+            (2 * eb + 2 * eb / 8) * cells, cyc64, cycmx)
         add("prolong L0 (%s)" % tag,
             lambda L0=L0, L1=L1: _hip.call("wf_mg_prolong", C.byref(L1), C.byref(L0), st),
-            (2 * eb + eb / 8) * cells, nv if tag == "fp64" else 0,
-            nv if tag == "fp32" else 0)
+            (2 * eb + eb / 8) * cells, cyc64, cycmx)
     return rows
 
 
@@ -231,7 +239,8 @@ def main(argv=None):
             "data": "synthetic (seeded hash-noise initial state)",
             "config": {"workload": "lid-driven cavity %d^3, Re %g, WALE cw 0.46, fixed dt %g, "
                                    "rmax<1e-6, %s form" % (a.n, a.reynolds, a.dt, head),
-                       "n": a.n, "mg_levels": a.levels, "vcycles_per_solve": 4,
+                       "n": a.n, "mg_levels": a.levels,
+                       "vcycles_per_solve": {"fp64": 4, "mixed": a.mixed_cycles},
                        "sweeps": "V(6,6)+300 coarsest"}}
     if "fp64" in res:
         line["fp64_value"] = round(res["fp64"]["mcell"], 3)

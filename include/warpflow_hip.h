@@ -46,13 +46,17 @@ typedef struct wf_grid {
  * h0..h2 are the level spacings dx*2^l, dy*2^l, dz*2^l (multigrid.py:253).
  * Coefficients are matrix-free: 1/h^2 per face, 0 across the six physical
  * faces of the single cavity domain, ap = -(aw+ae+as+an+ab+at)
- * (multigrid.py:23-67). */
+ * (multigrid.py:23-67).  tmp (nullable) is a second array of phi's shape,
+ * zero-initialised: with it, smoothing runs as fused full sweeps that
+ * ping-pong phi -> tmp -> phi (results always end in phi); without it, as
+ * two in-place half-sweep launches per sweep. */
 typedef struct wf_mg_level {
   void* phi;
   void* rhs;
   int32_t n0, n1, n2;
   int32_t prec;
   double h0, h1, h2;
+  void* tmp;
 } wf_mg_level;
 
 /* ------------------------------------------------------------------------

@@ -88,11 +88,15 @@ def test_rbgs_refill_restrict_prolong(gpu, n3):
     assert rel_err(f.phi.numpy(), rf.phi) < TOL
 
 
-@pytest.mark.parametrize("n3,nsweeps", [((8, 8, 8), 300), ((20, 20, 20), 6),
-                                        ((40, 40, 40), 3), ((16, 8, 24), 5)])
+@pytest.mark.parametrize("n3,nsweeps", [((8, 8, 8), 300), ((20, 20, 20), 300),
+                                        ((20, 20, 20), 6), ((40, 40, 40), 3),
+                                        ((16, 8, 24), 5), ((32, 32, 32), 4),
+                                        ((64, 48, 40), 6), ((24, 72, 100), 2)])
 def test_relax_paths(gpu, n3, nsweeps):
-    """_relax: small levels run in one LDS-resident workgroup, large ones as
-    colour half-sweep launches; both must match the oracle incl. the ring."""
+    """_relax: small levels (and long coarsest solves) run in one LDS-resident
+    workgroup, larger ones as fused out-of-place full sweeps (ping-pong, both
+    tile shapes, partial tiles), an odd leftover sweep as two half-sweep
+    launches; all must match the oracle incl. the ring."""
     mg, ref = build(n3, 1)
     load(mg, ref, seed=4)
     ref._relax(0, nsweeps)

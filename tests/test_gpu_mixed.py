@@ -31,7 +31,8 @@ def test_mixed_solve_matches_oracle_and_fp64(gpu, n3, levels):
     h = tuple(1.0 / n for n in n3)
     shape = tuple(n + 6 for n in n3)
     su = np.random.default_rng(2).standard_normal(shape)
-    mgm = MixedMultigrid(_Cfg(levels), topo_for(n3), None, *h, result_mean=False)
+    mgm = MixedMultigrid(_Cfg(levels), topo_for(n3), None, *h, max_cycles=4,
+                         result_mean=False)
     pp = zeros(shape)
     mgm.solve(array(su), pp, 3)
     got = pp.numpy()
@@ -90,16 +91,21 @@ def test_tolerance_mode_reaches_1e10(gpu):
     assert np.abs(O.residual(ref)).max() <= 1.01e-10 * bmax
 
 
-def test_flow_solver_mixed_tracks_fp64(gpu):
-    """Velocity rms deviation of the mixed form from the plain FP64 form
-    stays far below 1e-6 over 20 steps (32^3, Re 400, WALE)."""
-    a = gpu_solver(make_cfg(n3=(32, 32, 32), precision="mixed"))
+@pytest.mark.parametrize("cycles,bound", [(4, 1e-10), (2, 1e-7)])
+def test_flow_solver_mixed_tracks_fp64(gpu, cycles, bound):
+    """Velocity rms deviation of the mixed form from the plain FP64 form over
+    20 steps (32^3, Re 400, WALE): at the plain form's 4 cycles only FP32
+    rounding separates them; at the default 2 cycles the deviation stays far
+    below the 1e-6 gate, with the same outer-iteration and rmax history."""
+    cfg = make_cfg(n3=(32, 32, 32), precision="mixed")
+    cfg.numerics.mg_mixed_cycles = cycles
+    a = gpu_solver(cfg)
     b = gpu_solver(make_cfg(n3=(32, 32, 32), precision="fp64"))
     for _ in range(20):
         ia, ib = a.step(), b.step()
         assert ia["iters"] == ib["iters"]
-        assert ia["rmax"] == pytest.approx(ib["rmax"], rel=1e-3)
+        assert ia["rmax"] == pytest.approx(ib["rmax"], rel=1e-2)
     for nm in ("u", "v", "w"):
         x = a.f[nm].numpy()[3:-3, 3:-3, 3:-3]
         y = b.f[nm].numpy()[3:-3, 3:-3, 3:-3]
-        assert np.sqrt(np.mean((x - y) ** 2)) < 1e-8, nm
+        assert np.sqrt(np.mean((x - y) ** 2)) < bound, nm

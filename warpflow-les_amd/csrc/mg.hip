@@ -151,6 +151,176 @@ prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev 
   ring_images(phif, f, i, j, k, x, v);
 }
 
+// One full RBGS sweep (colour 0, then colour 1, then the ring refill) out of
+// place: pout = S(pin).  Colour-0 (odd i+j+k) cells read only even cells of
+// pin; colour-1 (even) cells read only the new odd values.  A workgroup owns a
+// TJ x TK (j, k) tile (512 cells = 256 (even, odd) pairs, one per thread) and
+// streams along i over a chunk of planes, keeping three planes of pin (tile +
+// 2-cell halo) and three planes of new odd values (tile + 1-cell halo) in LDS:
+// at plane q it computes the odd values of q, then the even values of q-1,
+// and writes plane q-1 of pout (both colours + face-ring images).  The next
+// plane of pin and the next cells' rhs are prefetched into registers while
+// the current plane computes.
+//
+// The loop body is branch-free: load addresses are clamped into the level
+// array (ring included).  A clamped value can only reach an odd cell outside
+// the domain, and those are forced to 0 (their coefficients are 0 in every
+// real cell's update), so clamping changes no result.  Per-thread offsets
+// within a plane depend only on the parity of the plane and are computed once.
+// HBM traffic per sweep: one read of pin and rhs, one write of pout; the
+// per-cell arithmetic is the half-sweep kernel's, in the same order.
+template <class T, int TJ, int TK>
+__global__ void __launch_bounds__(256)
+sweep_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
+             MgDev d, T c0, T c1, T c2, int ich) {
+  static_assert(TJ * TK == 512, "one (even, odd) pair per thread");
+  constexpr int PJ = TJ + 4, PK = TK + 4, PN = PJ * PK;
+  constexpr int OJ = TJ + 2, OK = TK + 2, OH = OK / 2;
+  constexpr int NP = (PN + 255) / 256;        // pin values per thread per plane
+  constexpr int NO = (OJ * OH + 255) / 256;   // odd cells per thread per plane
+  constexpr int HK = TK / 2;                  // pairs per tile row
+  __shared__ T P[3][PN];
+  __shared__ T O[3][OJ * OK];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.y * TJ, k0 = blockIdx.x * TK;
+  const int i0 = blockIdx.z * ich, i1 = min(i0 + ich, d.n0);
+  const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
+  const int s1 = (int)d.s1;
+  auto slot = [](int q) { return ((q % 3) + 3) % 3; };
+  auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  // offset of (j, k) inside one level plane, clamped into the ring
+  auto plane_off = [&](int j, int k) {
+    return (clampi(j, -1, n1) + 1) * s1 + (clampi(k, -1, n2) + 1);
+  };
+  auto plane_ptr = [&](const T* a, int q) {  // plane q (clamped into the ring)
+    return a + (int64_t)(clampi(q, -1, n0) + 1) * d.s0;
+  };
+
+  // pin-plane loads: l = tid + 256 e covers the (TJ+4) x (TK+4) halo tile
+  int poff[NP], pl_[NP];
+#pragma unroll
+  for (int e = 0; e < NP; ++e) {
+    const int l = min(tid + 256 * e, PN - 1);
+    pl_[e] = tid + 256 * e;
+    poff[e] = plane_off(j0 - 2 + l / PK, k0 - 2 + l % PK);
+  }
+  // odd cells of the 1-cell-halo tile, per plane parity par = q & 1
+  int ooff[2][NO], oP[2][NO], oO[2][NO], oj[NO], ok[2][NO];
+  bool olive[NO], oin[2][NO];
+#pragma unroll
+  for (int e = 0; e < NO; ++e) {
+    const int l = tid + 256 * e;
+    olive[e] = l < OJ * OH;
+    const int lc = olive[e] ? l : 0;
+    const int jj = lc / OH, m = lc % OH;
+    oj[e] = j0 - 1 + jj;
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const int kk = 2 * m + ((par + oj[e] + k0) & 1);
+      ok[par][e] = k0 - 1 + kk;
+      oin[par][e] = oj[e] >= 0 && oj[e] < n1 && ok[par][e] >= 0 && ok[par][e] < n2;
+      ooff[par][e] = plane_off(oj[e], ok[par][e]);
+      oP[par][e] = (jj + 1) * PK + (kk + 1);
+      oO[par][e] = jj * OK + kk;
+    }
+  }
+  // this thread's (even, odd) pair: k_a = k0 + 2m, k_b = k_a + 1; which one
+  // is even alternates with the plane parity
+  const int ej = j0 + tid / HK;
+  const int ka = k0 + 2 * (tid % HK);
+  const int erow = (tid / HK + 1) * OK + (ka - k0 + 1);  // O index of k_a
+  const int eoff = plane_off(ej, ka);                     // plane offset of k_a
+  const bool ej_in = ej < n1;
+
+  auto odd_cell_value = [&](int q, int par, int e, const T* pm, const T* p0, const T* pp,
+                            T r) {
+    const int c = oP[par][e];
+    const Coef<T> cf = mg_coef(d, clampi(q, 0, n0 - 1), clampi(oj[e], 0, n1 - 1),
+                               clampi(ok[par][e], 0, n2 - 1), c0, c1, c2);
+    const T sum = cf.aw * pm[c] + cf.ae * pp[c] + cf.as * p0[c - PK] + cf.an * p0[c + PK] +
+                  cf.ab * p0[c - 1] + cf.at * p0[c + 1];
+    return (r - sum) / cf.ap;
+  };
+
+  T pre[NP], rodd[NO], rodd_n[NO];
+  T reven = T(0), reven_n = T(0);
+  auto gload_plane = [&](int q) {
+    const T* b = plane_ptr(pin, q);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) pre[e] = b[poff[e]];
+  };
+  auto store_plane = [&](int q) {
+    T* dst = P[slot(q)];
+#pragma unroll
+    for (int e = 0; e < NP; ++e)
+      if (pl_[e] < PN) dst[pl_[e]] = pre[e];
+  };
+  auto gload_rhs_odd = [&](int q) {
+    const T* b = plane_ptr(rhs, q);
+    const int par = q & 1;
+#pragma unroll
+    for (int e = 0; e < NO; ++e) rodd_n[e] = b[ooff[par][e]];
+  };
+  auto gload_rhs_even = [&](int pl) {
+    const int kx = ((pl + ej + k0) & 1) ? 1 : 0;  // k_a even -> 0, else k_b
+    return plane_ptr(rhs, pl)[eoff + kx];
+  };
+
+  gload_plane(i0 - 2);
+  store_plane(i0 - 2);
+  gload_plane(i0 - 1);
+  store_plane(i0 - 1);
+  gload_plane(i0);
+  gload_rhs_odd(i0 - 1);
+  for (int q = i0 - 1; q <= i1; ++q) {
+    store_plane(q + 1);
+#pragma unroll
+    for (int e = 0; e < NO; ++e) rodd[e] = rodd_n[e];
+    reven = reven_n;
+    if (q + 2 <= i1 + 1) gload_plane(q + 2);
+    if (q + 1 <= i1) gload_rhs_odd(q + 1);
+    if (q >= i0 && q + 1 <= i1) reven_n = gload_rhs_even(q);
+    __syncthreads();
+    {  // odd values of plane q over the halo tile
+      const int par = q & 1;
+      const bool qin = q >= 0 && q < n0;
+      const T* pm = P[slot(q - 1)];
+      const T* p0 = P[slot(q)];
+      const T* pp = P[slot(q + 1)];
+      T* o = O[slot(q)];
+#pragma unroll
+      for (int e = 0; e < NO; ++e) {
+        const T v = odd_cell_value(q, par, e, pm, p0, pp, rodd[e]);
+        if (olive[e]) o[oO[par][e]] = (qin && oin[par][e]) ? v : T(0);
+      }
+    }
+    __syncthreads();
+    const int pl = q - 1;
+    if (pl >= i0 && ej_in) {  // even values of plane pl, write plane pl
+      const T* om = O[slot(pl - 1)];
+      const T* o0 = O[slot(pl)];
+      const T* op = O[slot(pl + 1)];
+      const int kx = ((pl + ej + k0) & 1) ? 1 : 0;  // offset of the even cell
+      const int ke = ka + kx, ko = ka + 1 - kx;
+      const int c = erow + kx;
+      const Coef<T> cf = mg_coef(d, pl, ej, min(ke, n2 - 1), c0, c1, c2);
+      const T sum = cf.aw * om[c] + cf.ae * op[c] + cf.as * o0[c - OK] + cf.an * o0[c + OK] +
+                    cf.ab * o0[c - 1] + cf.at * o0[c + 1];
+      const T ve = (reven - sum) / cf.ap;
+      const T vo = o0[erow + 1 - kx];
+      T* out = pout + (int64_t)(pl + 1) * d.s0;
+      if (ke < n2) {
+        out[eoff + kx] = ve;
+        ring_images(pout, d, pl, ej, ke, mg_at(d, pl, ej, ke), ve);
+      }
+      if (ko < n2) {
+        out[eoff + 1 - kx] = vo;
+        ring_images(pout, d, pl, ej, ko, mg_at(d, pl, ej, ko), vo);
+      }
+    }
+  }
+}
+
 // n sweeps of RBGS on one small level held in LDS by one workgroup
 // (multigrid.py:289-299 with the refill folded in).  The real cells of phi
 // live in LDS; ring terms carry zero coefficients and are skipped.  rhs is
@@ -321,14 +491,53 @@ int launch_relax_lds(const wf_mg_level& L, int nsweeps, size_t bytes, hipStream_
   return 0;
 }
 
+// Levels up to LDS_RELAX_MAX_CELLS cells -- or any level that fits LDS when
+// asked for many sweeps (the coarsest solve) -- relax in one LDS-resident
+// workgroup; the rest in fused full-sweep launches (or half-sweep launches
+// without tmp).
+constexpr size_t LDS_RELAX_MAX_CELLS = 4096;
+constexpr int LDS_RELAX_MIN_SWEEPS = 32;
+
+template <class T, int TJ, int TK>
+int launch_sweep(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
+  const MgDev d = mg_dev(L);
+  T c0, c1, c2;
+  inv_h2(L, c0, c1, c2);
+  const int kt = div_up(d.n2, TK), jt = div_up(d.n1, TJ);
+  const int64_t tiles = (int64_t)kt * jt;
+  const int ich = (int)std::max<int64_t>(4, std::min<int64_t>(16, tiles * d.n0 / 2048));
+  const dim3 grid(kt, jt, div_up(d.n0, ich));
+  hipLaunchKernelGGL((sweep_kernel<T, TJ, TK>), grid, dim3(256), 0, s, (const T*)pin, (T*)pout,
+                     (const T*)L.rhs, d, c0, c1, c2, ich);
+  WF_LAUNCHED();
+  return 0;
+}
+
+template <class T>
+int sweep_t(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
+  return L.n2 > 32 ? launch_sweep<T, 8, 64>(L, pin, pout, s)
+                   : launch_sweep<T, 16, 32>(L, pin, pout, s);
+}
+
 template <class T>
 int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s) {
   if (nsweeps <= 0) return 0;
   const size_t cells = (size_t)L.n0 * L.n1 * L.n2;
   const size_t lds = (size_t)max_lds();
-  if (2 * cells * sizeof(T) <= lds) return launch_relax_lds<T, true>(L, nsweeps, 2 * cells * sizeof(T), s);
-  if (cells * sizeof(T) <= lds) return launch_relax_lds<T, false>(L, nsweeps, cells * sizeof(T), s);
-  for (int q = 0; q < nsweeps; ++q) {
+  if (cells <= LDS_RELAX_MAX_CELLS || nsweeps >= LDS_RELAX_MIN_SWEEPS) {
+    if (2 * cells * sizeof(T) <= lds)
+      return launch_relax_lds<T, true>(L, nsweeps, 2 * cells * sizeof(T), s);
+    if (cells * sizeof(T) <= lds)
+      return launch_relax_lds<T, false>(L, nsweeps, cells * sizeof(T), s);
+  }
+  int q = 0;
+  if (L.tmp) {
+    for (; q + 1 < nsweeps; q += 2) {
+      WF_TRY((hipError_t)sweep_t<T>(L, L.phi, L.tmp, s));
+      WF_TRY((hipError_t)sweep_t<T>(L, L.tmp, L.phi, s));
+    }
+  }
+  for (; q < nsweeps; ++q) {
     WF_TRY((hipError_t)launch_rbgs<T>(L, 0, 1, s));
     WF_TRY((hipError_t)launch_rbgs<T>(L, 1, 1, s));
   }

@@ -79,7 +79,8 @@ fold_kernel(const double* __restrict__ partials, int64_t n, double scale, double
 __global__ void set_scalar_kernel(double* p, double v) { *p = v; }
 
 // r = b - A x (FP64; x == nullptr means x = 0), minus *mean if given, cast to
-// FP32 into the level-0 rhs; *rmax = max(*rmax, max|b - A x|).
+// FP32 into the level-0 rhs; rmax (nullable): each workgroup's max|b - A x|
+// into rmax[workgroup].
 __global__ void __launch_bounds__(256)
 defect_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_grid g,
               float* __restrict__ r32, MgDev d, double c0, double c1, double c2,
@@ -99,13 +100,30 @@ defect_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_gri
     if (mean) r = r - *mean;
     r32[mg_at(d, i, j, k)] = (float)r;
   }
-  if (rmax) {
+  if (rmax) {  // per-workgroup max into the partials (folded by max_fold_kernel)
     __shared__ double part[4];
     double m = wave_max(av);
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
     if ((t & 63) == 0) part[t >> 6] = m;
     __syncthreads();
-    if (t == 0) atomic_max_nonneg(rmax, nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3])));
+    if (t == 0) rmax[block_linear()] = nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3]));
+  }
+}
+
+// out = max(partials[0..n)), one 1024-thread workgroup, fixed order.
+__global__ void __launch_bounds__(1024)
+max_fold_kernel(const double* __restrict__ partials, int64_t n, double* __restrict__ out) {
+  __shared__ double s[16];
+  const int t = threadIdx.x;
+  double m = 0.0;
+  for (int64_t q = t; q < n; q += 1024) m = nan_max(m, partials[q]);
+  m = wave_max(m);
+  if ((t & 63) == 0) s[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    double r = s[0];
+    for (int w = 1; w < 16; ++w) r = nan_max(r, s[w]);
+    *out = r;
   }
 }
 
@@ -269,15 +287,15 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
   for (int c = 0; c <= max_cycles; ++c) {
     const bool last_check = c == max_cycles;
     if (last_check && !tol_mode) break;
-    if (tol_mode) {
-      hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, s, rmax, 0.0);
-      WF_LAUNCHED();
-    }
     hipLaunchKernelGGL(defect_kernel, field_grid(g), dim3(64, 4), 0, s, su, c == 0 ? nullptr : pp,
                        g, (float*)L0.rhs, d0, c0, c1, c2,
-                       (flags & MIX_DEFECT_MEAN) ? mean : nullptr, tol_mode ? rmax : nullptr);
+                       (flags & MIX_DEFECT_MEAN) ? mean : nullptr,
+                       tol_mode ? work + WORK_HEAD : nullptr);
     WF_LAUNCHED();
     if (tol_mode) {
+      hipLaunchKernelGGL(max_fold_kernel, dim3(1), dim3(1024), 0, s, work + WORK_HEAD,
+                         field_blocks(g), rmax);
+      WF_LAUNCHED();
       double h = 0.0;
       WF_TRY(hipMemcpyAsync(&h, rmax, sizeof(double), hipMemcpyDeviceToHost, s));
       WF_TRY(hipStreamSynchronize(s));

@@ -6,6 +6,8 @@
 // each wavefront reads and writes whole 512-byte k-rows.  Expressions keep
 // the reference's evaluation order (left to right) so results agree with
 // the FP64 oracle to a few ulps.
+#include <algorithm>
+
 #include "wf_common.h"
 
 namespace wf {
@@ -174,25 +176,73 @@ __device__ inline double block_min(double v) {
 }
 __device__ inline bool block_leader() { return threadIdx.x == 0 && threadIdx.y == 0; }
 
+// Reductions: a workgroup covers a 64 x 4 (k, j) tile over a chunk of `ich`
+// i-planes, reduces it in registers and LDS, and stores one value per
+// quantity in a partials slot of a static device array (referenced only by
+// device code); one workgroup then folds the slots.  No atomics on a shared
+// address, and max/min are order-independent, so results are deterministic.
+constexpr int RED_MAX_BLOCKS = 4096;
+constexpr int RED_NQ = 4;
+__device__ double g_red_part[RED_NQ * RED_MAX_BLOCKS];
+
+__device__ inline int red_block() {
+  return (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+}
+
+__device__ inline bool red_rows(const Box3& b, int ich, int& j, int& k, int& ibeg, int& iend) {
+  k = blockIdx.x * BX + threadIdx.x;
+  j = blockIdx.y * BY + threadIdx.y;
+  ibeg = blockIdx.z * ich;
+  iend = min(ibeg + ich, b.d0);
+  return k < b.d2 && j < b.d1;
+}
+
 __global__ void __launch_bounds__(256)
 calmas_kernel(const double* __restrict__ u, const double* __restrict__ v,
-              const double* __restrict__ w, double* __restrict__ su,
-              double* __restrict__ rmax, wf_grid g, Box3 b, double dx, double dy,
-              double dz, double dt) {
-  int i, j, k;
+              const double* __restrict__ w, double* __restrict__ su, wf_grid g, Box3 b,
+              int ich, double dx, double dy, double dz, double dt) {
+  int j, k, ibeg, iend;
   double val = 0.0;
-  if (box_thread(b, i, j, k)) {
-    const int64_t x = at(g, i, j, k);
+  if (red_rows(b, ich, j, k, ibeg, iend)) {
     const int64_t sx = g.si, sy = g.sj;
-    const double dudx = D4(u[x + sx], u[x], u[x - sx], u[x - 2 * sx], dx);
-    const double dvdy = D4(v[x + sy], v[x], v[x - sy], v[x - 2 * sy], dy);
-    const double dwdz = D4(w[x + 1], w[x], w[x - 1], w[x - 2], dz);
-    const double fact = dudx + dvdy + dwdz;
-    su[x] = fact / dt;
-    val = fabs(fact * dx * dy * dz);
+    for (int ii = ibeg; ii < iend; ++ii) {
+      const int64_t x = at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k);
+      const double dudx = D4(u[x + sx], u[x], u[x - sx], u[x - 2 * sx], dx);
+      const double dvdy = D4(v[x + sy], v[x], v[x - sy], v[x - 2 * sy], dy);
+      const double dwdz = D4(w[x + 1], w[x], w[x - 1], w[x - 2], dz);
+      const double fact = dudx + dvdy + dwdz;
+      su[x] = fact / dt;
+      val = nan_max(val, fabs(fact * dx * dy * dz));
+    }
   }
   const double r = block_max(val);
-  if (block_leader()) atomic_max_nonneg(rmax, r);
+  if (block_leader()) g_red_part[red_block()] = r;
+}
+
+// Fold the partials of nq quantities: bit q of min_mask selects min (else
+// max); combine != 0 folds the previous *out in too (the reference's
+// atomic_max/atomic_min into a pre-set buffer).
+__global__ void __launch_bounds__(1024)
+red_fold_kernel(double* __restrict__ out, int nblk, int nq, int min_mask, int combine) {
+  __shared__ double part[16];
+  const int t = threadIdx.x;
+  for (int q = 0; q < nq; ++q) {
+    const bool mn = (min_mask >> q) & 1;
+    double v = mn ? 1.0e10 : 0.0;
+    for (int p = t; p < nblk; p += 1024) {
+      const double y = g_red_part[q * RED_MAX_BLOCKS + p];
+      v = mn ? nan_min(v, y) : nan_max(v, y);
+    }
+    v = mn ? wave_min(v) : wave_max(v);
+    if ((t & 63) == 0) part[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+      double r = part[0];
+      for (int w = 1; w < 16; ++w) r = mn ? nan_min(r, part[w]) : nan_max(r, part[w]);
+      out[q] = combine ? (mn ? nan_min(out[q], r) : nan_max(out[q], r)) : r;
+    }
+    __syncthreads();
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -250,12 +300,14 @@ wale_kernel(const double* __restrict__ u, const double* __restrict__ v,
 }
 
 __global__ void __launch_bounds__(256)
-maxabs_kernel(const double* __restrict__ f, double* __restrict__ out, wf_grid g, Box3 b) {
-  int i, j, k;
+maxabs_kernel(const double* __restrict__ f, wf_grid g, Box3 b, int ich) {
+  int j, k, ibeg, iend;
   double val = 0.0;
-  if (box_thread(b, i, j, k)) val = fabs(f[at(g, i, j, k)]);
+  if (red_rows(b, ich, j, k, ibeg, iend))
+    for (int ii = ibeg; ii < iend; ++ii)
+      val = nan_max(val, fabs(f[at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k)]));
   const double r = block_max(val);
-  if (block_leader()) atomic_max_nonneg(out, r);
+  if (block_leader()) g_red_part[red_block()] = r;
 }
 
 // checkdt.for dt limit at one cell (kernels.py:394-401); rsum = 1/dx^2 +
@@ -275,37 +327,34 @@ __device__ inline double dtvisc_cell(const double* __restrict__ u,
 
 __global__ void __launch_bounds__(256)
 dtvisc_kernel(const double* __restrict__ u, const double* __restrict__ v,
-              const double* __restrict__ w, const double* __restrict__ vis,
-              double* __restrict__ out, wf_grid g, Box3 b, double dx, double dy,
-              double dz, double rsum) {
-  int i, j, k;
+              const double* __restrict__ w, const double* __restrict__ vis, wf_grid g,
+              Box3 b, int ich, double dx, double dy, double dz, double rsum) {
+  int j, k, ibeg, iend;
   double val = 1.0e10;
-  if (box_thread(b, i, j, k)) val = dtvisc_cell(u, v, w, vis, at(g, i, j, k), g, dx, dy, dz, rsum);
+  if (red_rows(b, ich, j, k, ibeg, iend))
+    for (int ii = ibeg; ii < iend; ++ii)
+      val = nan_min(val, dtvisc_cell(u, v, w, vis, at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k),
+                                     g, dx, dy, dz, rsum));
   const double r = block_min(val);
-  if (block_leader()) atomic_min_nonneg(out, r);
-}
-
-__global__ void init4_kernel(double* out) {
-  out[0] = 0.0;
-  out[1] = 0.0;
-  out[2] = 0.0;
-  out[3] = 1.0e10;
+  if (block_leader()) g_red_part[red_block()] = r;
 }
 
 __global__ void __launch_bounds__(256)
 checkdt_kernel(const double* __restrict__ u, const double* __restrict__ v,
-               const double* __restrict__ w, const double* __restrict__ vis,
-               double* __restrict__ out, wf_grid g, Box3 b, double dx, double dy,
-               double dz, double rsum) {
-  int i, j, k;
+               const double* __restrict__ w, const double* __restrict__ vis, wf_grid g,
+               Box3 b, int ich, double dx, double dy, double dz, double rsum) {
+  int j, k, ibeg, iend;
   double mu = 0.0, mv = 0.0, mw = 0.0, dtv = 1.0e10;
-  if (box_thread(b, i, j, k)) {
-    const int64_t x = at(g, i, j, k);
-    mu = fabs(u[x]);
-    mv = fabs(v[x]);
-    mw = fabs(w[x]);
-    dtv = dtvisc_cell(u, v, w, vis, x, g, dx, dy, dz, rsum);
+  if (red_rows(b, ich, j, k, ibeg, iend)) {
+    for (int ii = ibeg; ii < iend; ++ii) {
+      const int64_t x = at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k);
+      mu = nan_max(mu, fabs(u[x]));
+      mv = nan_max(mv, fabs(v[x]));
+      mw = nan_max(mw, fabs(w[x]));
+      dtv = nan_min(dtv, dtvisc_cell(u, v, w, vis, x, g, dx, dy, dz, rsum));
+    }
   }
+  const int blk = red_block();
   mu = block_max(mu);
   __syncthreads();
   mv = block_max(mv);
@@ -314,10 +363,10 @@ checkdt_kernel(const double* __restrict__ u, const double* __restrict__ v,
   __syncthreads();
   dtv = block_min(dtv);
   if (block_leader()) {
-    atomic_max_nonneg(out + 0, mu);
-    atomic_max_nonneg(out + 1, mv);
-    atomic_max_nonneg(out + 2, mw);
-    atomic_min_nonneg(out + 3, dtv);
+    g_red_part[0 * RED_MAX_BLOCKS + blk] = mu;
+    g_red_part[1 * RED_MAX_BLOCKS + blk] = mv;
+    g_red_part[2 * RED_MAX_BLOCKS + blk] = mw;
+    g_red_part[3 * RED_MAX_BLOCKS + blk] = dtv;
   }
 }
 
@@ -380,6 +429,31 @@ static bool box_ok(const wf_grid& g, const Box3& b, int reach_lo, int reach_hi) 
   }
   return true;
 }
+struct RedGeom {
+  dim3 grid;
+  int ich, nblk;
+};
+// Reduction launch shape: (k, j) tiles x i-chunks, at most RED_MAX_BLOCKS
+// workgroups; false when a single i-plane already needs more tiles.
+static bool red_geom(const Box3& b, RedGeom& r) {
+  const int kt = div_up(b.d2, BX), jt = div_up(b.d1, BY);
+  const int64_t tiles = (int64_t)kt * jt;
+  if (tiles > RED_MAX_BLOCKS || b.d0 <= 0) return false;
+  const int ic_max = (int)(RED_MAX_BLOCKS / tiles);
+  r.ich = div_up(b.d0, ic_max);
+  const int ic = div_up(b.d0, r.ich);
+  r.grid = dim3(kt, jt, ic);
+  r.nblk = (int)(tiles * ic);
+  return true;
+}
+static int fold(double* out, const RedGeom& r, int nq, int min_mask, int combine,
+                hipStream_t s) {
+  hipLaunchKernelGGL(red_fold_kernel, dim3(1), dim3(1024), 0, s, out, r.nblk, nq, min_mask,
+                     combine);
+  WF_LAUNCHED();
+  return 0;
+}
+
 static Box3 npl_box(const wf_grid& g, int d0, int d1, int d2) {
   return Box3{g.npl, g.npl, g.npl, d0, d1, d2};
 }
@@ -478,12 +552,14 @@ int wf_calmas(const double* u, const double* v, const double* w, double* su,
               double* rmax_dev, wf_grid g, int d0, int d1, int d2, double dx, double dy,
               double dz, double dt, void* stream) {
   const Box3 b = npl_box(g, d0, d1, d2);
-  if (!box_ok(g, b, 2, 1)) return WF_EINVAL;
+  RedGeom r;
+  if (!box_ok(g, b, 2, 1) || !rmax_dev) return WF_EINVAL;
   if (empty(b)) return 0;
-  hipLaunchKernelGGL(calmas_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, su,
-                     rmax_dev, g, b, dx, dy, dz, dt);
+  if (!red_geom(b, r)) return WF_EINVAL;
+  hipLaunchKernelGGL(calmas_kernel, r.grid, box_block(), 0, S_(stream), u, v, w, su, g, b,
+                     r.ich, dx, dy, dz, dt);
   WF_LAUNCHED();
-  return 0;
+  return fold(rmax_dev, r, 1, 0, 1, S_(stream));
 }
 
 int wf_wale(const double* u, const double* v, const double* w, double* vis, wf_grid g,
@@ -501,12 +577,13 @@ int wf_wale(const double* u, const double* v, const double* w, double* vis, wf_g
 int wf_maxabs(const double* f, double* out_dev, wf_grid g, int d0, int d1, int d2,
               void* stream) {
   const Box3 b = npl_box(g, d0, d1, d2);
-  if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
+  RedGeom r;
+  if (!box_ok(g, b, 0, 0) || !out_dev) return WF_EINVAL;
   if (empty(b)) return 0;
-  hipLaunchKernelGGL(maxabs_kernel, box_grid(b), box_block(), 0, S_(stream), f, out_dev, g,
-                     b);
+  if (!red_geom(b, r)) return WF_EINVAL;
+  hipLaunchKernelGGL(maxabs_kernel, r.grid, box_block(), 0, S_(stream), f, g, b, r.ich);
   WF_LAUNCHED();
-  return 0;
+  return fold(out_dev, r, 1, 0, 1, S_(stream));
 }
 
 static double inv_sq_sum(double dx, double dy, double dz) {
@@ -517,24 +594,25 @@ int wf_dtvisc(const double* u, const double* v, const double* w, const double* v
               double* out_dev, wf_grid g, int d0, int d1, int d2, double dx, double dy,
               double dz, void* stream) {
   const Box3 b = npl_box(g, d0, d1, d2);
-  if (!box_ok(g, b, 1, 0)) return WF_EINVAL;
+  RedGeom r;
+  if (!box_ok(g, b, 1, 0) || !out_dev) return WF_EINVAL;
   if (empty(b)) return 0;
-  hipLaunchKernelGGL(dtvisc_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, vis,
-                     out_dev, g, b, dx, dy, dz, inv_sq_sum(dx, dy, dz));
+  if (!red_geom(b, r)) return WF_EINVAL;
+  hipLaunchKernelGGL(dtvisc_kernel, r.grid, box_block(), 0, S_(stream), u, v, w, vis, g, b,
+                     r.ich, dx, dy, dz, inv_sq_sum(dx, dy, dz));
   WF_LAUNCHED();
-  return 0;
+  return fold(out_dev, r, 1, 1, 1, S_(stream));
 }
 
 int wf_checkdt(const double* u, const double* v, const double* w, const double* vis,
                double* out4_dev, wf_grid g, double dx, double dy, double dz, void* stream) {
   const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
-  if (!box_ok(g, b, 1, 0)) return WF_EINVAL;
-  hipLaunchKernelGGL(init4_kernel, dim3(1), dim3(1), 0, S_(stream), out4_dev);
+  RedGeom r;
+  if (!box_ok(g, b, 1, 0) || !out4_dev || !red_geom(b, r)) return WF_EINVAL;
+  hipLaunchKernelGGL(checkdt_kernel, r.grid, box_block(), 0, S_(stream), u, v, w, vis, g, b,
+                     r.ich, dx, dy, dz, inv_sq_sum(dx, dy, dz));
   WF_LAUNCHED();
-  hipLaunchKernelGGL(checkdt_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, vis,
-                     out4_dev, g, b, dx, dy, dz, inv_sq_sum(dx, dy, dz));
-  WF_LAUNCHED();
-  return 0;
+  return fold(out4_dev, r, 4, 8, 0, S_(stream));
 }
 
 int wf_add_ring(double* dst, const double* src, wf_grid g, void* stream) {

@@ -27,7 +27,8 @@ class WfMgLevel(C.Structure):
     _fields_ = [("phi", C.c_void_p), ("rhs", C.c_void_p),
                 ("n0", C.c_int32), ("n1", C.c_int32), ("n2", C.c_int32),
                 ("prec", C.c_int32),
-                ("h0", C.c_double), ("h1", C.c_double), ("h2", C.c_double)]
+                ("h0", C.c_double), ("h1", C.c_double), ("h2", C.c_double),
+                ("tmp", C.c_void_p)]
 
 
 _P = C.c_void_p

@@ -3,10 +3,11 @@
 Same schema and defaults as the reference's ``warpflow/config.py:15-132``
 (sections ``case, domain, grid, physics{reynolds, sgs{enabled, model, cw}},
 initial, boundaries, numerics, time, output, mpi``) so existing case files
-load unchanged, plus two optional keys this build reads:
+load unchanged, plus optional keys this build reads:
 ``numerics.mg_precision`` (``fp64`` = the reference's V-cycle, ``mixed`` =
-FP32 V-cycles inside FP64 defect correction) and ``initial.seed`` (the
-init_noise seed, 1 in the reference, solver.py:70-73).
+FP32 V-cycles inside FP64 defect correction), ``numerics.mg_mixed_cycles``
+(FP64-refined FP32 V-cycles per solve in the mixed form, default 2) and
+``initial.seed`` (the init_noise seed, 1 in the reference, solver.py:70-73).
 """
 from __future__ import annotations
 
@@ -67,6 +68,7 @@ class NumericsCfg:
     div_tolerance: float = 1.0e-6
     ghost_layers: int = 3
     mg_precision: str = "fp64"
+    mg_mixed_cycles: int = 2
 
 
 @dataclass
@@ -152,6 +154,8 @@ def validate(cfg: CaseConfig) -> None:
                              f"divisible by 2**(mg_levels-1)={2 ** (lev - 1)}")
     if cfg.numerics.mg_precision not in ("fp64", "mixed"):
         raise ValueError("numerics.mg_precision must be 'fp64' or 'mixed'")
+    if cfg.numerics.mg_mixed_cycles < 1:
+        raise ValueError("numerics.mg_mixed_cycles must be >= 1")
 
 
 def load_case(path) -> CaseConfig:

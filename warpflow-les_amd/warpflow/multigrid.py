@@ -34,11 +34,14 @@ class _Level:
         self.device = device
         self.phi = zeros(self.shape, self.dtype, device)
         self.rhs = zeros(self.shape, self.dtype, device)
+        # second phi buffer for the fused ping-pong sweeps (never read as a
+        # result: every relax leaves its result in phi)
+        self.tmp = zeros(self.shape, self.dtype, device)
         self._coef = None
 
     def desc(self) -> _hip.WfMgLevel:
         return _hip.WfMgLevel(self.phi.ptr, self.rhs.ptr, *self.n3,
-                              self.dtype.itemsize, *self.h3)
+                              self.dtype.itemsize, *self.h3, self.tmp.ptr)
 
     def _coefficients(self):
         if self._coef is None:
