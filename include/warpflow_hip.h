@@ -57,6 +57,9 @@ typedef struct wf_mg_level {
   int32_t prec;
   double h0, h1, h2;
   void* tmp;
+  void* cop;          /* nullable: coarse operator of this level (see below) */
+  int32_t cop_sweeps; /* sweeps the operator stands for */
+  int32_t reserved;
 } wf_mg_level;
 
 /* ------------------------------------------------------------------------
@@ -249,6 +252,20 @@ int wf_mg_vcycle(const wf_mg_level* levels, int nlev, int n_pre, int n_post,
 int wf_mg_solve(const wf_mg_level* levels, int nlev, const double* su,
                 double* pp, wf_grid g, int n_pre, int n_post, int n_coarse,
                 int n_vcycles, void* stream);
+
+/* Coarse operator of a small level: nsweeps RBGS sweeps started from
+ * phi = 0 are a fixed linear map phi = M rhs of the level's real cells.
+ * wf_mg_coarse_op_build forms M (column-major, ncell x ncell in the level
+ * precision; ncell = n0*n1*n2 <= 4096) by running exactly those sweeps on
+ * every unit rhs, one workgroup per column.  The V-cycle applies it to the
+ * coarsest level, whose phi restriction has just zeroed, when the level's cop
+ * is set and cop_sweeps equals the coarsest sweep count -- the same result as
+ * the sweeps (multigrid.py:313-316) up to rounding, as one matrix-vector
+ * product instead of 2*nsweeps dependent half-sweeps. */
+int wf_mg_coarse_op_bytes(const wf_mg_level* lev, size_t* bytes);
+int wf_mg_coarse_op_build(const wf_mg_level* lev, int nsweeps, void* op, void* stream);
+/* phi = M rhs on the real cells (+ face-ring images), M from the build. */
+int wf_mg_coarse_op_apply(const wf_mg_level* lev, void* stream);
 
 /* max|rhs - A phi| over the real cells of one level (FP64 or FP32 level),
  * written to *out_dev -- the residual norm of tests/test_mg.py:16-32. */

@@ -156,3 +156,25 @@ def test_reference_mg_script_flow(gpu):
         mg._vcycle()
         r.append(residual(lev0))
     assert all(b < 0.1 * a for a, b in zip(r, r[1:])), r
+
+
+@pytest.mark.parametrize("n3,prec", [((8, 8, 8), np.float64), ((8, 8, 8), np.float32),
+                                     ((4, 6, 8), np.float64), ((12, 12, 12), np.float64)])
+def test_coarse_operator_equals_sweeps(gpu, n3, prec):
+    """phi = M rhs (M formed by running the sweeps on unit vectors) equals
+    300 RBGS sweeps from phi = 0, ring included (multigrid.py:313-316)."""
+    import ctypes as C
+    from warpflow import _hip
+    from warpflow.multigrid import _Level
+    h = tuple(1.0 / (4 * n) for n in n3)
+    a, b = _Level(n3, h, None, prec), _Level(n3, h, None, prec)
+    assert a.build_coarse_operator(300)
+    rhs = np.random.default_rng(9).standard_normal(a.shape)
+    for lev in (a, b):
+        lev.rhs.assign(rhs)
+        lev.phi.zero_()
+    da, db = a.desc(), b.desc()
+    _hip.call("wf_mg_coarse_op_apply", C.byref(da), None)
+    _hip.call("wf_mg_relax", C.byref(db), 300, None)
+    tol = 1e-11 if prec == np.float64 else 2e-5
+    assert rel_err(a.phi.numpy(), b.phi.numpy()) < tol

@@ -60,7 +60,7 @@ def test_conv_diff_convdiff(gpu, n3, comp):
     fused = up(f["star0"])
     getattr(K, "convdiff4th_" + comp)(dims, d["u"], d["v"], d["w"], d["coo"], d["vis"], fused,
                                       *h, dt, alpha, N)
-    assert rel_err(fused.numpy() - f["coo"], got2 - f["coo"]) < 1e-14
+    assert rel_err(fused.numpy() - f["coo"], got2 - f["coo"]) < TOL
 
 
 @pytest.mark.parametrize("n3", SHAPES)

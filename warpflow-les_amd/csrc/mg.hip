@@ -376,6 +376,70 @@ relax_lds_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, 
   }
 }
 
+// Column `col` (= workgroup) of a small level's coarse operator: nsweeps RBGS
+// sweeps in LDS from phi = 0 with rhs = e_col, the relax_lds_kernel
+// arithmetic; M is column-major (M[col * ncell + row]).
+template <class T>
+__global__ void __launch_bounds__(256)
+coarse_op_build_kernel(T* __restrict__ M, MgDev d, T c0, T c1, T c2, int nsweeps) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sp = reinterpret_cast<T*>(smem_raw);
+  const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
+  const int ncell = n0 * n1 * n2;
+  const int col = blockIdx.x, nt = blockDim.x, tid = threadIdx.x;
+  for (int l = tid; l < ncell; l += nt) sp[l] = T(0);
+  __syncthreads();
+  const int np = (n2 + 1) / 2, nrow = n0 * n1, S0 = n1 * n2, S1 = n2;
+  for (int s = 0; s < nsweeps; ++s) {
+    for (int color = 0; color < 2; ++color) {
+      for (int q = tid; q < nrow * np; q += nt) {
+        const int m = q % np, r = q / np;
+        const int j = r % n1, i = r / n1;
+        const int k = 2 * m + ((i + j + color + 1) & 1);
+        if (k >= n2) continue;
+        const int l = (i * n1 + j) * n2 + k;
+        const Coef<T> c = mg_coef(d, i, j, k, c0, c1, c2);
+        const T W = i > 0 ? sp[l - S0] : T(0);
+        const T E = i < n0 - 1 ? sp[l + S0] : T(0);
+        const T S = j > 0 ? sp[l - S1] : T(0);
+        const T N = j < n1 - 1 ? sp[l + S1] : T(0);
+        const T B = k > 0 ? sp[l - 1] : T(0);
+        const T Tt = k < n2 - 1 ? sp[l + 1] : T(0);
+        const T rv = l == col ? T(1) : T(0);
+        const T sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt;
+        sp[l] = (rv - sum) / c.ap;
+      }
+      __syncthreads();
+    }
+  }
+  for (int l = tid; l < ncell; l += nt) M[(size_t)col * ncell + l] = sp[l];
+}
+
+// phi = M rhs on the real cells of a small level, plus face-ring images.
+template <class T>
+__global__ void __launch_bounds__(1024)
+coarse_op_apply_kernel(const T* __restrict__ M, T* __restrict__ phi, const T* __restrict__ rhs,
+                       MgDev d) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* r = reinterpret_cast<T*>(smem_raw);
+  const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
+  const int ncell = n0 * n1 * n2;
+  const int nt = blockDim.x, tid = threadIdx.x;
+  for (int l = tid; l < ncell; l += nt) {
+    const int k = l % n2, j = (l / n2) % n1, i = l / (n1 * n2);
+    r[l] = rhs[mg_at(d, i, j, k)];
+  }
+  __syncthreads();
+  for (int row = tid; row < ncell; row += nt) {
+    T acc = T(0);
+    for (int col = 0; col < ncell; ++col) acc = acc + M[(size_t)col * ncell + row] * r[col];
+    const int k = row % n2, j = (row / n2) % n1, i = row / (n1 * n2);
+    const int64_t x = mg_at(d, i, j, k);
+    phi[x] = acc;
+    ring_images(phi, d, i, j, k, x, acc);
+  }
+}
+
 __global__ void __launch_bounds__(256)
 copy_in_kernel(const double* __restrict__ su, wf_grid g, double* __restrict__ rhs, MgDev d) {
   const int k = blockIdx.x * 64 + threadIdx.x;
@@ -581,6 +645,51 @@ int relax_any(const wf_mg_level& L, int n, hipStream_t s) {
   return L.prec == 8 ? relax_t<double>(L, n, s) : relax_t<float>(L, n, s);
 }
 
+constexpr int COARSE_OP_MAX_CELLS = 4096;
+
+template <class T>
+int coarse_op_apply_t(const wf_mg_level& L, hipStream_t s) {
+  const MgDev d = mg_dev(L);
+  const int ncell = d.n0 * d.n1 * d.n2;
+  const size_t bytes = (size_t)ncell * sizeof(T);
+  if (ncell > COARSE_OP_MAX_CELLS || bytes > (size_t)max_lds()) return WF_EINVAL;
+  static bool attr_set = false;
+  if (!attr_set) {
+    WF_TRY(hipFuncSetAttribute((const void*)coarse_op_apply_kernel<T>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds()));
+    attr_set = true;
+  }
+  const int nt = ncell >= 1024 ? 1024 : ((ncell + 63) / 64) * 64;
+  hipLaunchKernelGGL(coarse_op_apply_kernel<T>, dim3(1), dim3(nt), bytes, s, (const T*)L.cop,
+                     (T*)L.phi, (const T*)L.rhs, d);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int coarse_op_apply_any(const wf_mg_level& L, hipStream_t s) {
+  return L.prec == 8 ? coarse_op_apply_t<double>(L, s) : coarse_op_apply_t<float>(L, s);
+}
+
+template <class T>
+int coarse_op_build_t(const wf_mg_level& L, int nsweeps, void* op, hipStream_t s) {
+  const MgDev d = mg_dev(L);
+  const int ncell = d.n0 * d.n1 * d.n2;
+  const size_t bytes = (size_t)ncell * sizeof(T);
+  if (ncell > COARSE_OP_MAX_CELLS || bytes > (size_t)max_lds()) return WF_EINVAL;
+  static bool attr_set = false;
+  if (!attr_set) {
+    WF_TRY(hipFuncSetAttribute((const void*)coarse_op_build_kernel<T>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds()));
+    attr_set = true;
+  }
+  T c0, c1, c2;
+  inv_h2(L, c0, c1, c2);
+  hipLaunchKernelGGL(coarse_op_build_kernel<T>, dim3(ncell), dim3(256), bytes, s, (T*)op, d, c0,
+                     c1, c2, nsweeps);
+  WF_LAUNCHED();
+  return 0;
+}
+
 int vcycle_any(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse,
                hipStream_t s) {
   for (int l = 0; l < nlev; ++l) {
@@ -588,6 +697,12 @@ int vcycle_any(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse
       const int r = lv[l].prec == 8 ? restrict_t<double>(lv[l - 1], lv[l], s)
                                     : restrict_t<float>(lv[l - 1], lv[l], s);
       if (r) return r;
+    }
+    if (l > 0 && l == nlev - 1 && lv[l].cop && lv[l].cop_sweeps == ncoarse && ncoarse > 0) {
+      // coarsest level right after restriction zeroed phi: phi = M rhs
+      const int r = coarse_op_apply_any(lv[l], s);
+      if (r) return r;
+      continue;
     }
     const int r = relax_any(lv[l], l < nlev - 1 ? npre : ncoarse, s);
     if (r) return r;
@@ -659,6 +774,26 @@ int wf_mg_coeff(const wf_mg_level* L, void* aw, void* ae, void* as, void* an, vo
   }
   WF_LAUNCHED();
   return 0;
+}
+
+int wf_mg_coarse_op_bytes(const wf_mg_level* L, size_t* bytes) {
+  if (!level_ok(L) || !bytes) return WF_EINVAL;
+  const size_t ncell = (size_t)L->n0 * L->n1 * L->n2;
+  if (ncell > (size_t)COARSE_OP_MAX_CELLS || ncell * L->prec > (size_t)max_lds())
+    return WF_EINVAL;
+  *bytes = ncell * ncell * (size_t)L->prec;
+  return 0;
+}
+
+int wf_mg_coarse_op_build(const wf_mg_level* L, int nsweeps, void* op, void* stream) {
+  if (!level_ok(L) || !op || nsweeps < 0) return WF_EINVAL;
+  return L->prec == 8 ? coarse_op_build_t<double>(*L, nsweeps, op, S_(stream))
+                      : coarse_op_build_t<float>(*L, nsweeps, op, S_(stream));
+}
+
+int wf_mg_coarse_op_apply(const wf_mg_level* L, void* stream) {
+  if (!level_ok(L) || !L->cop) return WF_EINVAL;
+  return coarse_op_apply_any(*L, S_(stream));
 }
 
 int wf_mg_rbgs(const wf_mg_level* L, int color, void* stream) {

@@ -127,6 +127,189 @@ convdiff4th_kernel(const double* __restrict__ u, const double* __restrict__ v,
   cstar[x] = t;
 }
 
+// ---------------------------------------------------------------------------
+// conv4th + diff4th of one component, streamed along i with LDS windows.
+//
+// A 256-thread workgroup owns a TJ x TK = 8 x 32 (j, k) tile of the operator
+// box and walks a chunk of i-planes.  Each field it reads is staged in LDS as
+// a rolling window of planes covering the tile plus a 3-cell (j, k) halo:
+// the component itself over planes i-3..i+3 and the two other components
+// over the planes their cross terms reach (C=0: v, w at i-1..i+2; C=1, C=2:
+// u at i-2..i+1 and the remaining component at plane i).  The next plane of
+// each window is prefetched into registers while the current output plane
+// computes, so every global value is read once per workgroup (plus the
+// halo), and all stencil taps come from LDS.  The arithmetic is conv_sum +
+// lap4 above, term for term, through an accessor with compile-time offsets.
+// ---------------------------------------------------------------------------
+constexpr int CD_TJ = 8, CD_TK = 32, CD_RJ = CD_TJ + 6, CD_RK = CD_TK + 6;
+constexpr int CD_PN = CD_RJ * CD_RK;  // 532 values per staged plane
+constexpr int CD_NPT = (CD_PN + 255) / 256;
+
+template <int C>
+struct CdWin {  // plane windows [lo, hi] of (own, q1, q2) for component C
+  static constexpr int lo0 = -3, hi0 = 3;
+  static constexpr int lo1 = C == 0 ? -1 : -2, hi1 = C == 0 ? 2 : 1;
+  static constexpr int lo2 = C == 0 ? -1 : 0, hi2 = C == 0 ? 2 : 0;
+  static constexpr int n0 = hi0 - lo0 + 1, n1 = hi1 - lo1 + 1, n2 = hi2 - lo2 + 1;
+};
+
+// Field accessor: value at plane offset di, row offset dj, column offset dk
+// from the thread's output cell.
+template <int LO, int N>
+struct CdField {
+  const double* p[N];  // plane pointers for offsets LO .. LO+N-1
+  int c;               // in-plane index of the output cell
+  __device__ double operator()(int di, int dj, int dk) const {
+    return p[di - LO][c + dj * CD_RK + dk];
+  }
+};
+
+template <int AX, class F>
+__device__ inline double I4w(const F& f, int di, int dj, int dk) {
+  constexpr int ex = AX == 0, ey = AX == 1, ez = AX == 2;
+  return (-f(di + ex, dj + ey, dk + ez) + 9.0 * f(di, dj, dk) +
+          9.0 * f(di - ex, dj - ey, dk - ez) - f(di - 2 * ex, dj - 2 * ey, dk - 2 * ez)) /
+         16.0;
+}
+
+template <int A, class F>
+__device__ inline double conv_self_w(const F& c, double ha) {
+  constexpr int ex = A == 0, ey = A == 1, ez = A == 2;
+  const double p2 = I4w<A>(c, 2 * ex, 2 * ey, 2 * ez), p1 = I4w<A>(c, ex, ey, ez);
+  const double p0 = I4w<A>(c, 0, 0, 0), m1 = I4w<A>(c, -ex, -ey, -ez);
+  return D4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, ha);
+}
+
+template <int A, int B, class F, class Q>
+__device__ inline double conv_cross_w(const F& c, const Q& q, double hb) {
+  constexpr int ax = A == 0, ay = A == 1, az = A == 2;
+  constexpr int bx = B == 0, by = B == 1, bz = B == 2;
+  const double sp1 = I4w<B>(c, 2 * bx, 2 * by, 2 * bz) * I4w<A>(q, ax + bx, ay + by, az + bz);
+  const double sn = I4w<B>(c, bx, by, bz) * I4w<A>(q, ax, ay, az);
+  const double sm1 = I4w<B>(c, 0, 0, 0) * I4w<A>(q, ax - bx, ay - by, az - bz);
+  const double sm2 =
+      I4w<B>(c, -bx, -by, -bz) * I4w<A>(q, ax - 2 * bx, ay - 2 * by, az - 2 * bz);
+  return D4(sp1, sn, sm1, sm2, hb);
+}
+
+template <class F>
+__device__ inline double lap4_w(const F& f, double r12xx, double r12yy, double r12zz) {
+  const double f0 = f(0, 0, 0);
+  const double lx = (-f(2, 0, 0) + 16.0 * f(1, 0, 0) - 30.0 * f0 + 16.0 * f(-1, 0, 0) -
+                     f(-2, 0, 0)) / r12xx;
+  const double ly = (-f(0, 2, 0) + 16.0 * f(0, 1, 0) - 30.0 * f0 + 16.0 * f(0, -1, 0) -
+                     f(0, -2, 0)) / r12yy;
+  const double lz = (-f(0, 0, 2) + 16.0 * f(0, 0, 1) - 30.0 * f0 + 16.0 * f(0, 0, -1) -
+                     f(0, 0, -2)) / r12zz;
+  return lx + ly + lz;
+}
+
+template <int C>
+__global__ void __launch_bounds__(256)
+convdiff_lds_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                    const double* __restrict__ w, const double* __restrict__ coo,
+                    const double* __restrict__ vis, double* __restrict__ cstar, wf_grid g,
+                    Box3 b, int ich, double dx, double dy, double dz, double r12xx,
+                    double r12yy, double r12zz, double dta) {
+  using W = CdWin<C>;
+  __shared__ double L0[W::n0][CD_PN];
+  __shared__ double L1[W::n1][CD_PN];
+  __shared__ double L2[W::n2][CD_PN];
+  const double* F0 = C == 0 ? u : (C == 1 ? v : w);  // own component
+  const double* F1 = C == 0 ? v : u;                  // first cross field
+  const double* F2 = C == 0 ? w : (C == 1 ? w : v);  // second cross field
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.y * CD_TJ, k0 = blockIdx.x * CD_TK;
+  const int i0 = blockIdx.z * ich, i1 = min(i0 + ich, b.d0);
+  const int nI = g.ni + 2 * g.npl, nJ = g.nj + 2 * g.npl, nK = g.nk + 2 * g.npl;
+  // staged-region offsets (full-array j, k of region row/column 0)
+  const int rj0 = b.lo1 + j0 - 3, rk0 = b.lo2 + k0 - 3;
+  int64_t roff[CD_NPT];
+#pragma unroll
+  for (int e = 0; e < CD_NPT; ++e) {
+    const int l = min(tid + 256 * e, CD_PN - 1);
+    const int jf = min(max(rj0 + l / CD_RK, 0), nJ - 1);
+    const int kf = min(max(rk0 + l % CD_RK, 0), nK - 1);
+    roff[e] = (int64_t)jf * g.sj + kf;
+  }
+  auto plane_base = [&](int i) {  // output plane i -> clamped full plane offset
+    return (int64_t)min(max(b.lo0 + i, 0), nI - 1) * g.si;
+  };
+  auto slot = [](int i, int n) { return ((i % n) + n) % n; };
+  double r0[CD_NPT], r1[CD_NPT], r2[CD_NPT];
+  auto gload = [&](const double* f, int i, double (&r)[CD_NPT]) {
+    const double* base = f + plane_base(i);
+#pragma unroll
+    for (int e = 0; e < CD_NPT; ++e) r[e] = base[roff[e]];
+  };
+  auto lstore = [&](double* dst, const double (&r)[CD_NPT]) {
+#pragma unroll
+    for (int e = 0; e < CD_NPT; ++e)
+      if (tid + 256 * e < CD_PN) dst[tid + 256 * e] = r[e];
+  };
+  // prologue: all but the newest plane of every window
+  for (int i = i0 + W::lo0; i < i0 + W::hi0; ++i) {
+    gload(F0, i, r0);
+    lstore(L0[slot(i, W::n0)], r0);
+  }
+  for (int i = i0 + W::lo1; i < i0 + W::hi1; ++i) {
+    gload(F1, i, r1);
+    lstore(L1[slot(i, W::n1)], r1);
+  }
+  for (int i = i0 + W::lo2; i < i0 + W::hi2; ++i) {
+    gload(F2, i, r2);
+    lstore(L2[slot(i, W::n2)], r2);
+  }
+  gload(F0, i0 + W::hi0, r0);
+  gload(F1, i0 + W::hi1, r1);
+  gload(F2, i0 + W::hi2, r2);
+
+  const int tj = tid / CD_TK, tk = tid % CD_TK;
+  const int jj = j0 + tj, kk = k0 + tk;
+  const bool live = jj < b.d1 && kk < b.d2;
+  const int cin = (tj + 3) * CD_RK + (tk + 3);
+  const int64_t voff = C == 0 ? 0 : (C == 1 ? g.sj : 1);
+  for (int i = i0; i < i1; ++i) {
+    lstore(L0[slot(i + W::hi0, W::n0)], r0);
+    lstore(L1[slot(i + W::hi1, W::n1)], r1);
+    lstore(L2[slot(i + W::hi2, W::n2)], r2);
+    if (i + 1 < i1) {
+      gload(F0, i + 1 + W::hi0, r0);
+      gload(F1, i + 1 + W::hi1, r1);
+      gload(F2, i + 1 + W::hi2, r2);
+    }
+    __syncthreads();
+    if (live) {
+      CdField<W::lo0, W::n0> fc;
+      CdField<W::lo1, W::n1> f1;
+      CdField<W::lo2, W::n2> f2;
+      fc.c = f1.c = f2.c = cin;
+#pragma unroll
+      for (int q = 0; q < W::n0; ++q) fc.p[q] = L0[slot(i + W::lo0 + q, W::n0)];
+#pragma unroll
+      for (int q = 0; q < W::n1; ++q) f1.p[q] = L1[slot(i + W::lo1 + q, W::n1)];
+#pragma unroll
+      for (int q = 0; q < W::n2; ++q) f2.p[q] = L2[slot(i + W::lo2 + q, W::n2)];
+      double nsum;
+      if (C == 0)
+        nsum = conv_self_w<0>(fc, dx) + conv_cross_w<0, 1>(fc, f1, dy) +
+               conv_cross_w<0, 2>(fc, f2, dz);
+      else if (C == 1)
+        nsum = conv_cross_w<1, 0>(fc, f1, dx) + conv_self_w<1>(fc, dy) +
+               conv_cross_w<1, 2>(fc, f2, dz);
+      else
+        nsum = conv_cross_w<2, 0>(fc, f1, dx) + conv_cross_w<2, 1>(fc, f2, dy) +
+               conv_self_w<2>(fc, dz);
+      const int64_t x = at(g, b.lo0 + i, b.lo1 + jj, b.lo2 + kk);
+      double t = coo[x] - dta * nsum;
+      const double diff = vis[x + voff] * lap4_w(fc, r12xx, r12yy, r12zz);
+      t = t + dta * diff;
+      cstar[x] = t;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void __launch_bounds__(256)
 calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
               double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double h, double dta) {
@@ -512,15 +695,20 @@ int wf_convdiff4th(int comp, const double* u, const double* v, const double* w,
   if (empty(b)) return 0;
   const double dta = dt * alpha;
   const double rx = 12.0 * dx * dx, ry = 12.0 * dy * dy, rz = 12.0 * dz * dz;
+  // i-chunk per workgroup: aim for >= 2048 workgroups, at least 16 planes
+  const int kt = div_up(b.d2, CD_TK), jt = div_up(b.d1, CD_TJ);
+  const int64_t tiles = (int64_t)kt * jt;
+  const int ich = (int)std::max<int64_t>(16, (tiles * b.d0 + 2047) / 2048);
+  const dim3 grid(kt, jt, div_up(b.d0, ich));
   if (comp == 0)
-    hipLaunchKernelGGL(convdiff4th_kernel<0>, box_grid(b), box_block(), 0, S_(stream), u, v,
-                       w, coo, vis, cstar, g, b, dx, dy, dz, rx, ry, rz, dta);
+    hipLaunchKernelGGL(convdiff_lds_kernel<0>, grid, dim3(256), 0, S_(stream), u, v, w, coo,
+                       vis, cstar, g, b, ich, dx, dy, dz, rx, ry, rz, dta);
   else if (comp == 1)
-    hipLaunchKernelGGL(convdiff4th_kernel<1>, box_grid(b), box_block(), 0, S_(stream), u, v,
-                       w, coo, vis, cstar, g, b, dx, dy, dz, rx, ry, rz, dta);
+    hipLaunchKernelGGL(convdiff_lds_kernel<1>, grid, dim3(256), 0, S_(stream), u, v, w, coo,
+                       vis, cstar, g, b, ich, dx, dy, dz, rx, ry, rz, dta);
   else
-    hipLaunchKernelGGL(convdiff4th_kernel<2>, box_grid(b), box_block(), 0, S_(stream), u, v,
-                       w, coo, vis, cstar, g, b, dx, dy, dz, rx, ry, rz, dta);
+    hipLaunchKernelGGL(convdiff_lds_kernel<2>, grid, dim3(256), 0, S_(stream), u, v, w, coo,
+                       vis, cstar, g, b, ich, dx, dy, dz, rx, ry, rz, dta);
   WF_LAUNCHED();
   return 0;
 }

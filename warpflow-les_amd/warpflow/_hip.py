@@ -28,7 +28,8 @@ class WfMgLevel(C.Structure):
                 ("n0", C.c_int32), ("n1", C.c_int32), ("n2", C.c_int32),
                 ("prec", C.c_int32),
                 ("h0", C.c_double), ("h1", C.c_double), ("h2", C.c_double),
-                ("tmp", C.c_void_p)]
+                ("tmp", C.c_void_p), ("cop", C.c_void_p),
+                ("cop_sweeps", C.c_int32), ("reserved", C.c_int32)]
 
 
 _P = C.c_void_p
@@ -90,6 +91,9 @@ _SIGS = {
     "wf_mg_vcycle": [_L, _I, _I, _I, _I, _P],
     "wf_mg_solve": [_L, _I, _P, _P, _G, _I, _I, _I, _I, _P],
     "wf_mg_residual_max": [_L, _P, _P],
+    "wf_mg_coarse_op_bytes": [_L, C.POINTER(C.c_size_t)],
+    "wf_mg_coarse_op_build": [_L, _I, _P, _P],
+    "wf_mg_coarse_op_apply": [_L, _P],
     # mixed-precision solve
     "wf_mg_mixed_workspace_bytes": [_G, C.POINTER(C.c_size_t)],
     "wf_mg_solve_mixed": [_L, _I, _P, _P, _G, _I, _I, _I, _I, _D, _I, _P, _Z,

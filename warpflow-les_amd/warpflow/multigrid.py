@@ -21,6 +21,7 @@ from .device import DeviceArray, current_stream, grid_of, zeros
 
 NPL_MG = 1
 SIDES = ("west", "east", "south", "north", "bottom", "top")
+COARSE_OPERATOR_MAX_CELLS = 1728
 
 
 class _Level:
@@ -38,10 +39,27 @@ class _Level:
         # result: every relax leaves its result in phi)
         self.tmp = zeros(self.shape, self.dtype, device)
         self._coef = None
+        self.cop = None        # coarse operator (coarsest level only)
+        self.cop_sweeps = 0
 
     def desc(self) -> _hip.WfMgLevel:
         return _hip.WfMgLevel(self.phi.ptr, self.rhs.ptr, *self.n3,
-                              self.dtype.itemsize, *self.h3, self.tmp.ptr)
+                              self.dtype.itemsize, *self.h3, self.tmp.ptr,
+                              self.cop.ptr if self.cop is not None else None,
+                              self.cop_sweeps, 0)
+
+    def build_coarse_operator(self, nsweeps):
+        """Form M with phi = M rhs equal to nsweeps RBGS sweeps from phi = 0
+        (the coarsest solve of every V-cycle), by running those sweeps on
+        every unit rhs.  Returns False when the level is too large."""
+        d = self.desc()
+        nbytes = C.c_size_t(0)
+        if _hip.lib.wf_mg_coarse_op_bytes(C.byref(d), C.byref(nbytes)) != 0:
+            return False
+        op = zeros(nbytes.value // self.dtype.itemsize, self.dtype, self.device)
+        _hip.call("wf_mg_coarse_op_build", C.byref(d), int(nsweeps), op.ptr, current_stream())
+        self.cop, self.cop_sweeps = op, int(nsweeps)
+        return True
 
     def _coefficients(self):
         if self._coef is None:
@@ -85,6 +103,11 @@ class Multigrid:
         self.n_post = 6
         self.n_coarse = 300
         self.n_vcycles = 4
+        # the coarsest solve as one matrix-vector product when the level is
+        # small (<= 12^3 cells: M <= 24 MB in FP64)
+        coarse = self.levels[-1]
+        if len(self.levels) > 1 and np.prod(coarse.n3) <= COARSE_OPERATOR_MAX_CELLS:
+            coarse.build_coarse_operator(self.n_coarse)
 
     def _descs(self):
         arr = (_hip.WfMgLevel * len(self.levels))()
