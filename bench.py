@@ -54,6 +54,8 @@ def parse(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel table")
     ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
+    ap.add_argument("--probe", action="store_true",
+                    help="only launch each hot kernel --reps times (hardware-counter runs)")
     ap.add_argument("--mixed-cycles", type=int, default=2,
                     help="FP64-refined FP32 V-cycles per pressure solve in the mixed form")
     return ap.parse_args(argv)
@@ -116,15 +118,17 @@ class Timer:
         return 1000.0 * ms.value / reps
 
 
-def kernel_table(a, s64, smx):
-    """Per-kernel HIP-event timings on the solvers' own arrays.  bytes =
-    algorithmic bytes per launch (distinct arrays read + written, RMW twice,
-    element size x launch cells); per_step = launches per step of the
-    schedule with 2 outer iterations (one pressure solve: 4 V-cycles in the
-    FP64 form, --mixed-cycles in the mixed form)."""
+def hot_kernels(a, s64, smx):
+    """The hot kernels of one step on the solvers' own arrays, as
+    (name, launch, algorithmic bytes per launch, launches per step FP64,
+    launches per step mixed).  bytes = distinct arrays read + written (RMW
+    twice) x element size x launch cells; launches per step are those of
+    the schedule with 2 outer iterations (one pressure solve: 4 V-cycles in
+    the FP64 form, --mixed-cycles in the mixed form).  The fused RBGS entry
+    is one call of two full sweeps (phi -> tmp -> phi), counted per sweep."""
     from warpflow import _hip
     from warpflow import kernels as K
-    from warpflow.device import current_stream
+    from warpflow.device import copy, current_stream
     from warpflow.topology import NPL
     n = a.n
     cells = float(n) ** 3
@@ -132,52 +136,88 @@ def kernel_table(a, s64, smx):
     h = (s64.dx, s64.dy, s64.dz)
     dt = s64.dt
     st = current_stream()
-    T = Timer()
-    rows = []
-
-    def add(name, fn, nbytes, per_step_64, per_step_mx):
-        us = T.avg_us(fn, a.reps)
-        rows.append({"kernel": name, "us": round(us, 2), "bytes": int(nbytes),
-                     "GBps": round(nbytes / us / 1e3, 1),
-                     "hbm_frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 3),
-                     "per_step_fp64": per_step_64, "per_step_mixed": per_step_mx})
-
     opu = s64._op_dim("u")
-    add("convdiff4th (u)", lambda: K.convdiff4th_u(opu, f["u"], f["v"], f["w"], f["uoo"],
-                                                   f["vis"], f["ustar"], *h, dt, 0.5, NPL),
-        48 * opu[0] * opu[1] * opu[2], 6, 6)
-    add("calvel3", lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
-                                     f["u"], f["v"], f["w"], *h, dt, 1.0, NPL),
-        56 * cells, 3, 3)
-    add("calmas", lambda: K.calmas(s64.ncells, f["u"], f["v"], f["w"], f["su"], s64._buf,
-                                   *h, dt, NPL), 32 * cells, 2, 2)
-    add("checkdt", lambda: K.checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"], s64._buf,
-                                     *h, NPL), 32 * cells, 1, 1)
-    add("wale", lambda: K.wale_viscosity(s64.ncells, f["u"], f["v"], f["w"], f["vis"], *h,
-                                         s64.rey, 0.46, NPL), 32 * cells, 1, 1)
-    add("add_ring", lambda: K.add_inplace(tuple(x + 2 for x in s64.ncells), f["p"], f["pp"],
-                                          NPL), 24 * (n + 2) ** 3, 1, 1)
-    add("bc_fill (u)", lambda: s64.fill_bc("u"), 0, 15, 15)
+    out = [
+        ("convdiff4th (u)", lambda: K.convdiff4th_u(opu, f["u"], f["v"], f["w"], f["uoo"],
+                                                    f["vis"], f["ustar"], *h, dt, 0.5, NPL),
+         48 * opu[0] * opu[1] * opu[2], 6, 6, 1),
+        ("calvel3", lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
+                                      f["u"], f["v"], f["w"], *h, dt, 1.0, NPL),
+         56 * cells, 3, 3, 1),
+        ("calmas", lambda: K.calmas(s64.ncells, f["u"], f["v"], f["w"], f["su"], s64._buf,
+                                    *h, dt, NPL), 32 * cells, 2, 2, 1),
+        ("checkdt", lambda: K.checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"], s64._buf,
+                                      *h, NPL), 32 * cells, 1, 1, 1),
+        ("wale", lambda: K.wale_viscosity(s64.ncells, f["u"], f["v"], f["w"], f["vis"], *h,
+                                          s64.rey, 0.46, NPL), 32 * cells, 1, 1, 1),
+        ("add_ring", lambda: K.add_inplace(tuple(x + 2 for x in s64.ncells), f["p"], f["pp"],
+                                           NPL), 24 * (n + 2) ** 3, 1, 1, 1),
+        ("copy (u -> uoo)", lambda: copy(f["uoo"], f["u"]), 2 * f["u"].nbytes, 3, 3, 1),
+        ("bc_fill (u)", lambda: s64.fill_bc("u"), 0, 15, 15, 1),
+    ]
     nv = s64.mg.n_vcycles
     nm = a.mixed_cycles
     sweeps = s64.mg.n_pre + s64.mg.n_post
     for tag, mg, eb, cyc64, cycmx in (("fp64", s64.mg, 8, nv, 0), ("fp32", smx.mg, 4, 0, nm)):
         L0, L1 = mg.levels[0].desc(), mg.levels[1].desc()
-        # one call = 2 fused full sweeps (phi -> tmp -> phi); bytes per sweep:
-        # read phi and rhs, write the other buffer
-        T2 = T.avg_us(lambda L0=L0: _hip.call("wf_mg_relax", C.byref(L0), 2, st), a.reps) / 2
-        nb = 3 * eb * cells
-        rows.append({"kernel": "rbgs full sweep L0 (%s)" % tag, "us": round(T2, 2),
-                     "bytes": int(nb), "GBps": round(nb / T2 / 1e3, 1),
-                     "hbm_frac": round(nb / T2 / 1e3 / HBM_PEAK_GBS, 3),
-                     "per_step_fp64": sweeps * cyc64, "per_step_mixed": sweeps * cycmx})
-        add("restrict L0 (%s)" % tag,
-            lambda L0=L0, L1=L1: _hip.call("wf_mg_restrict", C.byref(L0), C.byref(L1), st),
-            (2 * eb + 2 * eb / 8) * cells, cyc64, cycmx)
-        add("prolong L0 (%s)" % tag,
-            lambda L0=L0, L1=L1: _hip.call("wf_mg_prolong", C.byref(L1), C.byref(L0), st),
-            (2 * eb + eb / 8) * cells, cyc64, cycmx)
+        out.append(("rbgs full sweep L0 (%s)" % tag,
+                    lambda L0=L0: _hip.call("wf_mg_relax", C.byref(L0), 2, st),
+                    3 * eb * cells, sweeps * cyc64, sweeps * cycmx, 2))
+        out.append(("restrict L0 (%s)" % tag,
+                    lambda L0=L0, L1=L1: _hip.call("wf_mg_restrict", C.byref(L0), C.byref(L1),
+                                                   st),
+                    (2 * eb + 2 * eb / 8) * cells, cyc64, cycmx, 1))
+        out.append(("prolong L0 (%s)" % tag,
+                    lambda L0=L0, L1=L1: _hip.call("wf_mg_prolong", C.byref(L1), C.byref(L0),
+                                                   st),
+                    (2 * eb + eb / 8) * cells, cyc64, cycmx, 1))
+    return out
+
+
+def load_traffic():
+    """HBM bytes per launch of each hot kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    same counter's reading on the plain copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_cavity256.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return json.load(fh).get("kernels", {})
+
+
+def kernel_table(a, s64, smx):
+    """HIP-event timing of every hot kernel (hot_kernels) on the null stream,
+    the stream the solver launches on."""
+    T = Timer()
+    traffic = load_traffic()
+    rows = []
+    for name, fn, nbytes, ps64, psmx, per_call in hot_kernels(a, s64, smx):
+        us = T.avg_us(fn, a.reps) / per_call
+        row = {"kernel": name, "us": round(us, 2), "bytes": int(nbytes),
+               "GBps": round(nbytes / us / 1e3, 1),
+               "hbm_frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 3),
+               "per_step_fp64": ps64, "per_step_mixed": psmx}
+        if name in traffic:
+            row["traffic"] = traffic[name]["bytes_per_launch"]
+        rows.append(row)
     return rows
+
+
+def probe(a):
+    """Short run for hardware-counter passes: build both solvers (no steps),
+    then launch every hot kernel a.reps times, in order, on 256^3 arrays."""
+    from warpflow import FlowSolver, build_topology
+    from warpflow.device import synchronize
+    s64 = FlowSolver(make_cfg(a, "fp64"), build_topology(make_cfg(a, "fp64")), "hip:0")
+    smx = FlowSolver(make_cfg(a, "mixed"), build_topology(make_cfg(a, "mixed")), "hip:0")
+    synchronize()
+    for name, fn, nbytes, _, _, per_call in hot_kernels(a, s64, smx):
+        print("# probe %s: %d calls, %d algorithmic bytes per kernel launch"
+              % (name, a.reps, nbytes), file=sys.stderr, flush=True)
+        for _ in range(a.reps):
+            fn()
+        synchronize()
+    return 0
 
 
 def dominant(rows, key):
@@ -220,6 +260,8 @@ def main(argv=None):
         print("bench.py: this build runs one cavity on one GPU; --gpus %d is not supported "
               "(multi-GPU domain decomposition is out of scope)" % a.gpus, file=sys.stderr)
         return 2
+    if a.probe:
+        return probe(a)
     forms = [f.strip() for f in a.forms.split(",") if f.strip()]
     res = {}
     for form in forms:
@@ -258,7 +300,8 @@ def main(argv=None):
         line["roofline"] = {"bound": "hbm", "kernel": dom["kernel"],
                             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                            "traffic": None}
+                            "traffic": dom.get("traffic"),
+                            "algorithmic_bytes": dom["bytes"]}
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a)
     print(json.dumps(line), flush=True)

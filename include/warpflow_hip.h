@@ -77,6 +77,9 @@ int wf_free(void* ptr);
 int wf_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int wf_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int wf_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+/* Device-to-device copy by a kernel (wp.copy, solver.py:223-225): the same
+ * call is safe inside a captured graph. */
+int wf_copy(void* dst, const void* src, size_t bytes, void* stream);
 int wf_memset(void* ptr, int value, size_t bytes, void* stream);
 int wf_fill_f64(double* ptr, double value, size_t n, void* stream);
 int wf_fill_f32(float* ptr, float value, size_t n, void* stream);

@@ -24,6 +24,40 @@ int fill(T* p, T v, size_t n, hipStream_t s) {
   return 0;
 }
 
+// Plain device copy: 16-byte vector loads and stores when both pointers and
+// the size are 16-byte aligned, bytes otherwise.
+__global__ void __launch_bounds__(256) copy16_kernel(uint4* __restrict__ d,
+                                                     const uint4* __restrict__ s, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = t; q < n; q += stride) d[q] = s[q];
+}
+
+__global__ void __launch_bounds__(256) copy1_kernel(unsigned char* __restrict__ d,
+                                                    const unsigned char* __restrict__ s,
+                                                    size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = t; q < n; q += stride) d[q] = s[q];
+}
+
+int copy(void* d, const void* s, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return 0;
+  if (!d || !s) return WF_EINVAL;
+  if ((((uintptr_t)d | (uintptr_t)s | bytes) & 15) == 0) {
+    const size_t n = bytes / 16;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (uint4*)d,
+                       (const uint4*)s, n);
+  } else {
+    const size_t blocks = std::min<size_t>((bytes + 255) / 256, 65536);
+    hipLaunchKernelGGL(copy1_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
+                       (unsigned char*)d, (const unsigned char*)s, bytes);
+  }
+  WF_LAUNCHED();
+  return 0;
+}
+
 }  // namespace wf
 
 using namespace wf;
@@ -84,6 +118,10 @@ int wf_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
 
 int wf_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
   return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+}
+
+int wf_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  return copy(dst, src, bytes, (hipStream_t)stream);
 }
 
 int wf_memset(void* ptr, int value, size_t bytes, void* stream) {

@@ -101,18 +101,101 @@ static int launch_axis(double* f, const wf_grid& g, int axis, int a0, int na, in
   return 0;
 }
 
-static BcOp C_(int high, int g0, double val, int nlay) {
+__host__ __device__ inline BcOp C_(int high, int g0, double val, int nlay) {
   return BcOp{0, high, g0, 0, nlay, val};
 }
-static BcOp L_(int high, int g0, int s0, double sign, int nlay) {
+__host__ __device__ inline BcOp L_(int high, int g0, int s0, double sign, int nlay) {
   return BcOp{1, high, g0, s0, nlay, sign};
 }
-static BcOps ops2(BcOp a, BcOp b) {
+__host__ __device__ inline BcOps ops2(BcOp a, BcOp b) {
   BcOps o{};
   o.op[0] = a;
   o.op[1] = b;
   o.n = 2;
   return o;
+}
+
+// Launch range (a0, na) x (b0, nb) and op list of the `axis` launch of
+// BoundaryKernels.apply_u / _v / _w / _vis / _pbound / _p_mirror
+// (which = 0..5, bcs.py:201-311) on an N-ghost field of ni x nj x nk real
+// cells.  The reference uses one n for every axis; here each range uses its
+// own axis' count, identical for a cubic grid.  False for an unknown `which`.
+__host__ __device__ inline bool bc_axis_ops(int which, int axis, int N, int ni, int nj, int nk,
+                                            double param, int& a0, int& na, int& b0, int& nb,
+                                            BcOps& o) {
+  a0 = b0 = N - 1;
+  const int n3[3] = {ni, nj, nk};
+  const int pa = axis == 0 ? nj : ni, pb = axis == 2 ? nj : nk;  // perpendicular counts
+  switch (which) {
+    case 0:  // apply_u, bcs.py:201-223; param = lid velocity
+      if (axis == 0) {
+        na = nj + 2, nb = nk + 2;
+        o = ops2(C_(0, N - 1, 0.0, 3), C_(1, N + ni - 1, 0.0, 3));
+      } else if (axis == 1) {
+        na = ni + 1, nb = nk + 2;
+        o = ops2(L_(0, N - 1, N, 1.0, 3), L_(1, N + nj, N + nj - 1, 1.0, 3));
+      } else {
+        na = ni + 1, nb = nj + 2;
+        o = ops2(L_(0, N - 1, N, -1.0, 3), C_(1, N + nk, param, 3));
+      }
+      return true;
+    case 1:  // apply_v, bcs.py:225-244
+      if (axis == 0) {
+        na = nj + 1, nb = nk + 2;
+        o = ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + ni, N + ni - 1, -1.0, 3));
+      } else if (axis == 1) {
+        na = ni + 2, nb = nk + 2;
+        o = ops2(C_(0, N - 1, 0.0, 3), C_(1, N + nj - 1, 0.0, 3));
+      } else {
+        na = ni + 2, nb = nj + 1;
+        o = ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + nk, N + nk - 1, 1.0, 3));
+      }
+      return true;
+    case 2:  // apply_w, bcs.py:246-265
+      if (axis == 0) {
+        na = nj + 2, nb = nk + 1;
+        o = ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + ni, N + ni - 1, -1.0, 3));
+      } else if (axis == 1) {
+        na = ni + 2, nb = nk + 1;
+        o = ops2(L_(0, N - 1, N, 1.0, 3), L_(1, N + nj, N + nj - 1, 1.0, 3));
+      } else {
+        na = ni + 2, nb = nj + 2;
+        o = ops2(C_(0, N - 1, 0.0, 3), C_(1, N + nk - 1, 0.0, 3));
+      }
+      return true;
+    case 3:  // apply_vis, bcs.py:267-293; param = 1/Re
+      na = pa + 2, nb = pb + 2;
+      if (axis == 0) {
+        o = BcOps{};
+        o.op[0] = C_(0, N, param, 1);
+        o.op[1] = L_(0, N - 1, N, 1.0, 1);
+        o.op[2] = C_(1, N + ni - 1, param, 1);
+        o.op[3] = L_(1, N + ni, N + ni - 1, 1.0, 1);
+        o.n = 4;
+      } else if (axis == 1) {
+        o = ops2(L_(0, N - 1, N, 1.0, 1), L_(1, N + nj, N + nj - 1, 1.0, 1));
+      } else {
+        o = BcOps{};
+        o.op[0] = C_(0, N, param, 1);
+        o.op[1] = L_(0, N - 1, N, 1.0, 1);
+        o.op[2] = L_(1, N + nk, N + nk - 1, 1.0, 1);
+        o.n = 3;
+      }
+      return true;
+    case 4:  // apply_pbound, bcs.py:295-302
+      na = pa + 3, nb = pb + 3;
+      o = ops2(L_(0, N - 1, N, 1.0, 1), L_(1, N + n3[axis], N + n3[axis] - 1, 1.0, 1));
+      return true;
+    case 5: {  // apply_p_mirror, bcs.py:304-311; param = layers
+      const int nl = (int)param;
+      a0 = b0 = 0;
+      na = pa + 2 * N, nb = pb + 2 * N;
+      o = ops2(L_(0, N - 1, N, 1.0, nl), L_(1, N + n3[axis], N + n3[axis] - 1, 1.0, nl));
+      return true;
+    }
+    default:
+      return false;
+  }
 }
 
 extern "C" {
@@ -134,92 +217,18 @@ int wf_bc_const(double* fld, wf_grid g, int axis, int high, int g0, double val, 
 }
 
 // Per-axis restatement of BoundaryKernels.apply_u/v/w/vis/pbound/p_mirror
-// (bcs.py:201-311).  The reference uses one n for every axis; here each
-// range uses its own axis' count, identical for a cubic grid.
+// (bcs.py:201-311): the x, y and z launches of bc_axis_ops, in that order.
 int wf_bc_fill(double* f, wf_grid g, int which, double param, void* stream) {
   const hipStream_t s = (hipStream_t)stream;
-  const int N = g.npl, ni = g.ni, nj = g.nj, nk = g.nk;
-  if (N < 1) return WF_EINVAL;
-  switch (which) {
-    case 0: {  // apply_u, bcs.py:201-223; param = lid velocity
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 0, N - 1, nj + 2, N - 1, nk + 2,
-          ops2(C_(0, N - 1, 0.0, 3), C_(1, N + ni - 1, 0.0, 3)), s));
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 1, N - 1, ni + 1, N - 1, nk + 2,
-          ops2(L_(0, N - 1, N, 1.0, 3), L_(1, N + nj, N + nj - 1, 1.0, 3)), s));
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 2, N - 1, ni + 1, N - 1, nj + 2,
-          ops2(L_(0, N - 1, N, -1.0, 3), C_(1, N + nk, param, 3)), s));
-      return 0;
-    }
-    case 1: {  // apply_v, bcs.py:225-244
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 0, N - 1, nj + 1, N - 1, nk + 2,
-          ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + ni, N + ni - 1, -1.0, 3)), s));
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 1, N - 1, ni + 2, N - 1, nk + 2,
-          ops2(C_(0, N - 1, 0.0, 3), C_(1, N + nj - 1, 0.0, 3)), s));
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 2, N - 1, ni + 2, N - 1, nj + 1,
-          ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + nk, N + nk - 1, 1.0, 3)), s));
-      return 0;
-    }
-    case 2: {  // apply_w, bcs.py:246-265
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 0, N - 1, nj + 2, N - 1, nk + 1,
-          ops2(L_(0, N - 1, N, -1.0, 3), L_(1, N + ni, N + ni - 1, -1.0, 3)), s));
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 1, N - 1, ni + 2, N - 1, nk + 1,
-          ops2(L_(0, N - 1, N, 1.0, 3), L_(1, N + nj, N + nj - 1, 1.0, 3)), s));
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 2, N - 1, ni + 2, N - 1, nj + 2,
-          ops2(C_(0, N - 1, 0.0, 3), C_(1, N + nk - 1, 0.0, 3)), s));
-      return 0;
-    }
-    case 3: {  // apply_vis, bcs.py:267-293; param = 1/Re
-      BcOps x{};
-      x.op[0] = C_(0, N, param, 1);
-      x.op[1] = L_(0, N - 1, N, 1.0, 1);
-      x.op[2] = C_(1, N + ni - 1, param, 1);
-      x.op[3] = L_(1, N + ni, N + ni - 1, 1.0, 1);
-      x.n = 4;
-      WF_TRY((hipError_t)launch_axis(f, g, 0, N - 1, nj + 2, N - 1, nk + 2, x, s));
-      WF_TRY((hipError_t)launch_axis(
-          f, g, 1, N - 1, ni + 2, N - 1, nk + 2,
-          ops2(L_(0, N - 1, N, 1.0, 1), L_(1, N + nj, N + nj - 1, 1.0, 1)), s));
-      BcOps z{};
-      z.op[0] = C_(0, N, param, 1);
-      z.op[1] = L_(0, N - 1, N, 1.0, 1);
-      z.op[2] = L_(1, N + nk, N + nk - 1, 1.0, 1);
-      z.n = 3;
-      WF_TRY((hipError_t)launch_axis(f, g, 2, N - 1, ni + 2, N - 1, nj + 2, z, s));
-      return 0;
-    }
-    case 4: {  // apply_pbound, bcs.py:295-302
-      const int n3[3] = {ni, nj, nk};
-      for (int ax = 0; ax < 3; ++ax) {
-        const int pa = ax == 0 ? nj : ni, pb = ax == 2 ? nj : nk;
-        WF_TRY((hipError_t)launch_axis(
-            f, g, ax, N - 1, pa + 3, N - 1, pb + 3,
-            ops2(L_(0, N - 1, N, 1.0, 1), L_(1, N + n3[ax], N + n3[ax] - 1, 1.0, 1)), s));
-      }
-      return 0;
-    }
-    case 5: {  // apply_p_mirror, bcs.py:304-311; param = layers
-      const int nl = (int)param;
-      const int n3[3] = {ni, nj, nk};
-      for (int ax = 0; ax < 3; ++ax) {
-        const int pa = ax == 0 ? nj : ni, pb = ax == 2 ? nj : nk;
-        WF_TRY((hipError_t)launch_axis(
-            f, g, ax, 0, pa + 2 * N, 0, pb + 2 * N,
-            ops2(L_(0, N - 1, N, 1.0, nl), L_(1, N + n3[ax], N + n3[ax] - 1, 1.0, nl)), s));
-      }
-      return 0;
-    }
-    default:
+  if (g.npl < 1 || which < 0 || which > 5) return WF_EINVAL;
+  for (int axis = 0; axis < 3; ++axis) {
+    int a0, na, b0, nb;
+    BcOps o{};
+    if (!bc_axis_ops(which, axis, g.npl, g.ni, g.nj, g.nk, param, a0, na, b0, nb, o))
       return WF_EINVAL;
+    WF_TRY((hipError_t)launch_axis(f, g, axis, a0, na, b0, nb, o, s));
   }
+  return 0;
 }
 
 }  // extern "C"

@@ -98,12 +98,12 @@ __device__ inline T residual_at(const T* __restrict__ phi, const T* __restrict__
 
 // _restrict (multigrid.py:93-119) + coarse phi = 0 (multigrid.py:312).
 template <class T>
-__global__ void __launch_bounds__(256)
-restrict_kernel(const T* __restrict__ phif, const T* __restrict__ rhsf, MgDev f, T c0, T c1,
-                T c2, T* __restrict__ phic, T* __restrict__ rhsc, MgDev c) {
-  const int ck = blockIdx.x * 64 + threadIdx.x;
-  const int cj = blockIdx.y * 4 + threadIdx.y;
-  const int ci = blockIdx.z;
+__device__ inline void restrict_body(const T* __restrict__ phif, const T* __restrict__ rhsf,
+                                     const MgDev& f, T c0, T c1, T c2, T* __restrict__ phic,
+                                     T* __restrict__ rhsc, const MgDev& c, Blk B) {
+  const int ck = B.x * 64 + threadIdx.x;
+  const int cj = B.y * 4 + threadIdx.y;
+  const int ci = B.z;
   if (ck >= c.n2 || cj >= c.n1 || ci >= c.n0) return;
   T acc = T(0);
   for (int di = 0; di < 2; ++di)
@@ -116,14 +116,20 @@ restrict_kernel(const T* __restrict__ phif, const T* __restrict__ rhsf, MgDev f,
   phic[x] = T(0);
   ring_images(phic, c, ci, cj, ck, x, T(0));
 }
+template <class T>
+__global__ void __launch_bounds__(256)
+restrict_kernel(const T* __restrict__ phif, const T* __restrict__ rhsf, MgDev f, T c0, T c1,
+                T c2, T* __restrict__ phic, T* __restrict__ rhsc, MgDev c) {
+  restrict_body(phif, rhsf, f, c0, c1, c2, phic, rhsc, c, this_blk());
+}
 
 // _prolong (multigrid.py:122-146) + fine refill (multigrid.py:321).
 template <class T>
-__global__ void __launch_bounds__(256)
-prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev f) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  const int i = blockIdx.z;
+__device__ inline void prolong_body(const T* __restrict__ phic, const MgDev& c,
+                                    T* __restrict__ phif, const MgDev& f, Blk B) {
+  const int k = B.x * 64 + threadIdx.x;
+  const int j = B.y * 4 + threadIdx.y;
+  const int i = B.z;
   if (k >= f.n2 || j >= f.n1 || i >= f.n0) return;
   const int c0x = (i % 2 == 0) ? i / 2 - 1 : i / 2;
   const int c0y = (j % 2 == 0) ? j / 2 - 1 : j / 2;
@@ -150,6 +156,11 @@ prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev 
   phif[x] = v;
   ring_images(phif, f, i, j, k, x, v);
 }
+template <class T>
+__global__ void __launch_bounds__(256)
+prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev f) {
+  prolong_body(phic, c, phif, f, this_blk());
+}
 
 // One full RBGS sweep (colour 0, then colour 1, then the ring refill) out of
 // place: pout = S(pin).  Colour-0 (odd i+j+k) cells read only even cells of
@@ -170,9 +181,9 @@ prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev 
 // HBM traffic per sweep: one read of pin and rhs, one write of pout; the
 // per-cell arithmetic is the half-sweep kernel's, in the same order.
 template <class T, int TJ, int TK>
-__global__ void __launch_bounds__(256)
-sweep_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
-             MgDev d, T c0, T c1, T c2, int ich) {
+__device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pout,
+                                  const T* __restrict__ rhs, const MgDev& d, T c0, T c1, T c2,
+                                  int ich, Blk B) {
   static_assert(TJ * TK == 512, "one (even, odd) pair per thread");
   constexpr int PJ = TJ + 4, PK = TK + 4, PN = PJ * PK;
   constexpr int OJ = TJ + 2, OK = TK + 2, OH = OK / 2;
@@ -182,8 +193,8 @@ sweep_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restric
   __shared__ T P[3][PN];
   __shared__ T O[3][OJ * OK];
   const int tid = threadIdx.x;
-  const int j0 = blockIdx.y * TJ, k0 = blockIdx.x * TK;
-  const int i0 = blockIdx.z * ich, i1 = min(i0 + ich, d.n0);
+  const int j0 = B.y * TJ, k0 = B.x * TK;
+  const int i0 = B.z * ich, i1 = min(i0 + ich, d.n0);
   const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
   const int s1 = (int)d.s1;
   auto slot = [](int q) { return ((q % 3) + 3) % 3; };
@@ -320,17 +331,20 @@ sweep_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restric
     }
   }
 }
+template <class T, int TJ, int TK>
+__global__ void __launch_bounds__(256)
+sweep_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
+             MgDev d, T c0, T c1, T c2, int ich) {
+  sweep_body<T, TJ, TK>(pin, pout, rhs, d, c0, c1, c2, ich, this_blk());
+}
 
 // n sweeps of RBGS on one small level held in LDS by one workgroup
 // (multigrid.py:289-299 with the refill folded in).  The real cells of phi
 // live in LDS; ring terms carry zero coefficients and are skipped.  rhs is
 // in LDS too when RHS_LDS, else read from global memory (L2-resident).
 template <class T, bool RHS_LDS>
-__global__ void __launch_bounds__(1024)
-relax_lds_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, T c1, T c2,
-                 int nsweeps) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sp = reinterpret_cast<T*>(smem_raw);
+__device__ inline void relax_lds_body(T* __restrict__ phi, const T* __restrict__ rhs,
+                                      const MgDev& d, T c0, T c1, T c2, int nsweeps, T* sp) {
   const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
   const int ncell = n0 * n1 * n2;
   T* sr = sp + ncell;
@@ -375,18 +389,23 @@ relax_lds_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, 
     if (nsweeps > 0) ring_images(phi, d, i, j, k, x, v);
   }
 }
+template <class T, bool RHS_LDS>
+__global__ void __launch_bounds__(1024)
+relax_lds_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, T c1, T c2,
+                 int nsweeps) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  relax_lds_body<T, RHS_LDS>(phi, rhs, d, c0, c1, c2, nsweeps, reinterpret_cast<T*>(smem_raw));
+}
 
 // Column `col` (= workgroup) of a small level's coarse operator: nsweeps RBGS
 // sweeps in LDS from phi = 0 with rhs = e_col, the relax_lds_kernel
 // arithmetic; M is column-major (M[col * ncell + row]).
 template <class T>
-__global__ void __launch_bounds__(256)
-coarse_op_build_kernel(T* __restrict__ M, MgDev d, T c0, T c1, T c2, int nsweeps) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sp = reinterpret_cast<T*>(smem_raw);
+__device__ inline void coarse_op_build_body(T* __restrict__ M, const MgDev& d, T c0, T c1, T c2,
+                                            int nsweeps, int col, T* sp) {
   const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
   const int ncell = n0 * n1 * n2;
-  const int col = blockIdx.x, nt = blockDim.x, tid = threadIdx.x;
+  const int nt = blockDim.x, tid = threadIdx.x;
   for (int l = tid; l < ncell; l += nt) sp[l] = T(0);
   __syncthreads();
   const int np = (n2 + 1) / 2, nrow = n0 * n1, S0 = n1 * n2, S1 = n2;
@@ -414,14 +433,17 @@ coarse_op_build_kernel(T* __restrict__ M, MgDev d, T c0, T c1, T c2, int nsweeps
   }
   for (int l = tid; l < ncell; l += nt) M[(size_t)col * ncell + l] = sp[l];
 }
+template <class T>
+__global__ void __launch_bounds__(256)
+coarse_op_build_kernel(T* __restrict__ M, MgDev d, T c0, T c1, T c2, int nsweeps) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  coarse_op_build_body<T>(M, d, c0, c1, c2, nsweeps, blockIdx.x, reinterpret_cast<T*>(smem_raw));
+}
 
 // phi = M rhs on the real cells of a small level, plus face-ring images.
 template <class T>
-__global__ void __launch_bounds__(1024)
-coarse_op_apply_kernel(const T* __restrict__ M, T* __restrict__ phi, const T* __restrict__ rhs,
-                       MgDev d) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* r = reinterpret_cast<T*>(smem_raw);
+__device__ inline void coarse_op_apply_body(const T* __restrict__ M, T* __restrict__ phi,
+                                            const T* __restrict__ rhs, const MgDev& d, T* r) {
   const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
   const int ncell = n0 * n1 * n2;
   const int nt = blockDim.x, tid = threadIdx.x;
@@ -438,6 +460,13 @@ coarse_op_apply_kernel(const T* __restrict__ M, T* __restrict__ phi, const T* __
     phi[x] = acc;
     ring_images(phi, d, i, j, k, x, acc);
   }
+}
+template <class T>
+__global__ void __launch_bounds__(1024)
+coarse_op_apply_kernel(const T* __restrict__ M, T* __restrict__ phi, const T* __restrict__ rhs,
+                       MgDev d) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  coarse_op_apply_body<T>(M, phi, rhs, d, reinterpret_cast<T*>(smem_raw));
 }
 
 __global__ void __launch_bounds__(256)
@@ -588,12 +617,13 @@ int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s) {
   if (nsweeps <= 0) return 0;
   const size_t cells = (size_t)L.n0 * L.n1 * L.n2;
   const size_t lds = (size_t)max_lds();
-  if (cells <= LDS_RELAX_MAX_CELLS || nsweeps >= LDS_RELAX_MIN_SWEEPS) {
-    if (2 * cells * sizeof(T) <= lds)
-      return launch_relax_lds<T, true>(L, nsweeps, 2 * cells * sizeof(T), s);
-    if (cells * sizeof(T) <= lds)
-      return launch_relax_lds<T, false>(L, nsweeps, cells * sizeof(T), s);
-  }
+  // many sweeps in one workgroup pay off only with rhs in LDS too: with rhs
+  // read from global memory every colour pass waits on its latency
+  if ((cells <= LDS_RELAX_MAX_CELLS || nsweeps >= LDS_RELAX_MIN_SWEEPS) &&
+      2 * cells * sizeof(T) <= lds)
+    return launch_relax_lds<T, true>(L, nsweeps, 2 * cells * sizeof(T), s);
+  if (cells <= LDS_RELAX_MAX_CELLS && cells * sizeof(T) <= lds)
+    return launch_relax_lds<T, false>(L, nsweeps, cells * sizeof(T), s);
   int q = 0;
   if (L.tmp) {
     for (; q + 1 < nsweeps; q += 2) {

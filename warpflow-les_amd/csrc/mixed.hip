@@ -35,8 +35,10 @@ inline int64_t field_blocks(const wf_grid& g) {
   return (int64_t)b.x * b.y * b.z;
 }
 
-__device__ inline int64_t block_linear() {
-  return ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+// Linear index of block B of a field_grid launch (x fastest).
+__device__ inline int64_t block_linear(const wf_grid& g, Blk B) {
+  const int gx = (g.nk + 63) / 64, gy = (g.nj + 3) / 4;
+  return ((int64_t)B.z * gy + B.y) * gx + B.x;
 }
 
 // Block sum of 256 values in a fixed order; every thread must call it.
@@ -50,19 +52,23 @@ __device__ inline double block_sum_fixed(double v) {
 }
 
 // Partial sums of f over the real cells of a field, one per block.
-__global__ void __launch_bounds__(256)
-field_partials_kernel(const double* __restrict__ f, wf_grid g, double* __restrict__ partials) {
-  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+__device__ inline void field_partials_body(const double* __restrict__ f, const wf_grid& g,
+                                           double* __restrict__ partials, Blk B) {
+  const int k = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y, i = B.z;
   double v = 0.0;
   if (k < g.nk && j < g.nj) v = f[at(g, i + g.npl, j + g.npl, k + g.npl)];
   const double s = block_sum_fixed(v);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partials[block_linear()] = s;
+  if (threadIdx.x == 0 && threadIdx.y == 0) partials[block_linear(g, B)] = s;
+}
+__global__ void __launch_bounds__(256)
+field_partials_kernel(const double* __restrict__ f, wf_grid g, double* __restrict__ partials) {
+  field_partials_body(f, g, partials, this_blk());
 }
 
 // out = scale * sum(partials[0..n)), folded by one 1024-thread workgroup in
 // a fixed order.
-__global__ void __launch_bounds__(1024)
-fold_kernel(const double* __restrict__ partials, int64_t n, double scale, double* __restrict__ out) {
+__device__ inline void fold_body(const double* __restrict__ partials, int64_t n, double scale,
+                                 double* __restrict__ out) {
   __shared__ double s[1024];
   const int t = threadIdx.x;
   double acc = 0.0;
@@ -75,17 +81,22 @@ fold_kernel(const double* __restrict__ partials, int64_t n, double scale, double
   }
   if (t == 0) *out = s[0] * scale;
 }
+__global__ void __launch_bounds__(1024)
+fold_kernel(const double* __restrict__ partials, int64_t n, double scale, double* __restrict__ out) {
+  fold_body(partials, n, scale, out);
+}
 
 __global__ void set_scalar_kernel(double* p, double v) { *p = v; }
 
 // r = b - A x (FP64; x == nullptr means x = 0), minus *mean if given, cast to
 // FP32 into the level-0 rhs; rmax (nullable): each workgroup's max|b - A x|
 // into rmax[workgroup].
-__global__ void __launch_bounds__(256)
-defect_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_grid g,
-              float* __restrict__ r32, MgDev d, double c0, double c1, double c2,
-              const double* __restrict__ mean, double* __restrict__ rmax) {
-  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+__device__ inline void defect_body(const double* __restrict__ b, const double* __restrict__ x,
+                                   const wf_grid& g, float* __restrict__ r32, const MgDev& d,
+                                   double c0, double c1, double c2,
+                                   const double* __restrict__ mean, double* __restrict__ rmax,
+                                   Blk B) {
+  const int k = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y, i = B.z;
   double av = 0.0;
   if (k < g.nk && j < g.nj) {
     const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
@@ -106,8 +117,14 @@ defect_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_gri
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
     if ((t & 63) == 0) part[t >> 6] = m;
     __syncthreads();
-    if (t == 0) rmax[block_linear()] = nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3]));
+    if (t == 0) rmax[block_linear(g, B)] = nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3]));
   }
+}
+__global__ void __launch_bounds__(256)
+defect_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_grid g,
+              float* __restrict__ r32, MgDev d, double c0, double c1, double c2,
+              const double* __restrict__ mean, double* __restrict__ rmax) {
+  defect_body(b, x, g, r32, d, c0, c1, c2, mean, rmax, this_blk());
 }
 
 // out = max(partials[0..n)), one 1024-thread workgroup, fixed order.
@@ -128,26 +145,33 @@ max_fold_kernel(const double* __restrict__ partials, int64_t n, double* __restri
 }
 
 // x += e (FP32 correction widened to FP64) on the real cells.
-__global__ void __launch_bounds__(256)
-update_kernel(double* __restrict__ x, wf_grid g, const float* __restrict__ e, MgDev d) {
-  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+__device__ inline void update_body(double* __restrict__ x, const wf_grid& g,
+                                   const float* __restrict__ e, const MgDev& d, Blk B) {
+  const int k = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y, i = B.z;
   if (k >= g.nk || j >= g.nj) return;
   const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
   x[q] = x[q] + (double)e[mg_at(d, i, j, k)];
 }
-
 __global__ void __launch_bounds__(256)
-shift_kernel(double* __restrict__ x, wf_grid g, const double* __restrict__ mean) {
-  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+update_kernel(double* __restrict__ x, wf_grid g, const float* __restrict__ e, MgDev d) {
+  update_body(x, g, e, d, this_blk());
+}
+
+__device__ inline void shift_body(double* __restrict__ x, const wf_grid& g,
+                                  const double* __restrict__ mean, Blk B) {
+  const int k = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y, i = B.z;
   if (k >= g.nk || j >= g.nj) return;
   const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
   x[q] = x[q] - *mean;
 }
+__global__ void __launch_bounds__(256)
+shift_kernel(double* __restrict__ x, wf_grid g, const double* __restrict__ mean) {
+  shift_body(x, g, mean, this_blk());
+}
 
 // First ghost layer of the field: faces = mirror of the adjacent real cell,
 // edges/corners untouched (the ring state of the plain solve's copy-out).
-__global__ void __launch_bounds__(256) field_ring_kernel(double* __restrict__ x, wf_grid g) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void field_ring_body(double* __restrict__ x, const wf_grid& g, int t) {
   const int n0 = g.ni, n1 = g.nj, n2 = g.nk, P = g.npl;
   const int fx = n1 * n2, fy = n0 * n2, fz = n0 * n1;
   int q = t;
@@ -176,6 +200,9 @@ __global__ void __launch_bounds__(256) field_ring_kernel(double* __restrict__ x,
     const int64_t p = at(g, i + P, j + P, k + P);
     x[p + (hi ? 1 : -1)] = x[p];
   }
+}
+__global__ void __launch_bounds__(256) field_ring_kernel(double* __restrict__ x, wf_grid g) {
+  field_ring_body(x, g, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // max|rhs - A phi| over the real cells of one level (FP64 or FP32).

@@ -110,206 +110,6 @@ diff4th_kernel(const double* __restrict__ c, const double* __restrict__ vis,
   cstar[x] = cstar[x] + dta * diff;
 }
 
-template <int C>
-__global__ void __launch_bounds__(256)
-convdiff4th_kernel(const double* __restrict__ u, const double* __restrict__ v,
-                   const double* __restrict__ w, const double* __restrict__ coo,
-                   const double* __restrict__ vis, double* __restrict__ cstar, wf_grid g,
-                   Box3 b, double dx, double dy, double dz, double r12xx, double r12yy,
-                   double r12zz, double dta) {
-  int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
-  const int64_t x = at(g, i, j, k);
-  const double* __restrict__ c = C == 0 ? u : (C == 1 ? v : w);
-  double t = coo[x] - dta * conv_sum<C>(u, v, w, x, g, dx, dy, dz);
-  const double diff = vis[x + vis_offset<C>(g)] * lap4(c, x, g, r12xx, r12yy, r12zz);
-  t = t + dta * diff;
-  cstar[x] = t;
-}
-
-// ---------------------------------------------------------------------------
-// conv4th + diff4th of one component, streamed along i with LDS windows.
-//
-// A 256-thread workgroup owns a TJ x TK = 8 x 32 (j, k) tile of the operator
-// box and walks a chunk of i-planes.  Each field it reads is staged in LDS as
-// a rolling window of planes covering the tile plus a 3-cell (j, k) halo:
-// the component itself over planes i-3..i+3 and the two other components
-// over the planes their cross terms reach (C=0: v, w at i-1..i+2; C=1, C=2:
-// u at i-2..i+1 and the remaining component at plane i).  The next plane of
-// each window is prefetched into registers while the current output plane
-// computes, so every global value is read once per workgroup (plus the
-// halo), and all stencil taps come from LDS.  The arithmetic is conv_sum +
-// lap4 above, term for term, through an accessor with compile-time offsets.
-// ---------------------------------------------------------------------------
-constexpr int CD_TJ = 8, CD_TK = 32, CD_RJ = CD_TJ + 6, CD_RK = CD_TK + 6;
-constexpr int CD_PN = CD_RJ * CD_RK;  // 532 values per staged plane
-constexpr int CD_NPT = (CD_PN + 255) / 256;
-
-template <int C>
-struct CdWin {  // plane windows [lo, hi] of (own, q1, q2) for component C
-  static constexpr int lo0 = -3, hi0 = 3;
-  static constexpr int lo1 = C == 0 ? -1 : -2, hi1 = C == 0 ? 2 : 1;
-  static constexpr int lo2 = C == 0 ? -1 : 0, hi2 = C == 0 ? 2 : 0;
-  static constexpr int n0 = hi0 - lo0 + 1, n1 = hi1 - lo1 + 1, n2 = hi2 - lo2 + 1;
-};
-
-// Field accessor: value at plane offset di, row offset dj, column offset dk
-// from the thread's output cell.
-template <int LO, int N>
-struct CdField {
-  const double* p[N];  // plane pointers for offsets LO .. LO+N-1
-  int c;               // in-plane index of the output cell
-  __device__ double operator()(int di, int dj, int dk) const {
-    return p[di - LO][c + dj * CD_RK + dk];
-  }
-};
-
-template <int AX, class F>
-__device__ inline double I4w(const F& f, int di, int dj, int dk) {
-  constexpr int ex = AX == 0, ey = AX == 1, ez = AX == 2;
-  return (-f(di + ex, dj + ey, dk + ez) + 9.0 * f(di, dj, dk) +
-          9.0 * f(di - ex, dj - ey, dk - ez) - f(di - 2 * ex, dj - 2 * ey, dk - 2 * ez)) /
-         16.0;
-}
-
-template <int A, class F>
-__device__ inline double conv_self_w(const F& c, double ha) {
-  constexpr int ex = A == 0, ey = A == 1, ez = A == 2;
-  const double p2 = I4w<A>(c, 2 * ex, 2 * ey, 2 * ez), p1 = I4w<A>(c, ex, ey, ez);
-  const double p0 = I4w<A>(c, 0, 0, 0), m1 = I4w<A>(c, -ex, -ey, -ez);
-  return D4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, ha);
-}
-
-template <int A, int B, class F, class Q>
-__device__ inline double conv_cross_w(const F& c, const Q& q, double hb) {
-  constexpr int ax = A == 0, ay = A == 1, az = A == 2;
-  constexpr int bx = B == 0, by = B == 1, bz = B == 2;
-  const double sp1 = I4w<B>(c, 2 * bx, 2 * by, 2 * bz) * I4w<A>(q, ax + bx, ay + by, az + bz);
-  const double sn = I4w<B>(c, bx, by, bz) * I4w<A>(q, ax, ay, az);
-  const double sm1 = I4w<B>(c, 0, 0, 0) * I4w<A>(q, ax - bx, ay - by, az - bz);
-  const double sm2 =
-      I4w<B>(c, -bx, -by, -bz) * I4w<A>(q, ax - 2 * bx, ay - 2 * by, az - 2 * bz);
-  return D4(sp1, sn, sm1, sm2, hb);
-}
-
-template <class F>
-__device__ inline double lap4_w(const F& f, double r12xx, double r12yy, double r12zz) {
-  const double f0 = f(0, 0, 0);
-  const double lx = (-f(2, 0, 0) + 16.0 * f(1, 0, 0) - 30.0 * f0 + 16.0 * f(-1, 0, 0) -
-                     f(-2, 0, 0)) / r12xx;
-  const double ly = (-f(0, 2, 0) + 16.0 * f(0, 1, 0) - 30.0 * f0 + 16.0 * f(0, -1, 0) -
-                     f(0, -2, 0)) / r12yy;
-  const double lz = (-f(0, 0, 2) + 16.0 * f(0, 0, 1) - 30.0 * f0 + 16.0 * f(0, 0, -1) -
-                     f(0, 0, -2)) / r12zz;
-  return lx + ly + lz;
-}
-
-template <int C>
-__global__ void __launch_bounds__(256)
-convdiff_lds_kernel(const double* __restrict__ u, const double* __restrict__ v,
-                    const double* __restrict__ w, const double* __restrict__ coo,
-                    const double* __restrict__ vis, double* __restrict__ cstar, wf_grid g,
-                    Box3 b, int ich, double dx, double dy, double dz, double r12xx,
-                    double r12yy, double r12zz, double dta) {
-  using W = CdWin<C>;
-  __shared__ double L0[W::n0][CD_PN];
-  __shared__ double L1[W::n1][CD_PN];
-  __shared__ double L2[W::n2][CD_PN];
-  const double* F0 = C == 0 ? u : (C == 1 ? v : w);  // own component
-  const double* F1 = C == 0 ? v : u;                  // first cross field
-  const double* F2 = C == 0 ? w : (C == 1 ? w : v);  // second cross field
-  const int tid = threadIdx.x;
-  const int j0 = blockIdx.y * CD_TJ, k0 = blockIdx.x * CD_TK;
-  const int i0 = blockIdx.z * ich, i1 = min(i0 + ich, b.d0);
-  const int nI = g.ni + 2 * g.npl, nJ = g.nj + 2 * g.npl, nK = g.nk + 2 * g.npl;
-  // staged-region offsets (full-array j, k of region row/column 0)
-  const int rj0 = b.lo1 + j0 - 3, rk0 = b.lo2 + k0 - 3;
-  int64_t roff[CD_NPT];
-#pragma unroll
-  for (int e = 0; e < CD_NPT; ++e) {
-    const int l = min(tid + 256 * e, CD_PN - 1);
-    const int jf = min(max(rj0 + l / CD_RK, 0), nJ - 1);
-    const int kf = min(max(rk0 + l % CD_RK, 0), nK - 1);
-    roff[e] = (int64_t)jf * g.sj + kf;
-  }
-  auto plane_base = [&](int i) {  // output plane i -> clamped full plane offset
-    return (int64_t)min(max(b.lo0 + i, 0), nI - 1) * g.si;
-  };
-  auto slot = [](int i, int n) { return ((i % n) + n) % n; };
-  double r0[CD_NPT], r1[CD_NPT], r2[CD_NPT];
-  auto gload = [&](const double* f, int i, double (&r)[CD_NPT]) {
-    const double* base = f + plane_base(i);
-#pragma unroll
-    for (int e = 0; e < CD_NPT; ++e) r[e] = base[roff[e]];
-  };
-  auto lstore = [&](double* dst, const double (&r)[CD_NPT]) {
-#pragma unroll
-    for (int e = 0; e < CD_NPT; ++e)
-      if (tid + 256 * e < CD_PN) dst[tid + 256 * e] = r[e];
-  };
-  // prologue: all but the newest plane of every window
-  for (int i = i0 + W::lo0; i < i0 + W::hi0; ++i) {
-    gload(F0, i, r0);
-    lstore(L0[slot(i, W::n0)], r0);
-  }
-  for (int i = i0 + W::lo1; i < i0 + W::hi1; ++i) {
-    gload(F1, i, r1);
-    lstore(L1[slot(i, W::n1)], r1);
-  }
-  for (int i = i0 + W::lo2; i < i0 + W::hi2; ++i) {
-    gload(F2, i, r2);
-    lstore(L2[slot(i, W::n2)], r2);
-  }
-  gload(F0, i0 + W::hi0, r0);
-  gload(F1, i0 + W::hi1, r1);
-  gload(F2, i0 + W::hi2, r2);
-
-  const int tj = tid / CD_TK, tk = tid % CD_TK;
-  const int jj = j0 + tj, kk = k0 + tk;
-  const bool live = jj < b.d1 && kk < b.d2;
-  const int cin = (tj + 3) * CD_RK + (tk + 3);
-  const int64_t voff = C == 0 ? 0 : (C == 1 ? g.sj : 1);
-  for (int i = i0; i < i1; ++i) {
-    lstore(L0[slot(i + W::hi0, W::n0)], r0);
-    lstore(L1[slot(i + W::hi1, W::n1)], r1);
-    lstore(L2[slot(i + W::hi2, W::n2)], r2);
-    if (i + 1 < i1) {
-      gload(F0, i + 1 + W::hi0, r0);
-      gload(F1, i + 1 + W::hi1, r1);
-      gload(F2, i + 1 + W::hi2, r2);
-    }
-    __syncthreads();
-    if (live) {
-      CdField<W::lo0, W::n0> fc;
-      CdField<W::lo1, W::n1> f1;
-      CdField<W::lo2, W::n2> f2;
-      fc.c = f1.c = f2.c = cin;
-#pragma unroll
-      for (int q = 0; q < W::n0; ++q) fc.p[q] = L0[slot(i + W::lo0 + q, W::n0)];
-#pragma unroll
-      for (int q = 0; q < W::n1; ++q) f1.p[q] = L1[slot(i + W::lo1 + q, W::n1)];
-#pragma unroll
-      for (int q = 0; q < W::n2; ++q) f2.p[q] = L2[slot(i + W::lo2 + q, W::n2)];
-      double nsum;
-      if (C == 0)
-        nsum = conv_self_w<0>(fc, dx) + conv_cross_w<0, 1>(fc, f1, dy) +
-               conv_cross_w<0, 2>(fc, f2, dz);
-      else if (C == 1)
-        nsum = conv_cross_w<1, 0>(fc, f1, dx) + conv_self_w<1>(fc, dy) +
-               conv_cross_w<1, 2>(fc, f2, dz);
-      else
-        nsum = conv_cross_w<2, 0>(fc, f1, dx) + conv_cross_w<2, 1>(fc, f2, dy) +
-               conv_self_w<2>(fc, dz);
-      const int64_t x = at(g, b.lo0 + i, b.lo1 + jj, b.lo2 + kk);
-      double t = coo[x] - dta * nsum;
-      const double diff = vis[x + voff] * lap4_w(fc, r12xx, r12yy, r12zz);
-      t = t + dta * diff;
-      cstar[x] = t;
-    }
-    __syncthreads();
-  }
-}
-
 __global__ void __launch_bounds__(256)
 calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
               double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double h, double dta) {
@@ -321,18 +121,26 @@ calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
 
 // All three calvel launches over the real cells; component c is written
 // only inside its operator range (solver.py:99-110).
-__global__ void __launch_bounds__(256)
-calvel3_kernel(const double* __restrict__ us, const double* __restrict__ vs,
-               const double* __restrict__ ws, const double* __restrict__ p,
-               double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-               wf_grid g, Box3 b, double dx, double dy, double dz, double dta) {
+__device__ inline void calvel3_body(const double* __restrict__ us, const double* __restrict__ vs,
+                                    const double* __restrict__ ws, const double* __restrict__ p,
+                                    double* __restrict__ u, double* __restrict__ v,
+                                    double* __restrict__ w, const wf_grid& g, const Box3& b,
+                                    Blk B, double dx, double dy, double dz, double dta) {
   int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
+  if (!box_thread(b, B, i, j, k)) return;
   const int64_t x = at(g, i, j, k);
   const double p0 = p[x];
   if (i < g.npl + g.ni - 1) u[x] = us[x] - dta * (p[x + g.si] - p0) / dx;
   if (j < g.npl + g.nj - 1) v[x] = vs[x] - dta * (p[x + g.sj] - p0) / dy;
   if (k < g.npl + g.nk - 1) w[x] = ws[x] - dta * (p[x + 1] - p0) / dz;
+}
+
+__global__ void __launch_bounds__(256)
+calvel3_kernel(const double* __restrict__ us, const double* __restrict__ vs,
+               const double* __restrict__ ws, const double* __restrict__ p,
+               double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+               wf_grid g, Box3 b, double dx, double dy, double dz, double dta) {
+  calvel3_body(us, vs, ws, p, u, v, w, g, b, this_blk(), dx, dy, dz, dta);
 }
 
 // Block-wide reductions for 256-thread (4-wave) workgroups; every thread of
@@ -372,21 +180,26 @@ __device__ inline int red_block() {
   return (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
 }
 
-__device__ inline bool red_rows(const Box3& b, int ich, int& j, int& k, int& ibeg, int& iend) {
-  k = blockIdx.x * BX + threadIdx.x;
-  j = blockIdx.y * BY + threadIdx.y;
-  ibeg = blockIdx.z * ich;
+__device__ inline bool red_rows(const Box3& b, Blk B, int ich, int& j, int& k, int& ibeg,
+                                int& iend) {
+  k = B.x * BX + threadIdx.x;
+  j = B.y * BY + threadIdx.y;
+  ibeg = B.z * ich;
   iend = min(ibeg + ich, b.d0);
   return k < b.d2 && j < b.d1;
 }
+__device__ inline bool red_rows(const Box3& b, int ich, int& j, int& k, int& ibeg, int& iend) {
+  return red_rows(b, this_blk(), ich, j, k, ibeg, iend);
+}
 
-__global__ void __launch_bounds__(256)
-calmas_kernel(const double* __restrict__ u, const double* __restrict__ v,
-              const double* __restrict__ w, double* __restrict__ su, wf_grid g, Box3 b,
-              int ich, double dx, double dy, double dz, double dt) {
+// max|div dV| of one workgroup's rows into part[slot]; su = div / dt.
+__device__ inline void calmas_body(const double* __restrict__ u, const double* __restrict__ v,
+                                   const double* __restrict__ w, double* __restrict__ su,
+                                   const wf_grid& g, const Box3& b, Blk B, int ich, double dx,
+                                   double dy, double dz, double dt, double* part, int slot) {
   int j, k, ibeg, iend;
   double val = 0.0;
-  if (red_rows(b, ich, j, k, ibeg, iend)) {
+  if (red_rows(b, B, ich, j, k, ibeg, iend)) {
     const int64_t sx = g.si, sy = g.sj;
     for (int ii = ibeg; ii < iend; ++ii) {
       const int64_t x = at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k);
@@ -399,7 +212,14 @@ calmas_kernel(const double* __restrict__ u, const double* __restrict__ v,
     }
   }
   const double r = block_max(val);
-  if (block_leader()) g_red_part[red_block()] = r;
+  if (block_leader()) part[slot] = r;
+}
+
+__global__ void __launch_bounds__(256)
+calmas_kernel(const double* __restrict__ u, const double* __restrict__ v,
+              const double* __restrict__ w, double* __restrict__ su, wf_grid g, Box3 b,
+              int ich, double dx, double dy, double dz, double dt) {
+  calmas_body(u, v, w, su, g, b, this_blk(), ich, dx, dy, dz, dt, g_red_part, red_block());
 }
 
 // Fold the partials of nq quantities: bit q of min_mask selects min (else
@@ -428,12 +248,12 @@ red_fold_kernel(double* __restrict__ out, int nblk, int nq, int min_mask, int co
   }
 }
 
-__global__ void __launch_bounds__(256)
-wale_kernel(const double* __restrict__ u, const double* __restrict__ v,
-            const double* __restrict__ w, double* __restrict__ vis, wf_grid g, Box3 b,
-            double dx, double dy, double dz, double rrey, double ls) {
+__device__ inline void wale_body(const double* __restrict__ u, const double* __restrict__ v,
+                                 const double* __restrict__ w, double* __restrict__ vis,
+                                 const wf_grid& g, const Box3& b, Blk B, double dx, double dy,
+                                 double dz, double rrey, double ls) {
   int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
+  if (!box_thread(b, B, i, j, k)) return;
   const int64_t x = at(g, i, j, k);
   const int64_t X = g.si, Y = g.sj, Z = 1;
   // kernels.py:311-327
@@ -483,6 +303,13 @@ wale_kernel(const double* __restrict__ u, const double* __restrict__ v,
 }
 
 __global__ void __launch_bounds__(256)
+wale_kernel(const double* __restrict__ u, const double* __restrict__ v,
+            const double* __restrict__ w, double* __restrict__ vis, wf_grid g, Box3 b,
+            double dx, double dy, double dz, double rrey, double ls) {
+  wale_body(u, v, w, vis, g, b, this_blk(), dx, dy, dz, rrey, ls);
+}
+
+__global__ void __launch_bounds__(256)
 maxabs_kernel(const double* __restrict__ f, wf_grid g, Box3 b, int ich) {
   int j, k, ibeg, iend;
   double val = 0.0;
@@ -522,13 +349,15 @@ dtvisc_kernel(const double* __restrict__ u, const double* __restrict__ v,
   if (block_leader()) g_red_part[red_block()] = r;
 }
 
-__global__ void __launch_bounds__(256)
-checkdt_kernel(const double* __restrict__ u, const double* __restrict__ v,
-               const double* __restrict__ w, const double* __restrict__ vis, wf_grid g,
-               Box3 b, int ich, double dx, double dy, double dz, double rsum) {
+// umax, vmax, wmax, min dt of one workgroup's rows into part[q * stride + slot].
+__device__ inline void checkdt_body(const double* __restrict__ u, const double* __restrict__ v,
+                                    const double* __restrict__ w,
+                                    const double* __restrict__ vis, const wf_grid& g,
+                                    const Box3& b, Blk B, int ich, double dx, double dy,
+                                    double dz, double rsum, double* part, int stride, int slot) {
   int j, k, ibeg, iend;
   double mu = 0.0, mv = 0.0, mw = 0.0, dtv = 1.0e10;
-  if (red_rows(b, ich, j, k, ibeg, iend)) {
+  if (red_rows(b, B, ich, j, k, ibeg, iend)) {
     for (int ii = ibeg; ii < iend; ++ii) {
       const int64_t x = at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k);
       mu = nan_max(mu, fabs(u[x]));
@@ -537,7 +366,6 @@ checkdt_kernel(const double* __restrict__ u, const double* __restrict__ v,
       dtv = nan_min(dtv, dtvisc_cell(u, v, w, vis, x, g, dx, dy, dz, rsum));
     }
   }
-  const int blk = red_block();
   mu = block_max(mu);
   __syncthreads();
   mv = block_max(mv);
@@ -546,34 +374,50 @@ checkdt_kernel(const double* __restrict__ u, const double* __restrict__ v,
   __syncthreads();
   dtv = block_min(dtv);
   if (block_leader()) {
-    g_red_part[0 * RED_MAX_BLOCKS + blk] = mu;
-    g_red_part[1 * RED_MAX_BLOCKS + blk] = mv;
-    g_red_part[2 * RED_MAX_BLOCKS + blk] = mw;
-    g_red_part[3 * RED_MAX_BLOCKS + blk] = dtv;
+    part[0 * stride + slot] = mu;
+    part[1 * stride + slot] = mv;
+    part[2 * stride + slot] = mw;
+    part[3 * stride + slot] = dtv;
   }
 }
 
 __global__ void __launch_bounds__(256)
-add_kernel(double* __restrict__ dst, const double* __restrict__ src, wf_grid g, Box3 b) {
+checkdt_kernel(const double* __restrict__ u, const double* __restrict__ v,
+               const double* __restrict__ w, const double* __restrict__ vis, wf_grid g,
+               Box3 b, int ich, double dx, double dy, double dz, double rsum) {
+  checkdt_body(u, v, w, vis, g, b, this_blk(), ich, dx, dy, dz, rsum, g_red_part,
+               RED_MAX_BLOCKS, red_block());
+}
+
+__device__ inline void add_body(double* __restrict__ dst, const double* __restrict__ src,
+                                const wf_grid& g, const Box3& b, Blk B) {
   int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
+  if (!box_thread(b, B, i, j, k)) return;
   const int64_t x = at(g, i, j, k);
   dst[x] = dst[x] + src[x];
 }
+__global__ void __launch_bounds__(256)
+add_kernel(double* __restrict__ dst, const double* __restrict__ src, wf_grid g, Box3 b) {
+  add_body(dst, src, g, b, this_blk());
+}
 
+__device__ inline void fill_box_body(double* __restrict__ f, double val, const wf_grid& g,
+                                     const Box3& b, Blk B) {
+  int i, j, k;
+  if (!box_thread(b, B, i, j, k)) return;
+  f[at(g, i, j, k)] = val;
+}
 __global__ void __launch_bounds__(256)
 fill_box_kernel(double* __restrict__ f, double val, wf_grid g, Box3 b) {
-  int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
-  f[at(g, i, j, k)] = val;
+  fill_box_body(f, val, g, b, this_blk());
 }
 
 // kernels.py:460-471; uint32 wrap-around arithmetic on the full-array indices.
-__global__ void __launch_bounds__(256)
-noise_kernel(double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-             double amp, uint32_t seed, wf_grid g, Box3 b) {
+__device__ inline void noise_body(double* __restrict__ u, double* __restrict__ v,
+                                  double* __restrict__ w, double amp, uint32_t seed,
+                                  const wf_grid& g, const Box3& b, Blk B) {
   int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
+  if (!box_thread(b, B, i, j, k)) return;
   uint32_t h = ((uint32_t)i * 73856093u) ^ ((uint32_t)j * 19349663u) ^
                ((uint32_t)k * 83492791u) ^ seed;
   h = h ^ (h >> 13);
@@ -584,6 +428,11 @@ noise_kernel(double* __restrict__ u, double* __restrict__ v, double* __restrict_
   u[x] = amp * r;
   v[x] = amp * r;
   w[x] = -amp * r;
+}
+__global__ void __launch_bounds__(256)
+noise_kernel(double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+             double amp, uint32_t seed, wf_grid g, Box3 b) {
+  noise_body(u, v, w, amp, seed, g, b, this_blk());
 }
 
 }  // namespace wf
@@ -693,24 +542,8 @@ int wf_convdiff4th(int comp, const double* u, const double* v, const double* w,
   const Box3 b = npl_box(g, d0, d1, d2);
   if (comp < 0 || comp > 2 || !box_ok(g, b, 3, 3)) return WF_EINVAL;
   if (empty(b)) return 0;
-  const double dta = dt * alpha;
-  const double rx = 12.0 * dx * dx, ry = 12.0 * dy * dy, rz = 12.0 * dz * dz;
-  // i-chunk per workgroup: aim for >= 2048 workgroups, at least 16 planes
-  const int kt = div_up(b.d2, CD_TK), jt = div_up(b.d1, CD_TJ);
-  const int64_t tiles = (int64_t)kt * jt;
-  const int ich = (int)std::max<int64_t>(16, (tiles * b.d0 + 2047) / 2048);
-  const dim3 grid(kt, jt, div_up(b.d0, ich));
-  if (comp == 0)
-    hipLaunchKernelGGL(convdiff_lds_kernel<0>, grid, dim3(256), 0, S_(stream), u, v, w, coo,
-                       vis, cstar, g, b, ich, dx, dy, dz, rx, ry, rz, dta);
-  else if (comp == 1)
-    hipLaunchKernelGGL(convdiff_lds_kernel<1>, grid, dim3(256), 0, S_(stream), u, v, w, coo,
-                       vis, cstar, g, b, ich, dx, dy, dz, rx, ry, rz, dta);
-  else
-    hipLaunchKernelGGL(convdiff_lds_kernel<2>, grid, dim3(256), 0, S_(stream), u, v, w, coo,
-                       vis, cstar, g, b, ich, dx, dy, dz, rx, ry, rz, dta);
-  WF_LAUNCHED();
-  return 0;
+  return convdiff_launch(comp, u, v, w, coo, vis, cstar, g, b, dx, dy, dz, dt * alpha, 0,
+                         S_(stream));
 }
 
 int wf_calvel(int comp, const double* cstar, const double* p, double* c, wf_grid g, int d0,

@@ -41,15 +41,41 @@ inline dim3 box_grid(const Box3& b) {
 }
 inline dim3 box_block() { return dim3(BX, BY, 1); }
 
-__device__ inline bool box_thread(const Box3& b, int& i, int& j, int& k) {
-  k = blockIdx.x * BX + threadIdx.x;
-  j = blockIdx.y * BY + threadIdx.y;
-  i = blockIdx.z;
+// Block coordinates of a workgroup: blockIdx for a single-domain launch, or
+// the decoded work item of a batched (ensemble) launch, so that both run the
+// same kernel body.
+struct Blk {
+  int x, y, z;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b, linear
+// in x-fastest order, runs on XCD b % 8), each XCD with its own L2.  This
+// bijection hands XCD x the contiguous run of logical blocks
+// [x*q + min(x, r), ...), so the blocks that share halo rows and planes
+// share an L2.
+__host__ __device__ inline int xcd_linear(int b, int nb) {
+  const int q = nb / 8, r = nb % 8, x = b % 8;
+  return x * q + (x < r ? x : r) + b / 8;
+}
+__device__ inline Blk this_blk() {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int nb = gx * gy * (int)gridDim.z;
+  const int L = xcd_linear(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+  return Blk{L % gx, (L / gx) % gy, L / (gx * gy)};
+}
+
+__device__ inline bool box_thread(const Box3& b, Blk B, int& i, int& j, int& k) {
+  k = B.x * BX + threadIdx.x;
+  j = B.y * BY + threadIdx.y;
+  i = B.z;
   if (k >= b.d2 || j >= b.d1 || i >= b.d0) return false;
   i += b.lo0;
   j += b.lo1;
   k += b.lo2;
   return true;
+}
+__device__ inline bool box_thread(const Box3& b, int& i, int& j, int& k) {
+  return box_thread(b, this_blk(), i, j, k);
 }
 
 // Non-negative doubles order like their bit patterns, so max over them is an
@@ -77,5 +103,12 @@ __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+
+// Fused conv4th + diff4th of component comp over box b (convdiff.hip); ich =
+// i-planes per workgroup, 0 picks it from the box size.
+int convdiff_launch(int comp, const double* u, const double* v, const double* w,
+                    const double* coo, const double* vis, double* cstar, const wf_grid& g,
+                    const Box3& b, double dx, double dy, double dz, double dta, int ich,
+                    hipStream_t s);
 
 }  // namespace wf

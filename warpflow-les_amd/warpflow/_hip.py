@@ -50,6 +50,7 @@ _SIGS = {
     "wf_memcpy_h2d": [_P, _P, _Z, _P],
     "wf_memcpy_d2h": [_P, _P, _Z, _P],
     "wf_memcpy_d2d": [_P, _P, _Z, _P],
+    "wf_copy": [_P, _P, _Z, _P],
     "wf_memset": [_P, _I, _Z, _P],
     "wf_fill_f64": [_P, _D, _Z, _P],
     "wf_fill_f32": [_P, C.c_float, _Z, _P],

@@ -115,7 +115,7 @@ def copy(dst: DeviceArray, src: DeviceArray) -> None:
     if dst.shape != src.shape or dst.dtype != src.dtype:
         raise ValueError("copy: %r <- %r" % (dst, src))
     if dst.nbytes:
-        _hip.call("wf_memcpy_d2d", dst.ptr, src.ptr, dst.nbytes, _stream)
+        _hip.call("wf_copy", dst.ptr, src.ptr, dst.nbytes, _stream)
 
 
 def synchronize() -> None:
