@@ -54,6 +54,13 @@ def parse(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel table")
     ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
+    ap.add_argument("--workload", default="cavity256", choices=("cavity256", "ensemble64"),
+                    help="cavity256: the headline 256^3 step; ensemble64: 64 uneven cavities "
+                         "batched (mixed MG, graph-captured step) against sequential solves")
+    ap.add_argument("--seq-steps", type=int, default=3,
+                    help="ensemble64: timed steps of the sequential single-cavity baselines")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes instead of tuning them first")
     ap.add_argument("--probe", action="store_true",
                     help="only launch each hot kernel --reps times (hardware-counter runs)")
     ap.add_argument("--mixed-cycles", type=int, default=2,
@@ -82,9 +89,14 @@ def make_cfg(a, precision):
 
 def run_form(a, precision):
     from warpflow import FlowSolver, build_topology
+    from warpflow import autotune
     from warpflow.device import synchronize
     cfg = make_cfg(a, precision)
     sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
+    tuned = None
+    if not a.no_autotune:  # launch shapes for this form, untimed, arrays restored
+        autotune.apply({k: 0 for k in autotune.KNOBS})
+        tuned = autotune.tune_solver(sol, reps=5)["settings"]
     for _ in range(a.warmup):
         sol.step()
     synchronize()
@@ -92,6 +104,7 @@ def run_form(a, precision):
     infos = [sol.step() for _ in range(a.steps)]
     synchronize()
     sec = time.perf_counter() - t0
+    sol.tuned = tuned
     return sol, infos, sec
 
 
@@ -191,7 +204,11 @@ def kernel_table(a, s64, smx):
     T = Timer()
     traffic = load_traffic()
     rows = []
+    from warpflow import autotune
     for name, fn, nbytes, ps64, psmx, per_call in hot_kernels(a, s64, smx):
+        sol = smx if "(fp32)" in name else s64  # that form's tuned launch shapes
+        if getattr(sol, "tuned", None):
+            autotune.apply(sol.tuned)
         us = T.avg_us(fn, a.reps) / per_call
         row = {"kernel": name, "us": round(us, 2), "bytes": int(nbytes),
                "GBps": round(nbytes / us / 1e3, 1),
@@ -254,6 +271,67 @@ def deviation(sa, sb):
     return out
 
 
+def run_ensemble(a):
+    """64 cavities, nx, ny, nz each from {32, 48, 64, 96} and Re uniform in
+    [100, 1000] (seeded), WALE on, 1% noise (seed = member + 1), fixed dt
+    5e-4, 4 multigrid levels: the batched ensemble (mixed MG, graph-captured
+    step) against the same members solved one after another by single-cavity
+    solvers (plain FP64, and mixed)."""
+    from warpflow import FlowSolver, build_topology
+    from warpflow.device import synchronize
+    from warpflow.ensemble import Ensemble, draw_members, member_config
+    specs = draw_members(64)
+    dt = 5e-4
+    ens = Ensemble(specs, dt, mg_levels=4, graph=True)
+    cells = ens.cells()
+    for _ in range(a.warmup):
+        ens.step()
+    t0 = time.perf_counter()
+    iters = [ens.step()["iters"] for _ in range(a.steps)]
+    sec = time.perf_counter() - t0
+    ens.close()
+    out = {"batched": cells * a.steps / sec / 1e6, "batched_ms": 1000.0 * sec / a.steps,
+           "iters": [int(x) for x in np.max(iters, axis=0)]}
+    for form in ("fp64", "mixed"):
+        sols = []
+        for sp in specs:
+            cfg = member_config(sp, dt, mg_levels=4, precision=form)
+            sols.append(FlowSolver(cfg, build_topology(cfg), "hip:0"))
+        for sol in sols:
+            sol.step()
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.seq_steps):
+            for sol in sols:
+                sol.step()
+        synchronize()
+        sec = time.perf_counter() - t0
+        out["sequential_" + form] = cells * a.seq_steps / sec / 1e6
+        out["sequential_%s_ms" % form] = 1000.0 * sec / a.seq_steps
+        del sols
+    print("# ensemble64: %.1f Mcells, batched %.3f ms/step (%.1f Mcell/s); sequential fp64 %.3f "
+          "ms/step (%.1f Mcell/s), mixed %.3f ms/step (%.1f Mcell/s)"
+          % (cells / 1e6, out["batched_ms"], out["batched"], out["sequential_fp64_ms"],
+             out["sequential_fp64"], out["sequential_mixed_ms"], out["sequential_mixed"]),
+          file=sys.stderr, flush=True)
+    line = {"metric": "ensemble aggregate cell-updates/s, 64 uneven cavities",
+            "value": round(out["batched"], 3), "unit": "Mcell/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(out["batched_ms"], 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 (MG V-cycle f32)",
+            "data": "synthetic (seeded member draw and hash-noise initial states)",
+            "config": {"workload": "ensemble64: 64 cavities, n from {32,48,64,96}, Re in "
+                                   "[100,1000], WALE, fixed dt 5e-4, 4 MG levels",
+                       "cells": cells, "graph": True},
+            "sequential_fp64_value": round(out["sequential_fp64"], 3),
+            "sequential_mixed_value": round(out["sequential_mixed"], 3),
+            "speedup_vs_sequential_fp64": round(out["batched"] / out["sequential_fp64"], 3),
+            "speedup_vs_sequential_mixed": round(out["batched"] / out["sequential_mixed"], 3),
+            "max_outer_iters": out["iters"]}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     a = parse(argv)
     if a.gpus != 1:
@@ -262,11 +340,13 @@ def main(argv=None):
         return 2
     if a.probe:
         return probe(a)
+    if a.workload == "ensemble64":
+        return run_ensemble(a)
     forms = [f.strip() for f in a.forms.split(",") if f.strip()]
     res = {}
     for form in forms:
         sol, infos, sec = run_form(a, form)
-        res[form] = {"sol": sol, "infos": infos, "sec": sec,
+        res[form] = {"sol": sol, "infos": infos, "sec": sec, "tuned": sol.tuned,
                      "mcell": a.n ** 3 * a.steps / sec / 1e6,
                      "ms": 1000.0 * sec / a.steps}
         print("# %s: %.3f ms/step, %.1f Mcell/s, iters %s, rmax %.3e" %
@@ -284,6 +364,7 @@ def main(argv=None):
                        "n": a.n, "mg_levels": a.levels,
                        "vcycles_per_solve": {"fp64": 4, "mixed": a.mixed_cycles},
                        "sweeps": "V(6,6)+300 coarsest"}}
+    line["config"]["launch_shapes"] = {f: res[f]["tuned"] for f in res}
     if "fp64" in res:
         line["fp64_value"] = round(res["fp64"]["mcell"], 3)
         line["fp64_ms_per_step"] = round(res["fp64"]["ms"], 4)

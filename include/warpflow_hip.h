@@ -305,6 +305,106 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su,
 int wf_field_mean(const double* f, wf_grid g, double* work, size_t work_bytes,
                   double* out_dev, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Ensembles: many independent cavities of different sizes advanced
+ * together, one launch per operator for all members.  No counterpart in the
+ * reference, which runs one cavity per process (solver.py:29-271); each
+ * batched entry applies the single-domain entry named beside it to every
+ * member, with the same arithmetic.
+ *
+ * Layout.  wf_batch_create packs the members' arrays into pools:
+ * every field (u, v, w, p, ...) is one pool of field_elems doubles with
+ * member m's (n0+6)(n1+6)(n2+6) array at field_off[m]; multigrid level l is
+ * one FP32 pool per level array (phi, rhs, tmp) with member m's
+ * (n0/2^l+2)(n1/2^l+2)(n2/2^l+2) array at level_off[l*nmem+m]; the coarsest
+ * level's operators (wf_mg_coarse_op_build, column-major) share one FP32
+ * pool at cop_off[m] (-1 when the coarsest level has more than 4096 cells).
+ * wf_batch_layout reports the offsets and sizes; the caller allocates pools.
+ *
+ * Masks.  `active` (nullable device int[nmem]) limits a call to the members
+ * still iterating the projection loop; wf_batch_project_mask updates it on
+ * the device, so a whole step can be captured into a graph.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int32_t n0, n1, n2;  /* real cells, each divisible by 2^(nlev-1) */
+  uint32_t seed;       /* init_noise seed */
+  double dx, dy, dz;   /* cell sizes */
+  double rey;          /* 1/Re */
+  double ulid;         /* lid velocity */
+  double ls;           /* WALE length scale cw * delta (as wf_wale's caller forms it) */
+  double noise;        /* initial noise amplitude */
+  double dt;           /* fixed time step */
+} wf_member;
+
+typedef struct wf_batch_s* wf_batch;
+
+int wf_batch_create(const wf_member* members, int nmem, int nlev, wf_batch* out);
+int wf_batch_destroy(wf_batch b);
+int wf_batch_layout(wf_batch b, int64_t* field_off, int64_t* field_elems, int64_t* level_off,
+                    int64_t* level_elems, int64_t* cop_off, int64_t* cop_elems);
+/* wf_init_noise over the real cells of every member. */
+int wf_batch_init_noise(wf_batch b, double* u, double* v, double* w, void* stream);
+/* wf_fill_interior(vis, 1/Re) of every member (WALE off, solver.py:236). */
+int wf_batch_fill_vis(wf_batch b, double* vis, void* stream);
+/* wf_wale. */
+int wf_batch_wale(wf_batch b, const double* u, const double* v, const double* w, double* vis,
+                  void* stream);
+/* wf_checkdt: out4[4m .. 4m+3] = umax, vmax, wmax, dt limit of member m. */
+int wf_batch_checkdt(wf_batch b, const double* u, const double* v, const double* w,
+                     const double* vis, double* out4, void* stream);
+/* wf_bc_fill(which) with param = lid velocity (u), 1/Re (vis), 3 (p mirror). */
+int wf_batch_bc_fill(wf_batch b, double* f, int which, const int* active, void* stream);
+/* wf_convdiff4th over the operator box of component comp. */
+int wf_batch_convdiff(wf_batch b, int comp, const double* u, const double* v, const double* w,
+                      const double* coo, const double* vis, double* cstar, double alpha,
+                      void* stream);
+/* wf_calvel3. */
+int wf_batch_calvel3(wf_batch b, const double* us, const double* vs, const double* ws,
+                     const double* p, double* u, double* v, double* w, double alpha,
+                     const int* active, void* stream);
+/* wf_calmas: rmax[m] of member m. */
+int wf_batch_calmas(wf_batch b, const double* u, const double* v, const double* w, double* su,
+                    double* rmax, const int* active, void* stream);
+/* wf_add_ring. */
+int wf_batch_add_ring(wf_batch b, double* p, const double* pp, const int* active,
+                      void* stream);
+/* wf_mg_solve_mixed with a fixed ncycles (no tolerance test, no host sync):
+ * phi/rhs/tmp = FP32 level pools (nlev each), cop = operator pool or NULL
+ * (NULL: n_coarse sweeps on the coarsest level), flags 2 = remove the mean. */
+int wf_batch_mg_solve_mixed(wf_batch b, float* const* phi, float* const* rhs, float* const* tmp,
+                            const float* cop, const double* su, double* pp, int ncycles,
+                            int n_pre, int n_post, int n_coarse, int flags, const int* active,
+                            void* stream);
+/* Outer-loop bookkeeping of solver.py:185-197 for iteration it: each active
+ * member records iters[m] = it, then leaves when rmax[m] < eps and it > 1;
+ * rmax[m] > 10 sets status[m] = 1 (diverged) and leaves. */
+int wf_batch_project_mask(wf_batch b, const double* rmax, double eps, int it, int* active,
+                          int* iters, int* status, void* stream);
+/* p[0 .. n) = value on the device (graph-safe int fill). */
+int wf_batch_set_int(int* p, int n, int value, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Launch-shape tuning (no counterpart in the reference, whose Warp launches
+ * have no shape parameters).  Process-wide; 0 restores the built-in choice;
+ * results do not depend on any setting.
+ *   WF_TUNE_CD_ICH      i-planes per workgroup of wf_convdiff4th
+ *   WF_TUNE_SWEEP_TILE  fused RBGS sweep tile: 1 = 8 x 64, 2 = 16 x 32 (j x k)
+ *   WF_TUNE_SWEEP_ICH   i-planes per workgroup of the fused RBGS sweep
+ * ---------------------------------------------------------------------- */
+enum { WF_TUNE_CD_ICH = 0, WF_TUNE_SWEEP_TILE = 1, WF_TUNE_SWEEP_ICH = 2, WF_TUNE_COUNT = 3 };
+int wf_tune_set(int knob, int value);
+int wf_tune_get(int knob, int* value);
+
+/* ------------------------------------------------------------------------
+ * HIP graphs: capture the work enqueued on a (non-null) stream between begin
+ * and end, then replay it.  Replaces the reference's per-launch Python
+ * dispatch of a whole step.
+ * ---------------------------------------------------------------------- */
+int wf_graph_begin(void* stream);
+int wf_graph_end(void* stream, void** exec);
+int wf_graph_launch(void* exec, void* stream);
+int wf_graph_destroy(void* exec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,10 @@ int copy(void* d, const void* s, size_t bytes, hipStream_t st) {
   return 0;
 }
 
+int g_tune[WF_TUNE_COUNT] = {};
+
+int tune_get(int knob) { return knob >= 0 && knob < WF_TUNE_COUNT ? g_tune[knob] : 0; }
+
 }  // namespace wf
 
 using namespace wf;
@@ -164,6 +168,46 @@ int wf_event_elapsed_ms(float* ms, void* start, void* stop) {
 const char* wf_error_string(int code) {
   if (code == WF_EINVAL) return "warpflow: invalid argument (shape/range check failed)";
   return hipGetErrorString((hipError_t)code);
+}
+
+int wf_graph_begin(void* stream) {
+  if (!stream) return WF_EINVAL;  // the null stream cannot be captured
+  return (int)hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+}
+
+int wf_graph_end(void* stream, void** exec) {
+  if (!stream || !exec) return WF_EINVAL;
+  hipGraph_t graph = nullptr;
+  WF_TRY(hipStreamEndCapture((hipStream_t)stream, &graph));
+  hipGraphExec_t ge = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return (int)e;
+  *exec = (void*)ge;
+  return 0;
+}
+
+int wf_graph_launch(void* exec, void* stream) {
+  if (!exec) return WF_EINVAL;
+  return (int)hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream);
+}
+
+int wf_graph_destroy(void* exec) {
+  if (!exec) return 0;
+  return (int)hipGraphExecDestroy((hipGraphExec_t)exec);
+}
+
+int wf_tune_set(int knob, int value) {
+  if (knob < 0 || knob >= WF_TUNE_COUNT || value < 0) return WF_EINVAL;
+  if (knob == WF_TUNE_SWEEP_TILE && value > 2) return WF_EINVAL;
+  g_tune[knob] = value;
+  return 0;
+}
+
+int wf_tune_get(int knob, int* value) {
+  if (knob < 0 || knob >= WF_TUNE_COUNT || !value) return WF_EINVAL;
+  *value = g_tune[knob];
+  return 0;
 }
 
 }  // extern "C"

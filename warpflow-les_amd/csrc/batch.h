@@ -22,7 +22,9 @@ constexpr int BATCH_MAX_LEVELS = 8;
 struct MemDev {
   wf_grid g;  // field geometry, 3 ghost layers
   int64_t off;
-  double dx, dy, dz, rey, ulid, cw, noise, dt;
+  double dx, dy, dz, rey, ulid, noise, dt;
+  double ls;    // WALE length scale cw * delta
+  double rsum;  // 1/dx^2 + 1/dy^2 + 1/dz^2 (the checkdt viscous limit)
   uint32_t seed;
   int nlev;
   int ln[BATCH_MAX_LEVELS][3];
@@ -106,6 +108,7 @@ struct Batch {
   int64_t level_elems[BATCH_MAX_LEVELS] = {};
   int64_t cop_elems = 0;
   int max_coarse_cells = 0, max_lds_cells[BATCH_MAX_LEVELS] = {};
+  int lds = 0;  // dynamic LDS bytes available to one workgroup
   BView view() const { return BView{mem_dev, nmem}; }
 };
 
@@ -148,9 +151,13 @@ int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,
                    const double* w, const double* coo, const double* vis, double* cstar,
                    double alpha, hipStream_t s);
 int batch_bc_fill(const Batch& B, double* f, int which, const int* active, hipStream_t s);
+int batch_vcycle(const Batch& B, float* const* phi, float* const* rhs, float* const* tmp,
+                 const float* cop, int npre, int npost, int ncoarse, const int* active,
+                 hipStream_t s);
+int bc_blocks(int axis, int ni, int nj, int nk);
+int batch_mg_init(int* lds_out);
 int batch_mg_mixed(const Batch& B, float* const* phi, float* const* rhs, float* const* tmp,
                    const float* cop, const double* su, double* pp, int ncycles, int npre,
                    int npost, int ncoarse, int flags, const int* active, hipStream_t s);
-int batch_coarse_op_build(const Batch& B, float* cop, int nsweeps, hipStream_t s);
 
 }  // namespace wf

@@ -8,6 +8,7 @@
 // operations of its (a, b) line, which is exactly the reference's per-cell
 // order.  Different axes stay in separate launches because the y fill reads
 // ghosts written by the x fill at other (a, b) positions, and so on.
+#include "batch.h"
 #include "wf_common.h"
 
 namespace wf {
@@ -25,13 +26,9 @@ struct BcOps {
   int32_t n;
 };
 
-__global__ void __launch_bounds__(256)
-bc_axis_kernel(double* __restrict__ f, wf_grid g, int axis, int a0, int na, int b0, int nb,
-               BcOps ops) {
-  const int bb = blockIdx.x * blockDim.x + threadIdx.x;
-  const int aa = blockIdx.y * blockDim.y + threadIdx.y;
-  if (aa >= na || bb >= nb) return;
-  const int a = a0 + aa, b = b0 + bb;
+// The ops of one (a, b) line along `axis`, low side then high side.
+__device__ inline void bc_line(double* __restrict__ f, const wf_grid& g, int axis, int a, int b,
+                               const BcOps& ops) {
   int64_t base, s;
   if (axis == 0) {
     base = at(g, 0, a, b);
@@ -55,6 +52,15 @@ bc_axis_kernel(double* __restrict__ f, wf_grid g, int axis, int a0, int na, int 
       }
     }
   }
+}
+
+__global__ void __launch_bounds__(256)
+bc_axis_kernel(double* __restrict__ f, wf_grid g, int axis, int a0, int na, int b0, int nb,
+               BcOps ops) {
+  const int bb = blockIdx.x * blockDim.x + threadIdx.x;
+  const int aa = blockIdx.y * blockDim.y + threadIdx.y;
+  if (aa >= na || bb >= nb) return;
+  bc_line(f, g, axis, a0 + aa, b0 + bb, ops);
 }
 
 }  // namespace wf
@@ -197,6 +203,56 @@ __host__ __device__ inline bool bc_axis_ops(int which, int axis, int N, int ni, 
       return false;
   }
 }
+
+namespace wf {
+
+// Line blocks of one axis launch per member: 64 x 4 (b, a) blocks over the
+// widest range any fill uses ((perpendicular count + 6) per side).
+__host__ __device__ inline void bc_extent(int axis, int ni, int nj, int nk, int& namax,
+                                          int& nbmax) {
+  namax = (axis == 0 ? nj : ni) + 6;
+  nbmax = (axis == 2 ? nj : nk) + 6;
+}
+int bc_blocks(int axis, int ni, int nj, int nk) {
+  int na, nb;
+  bc_extent(axis, ni, nj, nk, na, nb);
+  return div_up(nb, 64) * div_up(na, 4);
+}
+
+__global__ void __launch_bounds__(256)
+batch_bc_kernel(BView V, Plan P, double* __restrict__ f, int which, int axis,
+                const int* active) {
+  for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
+    const int m = member_of(P.start, V.nmem, it);
+    if (active && !active[m]) continue;
+    const int q = it - P.start[m];
+    const MemDev& M = V.mem[m];
+    const wf_grid g = M.g;
+    const double param = which == 0 ? M.ulid : (which == 3 ? M.rey : (double)g.npl);
+    int a0, na, b0, nb;
+    BcOps o{};
+    bc_axis_ops(which, axis, g.npl, g.ni, g.nj, g.nk, param, a0, na, b0, nb, o);
+    int namax, nbmax;
+    bc_extent(axis, g.ni, g.nj, g.nk, namax, nbmax);
+    const int nx = (nbmax + 63) / 64;
+    const int bb = (q % nx) * 64 + threadIdx.x, aa = (q / nx) * 4 + threadIdx.y;
+    if (aa < na && bb < nb) bc_line(f + M.off, g, axis, a0 + aa, b0 + bb, o);
+  }
+}
+
+// The three axis launches of one fill over every (active) member.
+int batch_bc_fill(const Batch& B, double* f, int which, const int* active, hipStream_t s) {
+  if (which < 0 || which > 5) return WF_EINVAL;
+  for (int axis = 0; axis < 3; ++axis) {
+    const Plan& P = B.plan[PK_BC0 + axis];
+    hipLaunchKernelGGL(batch_bc_kernel, dim3(P.grid), dim3(64, 4), 0, s, B.view(), P, f, which,
+                       axis, active);
+    WF_LAUNCHED();
+  }
+  return 0;
+}
+
+}  // namespace wf
 
 extern "C" {
 

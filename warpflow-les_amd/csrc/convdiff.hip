@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "batch.h"
 #include "wf_common.h"
 
 namespace wf {
@@ -441,7 +442,56 @@ convdiff_kernel(const double* __restrict__ u, const double* __restrict__ v,
   convdiff_body<C>(u, v, w, coo, vis, cstar, g, b, ich, L % kt, (L / kt) % jt, L / (kt * jt), K);
 }
 
+// Ensemble form: a persistent grid over the (member, tile, chunk) items of
+// component C, chunks of BATCH_CD_ICH planes.
+template <int C>
+__global__ void __launch_bounds__(256)
+batch_convdiff_kernel(BView V, Plan P, const double* __restrict__ u,
+                      const double* __restrict__ v, const double* __restrict__ w,
+                      const double* __restrict__ coo, const double* __restrict__ vis,
+                      double* __restrict__ cstar, double alpha) {
+  for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
+    const int m = member_of(P.start, V.nmem, it);
+    const int q = it - P.start[m];
+    const MemDev& M = V.mem[m];
+    const wf_grid g = M.g;
+    int d0, d1, d2;
+    op_dims(C, g.ni, g.nj, g.nk, d0, d1, d2);
+    const Box3 b{g.npl, g.npl, g.npl, d0, d1, d2};
+    const int kt = (d2 + TK - 1) / TK, jt = (d1 + TJ - 1) / TJ;
+    Coefs K;  // as convdiff_launch forms them on the host
+    K.r24x = 1.0 / (24.0 * M.dx);
+    K.r24y = 1.0 / (24.0 * M.dy);
+    K.r24z = 1.0 / (24.0 * M.dz);
+    K.ir12x = 1.0 / (12.0 * M.dx * M.dx);
+    K.ir12y = 1.0 / (12.0 * M.dy * M.dy);
+    K.ir12z = 1.0 / (12.0 * M.dz * M.dz);
+    K.dta = M.dt * alpha;
+    convdiff_body<C>(u + M.off, v + M.off, w + M.off, coo + M.off, vis + M.off, cstar + M.off,
+                     g, b, BATCH_CD_ICH, q % kt, (q / kt) % jt, q / (kt * jt), K);
+    __syncthreads();  // LDS of this item is free for the next
+  }
+}
+
 }  // namespace cdv
+
+int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,
+                   const double* w, const double* coo, const double* vis, double* cstar,
+                   double alpha, hipStream_t s) {
+  using namespace cdv;
+  const Plan& P = B.plan[PK_CD0 + comp];
+  if (comp == 0)
+    hipLaunchKernelGGL(batch_convdiff_kernel<0>, dim3(P.grid), dim3(NT), 0, s, B.view(), P, u,
+                       v, w, coo, vis, cstar, alpha);
+  else if (comp == 1)
+    hipLaunchKernelGGL(batch_convdiff_kernel<1>, dim3(P.grid), dim3(NT), 0, s, B.view(), P, u,
+                       v, w, coo, vis, cstar, alpha);
+  else
+    hipLaunchKernelGGL(batch_convdiff_kernel<2>, dim3(P.grid), dim3(NT), 0, s, B.view(), P, u,
+                       v, w, coo, vis, cstar, alpha);
+  WF_LAUNCHED();
+  return 0;
+}
 
 // Launch of the fused operator on box b (checked by the caller).
 int convdiff_launch(int comp, const double* u, const double* v, const double* w,
@@ -450,6 +500,7 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
                     hipStream_t s) {
   using namespace cdv;
   const int kt = div_up(b.d2, TK), jt = div_up(b.d1, TJ);
+  if (ich <= 0) ich = tune_get(WF_TUNE_CD_ICH);
   if (ich <= 0) {  // aim for >= 2048 workgroups with at least 16 planes each
     const int64_t tiles = (int64_t)kt * jt;
     ich = (int)std::max<int64_t>(16, (tiles * b.d0 + 2047) / 2048);

@@ -20,6 +20,7 @@
 // reference could observe it.
 #include <algorithm>
 
+#include "batch.h"
 #include "mg_common.h"
 
 namespace wf {
@@ -513,6 +514,187 @@ coeff_kernel(MgDev d, T c0, T c1, T c2, T* aw, T* ae, T* as, T* an, T* ab, T* at
   ap[x] = c.ap;
 }
 
+// ---------------------------------------------------------------------------
+// Batched (ensemble) multigrid kernels over FP32 level pools: level l of
+// member m is the (n_l+2)^3 array at pool + loff[l], spacing h * 2^l.
+// ---------------------------------------------------------------------------
+__device__ inline MgDev level_dev(const MemDev& M, int l) {
+  MgDev d;
+  d.n0 = M.ln[l][0];
+  d.n1 = M.ln[l][1];
+  d.n2 = M.ln[l][2];
+  d.s1 = d.n2 + 2;
+  d.s0 = (int64_t)(d.n1 + 2) * d.s1;
+  return d;
+}
+// 1/h^2 of level l as the single-domain path forms it: (float)(1.0 / (h*h))
+// with h = spacing * 2^l.
+__device__ inline void level_coef(const MemDev& M, int l, float& c0, float& c1, float& c2) {
+  const double f = (double)(1 << l);
+  const double h0 = M.dx * f, h1 = M.dy * f, h2 = M.dz * f;
+  c0 = (float)(1.0 / (h0 * h0));
+  c1 = (float)(1.0 / (h1 * h1));
+  c2 = (float)(1.0 / (h2 * h2));
+}
+
+template <bool SYNC, class F>
+__device__ inline void mg_items(const Plan& P, const BView& V, const int* active, F&& f) {
+  for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
+    const int m = member_of(P.start, V.nmem, it);
+    if (active && !active[m]) continue;
+    f(it - P.start[m], V.mem[m]);
+    if (SYNC) __syncthreads();
+  }
+}
+
+// restrict level lc-1 -> lc
+__global__ void __launch_bounds__(256)
+batch_restrict_kernel(BView V, Plan P, int lc, const float* phif, const float* rhsf,
+                      float* phic, float* rhsc, const int* active) {
+  mg_items<false>(P, V, active, [&](int q, const MemDev& M) {
+    const MgDev f = level_dev(M, lc - 1), c = level_dev(M, lc);
+    float c0, c1, c2;
+    level_coef(M, lc - 1, c0, c1, c2);
+    restrict_body<float>(phif + M.loff[lc - 1], rhsf + M.loff[lc - 1], f, c0, c1, c2,
+                         phic + M.loff[lc], rhsc + M.loff[lc], c, cells_blk(q, c.n1, c.n2));
+  });
+}
+
+// prolong level lf+1 -> lf
+__global__ void __launch_bounds__(256)
+batch_prolong_kernel(BView V, Plan P, int lf, const float* phic, float* phif,
+                     const int* active) {
+  mg_items<false>(P, V, active, [&](int q, const MemDev& M) {
+    const MgDev c = level_dev(M, lf + 1), f = level_dev(M, lf);
+    prolong_body<float>(phic + M.loff[lf + 1], c, phif + M.loff[lf], f,
+                        cells_blk(q, f.n1, f.n2));
+  });
+}
+
+// one fused full sweep pout = S(pin) on level l, tile shape per member
+__global__ void __launch_bounds__(256)
+batch_sweep_kernel(BView V, Plan P, int l, const float* pin, float* pout, const float* rhs,
+                   const int* active) {
+  mg_items<true>(P, V, active, [&](int q, const MemDev& M) {
+    const MgDev d = level_dev(M, l);
+    float c0, c1, c2;
+    level_coef(M, l, c0, c1, c2);
+    const int TJ = sweep_tj(d.n2), TK = sweep_tk(d.n2);
+    const int kt = (d.n2 + TK - 1) / TK, jt = (d.n1 + TJ - 1) / TJ;
+    const Blk B{q % kt, (q / kt) % jt, q / (kt * jt)};
+    const int64_t o = M.loff[l];
+    if (TK == 64)
+      sweep_body<float, 8, 64>(pin + o, pout + o, rhs + o, d, c0, c1, c2, BATCH_SWEEP_ICH, B);
+    else
+      sweep_body<float, 16, 32>(pin + o, pout + o, rhs + o, d, c0, c1, c2, BATCH_SWEEP_ICH, B);
+  });
+}
+
+// nsweeps RBGS sweeps of level l, one LDS-resident workgroup per member
+__global__ void __launch_bounds__(1024)
+batch_relax_lds_kernel(BView V, int l, float* phi, const float* rhs, int nsweeps,
+                       const int* active) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int m = blockIdx.x;
+  if (active && !active[m]) return;  // uniform per workgroup
+  const MemDev& M = V.mem[m];
+  const MgDev d = level_dev(M, l);
+  float c0, c1, c2;
+  level_coef(M, l, c0, c1, c2);
+  relax_lds_body<float, true>(phi + M.loff[l], rhs + M.loff[l], d, c0, c1, c2, nsweeps,
+                              reinterpret_cast<float*>(smem_raw));
+}
+
+// phi = M rhs on the coarsest level l, one workgroup per member
+__global__ void __launch_bounds__(1024)
+batch_coarse_apply_kernel(BView V, int l, const float* cop, float* phi, const float* rhs,
+                          const int* active) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int m = blockIdx.x;
+  if (active && !active[m]) return;
+  const MemDev& M = V.mem[m];
+  coarse_op_apply_body<float>(cop + M.coff, phi + M.loff[l], rhs + M.loff[l], level_dev(M, l),
+                              reinterpret_cast<float*>(smem_raw));
+}
+
+int batch_lds_bytes() {
+  int dev = 0, v = 65536;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    v = 65536;
+  return v;
+}
+
+// Dynamic-LDS limits of the batched single-workgroup kernels, set once when a
+// batch is created (never inside a captured step).
+int batch_mg_init(int* lds_out) {
+  const int lds = batch_lds_bytes();
+  WF_TRY(hipFuncSetAttribute((const void*)batch_relax_lds_kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  WF_TRY(hipFuncSetAttribute((const void*)batch_coarse_apply_kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  *lds_out = lds;
+  return 0;
+}
+
+// One batched V-cycle of the FP32 levels (vcycle_any's order).
+int batch_vcycle(const Batch& B, float* const* phi, float* const* rhs, float* const* tmp,
+                 const float* cop, int npre, int npost, int ncoarse, const int* active,
+                 hipStream_t s) {
+  const int nlev = B.nlev;
+  const BView V = B.view();
+  const int lds = B.lds;
+  auto relax = [&](int l, int n) -> int {
+    if (n <= 0) return 0;
+    const size_t cells = (size_t)B.max_lds_cells[l];
+    if (cells <= 4096 && 2 * cells * sizeof(float) <= (size_t)lds) {
+      hipLaunchKernelGGL(batch_relax_lds_kernel, dim3(B.nmem), dim3(1024),
+                         2 * cells * sizeof(float), s, V, l, phi[l], (const float*)rhs[l], n,
+                         active);
+      WF_LAUNCHED();
+      return 0;
+    }
+    if (n % 2) return WF_EINVAL;  // the fused form runs sweeps in pairs
+    const Plan& P = B.plan[PK_SWEEP0 + l];
+    for (int q = 0; q < n; q += 2) {
+      hipLaunchKernelGGL(batch_sweep_kernel, dim3(P.grid), dim3(256), 0, s, V, P, l,
+                         (const float*)phi[l], tmp[l], (const float*)rhs[l], active);
+      WF_LAUNCHED();
+      hipLaunchKernelGGL(batch_sweep_kernel, dim3(P.grid), dim3(256), 0, s, V, P, l,
+                         (const float*)tmp[l], phi[l], (const float*)rhs[l], active);
+      WF_LAUNCHED();
+    }
+    return 0;
+  };
+  for (int l = 0; l < nlev; ++l) {
+    if (l > 0) {
+      const Plan& P = B.plan[PK_LCELLS0 + l];
+      hipLaunchKernelGGL(batch_restrict_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, l,
+                         (const float*)phi[l - 1], (const float*)rhs[l - 1], phi[l], rhs[l],
+                         active);
+      WF_LAUNCHED();
+    }
+    if (l > 0 && l == nlev - 1 && cop) {
+      hipLaunchKernelGGL(batch_coarse_apply_kernel, dim3(B.nmem), dim3(1024),
+                         (size_t)B.max_coarse_cells * sizeof(float), s, V, l, cop, phi[l],
+                         (const float*)rhs[l], active);
+      WF_LAUNCHED();
+      continue;
+    }
+    const int r = relax(l, l < nlev - 1 ? npre : ncoarse);
+    if (r) return r;
+  }
+  for (int l = nlev - 2; l >= 0; --l) {
+    const Plan& P = B.plan[PK_LCELLS0 + l];
+    hipLaunchKernelGGL(batch_prolong_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, l,
+                       (const float*)phi[l + 1], phi[l], active);
+    WF_LAUNCHED();
+    const int r = relax(l, npost);
+    if (r) return r;
+  }
+  return 0;
+}
+
 }  // namespace wf
 
 using namespace wf;
@@ -598,7 +780,9 @@ int launch_sweep(const wf_mg_level& L, const void* pin, void* pout, hipStream_t 
   inv_h2(L, c0, c1, c2);
   const int kt = div_up(d.n2, TK), jt = div_up(d.n1, TJ);
   const int64_t tiles = (int64_t)kt * jt;
-  const int ich = (int)std::max<int64_t>(4, std::min<int64_t>(16, tiles * d.n0 / 2048));
+  const int tuned = tune_get(WF_TUNE_SWEEP_ICH);
+  const int ich = tuned > 0 ? tuned
+                            : (int)std::max<int64_t>(4, std::min<int64_t>(16, tiles * d.n0 / 2048));
   const dim3 grid(kt, jt, div_up(d.n0, ich));
   hipLaunchKernelGGL((sweep_kernel<T, TJ, TK>), grid, dim3(256), 0, s, (const T*)pin, (T*)pout,
                      (const T*)L.rhs, d, c0, c1, c2, ich);
@@ -608,8 +792,9 @@ int launch_sweep(const wf_mg_level& L, const void* pin, void* pout, hipStream_t 
 
 template <class T>
 int sweep_t(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
-  return L.n2 > 32 ? launch_sweep<T, 8, 64>(L, pin, pout, s)
-                   : launch_sweep<T, 16, 32>(L, pin, pout, s);
+  const int tile = tune_get(WF_TUNE_SWEEP_TILE);
+  const bool wide = tile == 1 || (tile != 2 && L.n2 > 32);  // 8 x 64, else 16 x 32
+  return wide ? launch_sweep<T, 8, 64>(L, pin, pout, s) : launch_sweep<T, 16, 32>(L, pin, pout, s);
 }
 
 template <class T>

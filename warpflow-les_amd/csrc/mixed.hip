@@ -19,6 +19,9 @@
 //
 // Sums are deterministic: fixed-shape block reductions into a partials
 // buffer, then one workgroup folds the partials in a fixed order.
+#include <algorithm>
+
+#include "batch.h"
 #include "mg_common.h"
 
 namespace wf {
@@ -223,6 +226,132 @@ level_resmax_kernel(const T* __restrict__ phi, const T* __restrict__ rhs, MgDev 
   if ((t & 63) == 0) part[t >> 6] = m;
   __syncthreads();
   if (t == 0) atomic_max_nonneg(out, nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3])));
+}
+
+// ---------------------------------------------------------------------------
+// Batched (ensemble) mixed solve: the single-domain kernels' bodies over
+// every active member, level 0 of member m at level pool + loff[0].
+// ---------------------------------------------------------------------------
+__device__ inline MgDev level0_dev(const MemDev& M) {
+  MgDev d;
+  d.n0 = M.ln[0][0];
+  d.n1 = M.ln[0][1];
+  d.n2 = M.ln[0][2];
+  d.s1 = d.n2 + 2;
+  d.s0 = (int64_t)(d.n1 + 2) * d.s1;
+  return d;
+}
+
+template <bool SYNC, class F>
+__device__ inline void mx_items(const Plan& P, const BView& V, const int* active, F&& f) {
+  for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
+    const int m = member_of(P.start, V.nmem, it);
+    if (active && !active[m]) continue;
+    f(m, it - P.start[m], V.mem[m]);
+    if (SYNC) __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256)
+batch_defect_kernel(BView V, Plan P, const double* su, const double* pp, float* rhs0,
+                    const int* active) {
+  mx_items<false>(P, V, active, [&](int, int q, const MemDev& M) {
+    const wf_grid& g = M.g;
+    defect_body(su + M.off, pp ? pp + M.off : nullptr, g, rhs0 + M.loff[0], level0_dev(M),
+                1.0 / (M.dx * M.dx), 1.0 / (M.dy * M.dy), 1.0 / (M.dz * M.dz), nullptr,
+                nullptr, cells_blk(q, g.nj, g.nk));
+  });
+}
+
+__global__ void __launch_bounds__(256)
+batch_update_kernel(BView V, Plan P, double* pp, const float* phi0, const int* active) {
+  mx_items<false>(P, V, active, [&](int, int q, const MemDev& M) {
+    update_body(pp + M.off, M.g, phi0 + M.loff[0], level0_dev(M), cells_blk(q, M.g.nj, M.g.nk));
+  });
+}
+
+__global__ void __launch_bounds__(256)
+batch_partials_kernel(BView V, Plan P, const double* f, double* part, const int* active) {
+  mx_items<true>(P, V, active, [&](int, int q, const MemDev& M) {
+    field_partials_body(f + M.off, M.g, part + M.sum0, cells_blk(q, M.g.nj, M.g.nk));
+  });
+}
+
+__global__ void __launch_bounds__(1024)
+batch_sum_fold_kernel(BView V, const double* part, double* out, const int* active) {
+  const int m = blockIdx.x;
+  if (active && !active[m]) return;
+  const MemDev& M = V.mem[m];
+  const wf_grid& g = M.g;
+  fold_body(part + M.sum0, cells_blocks(g.ni, g.nj, g.nk), 1.0 / ((double)g.ni * g.nj * g.nk),
+            out + m);
+}
+
+__global__ void __launch_bounds__(256)
+batch_shift_kernel(BView V, Plan P, double* pp, const double* mean, const int* active) {
+  mx_items<false>(P, V, active, [&](int m, int q, const MemDev& M) {
+    shift_body(pp + M.off, M.g, mean + m, cells_blk(q, M.g.nj, M.g.nk));
+  });
+}
+
+__global__ void __launch_bounds__(256)
+batch_field_ring_kernel(BView V, double* pp, const int* active) {
+  const int m = blockIdx.x;
+  if (active && !active[m]) return;
+  const MemDev& M = V.mem[m];
+  const wf_grid& g = M.g;
+  const int nface = 2 * (g.nj * g.nk + g.ni * g.nk + g.ni * g.nj);
+  for (int t = threadIdx.x; t < nface; t += blockDim.x) field_ring_body(pp + M.off, g, t);
+}
+
+__global__ void __launch_bounds__(256) batch_fill_kernel(double* p, int64_t n, double v) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = v;
+}
+__global__ void __launch_bounds__(256) batch_fillf_kernel(float* p, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = 0.0f;
+}
+
+// wf_mg_solve_mixed over every active member with a fixed cycle count (the
+// plain iteration x <- x + M(b - A x), no tolerance test: no host sync).
+int batch_mg_mixed(const Batch& B, float* const* phi, float* const* rhs, float* const* tmp,
+                   const float* cop, const double* su, double* pp, int ncycles, int npre,
+                   int npost, int ncoarse, int flags, const int* active, hipStream_t s) {
+  const BView V = B.view();
+  const Plan& P = B.plan[PK_CELLS];
+  const int fb = (int)std::min<int64_t>((B.field_elems + 255) / 256, 8192);
+  hipLaunchKernelGGL(batch_fill_kernel, dim3(fb), dim3(256), 0, s, pp, B.field_elems, 0.0);
+  WF_LAUNCHED();
+  for (int c = 0; c < ncycles; ++c) {
+    hipLaunchKernelGGL(batch_defect_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, su,
+                       c == 0 ? (const double*)nullptr : (const double*)pp, rhs[0], active);
+    WF_LAUNCHED();
+    const int lb = (int)std::min<int64_t>((B.level_elems[0] + 255) / 256, 8192);
+    hipLaunchKernelGGL(batch_fillf_kernel, dim3(lb), dim3(256), 0, s, phi[0], B.level_elems[0]);
+    WF_LAUNCHED();
+    const int r = batch_vcycle(B, phi, rhs, tmp, cop, npre, npost, ncoarse, active, s);
+    if (r) return r;
+    hipLaunchKernelGGL(batch_update_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, pp,
+                       (const float*)phi[0], active);
+    WF_LAUNCHED();
+  }
+  if (flags & MIX_RESULT_MEAN) {
+    hipLaunchKernelGGL(batch_partials_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P,
+                       (const double*)pp, B.sum_part, active);
+    WF_LAUNCHED();
+    hipLaunchKernelGGL(batch_sum_fold_kernel, dim3(B.nmem), dim3(1024), 0, s, V,
+                       (const double*)B.sum_part, B.scal, active);
+    WF_LAUNCHED();
+    hipLaunchKernelGGL(batch_shift_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, pp,
+                       (const double*)B.scal, active);
+    WF_LAUNCHED();
+  }
+  hipLaunchKernelGGL(batch_field_ring_kernel, dim3(B.nmem), dim3(256), 0, s, V, pp, active);
+  WF_LAUNCHED();
+  return 0;
 }
 
 }  // namespace wf

@@ -8,6 +8,7 @@
 // the FP64 oracle to a few ulps.
 #include <algorithm>
 
+#include "batch.h"
 #include "wf_common.h"
 
 namespace wf {
@@ -22,6 +23,10 @@ __device__ inline double I4(const double* __restrict__ f, int64_t c, int64_t s) 
 // (-s[+] + 27 s[0] - 27 s[-] + s[--]) / (24 h)
 __device__ inline double D4(double sp, double sn, double sm, double smm, double h) {
   return (-sp + 27.0 * sn - 27.0 * sm + smm) / (24.0 * h);
+}
+// The same with the division replaced by r24h = 1 / (24 h) formed once.
+__device__ inline double D4r(double sp, double sn, double sm, double smm, double r24h) {
+  return (-sp + 27.0 * sn - 27.0 * sm + smm) * r24h;
 }
 
 // d(c c)/dx_a for the component c living on axis a (kernels.py:64-69,
@@ -112,11 +117,11 @@ diff4th_kernel(const double* __restrict__ c, const double* __restrict__ vis,
 
 __global__ void __launch_bounds__(256)
 calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
-              double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double h, double dta) {
+              double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double rh, double dta) {
   int i, j, k;
   if (!box_thread(b, i, j, k)) return;
   const int64_t x = at(g, i, j, k);
-  c[x] = cstar[x] - dta * (p[x + s] - p[x]) / h;
+  c[x] = cstar[x] - dta * (p[x + s] - p[x]) * rh;
 }
 
 // All three calvel launches over the real cells; component c is written
@@ -125,22 +130,22 @@ __device__ inline void calvel3_body(const double* __restrict__ us, const double*
                                     const double* __restrict__ ws, const double* __restrict__ p,
                                     double* __restrict__ u, double* __restrict__ v,
                                     double* __restrict__ w, const wf_grid& g, const Box3& b,
-                                    Blk B, double dx, double dy, double dz, double dta) {
+                                    Blk B, double rdx, double rdy, double rdz, double dta) {
   int i, j, k;
   if (!box_thread(b, B, i, j, k)) return;
   const int64_t x = at(g, i, j, k);
   const double p0 = p[x];
-  if (i < g.npl + g.ni - 1) u[x] = us[x] - dta * (p[x + g.si] - p0) / dx;
-  if (j < g.npl + g.nj - 1) v[x] = vs[x] - dta * (p[x + g.sj] - p0) / dy;
-  if (k < g.npl + g.nk - 1) w[x] = ws[x] - dta * (p[x + 1] - p0) / dz;
+  if (i < g.npl + g.ni - 1) u[x] = us[x] - dta * (p[x + g.si] - p0) * rdx;
+  if (j < g.npl + g.nj - 1) v[x] = vs[x] - dta * (p[x + g.sj] - p0) * rdy;
+  if (k < g.npl + g.nk - 1) w[x] = ws[x] - dta * (p[x + 1] - p0) * rdz;
 }
 
 __global__ void __launch_bounds__(256)
 calvel3_kernel(const double* __restrict__ us, const double* __restrict__ vs,
                const double* __restrict__ ws, const double* __restrict__ p,
                double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-               wf_grid g, Box3 b, double dx, double dy, double dz, double dta) {
-  calvel3_body(us, vs, ws, p, u, v, w, g, b, this_blk(), dx, dy, dz, dta);
+               wf_grid g, Box3 b, double rdx, double rdy, double rdz, double dta) {
+  calvel3_body(us, vs, ws, p, u, v, w, g, b, this_blk(), rdx, rdy, rdz, dta);
 }
 
 // Block-wide reductions for 256-thread (4-wave) workgroups; every thread of
@@ -197,17 +202,19 @@ __device__ inline void calmas_body(const double* __restrict__ u, const double* _
                                    const double* __restrict__ w, double* __restrict__ su,
                                    const wf_grid& g, const Box3& b, Blk B, int ich, double dx,
                                    double dy, double dz, double dt, double* part, int slot) {
+  const double r24x = 1.0 / (24.0 * dx), r24y = 1.0 / (24.0 * dy), r24z = 1.0 / (24.0 * dz);
+  const double rdt = 1.0 / dt;
   int j, k, ibeg, iend;
   double val = 0.0;
   if (red_rows(b, B, ich, j, k, ibeg, iend)) {
     const int64_t sx = g.si, sy = g.sj;
     for (int ii = ibeg; ii < iend; ++ii) {
       const int64_t x = at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k);
-      const double dudx = D4(u[x + sx], u[x], u[x - sx], u[x - 2 * sx], dx);
-      const double dvdy = D4(v[x + sy], v[x], v[x - sy], v[x - 2 * sy], dy);
-      const double dwdz = D4(w[x + 1], w[x], w[x - 1], w[x - 2], dz);
+      const double dudx = D4r(u[x + sx], u[x], u[x - sx], u[x - 2 * sx], r24x);
+      const double dvdy = D4r(v[x + sy], v[x], v[x - sy], v[x - 2 * sy], r24y);
+      const double dwdz = D4r(w[x + 1], w[x], w[x - 1], w[x - 2], r24z);
       const double fact = dudx + dvdy + dwdz;
-      su[x] = fact / dt;
+      su[x] = fact * rdt;
       val = nan_max(val, fabs(fact * dx * dy * dz));
     }
   }
@@ -252,20 +259,21 @@ __device__ inline void wale_body(const double* __restrict__ u, const double* __r
                                  const double* __restrict__ w, double* __restrict__ vis,
                                  const wf_grid& g, const Box3& b, Blk B, double dx, double dy,
                                  double dz, double rrey, double ls) {
+  const double rdx = 1.0 / dx, rdy = 1.0 / dy, rdz = 1.0 / dz;
   int i, j, k;
   if (!box_thread(b, B, i, j, k)) return;
   const int64_t x = at(g, i, j, k);
   const int64_t X = g.si, Y = g.sj, Z = 1;
   // kernels.py:311-327
-  const double dudx = (u[x] - u[x - X]) / dx;
-  const double dudy = (0.25 * (u[x + Y] + u[x - X + Y] - u[x - Y] - u[x - X - Y])) / dy;
-  const double dudz = (0.25 * (u[x + Z] + u[x - X + Z] - u[x - Z] - u[x - X - Z])) / dz;
-  const double dvdx = (0.25 * (v[x + X] + v[x + X - Y] - v[x - X] - v[x - X - Y])) / dx;
-  const double dvdy = (v[x] - v[x - Y]) / dy;
-  const double dvdz = (0.25 * (v[x + Z] + v[x - Y + Z] - v[x - Z] - v[x - Y - Z])) / dz;
-  const double dwdx = (0.25 * (w[x + X] + w[x + X - Z] - w[x - X] - w[x - X - Z])) / dx;
-  const double dwdy = (0.25 * (w[x + Y] + w[x + Y - Z] - w[x - Y] - w[x - Y - Z])) / dy;
-  const double dwdz = (w[x] - w[x - Z]) / dz;
+  const double dudx = (u[x] - u[x - X]) * rdx;
+  const double dudy = (0.25 * (u[x + Y] + u[x - X + Y] - u[x - Y] - u[x - X - Y])) * rdy;
+  const double dudz = (0.25 * (u[x + Z] + u[x - X + Z] - u[x - Z] - u[x - X - Z])) * rdz;
+  const double dvdx = (0.25 * (v[x + X] + v[x + X - Y] - v[x - X] - v[x - X - Y])) * rdx;
+  const double dvdy = (v[x] - v[x - Y]) * rdy;
+  const double dvdz = (0.25 * (v[x + Z] + v[x - Y + Z] - v[x - Z] - v[x - Y - Z])) * rdz;
+  const double dwdx = (0.25 * (w[x + X] + w[x + X - Z] - w[x - X] - w[x - X - Z])) * rdx;
+  const double dwdy = (0.25 * (w[x + Y] + w[x + Y - Z] - w[x - Y] - w[x - Y - Z])) * rdy;
+  const double dwdz = (w[x] - w[x - Z]) * rdz;
   // kernels.py:330-338, g2zx and g2zy with the reference's last terms
   const double g2xx = dudx * dudx + dudy * dvdx + dudz * dwdx;
   const double g2yy = dvdx * dudy + dvdy * dvdy + dvdz * dwdy;
@@ -435,6 +443,191 @@ noise_kernel(double* __restrict__ u, double* __restrict__ v, double* __restrict_
   noise_body(u, v, w, amp, seed, g, b, this_blk());
 }
 
+// ---------------------------------------------------------------------------
+// Batched (ensemble) kernels: a persistent grid walks the work items of a
+// plan; each item is one block of one member, run through the same body as
+// the single-domain kernel on that member's arrays.  Members whose `active`
+// flag is 0 are skipped.  SYNC: a barrier after every item, for bodies that
+// use LDS.
+// ---------------------------------------------------------------------------
+template <bool SYNC, class F>
+__device__ inline void for_items(const Plan& P, const BView& V, const int* active, F&& f) {
+  for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
+    const int m = member_of(P.start, V.nmem, it);
+    if (active && !active[m]) continue;
+    f(m, it - P.start[m], V.mem[m]);
+    if (SYNC) __syncthreads();
+  }
+}
+
+__device__ inline Box3 cells_box(const wf_grid& g) {
+  return Box3{g.npl, g.npl, g.npl, g.ni, g.nj, g.nk};
+}
+
+__global__ void __launch_bounds__(256)
+batch_noise_kernel(BView V, Plan P, double* u, double* v, double* w) {
+  for_items<false>(P, V, nullptr, [&](int, int q, const MemDev& M) {
+    const Box3 b = cells_box(M.g);
+    noise_body(u + M.off, v + M.off, w + M.off, M.noise, M.seed, M.g, b,
+               cells_blk(q, b.d1, b.d2));
+  });
+}
+
+__global__ void __launch_bounds__(256) batch_fill_vis_kernel(BView V, Plan P, double* vis) {
+  for_items<false>(P, V, nullptr, [&](int, int q, const MemDev& M) {
+    const Box3 b = cells_box(M.g);
+    fill_box_body(vis + M.off, M.rey, M.g, b, cells_blk(q, b.d1, b.d2));
+  });
+}
+
+__global__ void __launch_bounds__(256)
+batch_wale_kernel(BView V, Plan P, const double* u, const double* v, const double* w,
+                  double* vis) {
+  for_items<false>(P, V, nullptr, [&](int, int q, const MemDev& M) {
+    const Box3 b = cells_box(M.g);
+    wale_body(u + M.off, v + M.off, w + M.off, vis + M.off, M.g, b, cells_blk(q, b.d1, b.d2),
+              M.dx, M.dy, M.dz, M.rey, M.ls);
+  });
+}
+
+__global__ void __launch_bounds__(256)
+batch_calvel3_kernel(BView V, Plan P, const double* us, const double* vs, const double* ws,
+                     const double* p, double* u, double* v, double* w, double alpha,
+                     const int* active) {
+  for_items<false>(P, V, active, [&](int, int q, const MemDev& M) {
+    const Box3 b = cells_box(M.g);
+    calvel3_body(us + M.off, vs + M.off, ws + M.off, p + M.off, u + M.off, v + M.off,
+                 w + M.off, M.g, b, cells_blk(q, b.d1, b.d2), 1.0 / M.dx, 1.0 / M.dy, 1.0 / M.dz,
+                 M.dt * alpha);
+  });
+}
+
+__global__ void __launch_bounds__(256)
+batch_add_ring_kernel(BView V, Plan P, double* dst, const double* src, const int* active) {
+  for_items<false>(P, V, active, [&](int, int q, const MemDev& M) {
+    const wf_grid& g = M.g;
+    const Box3 b{g.npl - 1, g.npl - 1, g.npl - 1, g.ni + 2, g.nj + 2, g.nk + 2};
+    add_body(dst + M.off, src + M.off, g, b, cells_blk(q, b.d1, b.d2));
+  });
+}
+
+__global__ void __launch_bounds__(256)
+batch_calmas_kernel(BView V, Plan P, const double* u, const double* v, const double* w,
+                    double* su, double* part, const int* active) {
+  for_items<true>(P, V, active, [&](int, int q, const MemDev& M) {
+    const Box3 b = cells_box(M.g);
+    calmas_body(u + M.off, v + M.off, w + M.off, su + M.off, M.g, b, cells_blk(q, b.d1, b.d2),
+                BATCH_RED_ICH, M.dx, M.dy, M.dz, M.dt, part, M.red0 + q);
+  });
+}
+
+__global__ void __launch_bounds__(256)
+batch_checkdt_kernel(BView V, Plan P, const double* u, const double* v, const double* w,
+                     const double* vis, double* part, int stride) {
+  for_items<true>(P, V, nullptr, [&](int, int q, const MemDev& M) {
+    const Box3 b = cells_box(M.g);
+    checkdt_body(u + M.off, v + M.off, w + M.off, vis + M.off, M.g, b,
+                 cells_blk(q, b.d1, b.d2), BATCH_RED_ICH, M.dx, M.dy, M.dz, M.rsum, part,
+                 stride, M.red0 + q);
+  });
+}
+
+// One workgroup per member folds its reduction partials: out[m*nq + q] for
+// quantity q (bit q of min_mask: min, else max).
+__global__ void __launch_bounds__(1024)
+batch_red_fold_kernel(BView V, const double* part, int stride, int nq, int min_mask,
+                      double* out, const int* active) {
+  const int m = blockIdx.x;
+  if (active && !active[m]) return;  // uniform per workgroup
+  const MemDev& M = V.mem[m];
+  const int n = red_blocks(M.g.ni, M.g.nj, M.g.nk);
+  __shared__ double sh[16];
+  const int t = threadIdx.x;
+  for (int q = 0; q < nq; ++q) {
+    const bool mn = (min_mask >> q) & 1;
+    double val = mn ? 1.0e10 : 0.0;
+    for (int p = t; p < n; p += 1024) {
+      const double y = part[q * stride + M.red0 + p];
+      val = mn ? nan_min(val, y) : nan_max(val, y);
+    }
+    val = mn ? wave_min(val) : wave_max(val);
+    if ((t & 63) == 0) sh[t >> 6] = val;
+    __syncthreads();
+    if (t == 0) {
+      double r = sh[0];
+      for (int w = 1; w < 16; ++w) r = mn ? nan_min(r, sh[w]) : nan_max(r, sh[w]);
+      out[m * nq + q] = r;
+    }
+    __syncthreads();
+  }
+}
+
+int batch_noise(const Batch& B, double* u, double* v, double* w, hipStream_t s) {
+  const Plan& P = B.plan[PK_CELLS];
+  hipLaunchKernelGGL(batch_noise_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, u, v, w);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_fill_vis(const Batch& B, double* vis, hipStream_t s) {
+  const Plan& P = B.plan[PK_CELLS];
+  hipLaunchKernelGGL(batch_fill_vis_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, vis);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_wale(const Batch& B, const double* u, const double* v, const double* w, double* vis,
+               hipStream_t s) {
+  const Plan& P = B.plan[PK_CELLS];
+  hipLaunchKernelGGL(batch_wale_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, u, v, w,
+                     vis);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_calvel3(const Batch& B, const double* us, const double* vs, const double* ws,
+                  const double* p, double* u, double* v, double* w, double alpha,
+                  const int* active, hipStream_t s) {
+  const Plan& P = B.plan[PK_CELLS];
+  hipLaunchKernelGGL(batch_calvel3_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, us, vs,
+                     ws, p, u, v, w, alpha, active);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_add_ring(const Batch& B, double* p, const double* pp, const int* active,
+                   hipStream_t s) {
+  const Plan& P = B.plan[PK_RING];
+  hipLaunchKernelGGL(batch_add_ring_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, p, pp,
+                     active);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_calmas(const Batch& B, const double* u, const double* v, const double* w, double* su,
+                 double* rmax, const int* active, hipStream_t s) {
+  const Plan& P = B.plan[PK_RED];
+  hipLaunchKernelGGL(batch_calmas_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, u, v, w,
+                     su, B.red_part, active);
+  WF_LAUNCHED();
+  hipLaunchKernelGGL(batch_red_fold_kernel, dim3(B.nmem), dim3(1024), 0, s, B.view(),
+                     (const double*)B.red_part, B.nred, 1, 0, rmax, active);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_checkdt(const Batch& B, const double* u, const double* v, const double* w,
+                  const double* vis, double* out4, hipStream_t s) {
+  const Plan& P = B.plan[PK_RED];
+  hipLaunchKernelGGL(batch_checkdt_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, u, v, w,
+                     vis, B.red_part, B.nred);
+  WF_LAUNCHED();
+  hipLaunchKernelGGL(batch_red_fold_kernel, dim3(B.nmem), dim3(1024), 0, s, B.view(),
+                     (const double*)B.red_part, B.nred, 4, 8, out4, (const int*)nullptr);
+  WF_LAUNCHED();
+  return 0;
+}
+
 }  // namespace wf
 
 using namespace wf;
@@ -553,7 +746,7 @@ int wf_calvel(int comp, const double* cstar, const double* p, double* c, wf_grid
   if (empty(b)) return 0;
   const int64_t s = comp == 0 ? g.si : (comp == 1 ? g.sj : 1);
   hipLaunchKernelGGL(calvel_kernel, box_grid(b), box_block(), 0, S_(stream), cstar, p, c, g,
-                     b, s, h, dt * alpha);
+                     b, s, 1.0 / h, dt * alpha);
   WF_LAUNCHED();
   return 0;
 }
@@ -564,7 +757,7 @@ int wf_calvel3(const double* ustar, const double* vstar, const double* wstar,
   const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
   if (!box_ok(g, b, 0, 1)) return WF_EINVAL;
   hipLaunchKernelGGL(calvel3_kernel, box_grid(b), box_block(), 0, S_(stream), ustar, vstar,
-                     wstar, p, u, v, w, g, b, dx, dy, dz, dt * alpha);
+                     wstar, p, u, v, w, g, b, 1.0 / dx, 1.0 / dy, 1.0 / dz, dt * alpha);
   WF_LAUNCHED();
   return 0;
 }

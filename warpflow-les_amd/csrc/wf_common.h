@@ -104,6 +104,11 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Tuning knobs (api.hip): process-wide launch-shape choices set by the
+// autotuner; 0 = the built-in choice.  They change launch shapes only, never
+// results.
+int tune_get(int knob);
+
 // Fused conv4th + diff4th of component comp over box b (convdiff.hip); ich =
 // i-planes per workgroup, 0 picks it from the box size.
 int convdiff_launch(int comp, const double* u, const double* v, const double* w,

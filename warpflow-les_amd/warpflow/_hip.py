@@ -22,6 +22,14 @@ class WfGrid(C.Structure):
                 ("npl", C.c_int32), ("sj", C.c_int64), ("si", C.c_int64)]
 
 
+class WfMember(C.Structure):
+    """wf_member: one cavity of an ensemble."""
+    _fields_ = [("n0", C.c_int32), ("n1", C.c_int32), ("n2", C.c_int32),
+                ("seed", C.c_uint32), ("dx", C.c_double), ("dy", C.c_double),
+                ("dz", C.c_double), ("rey", C.c_double), ("ulid", C.c_double),
+                ("ls", C.c_double), ("noise", C.c_double), ("dt", C.c_double)]
+
+
 class WfMgLevel(C.Structure):
     """wf_mg_level: one multigrid level (arrays with a one-cell ring)."""
     _fields_ = [("phi", C.c_void_p), ("rhs", C.c_void_p),
@@ -55,6 +63,27 @@ _SIGS = {
     "wf_fill_f64": [_P, _D, _Z, _P],
     "wf_fill_f32": [_P, C.c_float, _Z, _P],
     "wf_synchronize": [],
+    "wf_batch_create": [_P, _I, _I, C.POINTER(C.c_void_p)],
+    "wf_batch_destroy": [_P],
+    "wf_batch_layout": [_P, _P, _P, _P, _P, _P, _P],
+    "wf_batch_init_noise": [_P, _P, _P, _P, _P],
+    "wf_batch_fill_vis": [_P, _P, _P],
+    "wf_batch_wale": [_P, _P, _P, _P, _P, _P],
+    "wf_batch_checkdt": [_P, _P, _P, _P, _P, _P, _P],
+    "wf_batch_bc_fill": [_P, _P, _I, _P, _P],
+    "wf_batch_convdiff": [_P, _I, _P, _P, _P, _P, _P, _P, C.c_double, _P],
+    "wf_batch_calvel3": [_P, _P, _P, _P, _P, _P, _P, _P, C.c_double, _P, _P],
+    "wf_batch_calmas": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "wf_batch_add_ring": [_P, _P, _P, _P, _P],
+    "wf_batch_mg_solve_mixed": [_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
+    "wf_batch_project_mask": [_P, _P, C.c_double, _I, _P, _P, _P, _P],
+    "wf_batch_set_int": [_P, _I, _I, _P],
+    "wf_tune_set": [_I, _I],
+    "wf_tune_get": [_I, C.POINTER(C.c_int)],
+    "wf_graph_begin": [_P],
+    "wf_graph_end": [_P, C.POINTER(C.c_void_p)],
+    "wf_graph_launch": [_P, _P],
+    "wf_graph_destroy": [_P],
     "wf_stream_create": [C.POINTER(C.c_void_p)],
     "wf_stream_destroy": [_P],
     "wf_stream_synchronize": [_P],

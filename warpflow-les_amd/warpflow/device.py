@@ -82,8 +82,25 @@ class DeviceArray:
             self.assign(np.full(self.shape, value, dtype=self.dtype))
         return self
 
+    def view(self, offset, shape) -> "DeviceArray":
+        """A C-ordered array of `shape` over this array's memory from element
+        `offset` on (no copy; keeps this array alive)."""
+        shape = tuple(int(s) for s in shape)
+        size = int(np.prod(shape, dtype=np.int64))
+        if offset < 0 or offset + size > self.size:
+            raise ValueError("view [%d, %d) outside %r" % (offset, offset + size, self))
+        v = DeviceArray.__new__(DeviceArray)
+        v.shape, v.dtype, v.size = shape, self.dtype, size
+        v.nbytes = size * self.dtype.itemsize
+        v.ptr = self.ptr + int(offset) * self.dtype.itemsize
+        v.device = self.device
+        v._base = self
+        return v
+
     # -------------------------------------------------------------------- #
     def __del__(self):
+        if getattr(self, "_base", None) is not None:
+            return  # a view: the base array owns the memory
         p = getattr(self, "ptr", None)
         if p:
             try:
