@@ -94,9 +94,8 @@ def run_form(a, precision):
     cfg = make_cfg(a, precision)
     sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
     tuned = None
-    if not a.no_autotune:  # launch shapes for this form, untimed, arrays restored
-        autotune.apply({k: 0 for k in autotune.KNOBS})
-        tuned = autotune.tune_solver(sol, reps=5)["settings"]
+    if not a.no_autotune:  # launch shapes for this grid and form, untimed, arrays restored
+        tuned = [[k, list(n3), v] for k, n3, v in autotune.tune_solver(sol, reps=5)["entries"]]
     for _ in range(a.warmup):
         sol.step()
     synchronize()
@@ -204,11 +203,7 @@ def kernel_table(a, s64, smx):
     T = Timer()
     traffic = load_traffic()
     rows = []
-    from warpflow import autotune
     for name, fn, nbytes, ps64, psmx, per_call in hot_kernels(a, s64, smx):
-        sol = smx if "(fp32)" in name else s64  # that form's tuned launch shapes
-        if getattr(sol, "tuned", None):
-            autotune.apply(sol.tuned)
         us = T.avg_us(fn, a.reps) / per_call
         row = {"kernel": name, "us": round(us, 2), "bytes": int(nbytes),
                "GBps": round(nbytes / us / 1e3, 1),
@@ -364,7 +359,7 @@ def main(argv=None):
                        "n": a.n, "mg_levels": a.levels,
                        "vcycles_per_solve": {"fp64": 4, "mixed": a.mixed_cycles},
                        "sweeps": "V(6,6)+300 coarsest"}}
-    line["config"]["launch_shapes"] = {f: res[f]["tuned"] for f in res}
+    line["config"]["launch_shapes"] = {f: res[f]["tuned"] for f in res}  # [knob, n3, value]
     if "fp64" in res:
         line["fp64_value"] = round(res["fp64"]["mcell"], 3)
         line["fp64_ms_per_step"] = round(res["fp64"]["ms"], 4)

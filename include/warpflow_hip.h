@@ -385,15 +385,26 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
 
 /* ------------------------------------------------------------------------
  * Launch-shape tuning (no counterpart in the reference, whose Warp launches
- * have no shape parameters).  Process-wide; 0 restores the built-in choice;
- * results do not depend on any setting.
- *   WF_TUNE_CD_ICH      i-planes per workgroup of wf_convdiff4th
- *   WF_TUNE_SWEEP_TILE  fused RBGS sweep tile: 1 = 8 x 64, 2 = 16 x 32 (j x k)
- *   WF_TUNE_SWEEP_ICH   i-planes per workgroup of the fused RBGS sweep
+ * have no shape parameters).  A setting applies to launches of exactly the
+ * shape it names -- the real cells (n0, n1, n2) of the field (convdiff) or of
+ * the multigrid level (sweeps) -- so tuning one grid never changes another.
+ * value 0 removes the entry (the built-in choice); results never depend on
+ * any setting.  At most 64 entries.
+ *   WF_TUNE_CD_ICH              i-planes per workgroup of wf_convdiff4th
+ *   WF_TUNE_SWEEP_TILE_F64/F32  fused RBGS sweep tile: 1 = 8 x 64, 2 = 16 x 32
+ *   WF_TUNE_SWEEP_ICH_F64/F32   i-planes per workgroup of the fused sweep
  * ---------------------------------------------------------------------- */
-enum { WF_TUNE_CD_ICH = 0, WF_TUNE_SWEEP_TILE = 1, WF_TUNE_SWEEP_ICH = 2, WF_TUNE_COUNT = 3 };
-int wf_tune_set(int knob, int value);
-int wf_tune_get(int knob, int* value);
+enum {
+  WF_TUNE_CD_ICH = 0,
+  WF_TUNE_SWEEP_TILE_F64 = 1,
+  WF_TUNE_SWEEP_TILE_F32 = 2,
+  WF_TUNE_SWEEP_ICH_F64 = 3,
+  WF_TUNE_SWEEP_ICH_F32 = 4,
+  WF_TUNE_COUNT = 5
+};
+int wf_tune_set(int knob, int n0, int n1, int n2, int value);
+int wf_tune_get(int knob, int n0, int n1, int n2, int* value);
+int wf_tune_reset(void);
 
 /* ------------------------------------------------------------------------
  * HIP graphs: capture the work enqueued on a (non-null) stream between begin

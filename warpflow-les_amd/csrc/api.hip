@@ -58,9 +58,25 @@ int copy(void* d, const void* s, size_t bytes, hipStream_t st) {
   return 0;
 }
 
-int g_tune[WF_TUNE_COUNT] = {};
+struct TuneEntry {
+  int knob, n0, n1, n2, value;
+};
+constexpr int TUNE_MAX = 64;
+TuneEntry g_tune[TUNE_MAX];
+int g_ntune = 0;
 
-int tune_get(int knob) { return knob >= 0 && knob < WF_TUNE_COUNT ? g_tune[knob] : 0; }
+int tune_find(int knob, int n0, int n1, int n2) {
+  for (int q = 0; q < g_ntune; ++q) {
+    const TuneEntry& e = g_tune[q];
+    if (e.knob == knob && e.n0 == n0 && e.n1 == n1 && e.n2 == n2) return q;
+  }
+  return -1;
+}
+
+int tune_get(int knob, int n0, int n1, int n2) {
+  const int q = tune_find(knob, n0, n1, n2);
+  return q < 0 ? 0 : g_tune[q].value;
+}
 
 }  // namespace wf
 
@@ -197,16 +213,33 @@ int wf_graph_destroy(void* exec) {
   return (int)hipGraphExecDestroy((hipGraphExec_t)exec);
 }
 
-int wf_tune_set(int knob, int value) {
-  if (knob < 0 || knob >= WF_TUNE_COUNT || value < 0) return WF_EINVAL;
-  if (knob == WF_TUNE_SWEEP_TILE && value > 2) return WF_EINVAL;
-  g_tune[knob] = value;
+int wf_tune_set(int knob, int n0, int n1, int n2, int value) {
+  if (knob < 0 || knob >= WF_TUNE_COUNT || value < 0 || n0 <= 0 || n1 <= 0 || n2 <= 0)
+    return WF_EINVAL;
+  if ((knob == WF_TUNE_SWEEP_TILE_F64 || knob == WF_TUNE_SWEEP_TILE_F32) && value > 2)
+    return WF_EINVAL;
+  const int q = tune_find(knob, n0, n1, n2);
+  if (value == 0) {  // remove
+    if (q >= 0) g_tune[q] = g_tune[--g_ntune];
+    return 0;
+  }
+  if (q >= 0) {
+    g_tune[q].value = value;
+    return 0;
+  }
+  if (g_ntune >= TUNE_MAX) return WF_EINVAL;
+  g_tune[g_ntune++] = TuneEntry{knob, n0, n1, n2, value};
   return 0;
 }
 
-int wf_tune_get(int knob, int* value) {
+int wf_tune_get(int knob, int n0, int n1, int n2, int* value) {
   if (knob < 0 || knob >= WF_TUNE_COUNT || !value) return WF_EINVAL;
-  *value = g_tune[knob];
+  *value = tune_get(knob, n0, n1, n2);
+  return 0;
+}
+
+int wf_tune_reset(void) {
+  g_ntune = 0;
   return 0;
 }
 

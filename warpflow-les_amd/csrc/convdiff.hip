@@ -500,7 +500,7 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
                     hipStream_t s) {
   using namespace cdv;
   const int kt = div_up(b.d2, TK), jt = div_up(b.d1, TJ);
-  if (ich <= 0) ich = tune_get(WF_TUNE_CD_ICH);
+  if (ich <= 0) ich = tune_get(WF_TUNE_CD_ICH, g.ni, g.nj, g.nk);
   if (ich <= 0) {  // aim for >= 2048 workgroups with at least 16 planes each
     const int64_t tiles = (int64_t)kt * jt;
     ich = (int)std::max<int64_t>(16, (tiles * b.d0 + 2047) / 2048);

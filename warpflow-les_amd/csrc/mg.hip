@@ -120,8 +120,9 @@ __device__ inline void restrict_body(const T* __restrict__ phif, const T* __rest
 template <class T>
 __global__ void __launch_bounds__(256)
 restrict_kernel(const T* __restrict__ phif, const T* __restrict__ rhsf, MgDev f, T c0, T c1,
-                T c2, T* __restrict__ phic, T* __restrict__ rhsc, MgDev c) {
-  restrict_body(phif, rhsf, f, c0, c1, c2, phic, rhsc, c, this_blk());
+                T c2, T* __restrict__ phic, T* __restrict__ rhsc, MgDev c, int ich) {
+  for_planes(c.n0, ich,
+             [&](Blk B) { restrict_body(phif, rhsf, f, c0, c1, c2, phic, rhsc, c, B); });
 }
 
 // _prolong (multigrid.py:122-146) + fine refill (multigrid.py:321).
@@ -159,8 +160,8 @@ __device__ inline void prolong_body(const T* __restrict__ phic, const MgDev& c,
 }
 template <class T>
 __global__ void __launch_bounds__(256)
-prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev f) {
-  prolong_body(phic, c, phif, f, this_blk());
+prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev f, int ich) {
+  for_planes(f.n0, ich, [&](Blk B) { prolong_body(phic, c, phif, f, B); });
 }
 
 // One full RBGS sweep (colour 0, then colour 1, then the ring refill) out of
@@ -780,7 +781,8 @@ int launch_sweep(const wf_mg_level& L, const void* pin, void* pout, hipStream_t 
   inv_h2(L, c0, c1, c2);
   const int kt = div_up(d.n2, TK), jt = div_up(d.n1, TJ);
   const int64_t tiles = (int64_t)kt * jt;
-  const int tuned = tune_get(WF_TUNE_SWEEP_ICH);
+  const int tuned = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP_ICH_F64 : WF_TUNE_SWEEP_ICH_F32,
+                             d.n0, d.n1, d.n2);
   const int ich = tuned > 0 ? tuned
                             : (int)std::max<int64_t>(4, std::min<int64_t>(16, tiles * d.n0 / 2048));
   const dim3 grid(kt, jt, div_up(d.n0, ich));
@@ -792,7 +794,8 @@ int launch_sweep(const wf_mg_level& L, const void* pin, void* pout, hipStream_t 
 
 template <class T>
 int sweep_t(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
-  const int tile = tune_get(WF_TUNE_SWEEP_TILE);
+  const int tile = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP_TILE_F64 : WF_TUNE_SWEEP_TILE_F32,
+                            L.n0, L.n1, L.n2);
   const bool wide = tile == 1 || (tile != 2 && L.n2 > 32);  // 8 x 64, else 16 x 32
   return wide ? launch_sweep<T, 8, 64>(L, pin, pout, s) : launch_sweep<T, 16, 32>(L, pin, pout, s);
 }
@@ -828,8 +831,10 @@ int restrict_t(const wf_mg_level& F, const wf_mg_level& C, hipStream_t s) {
   const MgDev f = mg_dev(F), c = mg_dev(C);
   T c0, c1, c2;
   inv_h2(F, c0, c1, c2);
-  hipLaunchKernelGGL(restrict_kernel<T>, cell_grid(c.n0, c.n1, c.n2), dim3(64, 4), 0, s,
-                     (const T*)F.phi, (const T*)F.rhs, f, c0, c1, c2, (T*)C.phi, (T*)C.rhs, c);
+  const int ich = plane_chunk(c.n0, c.n1, c.n2);
+  hipLaunchKernelGGL(restrict_kernel<T>, chunk_grid(c.n0, c.n1, c.n2, ich), dim3(64, 4), 0, s,
+                     (const T*)F.phi, (const T*)F.rhs, f, c0, c1, c2, (T*)C.phi, (T*)C.rhs, c,
+                     ich);
   WF_LAUNCHED();
   return 0;
 }
@@ -837,8 +842,9 @@ int restrict_t(const wf_mg_level& F, const wf_mg_level& C, hipStream_t s) {
 template <class T>
 int prolong_t(const wf_mg_level& C, const wf_mg_level& F, hipStream_t s) {
   const MgDev f = mg_dev(F), c = mg_dev(C);
-  hipLaunchKernelGGL(prolong_kernel<T>, cell_grid(f.n0, f.n1, f.n2), dim3(64, 4), 0, s,
-                     (const T*)C.phi, c, (T*)F.phi, f);
+  const int ich = plane_chunk(f.n0, f.n1, f.n2);
+  hipLaunchKernelGGL(prolong_kernel<T>, chunk_grid(f.n0, f.n1, f.n2, ich), dim3(64, 4), 0, s,
+                     (const T*)C.phi, c, (T*)F.phi, f, ich);
   WF_LAUNCHED();
   return 0;
 }

@@ -33,6 +33,8 @@ constexpr int WORK_HEAD = 4;  // work[0] mean, work[1] max|r|, work[2] sum, work
 inline dim3 field_grid(const wf_grid& g) {
   return dim3(div_up(g.nk, 64), div_up(g.nj, 4), g.ni);
 }
+inline int field_ich(const wf_grid& g) { return plane_chunk(g.ni, g.nj, g.nk); }
+inline dim3 field_cgrid(const wf_grid& g) { return chunk_grid(g.ni, g.nj, g.nk, field_ich(g)); }
 inline int64_t field_blocks(const wf_grid& g) {
   const dim3 b = field_grid(g);
   return (int64_t)b.x * b.y * b.z;
@@ -64,8 +66,9 @@ __device__ inline void field_partials_body(const double* __restrict__ f, const w
   if (threadIdx.x == 0 && threadIdx.y == 0) partials[block_linear(g, B)] = s;
 }
 __global__ void __launch_bounds__(256)
-field_partials_kernel(const double* __restrict__ f, wf_grid g, double* __restrict__ partials) {
-  field_partials_body(f, g, partials, this_blk());
+field_partials_kernel(const double* __restrict__ f, wf_grid g, double* __restrict__ partials,
+                      int ich) {
+  for_planes<true>(g.ni, ich, [&](Blk B) { field_partials_body(f, g, partials, B); });
 }
 
 // out = scale * sum(partials[0..n)), folded by one 1024-thread workgroup in
@@ -126,8 +129,9 @@ __device__ inline void defect_body(const double* __restrict__ b, const double* _
 __global__ void __launch_bounds__(256)
 defect_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_grid g,
               float* __restrict__ r32, MgDev d, double c0, double c1, double c2,
-              const double* __restrict__ mean, double* __restrict__ rmax) {
-  defect_body(b, x, g, r32, d, c0, c1, c2, mean, rmax, this_blk());
+              const double* __restrict__ mean, double* __restrict__ rmax, int ich) {
+  for_planes<true>(g.ni, ich,
+                   [&](Blk B) { defect_body(b, x, g, r32, d, c0, c1, c2, mean, rmax, B); });
 }
 
 // out = max(partials[0..n)), one 1024-thread workgroup, fixed order.
@@ -156,8 +160,8 @@ __device__ inline void update_body(double* __restrict__ x, const wf_grid& g,
   x[q] = x[q] + (double)e[mg_at(d, i, j, k)];
 }
 __global__ void __launch_bounds__(256)
-update_kernel(double* __restrict__ x, wf_grid g, const float* __restrict__ e, MgDev d) {
-  update_body(x, g, e, d, this_blk());
+update_kernel(double* __restrict__ x, wf_grid g, const float* __restrict__ e, MgDev d, int ich) {
+  for_planes(g.ni, ich, [&](Blk B) { update_body(x, g, e, d, B); });
 }
 
 __device__ inline void shift_body(double* __restrict__ x, const wf_grid& g,
@@ -168,8 +172,8 @@ __device__ inline void shift_body(double* __restrict__ x, const wf_grid& g,
   x[q] = x[q] - *mean;
 }
 __global__ void __launch_bounds__(256)
-shift_kernel(double* __restrict__ x, wf_grid g, const double* __restrict__ mean) {
-  shift_body(x, g, mean, this_blk());
+shift_kernel(double* __restrict__ x, wf_grid g, const double* __restrict__ mean, int ich) {
+  for_planes(g.ni, ich, [&](Blk B) { shift_body(x, g, mean, B); });
 }
 
 // First ghost layer of the field: faces = mirror of the adjacent real cell,
@@ -373,7 +377,8 @@ MgDev dev_of(const wf_mg_level& L) {
 int field_sum(const double* f, const wf_grid& g, double scale, double* work, double* out,
               hipStream_t s) {
   double* partials = work + WORK_HEAD;
-  hipLaunchKernelGGL(field_partials_kernel, field_grid(g), dim3(64, 4), 0, s, f, g, partials);
+  hipLaunchKernelGGL(field_partials_kernel, field_cgrid(g), dim3(64, 4), 0, s, f, g, partials,
+                     field_ich(g));
   WF_LAUNCHED();
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(1024), 0, s, partials, field_blocks(g), scale,
                      out);
@@ -443,10 +448,10 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
   for (int c = 0; c <= max_cycles; ++c) {
     const bool last_check = c == max_cycles;
     if (last_check && !tol_mode) break;
-    hipLaunchKernelGGL(defect_kernel, field_grid(g), dim3(64, 4), 0, s, su, c == 0 ? nullptr : pp,
+    hipLaunchKernelGGL(defect_kernel, field_cgrid(g), dim3(64, 4), 0, s, su, c == 0 ? nullptr : pp,
                        g, (float*)L0.rhs, d0, c0, c1, c2,
                        (flags & MIX_DEFECT_MEAN) ? mean : nullptr,
-                       tol_mode ? work + WORK_HEAD : nullptr);
+                       tol_mode ? work + WORK_HEAD : nullptr, field_ich(g));
     WF_LAUNCHED();
     if (tol_mode) {
       hipLaunchKernelGGL(max_fold_kernel, dim3(1), dim3(1024), 0, s, work + WORK_HEAD,
@@ -463,15 +468,16 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
     if (r) return r;
     r = mg_vcycle_levels(levels, nlev, n_pre, n_post, n_coarse, s);
     if (r) return r;
-    hipLaunchKernelGGL(update_kernel, field_grid(g), dim3(64, 4), 0, s, pp, g,
-                       (const float*)L0.phi, d0);
+    hipLaunchKernelGGL(update_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g,
+                       (const float*)L0.phi, d0, field_ich(g));
     WF_LAUNCHED();
     ++cycles;
   }
   if (flags & MIX_RESULT_MEAN) {
     r = field_sum(pp, g, 1.0 / ncell, work, xmean, s);
     if (r) return r;
-    hipLaunchKernelGGL(shift_kernel, field_grid(g), dim3(64, 4), 0, s, pp, g, xmean);
+    hipLaunchKernelGGL(shift_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g, xmean,
+                       field_ich(g));
     WF_LAUNCHED();
   }
   const int nface = 2 * (g.nj * g.nk + g.ni * g.nk + g.ni * g.nj);

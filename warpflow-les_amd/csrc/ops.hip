@@ -96,32 +96,39 @@ __global__ void __launch_bounds__(256)
 conv4th_kernel(const double* __restrict__ u, const double* __restrict__ v,
                const double* __restrict__ w, const double* __restrict__ coo,
                double* __restrict__ cstar, wf_grid g, Box3 b, double dx, double dy,
-               double dz, double dta) {
-  int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
-  const int64_t x = at(g, i, j, k);
-  cstar[x] = coo[x] - dta * conv_sum<C>(u, v, w, x, g, dx, dy, dz);
+               double dz, double dta, int ich) {
+  for_planes(b.d0, ich, [&](Blk B) {
+    int i, j, k;
+    if (!box_thread(b, B, i, j, k)) return;
+    const int64_t x = at(g, i, j, k);
+    cstar[x] = coo[x] - dta * conv_sum<C>(u, v, w, x, g, dx, dy, dz);
+  });
 }
 
 template <int C>
 __global__ void __launch_bounds__(256)
 diff4th_kernel(const double* __restrict__ c, const double* __restrict__ vis,
                double* __restrict__ cstar, wf_grid g, Box3 b, double r12xx,
-               double r12yy, double r12zz, double dta) {
-  int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
-  const int64_t x = at(g, i, j, k);
-  const double diff = vis[x + vis_offset<C>(g)] * lap4(c, x, g, r12xx, r12yy, r12zz);
-  cstar[x] = cstar[x] + dta * diff;
+               double r12yy, double r12zz, double dta, int ich) {
+  for_planes(b.d0, ich, [&](Blk B) {
+    int i, j, k;
+    if (!box_thread(b, B, i, j, k)) return;
+    const int64_t x = at(g, i, j, k);
+    const double diff = vis[x + vis_offset<C>(g)] * lap4(c, x, g, r12xx, r12yy, r12zz);
+    cstar[x] = cstar[x] + dta * diff;
+  });
 }
 
 __global__ void __launch_bounds__(256)
 calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
-              double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double rh, double dta) {
-  int i, j, k;
-  if (!box_thread(b, i, j, k)) return;
-  const int64_t x = at(g, i, j, k);
-  c[x] = cstar[x] - dta * (p[x + s] - p[x]) * rh;
+              double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double rh, double dta,
+              int ich) {
+  for_planes(b.d0, ich, [&](Blk B) {
+    int i, j, k;
+    if (!box_thread(b, B, i, j, k)) return;
+    const int64_t x = at(g, i, j, k);
+    c[x] = cstar[x] - dta * (p[x + s] - p[x]) * rh;
+  });
 }
 
 // All three calvel launches over the real cells; component c is written
@@ -144,8 +151,9 @@ __global__ void __launch_bounds__(256)
 calvel3_kernel(const double* __restrict__ us, const double* __restrict__ vs,
                const double* __restrict__ ws, const double* __restrict__ p,
                double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-               wf_grid g, Box3 b, double rdx, double rdy, double rdz, double dta) {
-  calvel3_body(us, vs, ws, p, u, v, w, g, b, this_blk(), rdx, rdy, rdz, dta);
+               wf_grid g, Box3 b, double rdx, double rdy, double rdz, double dta, int ich) {
+  for_planes(b.d0, ich,
+             [&](Blk B) { calvel3_body(us, vs, ws, p, u, v, w, g, b, B, rdx, rdy, rdz, dta); });
 }
 
 // Block-wide reductions for 256-thread (4-wave) workgroups; every thread of
@@ -313,8 +321,8 @@ __device__ inline void wale_body(const double* __restrict__ u, const double* __r
 __global__ void __launch_bounds__(256)
 wale_kernel(const double* __restrict__ u, const double* __restrict__ v,
             const double* __restrict__ w, double* __restrict__ vis, wf_grid g, Box3 b,
-            double dx, double dy, double dz, double rrey, double ls) {
-  wale_body(u, v, w, vis, g, b, this_blk(), dx, dy, dz, rrey, ls);
+            double dx, double dy, double dz, double rrey, double ls, int ich) {
+  for_planes(b.d0, ich, [&](Blk B) { wale_body(u, v, w, vis, g, b, B, dx, dy, dz, rrey, ls); });
 }
 
 __global__ void __launch_bounds__(256)
@@ -405,8 +413,8 @@ __device__ inline void add_body(double* __restrict__ dst, const double* __restri
   dst[x] = dst[x] + src[x];
 }
 __global__ void __launch_bounds__(256)
-add_kernel(double* __restrict__ dst, const double* __restrict__ src, wf_grid g, Box3 b) {
-  add_body(dst, src, g, b, this_blk());
+add_kernel(double* __restrict__ dst, const double* __restrict__ src, wf_grid g, Box3 b, int ich) {
+  for_planes(b.d0, ich, [&](Blk B) { add_body(dst, src, g, b, B); });
 }
 
 __device__ inline void fill_box_body(double* __restrict__ f, double val, const wf_grid& g,
@@ -416,8 +424,8 @@ __device__ inline void fill_box_body(double* __restrict__ f, double val, const w
   f[at(g, i, j, k)] = val;
 }
 __global__ void __launch_bounds__(256)
-fill_box_kernel(double* __restrict__ f, double val, wf_grid g, Box3 b) {
-  fill_box_body(f, val, g, b, this_blk());
+fill_box_kernel(double* __restrict__ f, double val, wf_grid g, Box3 b, int ich) {
+  for_planes(b.d0, ich, [&](Blk B) { fill_box_body(f, val, g, b, B); });
 }
 
 // kernels.py:460-471; uint32 wrap-around arithmetic on the full-array indices.
@@ -439,8 +447,8 @@ __device__ inline void noise_body(double* __restrict__ u, double* __restrict__ v
 }
 __global__ void __launch_bounds__(256)
 noise_kernel(double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-             double amp, uint32_t seed, wf_grid g, Box3 b) {
-  noise_body(u, v, w, amp, seed, g, b, this_blk());
+             double amp, uint32_t seed, wf_grid g, Box3 b, int ich) {
+  for_planes(b.d0, ich, [&](Blk B) { noise_body(u, v, w, amp, seed, g, b, B); });
 }
 
 // ---------------------------------------------------------------------------
@@ -684,6 +692,8 @@ static Box3 npl_box(const wf_grid& g, int d0, int d1, int d2) {
 }
 static bool empty(const Box3& b) { return b.d0 == 0 || b.d1 == 0 || b.d2 == 0; }
 #define S_(s) ((hipStream_t)(s))
+#define CH(b) plane_chunk((b).d0, (b).d1, (b).d2)
+#define CG(b) chunk_grid((b).d0, (b).d1, (b).d2, CH(b))
 
 extern "C" {
 
@@ -695,14 +705,14 @@ int wf_conv4th(int comp, const double* u, const double* v, const double* w,
   if (empty(b)) return 0;
   const double dta = dt * alpha;
   if (comp == 0)
-    hipLaunchKernelGGL(conv4th_kernel<0>, box_grid(b), box_block(), 0, S_(stream), u, v, w,
-                       coo, cstar, g, b, dx, dy, dz, dta);
+    hipLaunchKernelGGL(conv4th_kernel<0>, CG(b), box_block(), 0, S_(stream), u, v, w,
+                       coo, cstar, g, b, dx, dy, dz, dta, CH(b));
   else if (comp == 1)
-    hipLaunchKernelGGL(conv4th_kernel<1>, box_grid(b), box_block(), 0, S_(stream), u, v, w,
-                       coo, cstar, g, b, dx, dy, dz, dta);
+    hipLaunchKernelGGL(conv4th_kernel<1>, CG(b), box_block(), 0, S_(stream), u, v, w,
+                       coo, cstar, g, b, dx, dy, dz, dta, CH(b));
   else
-    hipLaunchKernelGGL(conv4th_kernel<2>, box_grid(b), box_block(), 0, S_(stream), u, v, w,
-                       coo, cstar, g, b, dx, dy, dz, dta);
+    hipLaunchKernelGGL(conv4th_kernel<2>, CG(b), box_block(), 0, S_(stream), u, v, w,
+                       coo, cstar, g, b, dx, dy, dz, dta, CH(b));
   WF_LAUNCHED();
   return 0;
 }
@@ -716,14 +726,14 @@ int wf_diff4th(int comp, const double* c, const double* vis, double* cstar, wf_g
   const double dta = dt * alpha;
   const double rx = 12.0 * dx * dx, ry = 12.0 * dy * dy, rz = 12.0 * dz * dz;
   if (comp == 0)
-    hipLaunchKernelGGL(diff4th_kernel<0>, box_grid(b), box_block(), 0, S_(stream), c, vis,
-                       cstar, g, b, rx, ry, rz, dta);
+    hipLaunchKernelGGL(diff4th_kernel<0>, CG(b), box_block(), 0, S_(stream), c, vis,
+                       cstar, g, b, rx, ry, rz, dta, CH(b));
   else if (comp == 1)
-    hipLaunchKernelGGL(diff4th_kernel<1>, box_grid(b), box_block(), 0, S_(stream), c, vis,
-                       cstar, g, b, rx, ry, rz, dta);
+    hipLaunchKernelGGL(diff4th_kernel<1>, CG(b), box_block(), 0, S_(stream), c, vis,
+                       cstar, g, b, rx, ry, rz, dta, CH(b));
   else
-    hipLaunchKernelGGL(diff4th_kernel<2>, box_grid(b), box_block(), 0, S_(stream), c, vis,
-                       cstar, g, b, rx, ry, rz, dta);
+    hipLaunchKernelGGL(diff4th_kernel<2>, CG(b), box_block(), 0, S_(stream), c, vis,
+                       cstar, g, b, rx, ry, rz, dta, CH(b));
   WF_LAUNCHED();
   return 0;
 }
@@ -745,8 +755,8 @@ int wf_calvel(int comp, const double* cstar, const double* p, double* c, wf_grid
   if (comp < 0 || comp > 2 || !box_ok(g, b, 0, 1)) return WF_EINVAL;
   if (empty(b)) return 0;
   const int64_t s = comp == 0 ? g.si : (comp == 1 ? g.sj : 1);
-  hipLaunchKernelGGL(calvel_kernel, box_grid(b), box_block(), 0, S_(stream), cstar, p, c, g,
-                     b, s, 1.0 / h, dt * alpha);
+  hipLaunchKernelGGL(calvel_kernel, CG(b), box_block(), 0, S_(stream), cstar, p, c, g,
+                     b, s, 1.0 / h, dt * alpha, CH(b));
   WF_LAUNCHED();
   return 0;
 }
@@ -756,8 +766,8 @@ int wf_calvel3(const double* ustar, const double* vstar, const double* wstar,
                double dy, double dz, double dt, double alpha, void* stream) {
   const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
   if (!box_ok(g, b, 0, 1)) return WF_EINVAL;
-  hipLaunchKernelGGL(calvel3_kernel, box_grid(b), box_block(), 0, S_(stream), ustar, vstar,
-                     wstar, p, u, v, w, g, b, 1.0 / dx, 1.0 / dy, 1.0 / dz, dt * alpha);
+  hipLaunchKernelGGL(calvel3_kernel, CG(b), box_block(), 0, S_(stream), ustar, vstar,
+                     wstar, p, u, v, w, g, b, 1.0 / dx, 1.0 / dy, 1.0 / dz, dt * alpha, CH(b));
   WF_LAUNCHED();
   return 0;
 }
@@ -782,8 +792,8 @@ int wf_wale(const double* u, const double* v, const double* w, double* vis, wf_g
   const Box3 b = npl_box(g, d0, d1, d2);
   if (!box_ok(g, b, 1, 1)) return WF_EINVAL;
   if (empty(b)) return 0;
-  hipLaunchKernelGGL(wale_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, vis, g,
-                     b, dx, dy, dz, rrey, cw * delta);
+  hipLaunchKernelGGL(wale_kernel, CG(b), box_block(), 0, S_(stream), u, v, w, vis, g,
+                     b, dx, dy, dz, rrey, cw * delta, CH(b));
   WF_LAUNCHED();
   return 0;
 }
@@ -833,7 +843,7 @@ int wf_add_ring(double* dst, const double* src, wf_grid g, void* stream) {
   if (g.npl < 1) return WF_EINVAL;
   const Box3 b{g.npl - 1, g.npl - 1, g.npl - 1, g.ni + 2, g.nj + 2, g.nk + 2};
   if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
-  hipLaunchKernelGGL(add_kernel, box_grid(b), box_block(), 0, S_(stream), dst, src, g, b);
+  hipLaunchKernelGGL(add_kernel, CG(b), box_block(), 0, S_(stream), dst, src, g, b, CH(b));
   WF_LAUNCHED();
   return 0;
 }
@@ -843,8 +853,8 @@ int wf_fill_interior(double* f, double val, wf_grid g, int d0, int d1, int d2,
   const Box3 b = npl_box(g, d0, d1, d2);
   if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
   if (empty(b)) return 0;
-  hipLaunchKernelGGL(fill_box_kernel, box_grid(b), box_block(), 0, S_(stream), f, val, g,
-                     b);
+  hipLaunchKernelGGL(fill_box_kernel, CG(b), box_block(), 0, S_(stream), f, val, g,
+                     b, CH(b));
   WF_LAUNCHED();
   return 0;
 }
@@ -854,8 +864,8 @@ int wf_init_noise(double* u, double* v, double* w, double amp, int32_t seed, wf_
   const Box3 b = npl_box(g, d0, d1, d2);
   if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
   if (empty(b)) return 0;
-  hipLaunchKernelGGL(noise_kernel, box_grid(b), box_block(), 0, S_(stream), u, v, w, amp,
-                     (uint32_t)seed, g, b);
+  hipLaunchKernelGGL(noise_kernel, CG(b), box_block(), 0, S_(stream), u, v, w, amp,
+                     (uint32_t)seed, g, b, CH(b));
   WF_LAUNCHED();
   return 0;
 }

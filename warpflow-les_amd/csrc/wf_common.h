@@ -64,6 +64,30 @@ __device__ inline Blk this_blk() {
   return Blk{L % gx, (L / gx) % gy, L / (gx * gy)};
 }
 
+// Elementwise kernels give each workgroup a chunk of `ich` i-planes of its
+// (k, j) tile: the block decode above is paid once per chunk, and the
+// i-neighbours a plane reads were just read by the same CU.  The chunk is
+// sized to keep >= 8192 workgroups in flight.
+inline int plane_chunk(int d0, int d1, int d2) {
+  const int64_t tiles = (int64_t)div_up(d2, BX) * div_up(d1, BY);
+  const int64_t c = tiles * d0 / 8192;
+  return (int)(c < 1 ? 1 : (c > 8 ? 8 : c));
+}
+inline dim3 chunk_grid(int d0, int d1, int d2, int ich) {
+  return dim3(div_up(d2, BX), div_up(d1, BY), div_up(d0, ich));
+}
+// f(B) for every plane B.z of this workgroup's chunk; SYNC: a barrier after
+// each plane, for bodies that reduce through LDS.
+template <bool SYNC = false, class F>
+__device__ inline void for_planes(int d0, int ich, F&& f) {
+  const Blk C = this_blk();
+  const int z0 = C.z * ich, z1 = min(z0 + ich, d0);
+  for (int z = z0; z < z1; ++z) {
+    f(Blk{C.x, C.y, z});
+    if (SYNC) __syncthreads();
+  }
+}
+
 __device__ inline bool box_thread(const Box3& b, Blk B, int& i, int& j, int& k) {
   k = B.x * BX + threadIdx.x;
   j = B.y * BY + threadIdx.y;
@@ -104,10 +128,10 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Tuning knobs (api.hip): process-wide launch-shape choices set by the
+// Tuning table (api.hip): launch-shape choices per (knob, shape) set by the
 // autotuner; 0 = the built-in choice.  They change launch shapes only, never
 // results.
-int tune_get(int knob);
+int tune_get(int knob, int n0, int n1, int n2);
 
 // Fused conv4th + diff4th of component comp over box b (convdiff.hip); ich =
 // i-planes per workgroup, 0 picks it from the box size.

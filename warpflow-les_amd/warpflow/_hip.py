@@ -78,8 +78,9 @@ _SIGS = {
     "wf_batch_mg_solve_mixed": [_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
     "wf_batch_project_mask": [_P, _P, C.c_double, _I, _P, _P, _P, _P],
     "wf_batch_set_int": [_P, _I, _I, _P],
-    "wf_tune_set": [_I, _I],
-    "wf_tune_get": [_I, C.POINTER(C.c_int)],
+    "wf_tune_set": [_I, _I, _I, _I, _I],
+    "wf_tune_get": [_I, _I, _I, _I, C.POINTER(C.c_int)],
+    "wf_tune_reset": [],
     "wf_graph_begin": [_P],
     "wf_graph_end": [_P, C.POINTER(C.c_void_p)],
     "wf_graph_launch": [_P, _P],

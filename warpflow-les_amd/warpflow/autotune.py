@@ -1,13 +1,15 @@
 """Launch-shape autotuner for the hot kernels.
 
 The fused momentum kernel (``wf_convdiff4th``) and the fused RBGS sweep have
-launch-shape parameters that do not change any result: the i-planes each
-workgroup streams through (``cd_ich``, ``sweep_ich``) and the sweep's (j, k)
-tile (``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32).  ``tune_solver`` times every
-candidate with HIP events on a solver's own arrays, checks that each
+launch-shape parameters that change no result: the i-planes each workgroup
+streams through (``cd_ich``, ``sweep_ich``) and the sweep's (j, k) tile
+(``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32).  The library keeps one setting per
+(parameter, launch shape); ``tune_solver`` times every candidate with HIP
+events on a solver's own arrays -- the momentum kernel on the solver's grid,
+the sweep on each multigrid level that uses it -- checks that each
 candidate's output equals the built-in choice bit for bit, restores the
-arrays, and sets the fastest values process-wide (``wf_tune_set``).  Results
-can be cached in a JSON file keyed by device and grid.
+arrays and records the fastest.  Results can be cached in a JSON file keyed
+by device and grid.
 """
 from __future__ import annotations
 
@@ -20,29 +22,32 @@ import numpy as np
 from . import _hip
 from .device import copy, current_stream, zeros
 
-KNOBS = {"cd_ich": 0, "sweep_tile": 1, "sweep_ich": 2}
-CD_ICH = (0, 8, 16, 24, 32, 48, 64)
-SWEEP_TILE = (1, 2)
-SWEEP_ICH = (0, 4, 8, 16, 32)
+# knob ids (include/warpflow_hip.h)
+CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32 = range(5)
+CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
+SWEEP_TILE_CANDIDATES = (1, 2)
+SWEEP_ICH_CANDIDATES = (0, 4, 8, 16, 32)
+SWEEP_MIN_CELLS = 4096  # smaller levels relax in one LDS-resident workgroup
 
 
-def get(name) -> int:
+def set_shape(knob, n3, value) -> None:
+    _hip.call("wf_tune_set", int(knob), *[int(x) for x in n3], int(value))
+
+
+def get_shape(knob, n3) -> int:
     v = C.c_int(0)
-    _hip.call("wf_tune_get", KNOBS[name], C.byref(v))
+    _hip.call("wf_tune_get", int(knob), *[int(x) for x in n3], C.byref(v))
     return v.value
 
 
-def set(name, value) -> None:  # noqa: A001 - mirrors wf_tune_set
-    _hip.call("wf_tune_set", KNOBS[name], int(value))
+def reset() -> None:
+    _hip.call("wf_tune_reset")
 
 
-def settings() -> dict:
-    return {k: get(k) for k in KNOBS}
-
-
-def apply(values: dict) -> None:
-    for k, v in values.items():
-        set(k, v)
+def apply(entries) -> None:
+    """entries: [(knob, (n0, n1, n2), value), ...] as tune_solver returns."""
+    for knob, n3, value in entries:
+        set_shape(knob, n3, value)
 
 
 class _Timer:
@@ -73,86 +78,99 @@ def _device_name():
 
 
 def _key(sol):
-    return "%s|%dx%dx%d|%s" % (_device_name(), *sol.ncells, np.dtype(sol.mg.levels[0].dtype).name)
+    return "%s|%dx%dx%d|%s|L%d" % (_device_name(), *sol.ncells,
+                                    np.dtype(sol.mg.levels[0].dtype).name, len(sol.mg.levels))
+
+
+def _tune_convdiff(sol, T, reps, out):
+    from . import kernels as K
+    from .topology import NPL
+    f = sol.f
+    h = (sol.dx, sol.dy, sol.dz)
+    opu = sol._op_dim("u")
+    n3 = sol.ncells
+    saved = zeros(f["ustar"].shape)
+    copy(saved, f["ustar"])
+
+    def cd():
+        K.convdiff4th_u(opu, f["u"], f["v"], f["w"], f["uoo"], f["vis"], f["ustar"], *h, sol.dt,
+                        0.5, NPL)
+    set_shape(CD_ICH, n3, 0)
+    cd()
+    want = f["ustar"].numpy()
+    best = None
+    for ich in CD_ICH_CANDIDATES:
+        set_shape(CD_ICH, n3, ich)
+        t = T.us(cd, reps)
+        if not np.array_equal(f["ustar"].numpy(), want):
+            raise RuntimeError("convdiff result depends on cd_ich=%d" % ich)
+        out["timings_us"]["convdiff %s ich=%d" % ("x".join(map(str, n3)), ich)] = round(t, 2)
+        if best is None or t < best[0]:
+            best = (t, ich)
+    set_shape(CD_ICH, n3, best[1])
+    out["entries"].append((CD_ICH, tuple(n3), best[1]))
+    copy(f["ustar"], saved)
+
+
+def _tune_level(lev, T, reps, out):
+    n3 = lev.n3
+    f64 = np.dtype(lev.dtype) == np.float64
+    ktile = SWEEP_TILE_F64 if f64 else SWEEP_TILE_F32
+    kich = SWEEP_ICH_F64 if f64 else SWEEP_ICH_F32
+    keep_phi, keep_tmp = zeros(lev.phi.shape, lev.dtype), zeros(lev.tmp.shape, lev.dtype)
+    copy(keep_phi, lev.phi)
+    copy(keep_tmp, lev.tmp)
+    d = lev.desc()
+
+    def relax():
+        copy(lev.phi, keep_phi)
+        _hip.call("wf_mg_relax", C.byref(d), 2, current_stream())
+    set_shape(ktile, n3, 0)
+    set_shape(kich, n3, 0)
+    relax()
+    want = lev.phi.numpy()
+    t_copy = T.us(lambda: copy(lev.phi, keep_phi), reps)
+    best = None
+    for tile in SWEEP_TILE_CANDIDATES:
+        for ich in SWEEP_ICH_CANDIDATES:
+            set_shape(ktile, n3, tile)
+            set_shape(kich, n3, ich)
+            t = (T.us(relax, reps) - t_copy) / 2
+            if not np.array_equal(lev.phi.numpy(), want):
+                raise RuntimeError("sweep result depends on tile=%d ich=%d" % (tile, ich))
+            out["timings_us"]["sweep %s %s tile=%d ich=%d" % (
+                "x".join(map(str, n3)), "f64" if f64 else "f32", tile, ich)] = round(t, 2)
+            if best is None or t < best[0]:
+                best = (t, tile, ich)
+    set_shape(ktile, n3, best[1])
+    set_shape(kich, n3, best[2])
+    out["entries"] += [(ktile, tuple(n3), best[1]), (kich, tuple(n3), best[2])]
+    copy(lev.phi, keep_phi)
+    copy(lev.tmp, keep_tmp)
 
 
 def tune_solver(sol, reps=10, cache=None):
     """Tune on FlowSolver `sol` (its arrays are restored afterwards); returns
-    {"settings": {...}, "timings_us": {...}}."""
-    from . import kernels as K
-    from .topology import NPL
+    {"entries": [(knob, n3, value), ...], "timings_us": {...}} and leaves
+    the entries set."""
     if cache and os.path.exists(cache):
         with open(cache) as fh:
-            known = json.load(fh)
-        hit = known.get(_key(sol))
+            hit = json.load(fh).get(_key(sol))
         if hit:
-            apply(hit["settings"])
+            hit["entries"] = [(k, tuple(n3), v) for k, n3, v in hit["entries"]]
+            apply(hit["entries"])
             return hit
-    before = settings()
+    out = {"entries": [], "timings_us": {}}
     T = _Timer()
-    f = sol.f
-    h = (sol.dx, sol.dy, sol.dz)
-    opu = sol._op_dim("u")
-    out = {"settings": {}, "timings_us": {}}
     try:
-        # --- fused momentum kernel: output ustar, inputs untouched ---------
-        saved = zeros(f["ustar"].shape)
-        copy(saved, f["ustar"])
-
-        def cd():
-            K.convdiff4th_u(opu, f["u"], f["v"], f["w"], f["uoo"], f["vis"], f["ustar"], *h,
-                            sol.dt, 0.5, NPL)
-        set("cd_ich", 0)
-        cd()
-        want = f["ustar"].numpy()
-        best = None
-        for ich in CD_ICH:
-            set("cd_ich", ich)
-            t = T.us(cd, reps)
-            if not np.array_equal(f["ustar"].numpy(), want):
-                raise RuntimeError("convdiff result depends on cd_ich=%d" % ich)
-            out["timings_us"]["cd_ich=%d" % ich] = round(t, 2)
-            if best is None or t < best[0]:
-                best = (t, ich)
-        set("cd_ich", best[1])
-        out["settings"]["cd_ich"] = best[1]
-        copy(f["ustar"], saved)
-        # --- fused RBGS sweep on level 0: two sweeps phi -> tmp -> phi ---------
-        lev = sol.mg.levels[0]
-        if lev.tmp is not None:
-            keep_phi, keep_tmp = zeros(lev.phi.shape, lev.dtype), zeros(lev.tmp.shape, lev.dtype)
-            copy(keep_phi, lev.phi)
-            copy(keep_tmp, lev.tmp)
-            d = lev.desc()
-
-            def relax():
-                copy(lev.phi, keep_phi)
-                _hip.call("wf_mg_relax", C.byref(d), 2, current_stream())
-            set("sweep_tile", 0)
-            set("sweep_ich", 0)
-            relax()
-            want = lev.phi.numpy()
-            t_copy = T.us(lambda: copy(lev.phi, keep_phi), reps)
-            best = None
-            for tile in SWEEP_TILE:
-                for ich in SWEEP_ICH:
-                    set("sweep_tile", tile)
-                    set("sweep_ich", ich)
-                    t = (T.us(relax, reps) - t_copy) / 2
-                    if not np.array_equal(lev.phi.numpy(), want):
-                        raise RuntimeError("sweep result depends on tile=%d ich=%d" % (tile, ich))
-                    out["timings_us"]["sweep_tile=%d,ich=%d" % (tile, ich)] = round(t, 2)
-                    if best is None or t < best[0]:
-                        best = (t, tile, ich)
-            set("sweep_tile", best[1])
-            set("sweep_ich", best[2])
-            out["settings"]["sweep_tile"] = best[1]
-            out["settings"]["sweep_ich"] = best[2]
-            copy(lev.phi, keep_phi)
-            copy(lev.tmp, keep_tmp)
+        _tune_convdiff(sol, T, reps, out)
+        for lev in sol.mg.levels:
+            if lev.tmp is not None and int(np.prod(lev.n3)) > SWEEP_MIN_CELLS:
+                _tune_level(lev, T, reps, out)
         _hip.call("wf_synchronize")
     except Exception:
-        apply(before)
+        for knob, n3, _ in out["entries"]:
+            set_shape(knob, n3, 0)
         raise
     finally:
         T.close()

@@ -188,8 +188,10 @@ class Ensemble:
     # ------------------------------------------------------------------ #
     # pieces of one step (FlowSolver.step)                                #
     # ------------------------------------------------------------------ #
-    def _fill(self, name, active=None):
-        self._call("wf_batch_bc_fill", self.pool[name].ptr, _BC[name],
+    def _fill(self, field, which=None, active=None):
+        """Ghost fill `which` (default: the field's own, bcs.py:201-311) of
+        pool `field` for every (active) member."""
+        self._call("wf_batch_bc_fill", self.pool[field].ptr, _BC[which or field],
                    active.ptr if active is not None else None, self.stream)
 
     def _calvel(self, alpha, active=None):
@@ -240,7 +242,7 @@ class Ensemble:
                    P["pp"].ptr, self.mixed_cycles, self.N_PRE, self.N_POST, self.N_COARSE, 2,
                    a, s)
         self._call("wf_batch_add_ring", P["p"].ptr, P["pp"].ptr, a, s)
-        self._fill("pbound", self.active)
+        self._fill("p", "pbound", self.active)
         self._calvel(1.0, self.active)
 
     def _part_a(self):
