@@ -74,8 +74,11 @@ def test_fused_equals_reference_launch_order(gpu):
     for _ in range(4):
         ia, ib = a.step(), b.step()
         assert ia["iters"] == ib["iters"]
+    # the fused momentum kernel multiplies by 1/(24h) and 1/(12h^2) where the
+    # reference-order kernels divide: ulp-level differences per operator,
+    # carried through four steps of pressure solves
     for nm in ("u", "v", "w", "p", "vis", "ustar", "vstar", "wstar"):
-        assert rel_err(a.f[nm].numpy(), b.f[nm].numpy()) < 1e-13, nm
+        assert rel_err(a.f[nm].numpy(), b.f[nm].numpy()) < 1e-12, nm
 
 
 def test_step1_rmax_matches_report(gpu):

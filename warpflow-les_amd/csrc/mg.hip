@@ -19,6 +19,7 @@
 // zero they were allocated with -- the same ring state at every point the
 // reference could observe it.
 #include <algorithm>
+#include <type_traits>
 
 #include "batch.h"
 #include "mg_common.h"
@@ -44,7 +45,7 @@ rbgs_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, T c1,
   if (!colour_cell(d, color, i, j, k)) return;
   const int64_t x = mg_at(d, i, j, k);
   const Coef<T> c = mg_coef(d, i, j, k, c0, c1, c2);
-  const T v = (rhs[x] - nb_sum(phi, x, d, c)) / c.ap;
+  const T v = (rhs[x] - nb_sum(phi, x, d, c)) * (T(1) / c.ap);
   phi[x] = v;
   if (write_ring) ring_images(phi, d, i, j, k, x, v);
 }
@@ -199,7 +200,6 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   const int i0 = B.z * ich, i1 = min(i0 + ich, d.n0);
   const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
   const int s1 = (int)d.s1;
-  auto slot = [](int q) { return ((q % 3) + 3) % 3; };
   auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
   // offset of (j, k) inside one level plane, clamped into the ring
   auto plane_off = [&](int j, int k) {
@@ -208,6 +208,10 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   auto plane_ptr = [&](const T* a, int q) {  // plane q (clamped into the ring)
     return a + (int64_t)(clampi(q, -1, n0) + 1) * d.s0;
   };
+  // Inner tile: every odd cell of the 1-cell-halo tile is a real cell off
+  // the j and k faces, so its j/k coefficients are c1, c1, c2, c2 and only
+  // the plane decides aw, ae and ap (uniform over the workgroup).
+  const bool inner = j0 >= 2 && j0 + TJ <= n1 - 2 && k0 >= 2 && k0 + TK <= n2 - 2;
 
   // pin-plane loads: l = tid + 256 e covers the (TJ+4) x (TK+4) halo tile
   int poff[NP], pl_[NP];
@@ -245,14 +249,17 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   const int eoff = plane_off(ej, ka);                     // plane offset of k_a
   const bool ej_in = ej < n1;
 
-  auto odd_cell_value = [&](int q, int par, int e, const T* pm, const T* p0, const T* pp,
-                            T r) {
-    const int c = oP[par][e];
-    const Coef<T> cf = mg_coef(d, clampi(q, 0, n0 - 1), clampi(oj[e], 0, n1 - 1),
-                               clampi(ok[par][e], 0, n2 - 1), c0, c1, c2);
-    const T sum = cf.aw * pm[c] + cf.ae * pp[c] + cf.as * p0[c - PK] + cf.an * p0[c + PK] +
-                  cf.ab * p0[c - 1] + cf.at * p0[c + 1];
-    return (r - sum) / cf.ap;
+  // coefficients of plane q for an inner tile, in mg_coef's order
+  auto plane_coef = [&](int q) {
+    Coef<T> cf;
+    cf.aw = q == 0 ? T(0) : c0;
+    cf.ae = q == n0 - 1 ? T(0) : c0;
+    cf.as = c1;
+    cf.an = c1;
+    cf.ab = c2;
+    cf.at = c2;
+    cf.ap = -(cf.aw + cf.ae + cf.as + cf.an + cf.ab + cf.at);
+    return cf;
   };
 
   T pre[NP], rodd[NO], rodd_n[NO];
@@ -262,8 +269,7 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
 #pragma unroll
     for (int e = 0; e < NP; ++e) pre[e] = b[poff[e]];
   };
-  auto store_plane = [&](int q) {
-    T* dst = P[slot(q)];
+  auto store_plane = [&](T* dst) {
 #pragma unroll
     for (int e = 0; e < NP; ++e)
       if (pl_[e] < PN) dst[pl_[e]] = pre[e];
@@ -279,14 +285,12 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
     return plane_ptr(rhs, pl)[eoff + kx];
   };
 
-  gload_plane(i0 - 2);
-  store_plane(i0 - 2);
-  gload_plane(i0 - 1);
-  store_plane(i0 - 1);
-  gload_plane(i0);
-  gload_rhs_odd(i0 - 1);
-  for (int q = i0 - 1; q <= i1; ++q) {
-    store_plane(q + 1);
+  // LDS slot of plane q is (q - i0 + 2) mod 3; iteration t handles q = i0-1+t,
+  // so with R = t mod 3 every slot below is a compile-time constant.
+  auto step = [&](auto Rc, int q) {
+    constexpr int R = decltype(Rc)::value;
+    constexpr int SM = R, S0 = (R + 1) % 3, SP = (R + 2) % 3;  // planes q-1, q, q+1
+    store_plane(P[SP]);
 #pragma unroll
     for (int e = 0; e < NO; ++e) rodd[e] = rodd_n[e];
     reven = reven_n;
@@ -297,40 +301,76 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
     {  // odd values of plane q over the halo tile
       const int par = q & 1;
       const bool qin = q >= 0 && q < n0;
-      const T* pm = P[slot(q - 1)];
-      const T* p0 = P[slot(q)];
-      const T* pp = P[slot(q + 1)];
-      T* o = O[slot(q)];
+      const T* pm = P[SM];
+      const T* p0 = P[S0];
+      const T* pp = P[SP];
+      T* o = O[S0];
+      const Coef<T> cq = plane_coef(clampi(q, 0, n0 - 1));
+      const T rq = T(1) / cq.ap;
 #pragma unroll
       for (int e = 0; e < NO; ++e) {
-        const T v = odd_cell_value(q, par, e, pm, p0, pp, rodd[e]);
-        if (olive[e]) o[oO[par][e]] = (qin && oin[par][e]) ? v : T(0);
+        const int c = oP[par][e];
+        const Coef<T> cf =
+            inner ? cq
+                  : mg_coef(d, clampi(q, 0, n0 - 1), clampi(oj[e], 0, n1 - 1),
+                            clampi(ok[par][e], 0, n2 - 1), c0, c1, c2);
+        const T rap = inner ? rq : T(1) / cf.ap;
+        const T sum = cf.aw * pm[c] + cf.ae * pp[c] + cf.as * p0[c - PK] +
+                      cf.an * p0[c + PK] + cf.ab * p0[c - 1] + cf.at * p0[c + 1];
+        const T v = (rodd[e] - sum) * rap;
+        if (olive[e]) o[oO[par][e]] = (qin && (inner || oin[par][e])) ? v : T(0);
       }
     }
     __syncthreads();
     const int pl = q - 1;
     if (pl >= i0 && ej_in) {  // even values of plane pl, write plane pl
-      const T* om = O[slot(pl - 1)];
-      const T* o0 = O[slot(pl)];
-      const T* op = O[slot(pl + 1)];
+      const T* om = O[SP];  // plane pl-1
+      const T* o0 = O[SM];  // plane pl
+      const T* op = O[S0];  // plane pl+1
       const int kx = ((pl + ej + k0) & 1) ? 1 : 0;  // offset of the even cell
       const int ke = ka + kx, ko = ka + 1 - kx;
       const int c = erow + kx;
-      const Coef<T> cf = mg_coef(d, pl, ej, min(ke, n2 - 1), c0, c1, c2);
+      const Coef<T> cf = inner ? plane_coef(pl) : mg_coef(d, pl, ej, min(ke, n2 - 1), c0, c1, c2);
       const T sum = cf.aw * om[c] + cf.ae * op[c] + cf.as * o0[c - OK] + cf.an * o0[c + OK] +
                     cf.ab * o0[c - 1] + cf.at * o0[c + 1];
-      const T ve = (reven - sum) / cf.ap;
+      const T ve = (reven - sum) * (T(1) / cf.ap);
       const T vo = o0[erow + 1 - kx];
       T* out = pout + (int64_t)(pl + 1) * d.s0;
-      if (ke < n2) {
+      if (inner) {  // only the i faces can have ring images
         out[eoff + kx] = ve;
-        ring_images(pout, d, pl, ej, ke, mg_at(d, pl, ej, ke), ve);
-      }
-      if (ko < n2) {
         out[eoff + 1 - kx] = vo;
-        ring_images(pout, d, pl, ej, ko, mg_at(d, pl, ej, ko), vo);
+        if (pl == 0) {
+          out[eoff + kx - d.s0] = ve;
+          out[eoff + 1 - kx - d.s0] = vo;
+        }
+        if (pl == n0 - 1) {
+          out[eoff + kx + d.s0] = ve;
+          out[eoff + 1 - kx + d.s0] = vo;
+        }
+      } else {
+        if (ke < n2) {
+          out[eoff + kx] = ve;
+          ring_images(pout, d, pl, ej, ke, mg_at(d, pl, ej, ke), ve);
+        }
+        if (ko < n2) {
+          out[eoff + 1 - kx] = vo;
+          ring_images(pout, d, pl, ej, ko, mg_at(d, pl, ej, ko), vo);
+        }
       }
     }
+  };
+
+  gload_plane(i0 - 2);
+  store_plane(P[0]);
+  gload_plane(i0 - 1);
+  store_plane(P[1]);
+  gload_plane(i0);
+  gload_rhs_odd(i0 - 1);
+  const int cnt = i1 - i0 + 2;  // q = i0-1 .. i1
+  for (int t = 0; t < cnt; t += 3) {
+    step(std::integral_constant<int, 0>{}, i0 - 1 + t);
+    if (t + 1 < cnt) step(std::integral_constant<int, 1>{}, i0 + t);
+    if (t + 2 < cnt) step(std::integral_constant<int, 2>{}, i0 + 1 + t);
   }
 }
 template <class T, int TJ, int TK>
@@ -378,7 +418,7 @@ __device__ inline void relax_lds_body(T* __restrict__ phi, const T* __restrict__
         const T Tt = k < n2 - 1 ? sp[l + 1] : T(0);
         const T rv = RHS_LDS ? sr[l] : rhs[mg_at(d, i, j, k)];
         const T sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt;
-        sp[l] = (rv - sum) / c.ap;
+        sp[l] = (rv - sum) * (T(1) / c.ap);
       }
       __syncthreads();
     }
@@ -428,7 +468,7 @@ __device__ inline void coarse_op_build_body(T* __restrict__ M, const MgDev& d, T
         const T Tt = k < n2 - 1 ? sp[l + 1] : T(0);
         const T rv = l == col ? T(1) : T(0);
         const T sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt;
-        sp[l] = (rv - sum) / c.ap;
+        sp[l] = (rv - sum) * (T(1) / c.ap);
       }
       __syncthreads();
     }
@@ -455,8 +495,18 @@ __device__ inline void coarse_op_apply_body(const T* __restrict__ M, T* __restri
   }
   __syncthreads();
   for (int row = tid; row < ncell; row += nt) {
+    // serial sum in column order; the loads of 16 columns are issued
+    // before their products so that one workgroup keeps them in flight
     T acc = T(0);
-    for (int col = 0; col < ncell; ++col) acc = acc + M[(size_t)col * ncell + row] * r[col];
+    int col = 0;
+    for (; col + 16 <= ncell; col += 16) {
+      T m[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) m[q] = M[(size_t)(col + q) * ncell + row];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc = acc + m[q] * r[col + q];
+    }
+    for (; col < ncell; ++col) acc = acc + M[(size_t)col * ncell + row] * r[col];
     const int k = row % n2, j = (row / n2) % n1, i = row / (n1 * n2);
     const int64_t x = mg_at(d, i, j, k);
     phi[x] = acc;

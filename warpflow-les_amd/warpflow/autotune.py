@@ -26,7 +26,7 @@ from .device import copy, current_stream, zeros
 CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32 = range(5)
 CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
 SWEEP_TILE_CANDIDATES = (1, 2)
-SWEEP_ICH_CANDIDATES = (0, 4, 8, 16, 32)
+SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32)
 SWEEP_MIN_CELLS = 4096  # smaller levels relax in one LDS-resident workgroup
 
 
