@@ -391,7 +391,8 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  * value 0 removes the entry (the built-in choice); results never depend on
  * any setting.  At most 64 entries.
  *   WF_TUNE_CD_ICH              i-planes per workgroup of wf_convdiff4th
- *   WF_TUNE_SWEEP_TILE_F64/F32  fused RBGS sweep tile: 1 = 8 x 64, 2 = 16 x 32
+ *   WF_TUNE_SWEEP_TILE_F64/F32  fused RBGS sweep tile (j x k): 1 = 8 x 64,
+ *                               2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32
  *   WF_TUNE_SWEEP_ICH_F64/F32   i-planes per workgroup of the fused sweep
  * ---------------------------------------------------------------------- */
 enum {

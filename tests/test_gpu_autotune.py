@@ -54,7 +54,7 @@ def test_tuning_table(gpu):
     from warpflow import _hip, autotune
     autotune.reset()
     with pytest.raises(_hip.HipError):
-        autotune.set_shape(autotune.SWEEP_TILE_F32, (8, 8, 8), 3)
+        autotune.set_shape(autotune.SWEEP_TILE_F32, (8, 8, 8), 5)
     autotune.set_shape(autotune.CD_ICH, (64, 64, 64), 24)
     assert autotune.get_shape(autotune.CD_ICH, (64, 64, 64)) == 24
     assert autotune.get_shape(autotune.CD_ICH, (64, 64, 32)) == 0

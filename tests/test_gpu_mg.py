@@ -178,3 +178,37 @@ def test_coarse_operator_equals_sweeps(gpu, n3, prec):
     _hip.call("wf_mg_relax", C.byref(db), 300, None)
     tol = 1e-11 if prec == np.float64 else 2e-5
     assert rel_err(a.phi.numpy(), b.phi.numpy()) < tol
+
+
+@pytest.mark.parametrize("prec", [np.float64, np.float32])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+def test_sweep_tiles_agree_bitwise(gpu, prec, tile):
+    """Every fused-sweep tile shape (8x64, 16x32, 16x64, 32x32; one or two
+    (even, odd) pairs per thread) and i-chunk gives the built-in result bit
+    for bit, on a level whose extents are multiples of none of them."""
+    import ctypes as C
+    from warpflow import _hip, autotune
+    from warpflow.multigrid import _Level
+    n3 = (37, 42, 70)
+    h = tuple(1.0 / n for n in n3)
+    lev = _Level(n3, h, None, prec)
+    rng = np.random.default_rng(11)
+    rhs = rng.standard_normal(lev.shape)
+    phi0 = rng.standard_normal(lev.shape)
+    d = lev.desc()
+    knob_t = autotune.SWEEP_TILE_F64 if prec == np.float64 else autotune.SWEEP_TILE_F32
+    knob_i = autotune.SWEEP_ICH_F64 if prec == np.float64 else autotune.SWEEP_ICH_F32
+    lev.rhs.assign(rhs)
+    autotune.reset()
+    try:
+        lev.phi.assign(phi0)
+        _hip.call("wf_mg_relax", C.byref(d), 4, None)
+        want = lev.phi.numpy()
+        for ich in (1, 3, 16):
+            autotune.set_shape(knob_t, n3, tile)
+            autotune.set_shape(knob_i, n3, ich)
+            lev.phi.assign(phi0)
+            _hip.call("wf_mg_relax", C.byref(d), 4, None)
+            assert np.array_equal(lev.phi.numpy(), want), (tile, ich)
+    finally:
+        autotune.reset()

@@ -53,3 +53,83 @@ def test_80cube_1000_steps_stable(gpu):
     assert infos[0]["dt"] == pytest.approx(3.3e-4)
     late = [i["cfl"] for i in infos[500:]]
     assert 0.78 < min(late) and max(late) < 0.82, (min(late), max(late))
+
+
+CASE32 = os.path.join(PKG, "cases", "cavity_re100_32.yaml")
+
+
+def test_32cube_re100_200_steps_vs_c_oracle(gpu):
+    """The CPU-runnable reference case (32^3, Re 100, WALE off, fixed dt
+    1e-3, 3 levels, 200 steps): identical outer-iteration counts and rmax
+    trajectory, fields to rounding."""
+    from warpflow import FlowSolver, build_topology, load_case
+    cfg = load_case(CASE32)
+    sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
+    ref = CO.CSolver(O.Case.from_config(cfg))
+    for it in range(1, cfg.time.n_steps + 1):
+        a, b = sol.step(), ref.step()
+        assert a["iters"] == b["iters"], (it, a, b)
+        assert a["rmax"] == pytest.approx(b["rmax"], rel=1e-6, abs=1e-15), it
+    for c in "uvw":
+        assert rms(sol.f[c].numpy(), ref.field(c)) <= 1e-12, c
+    assert sol.ctime == pytest.approx(0.2, rel=1e-12)
+
+
+@pytest.mark.parametrize("nsteps", [1, 100])
+def test_operators_on_evolved_80cube_state(gpu, nsteps):
+    """Operator parity (1e-12) on the GPU's own 80^3 state after 1 and 100
+    steps of cases/cavity_serial.yaml, each operator applied by the GPU and
+    by the oracle to the same input arrays."""
+    from warpflow import FlowSolver, build_topology, load_case
+    from warpflow import kernels as K
+    from warpflow.device import array, zeros
+    from conftest import rel_err
+    cfg = load_case(CASE80)
+    sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
+    for _ in range(nsteps):
+        sol.step()
+    N = O.NPL
+    n3 = sol.ncells
+    h = (sol.dx, sol.dy, sol.dz)
+    dt = sol.dt
+    st = {k: sol.f[k].numpy() for k in ("u", "v", "w", "vis", "p", "uoo", "voo", "woo")}
+    d = {k: array(v, np.float64) for k, v in st.items()}
+    ops = {"u": (n3[0] - 1, n3[1], n3[2]), "v": (n3[0], n3[1] - 1, n3[2]),
+           "w": (n3[0], n3[1], n3[2] - 1)}
+    for c in "uvw":  # conv4th + diff4th (one RK stage, alpha 0.5)
+        ref = np.zeros_like(st["u"])
+        getattr(O, "conv4th_" + c)(st["u"], st["v"], st["w"], st[c + "oo"], ref, *h, dt, 0.5,
+                                   N, ops[c])
+        getattr(O, "diff4th_" + c)(st[c], st["vis"], ref, *h, dt, 0.5, N, ops[c])
+        got = zeros(st["u"].shape)
+        getattr(K, "convdiff4th_" + c)(ops[c], d["u"], d["v"], d["w"], d[c + "oo"], d["vis"],
+                                        got, *h, dt, 0.5, N)
+        assert rel_err(got.numpy() - st[c + "oo"], ref - st[c + "oo"]) < 1e-12, c
+    # WALE
+    vref = st["vis"].copy()
+    O.wale_viscosity(st["u"], st["v"], st["w"], vref, *h, sol.rey, cfg.physics.sgs_cw, N, n3)
+    vg = array(st["vis"], np.float64)
+    K.wale_viscosity(n3, d["u"], d["v"], d["w"], vg, *h, sol.rey, cfg.physics.sgs_cw, N)
+    assert rel_err(vg.numpy(), vref) < 1e-12
+    # calvel3 then calmas on the corrected velocities
+    ref = {c: st[c].copy() for c in "uvw"}
+    for c, a in (("u", 0), ("v", 1), ("w", 2)):
+        getattr(O, "calvel_" + c)(st[c + "oo"], st["p"], ref[c], h[a], dt, 1.0, N, ops[c])
+    gu, gv, gw = (array(st[c], np.float64) for c in "uvw")
+    K.calvel3(n3, d["uoo"], d["voo"], d["woo"], d["p"], gu, gv, gw, *h, dt, 1.0, N)
+    for c, g in zip("uvw", (gu, gv, gw)):
+        assert rel_err(g.numpy() - st[c + "oo"], ref[c] - st[c + "oo"]) < 1e-12, c
+    su_ref = np.zeros_like(st["u"])
+    rmax_ref = O.calmas(ref["u"], ref["v"], ref["w"], su_ref, *h, dt, N, n3)
+    su, rb = zeros(st["u"].shape), zeros(1)
+    K.calmas(n3, gu, gv, gw, su, rb, *h, dt, N)
+    assert rel_err(su.numpy(), su_ref) < 1e-12
+    assert abs(rb.numpy()[0] - rmax_ref) <= 1e-12 * rmax_ref
+    # checkdt reductions
+    out4 = zeros(4)
+    K.checkdt(n3, d["u"], d["v"], d["w"], d["vis"], out4, *h, N)
+    exp = [O.maxabs_reduce(st[c], N, n3) for c in "uvw"]
+    exp.append(O.dtvisc_reduce(st["u"], st["v"], st["w"], st["vis"], *h, N, n3))
+    got4 = out4.numpy()
+    assert np.array_equal(got4[:3], exp[:3])
+    assert abs(got4[3] - exp[3]) <= 1e-14 * exp[3]

@@ -187,7 +187,8 @@ template <class T, int TJ, int TK>
 __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pout,
                                   const T* __restrict__ rhs, const MgDev& d, T c0, T c1, T c2,
                                   int ich, Blk B) {
-  static_assert(TJ * TK == 512, "one (even, odd) pair per thread");
+  constexpr int NPAIR = TJ * TK / 512;  // (even, odd) pairs per thread
+  static_assert(TJ * TK == 512 * NPAIR && (NPAIR == 1 || NPAIR == 2), "256 threads, 1-2 pairs");
   constexpr int PJ = TJ + 4, PK = TK + 4, PN = PJ * PK;
   constexpr int OJ = TJ + 2, OK = TK + 2, OH = OK / 2;
   constexpr int NP = (PN + 255) / 256;        // pin values per thread per plane
@@ -241,13 +242,19 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
       oO[par][e] = jj * OK + kk;
     }
   }
-  // this thread's (even, odd) pair: k_a = k0 + 2m, k_b = k_a + 1; which one
-  // is even alternates with the plane parity
-  const int ej = j0 + tid / HK;
-  const int ka = k0 + 2 * (tid % HK);
-  const int erow = (tid / HK + 1) * OK + (ka - k0 + 1);  // O index of k_a
-  const int eoff = plane_off(ej, ka);                     // plane offset of k_a
-  const bool ej_in = ej < n1;
+  // this thread's (even, odd) pairs p = tid + 256 e: k_a = k0 + 2m,
+  // k_b = k_a + 1; which one is even alternates with the plane parity
+  int ej[NPAIR], ka[NPAIR], erow[NPAIR], eoff[NPAIR];
+  bool ej_in[NPAIR];
+#pragma unroll
+  for (int e = 0; e < NPAIR; ++e) {
+    const int pr = tid + 256 * e;
+    ej[e] = j0 + pr / HK;
+    ka[e] = k0 + 2 * (pr % HK);
+    erow[e] = (pr / HK + 1) * OK + (ka[e] - k0 + 1);  // O index of k_a
+    eoff[e] = plane_off(ej[e], ka[e]);                 // plane offset of k_a
+    ej_in[e] = ej[e] < n1;
+  }
 
   // coefficients of plane q for an inner tile, in mg_coef's order
   auto plane_coef = [&](int q) {
@@ -262,27 +269,32 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
     return cf;
   };
 
+  // Loads run one plane ahead of their use: pre holds pin plane q+1,
+  // rodd/rodd_n the odd-cell rhs of planes q, q+1, and reven/reven_n the
+  // even-cell rhs of planes q-1, q.
   T pre[NP], rodd[NO], rodd_n[NO];
-  T reven = T(0), reven_n = T(0);
-  auto gload_plane = [&](int q) {
+  T reven[NPAIR], reven_n[NPAIR];
+#pragma unroll
+  for (int e = 0; e < NPAIR; ++e) reven[e] = reven_n[e] = T(0);
+  auto gload_plane = [&](int q, T (&dst)[NP]) {
     const T* b = plane_ptr(pin, q);
 #pragma unroll
-    for (int e = 0; e < NP; ++e) pre[e] = b[poff[e]];
+    for (int e = 0; e < NP; ++e) dst[e] = b[poff[e]];
   };
-  auto store_plane = [&](T* dst) {
+  auto store_plane = [&](T* dst, const T (&src)[NP]) {
 #pragma unroll
     for (int e = 0; e < NP; ++e)
-      if (pl_[e] < PN) dst[pl_[e]] = pre[e];
+      if (pl_[e] < PN) dst[pl_[e]] = src[e];
   };
-  auto gload_rhs_odd = [&](int q) {
+  auto gload_rhs_odd = [&](int q, T (&dst)[NO]) {
     const T* b = plane_ptr(rhs, q);
     const int par = q & 1;
 #pragma unroll
-    for (int e = 0; e < NO; ++e) rodd_n[e] = b[ooff[par][e]];
+    for (int e = 0; e < NO; ++e) dst[e] = b[ooff[par][e]];
   };
-  auto gload_rhs_even = [&](int pl) {
-    const int kx = ((pl + ej + k0) & 1) ? 1 : 0;  // k_a even -> 0, else k_b
-    return plane_ptr(rhs, pl)[eoff + kx];
+  auto gload_rhs_even = [&](int pl, int e) {
+    const int kx = ((pl + ej[e] + k0) & 1) ? 1 : 0;  // k_a even -> 0, else k_b
+    return plane_ptr(rhs, pl)[eoff[e] + kx];
   };
 
   // LDS slot of plane q is (q - i0 + 2) mod 3; iteration t handles q = i0-1+t,
@@ -290,13 +302,17 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   auto step = [&](auto Rc, int q) {
     constexpr int R = decltype(Rc)::value;
     constexpr int SM = R, S0 = (R + 1) % 3, SP = (R + 2) % 3;  // planes q-1, q, q+1
-    store_plane(P[SP]);
+    store_plane(P[SP], pre);  // plane q+1
 #pragma unroll
     for (int e = 0; e < NO; ++e) rodd[e] = rodd_n[e];
-    reven = reven_n;
-    if (q + 2 <= i1 + 1) gload_plane(q + 2);
-    if (q + 1 <= i1) gload_rhs_odd(q + 1);
-    if (q >= i0 && q + 1 <= i1) reven_n = gload_rhs_even(q);
+#pragma unroll
+    for (int e = 0; e < NPAIR; ++e) reven[e] = reven_n[e];
+    if (q + 2 <= i1 + 1) gload_plane(q + 2, pre);
+    if (q + 1 <= i1) gload_rhs_odd(q + 1, rodd_n);
+    if (q >= i0 && q + 1 <= i1) {
+#pragma unroll
+      for (int e = 0; e < NPAIR; ++e) reven_n[e] = gload_rhs_even(q, e);
+    }
     __syncthreads();
     {  // odd values of plane q over the halo tile
       const int par = q & 1;
@@ -323,49 +339,53 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
     }
     __syncthreads();
     const int pl = q - 1;
-    if (pl >= i0 && ej_in) {  // even values of plane pl, write plane pl
+#pragma unroll
+    for (int e = 0; e < NPAIR; ++e) {
+      if (!(pl >= i0 && ej_in[e])) continue;  // even values of plane pl, write plane pl
       const T* om = O[SP];  // plane pl-1
       const T* o0 = O[SM];  // plane pl
       const T* op = O[S0];  // plane pl+1
-      const int kx = ((pl + ej + k0) & 1) ? 1 : 0;  // offset of the even cell
-      const int ke = ka + kx, ko = ka + 1 - kx;
-      const int c = erow + kx;
-      const Coef<T> cf = inner ? plane_coef(pl) : mg_coef(d, pl, ej, min(ke, n2 - 1), c0, c1, c2);
+      const int kx = ((pl + ej[e] + k0) & 1) ? 1 : 0;  // offset of the even cell
+      const int ke = ka[e] + kx, ko = ka[e] + 1 - kx;
+      const int c = erow[e] + kx;
+      const Coef<T> cf =
+          inner ? plane_coef(pl) : mg_coef(d, pl, ej[e], min(ke, n2 - 1), c0, c1, c2);
       const T sum = cf.aw * om[c] + cf.ae * op[c] + cf.as * o0[c - OK] + cf.an * o0[c + OK] +
                     cf.ab * o0[c - 1] + cf.at * o0[c + 1];
-      const T ve = (reven - sum) * (T(1) / cf.ap);
-      const T vo = o0[erow + 1 - kx];
+      const T ve = (reven[e] - sum) * (T(1) / cf.ap);
+      const T vo = o0[erow[e] + 1 - kx];
       T* out = pout + (int64_t)(pl + 1) * d.s0;
+      const int eo = eoff[e];
       if (inner) {  // only the i faces can have ring images
-        out[eoff + kx] = ve;
-        out[eoff + 1 - kx] = vo;
+        out[eo + kx] = ve;
+        out[eo + 1 - kx] = vo;
         if (pl == 0) {
-          out[eoff + kx - d.s0] = ve;
-          out[eoff + 1 - kx - d.s0] = vo;
+          out[eo + kx - d.s0] = ve;
+          out[eo + 1 - kx - d.s0] = vo;
         }
         if (pl == n0 - 1) {
-          out[eoff + kx + d.s0] = ve;
-          out[eoff + 1 - kx + d.s0] = vo;
+          out[eo + kx + d.s0] = ve;
+          out[eo + 1 - kx + d.s0] = vo;
         }
       } else {
         if (ke < n2) {
-          out[eoff + kx] = ve;
-          ring_images(pout, d, pl, ej, ke, mg_at(d, pl, ej, ke), ve);
+          out[eo + kx] = ve;
+          ring_images(pout, d, pl, ej[e], ke, mg_at(d, pl, ej[e], ke), ve);
         }
         if (ko < n2) {
-          out[eoff + 1 - kx] = vo;
-          ring_images(pout, d, pl, ej, ko, mg_at(d, pl, ej, ko), vo);
+          out[eo + 1 - kx] = vo;
+          ring_images(pout, d, pl, ej[e], ko, mg_at(d, pl, ej[e], ko), vo);
         }
       }
     }
   };
 
-  gload_plane(i0 - 2);
-  store_plane(P[0]);
-  gload_plane(i0 - 1);
-  store_plane(P[1]);
-  gload_plane(i0);
-  gload_rhs_odd(i0 - 1);
+  gload_plane(i0 - 2, pre);
+  store_plane(P[0], pre);
+  gload_plane(i0 - 1, pre);
+  store_plane(P[1], pre);
+  gload_plane(i0, pre);
+  gload_rhs_odd(i0 - 1, rodd_n);
   const int cnt = i1 - i0 + 2;  // q = i0-1 .. i1
   for (int t = 0; t < cnt; t += 3) {
     step(std::integral_constant<int, 0>{}, i0 - 1 + t);
@@ -846,8 +866,19 @@ template <class T>
 int sweep_t(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
   const int tile = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP_TILE_F64 : WF_TUNE_SWEEP_TILE_F32,
                             L.n0, L.n1, L.n2);
-  const bool wide = tile == 1 || (tile != 2 && L.n2 > 32);  // 8 x 64, else 16 x 32
-  return wide ? launch_sweep<T, 8, 64>(L, pin, pout, s) : launch_sweep<T, 16, 32>(L, pin, pout, s);
+  switch (tile) {  // (j x k): 1 = 8 x 64, 2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32
+    case 1:
+      return launch_sweep<T, 8, 64>(L, pin, pout, s);
+    case 2:
+      return launch_sweep<T, 16, 32>(L, pin, pout, s);
+    case 3:
+      return launch_sweep<T, 16, 64>(L, pin, pout, s);
+    case 4:
+      return launch_sweep<T, 32, 32>(L, pin, pout, s);
+    default:
+      return L.n2 > 32 ? launch_sweep<T, 8, 64>(L, pin, pout, s)
+                       : launch_sweep<T, 16, 32>(L, pin, pout, s);
+  }
 }
 
 template <class T>

@@ -3,7 +3,7 @@
 The fused momentum kernel (``wf_convdiff4th``) and the fused RBGS sweep have
 launch-shape parameters that change no result: the i-planes each workgroup
 streams through (``cd_ich``, ``sweep_ich``) and the sweep's (j, k) tile
-(``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32).  The library keeps one setting per
+(``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32).  The library keeps one setting per
 (parameter, launch shape); ``tune_solver`` times every candidate with HIP
 events on a solver's own arrays -- the momentum kernel on the solver's grid,
 the sweep on each multigrid level that uses it -- checks that each
@@ -25,7 +25,7 @@ from .device import copy, current_stream, zeros
 # knob ids (include/warpflow_hip.h)
 CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32 = range(5)
 CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
-SWEEP_TILE_CANDIDATES = (1, 2)
+SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
 SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32)
 SWEEP_MIN_CELLS = 4096  # smaller levels relax in one LDS-resident workgroup
 
