@@ -164,7 +164,7 @@ def hot_kernels(a, s64, smx):
                                           s64.rey, 0.46, NPL), 32 * cells, 1, 1, 1),
         ("add_ring", lambda: K.add_inplace(tuple(x + 2 for x in s64.ncells), f["p"], f["pp"],
                                            NPL), 24 * (n + 2) ** 3, 1, 1, 1),
-        ("copy (u -> uoo)", lambda: copy(f["uoo"], f["u"]), 2 * f["u"].nbytes, 3, 3, 1),
+        ("copy (u -> uoo)", lambda: copy(f["uoo"], f["u"]), 2 * f["u"].phys_nbytes, 3, 3, 1),
         ("bc_fill (u)", lambda: s64.fill_bc("u"), 0, 15, 15, 1),
     ]
     nv = s64.mg.n_vcycles

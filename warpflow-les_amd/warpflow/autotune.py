@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 from . import _hip
-from .device import copy, current_stream, zeros
+from .device import copy, current_stream, zeros, zeros_like
 
 # knob ids (include/warpflow_hip.h)
 CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32 = range(5)
@@ -89,7 +89,7 @@ def _tune_convdiff(sol, T, reps, out):
     h = (sol.dx, sol.dy, sol.dz)
     opu = sol._op_dim("u")
     n3 = sol.ncells
-    saved = zeros(f["ustar"].shape)
+    saved = zeros_like(f["ustar"])
     copy(saved, f["ustar"])
 
     def cd():
@@ -117,7 +117,7 @@ def _tune_level(lev, T, reps, out):
     f64 = np.dtype(lev.dtype) == np.float64
     ktile = SWEEP_TILE_F64 if f64 else SWEEP_TILE_F32
     kich = SWEEP_ICH_F64 if f64 else SWEEP_ICH_F32
-    keep_phi, keep_tmp = zeros(lev.phi.shape, lev.dtype), zeros(lev.tmp.shape, lev.dtype)
+    keep_phi, keep_tmp = zeros_like(lev.phi), zeros_like(lev.tmp)
     copy(keep_phi, lev.phi)
     copy(keep_tmp, lev.tmp)
     d = lev.desc()

@@ -45,10 +45,14 @@ def launch(kernel, dim, inputs, device=None, stream=None):
 
 
 def _same(*arrs):
-    s = arrs[0].shape
+    """One shape and one memory layout (row pitch, alignment shift): a kernel
+    addresses every array it is given through one wf_grid."""
+    a0 = arrs[0]
     for a in arrs:
-        if not isinstance(a, DeviceArray) or a.shape != s:
-            raise ValueError("arrays must be DeviceArrays of one shape: %r" % (arrs,))
+        if (not isinstance(a, DeviceArray) or a.shape != a0.shape or a.pitch != a0.pitch
+                or a.lead != a0.lead):
+            raise ValueError("arrays must be DeviceArrays of one shape and layout: %r"
+                             % (arrs,))
 
 
 def _conv(comp):

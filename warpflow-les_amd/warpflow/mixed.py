@@ -53,8 +53,8 @@ class MixedMultigrid(Multigrid):
 
     def solve(self, su3: DeviceArray, pp3: DeviceArray, npl: int):
         g = grid_of(pp3, npl)
-        if su3.shape != pp3.shape:
-            raise ValueError("su and pp must have one shape")
+        if su3.shape != pp3.shape or su3.pitch != pp3.pitch or su3.lead != pp3.lead:
+            raise ValueError("su and pp must have one shape and memory layout")
         work = self._workspace(g)
         descs = self._descs()
         cycles = C.c_int(0)

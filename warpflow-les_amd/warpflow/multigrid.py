@@ -144,7 +144,7 @@ class Multigrid:
         n_vcycles V-cycles, copy-out including the ring."""
         descs = self._descs()
         g = grid_of(pp3, npl)
-        if su3.shape != pp3.shape:
-            raise ValueError("su and pp must have one shape")
+        if su3.shape != pp3.shape or su3.pitch != pp3.pitch or su3.lead != pp3.lead:
+            raise ValueError("su and pp must have one shape and memory layout")
         _hip.call("wf_mg_solve", descs, len(self.levels), su3.ptr, pp3.ptr, g, self.n_pre,
                   self.n_post, self.n_coarse, self.n_vcycles, current_stream())

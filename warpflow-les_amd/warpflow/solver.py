@@ -49,7 +49,8 @@ class FlowSolver:
         self.rey = 1.0 / cfg.physics.reynolds
 
         shape = (ni + 2 * NPL, nj + 2 * NPL, nk + 2 * NPL)
-        self.f = {name: zeros(shape, np.float64, device) for name in FIELDS}
+        # rows padded so that real cell k = NPL of every row starts a 128-byte line
+        self.f = {name: zeros(shape, np.float64, device, aligned=True) for name in FIELDS}
 
         self.halo = HaloExchanger(topo, device, NPL)
         b = cfg.boundaries
