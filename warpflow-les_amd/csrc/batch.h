@@ -86,6 +86,7 @@ enum PlanKind : int {
   PK_BC0,           // ghost-fill lines of the x, y, z axis launches
   PK_BC1,
   PK_BC2,
+  PK_WALE,          // WALE tiles (8 x 32 (j, k) x BATCH_CD_ICH planes) over the real cells
   PK_MEMBER,        // one item per member
   PK_LCELLS0,       // 64x4 blocks over level-l real cells, l = 0 .. 7
   PK_SWEEP0 = PK_LCELLS0 + BATCH_MAX_LEVELS,  // fused-sweep tiles of level l

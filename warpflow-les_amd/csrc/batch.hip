@@ -64,6 +64,8 @@ int member_count(int kind, const MemDev& M) {
     case PK_BC1:
     case PK_BC2:
       return bc_blocks(kind - PK_BC0, n0, n1, n2);
+    case PK_WALE:
+      return cd_tiles(n0, n1, n2);
     case PK_MEMBER:
       return 1;
     default:

@@ -7,6 +7,7 @@
 // the reference's evaluation order (left to right) so results agree with
 // the FP64 oracle to a few ulps.
 #include <algorithm>
+#include <type_traits>
 
 #include "batch.h"
 #include "wf_common.h"
@@ -263,25 +264,21 @@ red_fold_kernel(double* __restrict__ out, int nblk, int nq, int min_mask, int co
   }
 }
 
-__device__ inline void wale_body(const double* __restrict__ u, const double* __restrict__ v,
-                                 const double* __restrict__ w, double* __restrict__ vis,
-                                 const wf_grid& g, const Box3& b, Blk B, double dx, double dy,
-                                 double dz, double rrey, double ls) {
-  const double rdx = 1.0 / dx, rdy = 1.0 / dy, rdz = 1.0 / dz;
-  int i, j, k;
-  if (!box_thread(b, B, i, j, k)) return;
-  const int64_t x = at(g, i, j, k);
-  const int64_t X = g.si, Y = g.sj, Z = 1;
+// WALE viscosity at one cell (kernels.py:297-367) from accessors U, V, W
+// returning the velocity component at plane/row/column offset (di, dj, dk).
+template <class FU, class FV, class FW>
+__device__ inline double wale_value(const FU& U, const FV& V, const FW& W, double rdx,
+                                    double rdy, double rdz, double rrey, double ls) {
   // kernels.py:311-327
-  const double dudx = (u[x] - u[x - X]) * rdx;
-  const double dudy = (0.25 * (u[x + Y] + u[x - X + Y] - u[x - Y] - u[x - X - Y])) * rdy;
-  const double dudz = (0.25 * (u[x + Z] + u[x - X + Z] - u[x - Z] - u[x - X - Z])) * rdz;
-  const double dvdx = (0.25 * (v[x + X] + v[x + X - Y] - v[x - X] - v[x - X - Y])) * rdx;
-  const double dvdy = (v[x] - v[x - Y]) * rdy;
-  const double dvdz = (0.25 * (v[x + Z] + v[x - Y + Z] - v[x - Z] - v[x - Y - Z])) * rdz;
-  const double dwdx = (0.25 * (w[x + X] + w[x + X - Z] - w[x - X] - w[x - X - Z])) * rdx;
-  const double dwdy = (0.25 * (w[x + Y] + w[x + Y - Z] - w[x - Y] - w[x - Y - Z])) * rdy;
-  const double dwdz = (w[x] - w[x - Z]) * rdz;
+  const double dudx = (U(0, 0, 0) - U(-1, 0, 0)) * rdx;
+  const double dudy = (0.25 * (U(0, 1, 0) + U(-1, 1, 0) - U(0, -1, 0) - U(-1, -1, 0))) * rdy;
+  const double dudz = (0.25 * (U(0, 0, 1) + U(-1, 0, 1) - U(0, 0, -1) - U(-1, 0, -1))) * rdz;
+  const double dvdx = (0.25 * (V(1, 0, 0) + V(1, -1, 0) - V(-1, 0, 0) - V(-1, -1, 0))) * rdx;
+  const double dvdy = (V(0, 0, 0) - V(0, -1, 0)) * rdy;
+  const double dvdz = (0.25 * (V(0, 0, 1) + V(0, -1, 1) - V(0, 0, -1) - V(0, -1, -1))) * rdz;
+  const double dwdx = (0.25 * (W(1, 0, 0) + W(1, 0, -1) - W(-1, 0, 0) - W(-1, 0, -1))) * rdx;
+  const double dwdy = (0.25 * (W(0, 1, 0) + W(0, 1, -1) - W(0, -1, 0) - W(0, -1, -1))) * rdy;
+  const double dwdz = (W(0, 0, 0) - W(0, 0, -1)) * rdz;
   // kernels.py:330-338, g2zx and g2zy with the reference's last terms
   const double g2xx = dudx * dudx + dudy * dvdx + dudz * dwdx;
   const double g2yy = dvdx * dudy + dvdy * dvdy + dvdz * dwdy;
@@ -315,14 +312,105 @@ __device__ inline void wale_body(const double* __restrict__ u, const double* __r
     const double denom = ass * ass * sqrt(ass) + sdsd * sqrt(rs);
     nut = ls * ls * sdsd15 / denom;
   }
-  vis[x] = rrey + nut;
+  return rrey + nut;
+}
+
+// WALE streamed along i: an 8 x 32 (j, k) tile walks a chunk of planes with
+// u, v, w staged in four rolling LDS planes (tile + 1-cell halo), so every
+// global value is read once per tile and the plane written at iteration i
+// (plane i+1) is the one last read at iteration i-2 -- one barrier per plane.
+constexpr int WS_TJ = 8, WS_TK = 32, WS_RK = WS_TK + 2, WS_RN = (WS_TJ + 2) * WS_RK;
+constexpr int WS_NL = (WS_RN + 255) / 256;  // staged values per thread per field
+
+__device__ inline void wale_stream_body(const double* __restrict__ u,
+                                        const double* __restrict__ v,
+                                        const double* __restrict__ w, double* __restrict__ vis,
+                                        const wf_grid& g, const Box3& b, int ich, int ktile,
+                                        int jtile, int chunk, double rdx, double rdy, double rdz,
+                                        double rrey, double ls) {
+  __shared__ double S[4][3][WS_RN];
+  const int tid = threadIdx.x;
+  const int tj = tid / WS_TK, tk = tid % WS_TK;
+  const int j0 = jtile * WS_TJ, k0 = ktile * WS_TK;
+  const int i0 = chunk * ich, i1 = min(i0 + ich, b.d0);
+  if (i0 >= i1) return;
+  const int nI = g.ni + 2 * g.npl, nJ = g.nj + 2 * g.npl, nK = g.nk + 2 * g.npl;
+  int64_t off[WS_NL];
+  int lds[WS_NL];
+  bool on[WS_NL];
+#pragma unroll
+  for (int e = 0; e < WS_NL; ++e) {
+    const int l = tid + 256 * e;
+    on[e] = l < WS_RN;
+    const int lc = on[e] ? l : 0;
+    const int jf = min(max(b.lo1 + j0 + lc / WS_RK - 1, 0), nJ - 1);
+    const int kf = min(max(b.lo2 + k0 + lc % WS_RK - 1, 0), nK - 1);
+    off[e] = (int64_t)jf * g.sj + kf;
+    lds[e] = lc;
+  }
+  auto plane = [&](int i) { return (int64_t)min(max(b.lo0 + i, 0), nI - 1) * g.si; };
+  double ru[WS_NL], rv[WS_NL], rw[WS_NL];
+  auto load = [&](int i) {
+    const int64_t pb = plane(i);
+#pragma unroll
+    for (int e = 0; e < WS_NL; ++e) {
+      ru[e] = u[pb + off[e]];
+      rv[e] = v[pb + off[e]];
+      rw[e] = w[pb + off[e]];
+    }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int e = 0; e < WS_NL; ++e)
+      if (on[e]) {
+        S[slot][0][lds[e]] = ru[e];
+        S[slot][1][lds[e]] = rv[e];
+        S[slot][2][lds[e]] = rw[e];
+      }
+  };
+  const bool live = j0 + tj < b.d1 && k0 + tk < b.d2;
+  const int own = (tj + 1) * WS_RK + (tk + 1);
+  // slot of plane p: (p - i0 + 1) mod 4
+  load(i0 - 1);
+  store(0);
+  load(i0);
+  store(1);
+  load(i0 + 1);
+  auto step = [&](auto Rc, int i) {
+    constexpr int R = decltype(Rc)::value;  // (i - i0) mod 4
+    constexpr int SPv = R, S0v = (R + 1) % 4, SNv = (R + 2) % 4;
+    store(SNv);  // plane i+1
+    if (i + 2 <= i1) load(i + 2);
+    __syncthreads();
+    if (live) {
+      auto F = [&](int f, int di, int dj, int dk) {
+        const int sl = di < 0 ? SPv : (di > 0 ? SNv : S0v);
+        return S[sl][f][own + dj * WS_RK + dk];
+      };
+      auto U = [&](int di, int dj, int dk) { return F(0, di, dj, dk); };
+      auto V = [&](int di, int dj, int dk) { return F(1, di, dj, dk); };
+      auto W = [&](int di, int dj, int dk) { return F(2, di, dj, dk); };
+      vis[plane(i) + (int64_t)(b.lo1 + j0 + tj) * g.sj + (b.lo2 + k0 + tk)] =
+          wale_value(U, V, W, rdx, rdy, rdz, rrey, ls);
+    }
+  };
+  for (int t = 0; t < i1 - i0; t += 4) {
+    step(std::integral_constant<int, 0>{}, i0 + t);
+    if (t + 1 < i1 - i0) step(std::integral_constant<int, 1>{}, i0 + t + 1);
+    if (t + 2 < i1 - i0) step(std::integral_constant<int, 2>{}, i0 + t + 2);
+    if (t + 3 < i1 - i0) step(std::integral_constant<int, 3>{}, i0 + t + 3);
+  }
 }
 
 __global__ void __launch_bounds__(256)
-wale_kernel(const double* __restrict__ u, const double* __restrict__ v,
-            const double* __restrict__ w, double* __restrict__ vis, wf_grid g, Box3 b,
-            double dx, double dy, double dz, double rrey, double ls, int ich) {
-  for_planes(b.d0, ich, [&](Blk B) { wale_body(u, v, w, vis, g, b, B, dx, dy, dz, rrey, ls); });
+wale_stream_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                   const double* __restrict__ w, double* __restrict__ vis, wf_grid g, Box3 b,
+                   int ich, int kt, int jt, double rdx, double rdy, double rdz, double rrey,
+                   double ls) {
+  const int nb = kt * jt * gridDim.y;
+  const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
+  wale_stream_body(u, v, w, vis, g, b, ich, L % kt, (L / kt) % jt, L / (kt * jt), rdx, rdy, rdz,
+                   rrey, ls);
 }
 
 __global__ void __launch_bounds__(256)
@@ -491,10 +579,12 @@ __global__ void __launch_bounds__(256) batch_fill_vis_kernel(BView V, Plan P, do
 __global__ void __launch_bounds__(256)
 batch_wale_kernel(BView V, Plan P, const double* u, const double* v, const double* w,
                   double* vis) {
-  for_items<false>(P, V, nullptr, [&](int, int q, const MemDev& M) {
+  for_items<true>(P, V, nullptr, [&](int, int q, const MemDev& M) {
     const Box3 b = cells_box(M.g);
-    wale_body(u + M.off, v + M.off, w + M.off, vis + M.off, M.g, b, cells_blk(q, b.d1, b.d2),
-              M.dx, M.dy, M.dz, M.rey, M.ls);
+    const int kt = (b.d2 + WS_TK - 1) / WS_TK, jt = (b.d1 + WS_TJ - 1) / WS_TJ;
+    wale_stream_body(u + M.off, v + M.off, w + M.off, vis + M.off, M.g, b, BATCH_CD_ICH, q % kt,
+                     (q / kt) % jt, q / (kt * jt), 1.0 / M.dx, 1.0 / M.dy, 1.0 / M.dz, M.rey,
+                     M.ls);
   });
 }
 
@@ -586,8 +676,8 @@ int batch_fill_vis(const Batch& B, double* vis, hipStream_t s) {
 
 int batch_wale(const Batch& B, const double* u, const double* v, const double* w, double* vis,
                hipStream_t s) {
-  const Plan& P = B.plan[PK_CELLS];
-  hipLaunchKernelGGL(batch_wale_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, u, v, w,
+  const Plan& P = B.plan[PK_WALE];
+  hipLaunchKernelGGL(batch_wale_kernel, dim3(P.grid), dim3(256), 0, s, B.view(), P, u, v, w,
                      vis);
   WF_LAUNCHED();
   return 0;
@@ -792,8 +882,11 @@ int wf_wale(const double* u, const double* v, const double* w, double* vis, wf_g
   const Box3 b = npl_box(g, d0, d1, d2);
   if (!box_ok(g, b, 1, 1)) return WF_EINVAL;
   if (empty(b)) return 0;
-  hipLaunchKernelGGL(wale_kernel, CG(b), box_block(), 0, S_(stream), u, v, w, vis, g,
-                     b, dx, dy, dz, rrey, cw * delta, CH(b));
+  const int kt = div_up(b.d2, WS_TK), jt = div_up(b.d1, WS_TJ);
+  const int ich = (int)std::max<int64_t>(16, ((int64_t)kt * jt * b.d0 + 2047) / 2048);
+  hipLaunchKernelGGL(wale_stream_kernel, dim3(kt * jt, div_up(b.d0, ich)), dim3(256), 0,
+                     S_(stream), u, v, w, vis, g, b, ich, kt, jt, 1.0 / dx, 1.0 / dy, 1.0 / dz,
+                     rrey, cw * delta);
   WF_LAUNCHED();
   return 0;
 }
