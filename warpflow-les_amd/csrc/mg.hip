@@ -165,232 +165,362 @@ prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev 
   for_planes(f.n0, ich, [&](Blk B) { prolong_body(phic, c, phif, f, B); });
 }
 
+// Buffer access to a level array: the base, chosen per workgroup, sits in a
+// scalar buffer descriptor, the plane offset in the scalar soffset operand,
+// and each lane supplies a 32-bit byte offset within the plane, so a load or
+// store costs no vector address arithmetic.  A lane offset of SWEEP_DEAD
+// (or SWEEP_DEAD plus or minus one row) lies past every descriptor's range --
+// the host keeps a chunk's plane offsets below 2^30 -- so its load returns
+// zero and its store is dropped, which is how cells outside the domain are
+// masked.  Descriptor
+// word 3 = 0x00020000: untyped 32-bit data format, the value the gfx9 family
+// (gfx950 included) expects for raw buffer instructions.
+constexpr uint32_t SWEEP_DEAD = 0xc0000000u;
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+template <class T>
+__device__ inline T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t boff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)boff, soff, 0));
+}
+template <class T>
+__device__ inline void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t boff, int soff, T v) {
+  if constexpr (sizeof(T) == 8) {
+    using U2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)boff, soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)boff, soff,
+                                          0);
+  }
+}
+
 // One full RBGS sweep (colour 0, then colour 1, then the ring refill) out of
 // place: pout = S(pin).  Colour-0 (odd i+j+k) cells read only even cells of
 // pin; colour-1 (even) cells read only the new odd values.  A workgroup owns a
-// TJ x TK (j, k) tile (512 cells = 256 (even, odd) pairs, one per thread) and
-// streams along i over a chunk of planes, keeping three planes of pin (tile +
-// 2-cell halo) and three planes of new odd values (tile + 1-cell halo) in LDS:
-// at plane q it computes the odd values of q, then the even values of q-1,
-// and writes plane q-1 of pout (both colours + face-ring images).  The next
-// plane of pin and the next cells' rhs are prefetched into registers while
-// the current plane computes.
+// TJ x TK (j, k) tile (TJ*TK/2 (even, odd) pairs, NPAIR per thread) and
+// streams along i over a chunk of planes.  LDS holds, for three planes, the
+// even cells of pin over the tile + 2-cell halo and the new odd values over
+// the tile + 1-cell halo, each packed checkerboard-wise (a row keeps only its
+// cells of the one parity, so neighbouring lanes touch neighbouring words).
+// At plane q the workgroup computes the odd values of q, then the even values
+// of q-1, and writes plane q-1 of pout (both colours + face-ring images).
+// Global loads run two planes ahead of their use.
 //
-// The loop body is branch-free: load addresses are clamped into the level
-// array (ring included).  A clamped value can only reach an odd cell outside
-// the domain, and those are forced to 0 (their coefficients are 0 in every
-// real cell's update), so clamping changes no result.  Per-thread offsets
-// within a plane depend only on the parity of the plane and are computed once.
-// HBM traffic per sweep: one read of pin and rhs, one write of pout; the
-// per-cell arithmetic is the half-sweep kernel's, in the same order.
+// Coefficients.  A neighbour outside the domain carries a zero coefficient in
+// _mgcoeff; here its value enters LDS as zero instead (pin cells outside the
+// domain, odd cells outside the domain: their reciprocal diagonal is zero),
+// so every cell uses the uniform coefficients c0, c1, c2 in the same order of
+// terms, and only the diagonal depends on the cell.  A zero product adds
+// nothing either way, so the sums are the coefficient-array sums bit for bit.
+// The reciprocal diagonal of each cell the thread owns is computed once for
+// the interior planes; the planes at the i faces recompute it.
+//
+// Plane parity.  Which k of a row holds a colour alternates with the plane;
+// the loop is unrolled over six planes (3 LDS slots x 2 parities) so every
+// per-parity offset is a compile-time choice between two registers, and
+// every iteration issues the same memory instructions (loads that are not
+// needed read a valid plane and are ignored), so the waits on the prefetched
+// planes stay exact.  HBM traffic per sweep: one read of pin and rhs, one
+// write of pout.
 template <class T, int TJ, int TK>
 __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pout,
                                   const T* __restrict__ rhs, const MgDev& d, T c0, T c1, T c2,
                                   int ich, Blk B) {
   constexpr int NPAIR = TJ * TK / 512;  // (even, odd) pairs per thread
   static_assert(TJ * TK == 512 * NPAIR && (NPAIR == 1 || NPAIR == 2), "256 threads, 1-2 pairs");
-  constexpr int PJ = TJ + 4, PK = TK + 4, PN = PJ * PK;
-  constexpr int OJ = TJ + 2, OK = TK + 2, OH = OK / 2;
-  constexpr int NP = (PN + 255) / 256;        // pin values per thread per plane
-  constexpr int NO = (OJ * OH + 255) / 256;   // odd cells per thread per plane
-  constexpr int HK = TK / 2;                  // pairs per tile row
-  __shared__ T P[3][PN];
-  __shared__ T O[3][OJ * OK];
+  static_assert(TK % 2 == 0, "pairs along k");
+  constexpr int PJ = TJ + 4, PH = (TK + 4) / 2;  // pin halo rows, even cells per row
+  constexpr int OJ = TJ + 2, OH = (TK + 2) / 2;  // odd halo rows, odd cells per row
+  constexpr int HK = TK / 2;                     // pairs per tile row
+  constexpr int NPE = (PJ * PH + 255) / 256;     // pin cells per thread per plane
+  constexpr int NO = (OJ * OH + 255) / 256;      // odd cells per thread per plane
+  __shared__ T P[3][NPE * 256];
+  __shared__ T O[3][NO * 256];
   const int tid = threadIdx.x;
   const int j0 = B.y * TJ, k0 = B.x * TK;
   const int i0 = B.z * ich, i1 = min(i0 + ich, d.n0);
   const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
   const int s1 = (int)d.s1;
+  const int sb = (i0 - 1 + j0 + k0) & 1;  // parity class of plane i0-1; index r: sb ^ r
+  const int imid = n0 >= 3 ? 1 : 0;       // an interior plane (unused when n0 < 3)
   auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
-  // offset of (j, k) inside one level plane, clamped into the ring
-  auto plane_off = [&](int j, int k) {
-    return (clampi(j, -1, n1) + 1) * s1 + (clampi(k, -1, n2) + 1);
+  // byte offset of (j, k) inside one level plane
+  auto plane_boff = [&](int j, int k) {
+    return (uint32_t)(((j + 1) * s1 + (k + 1)) * (int)sizeof(T));
   };
-  auto plane_ptr = [&](const T* a, int q) {  // plane q (clamped into the ring)
-    return a + (int64_t)(clampi(q, -1, n0) + 1) * d.s0;
+  auto in_jk = [&](int j, int k) { return j >= 0 && j < n1 && k >= 0 && k < n2; };
+  auto rdiag = [&](int i, int j, int k) {
+    return T(1) / mg_coef(d, i, clampi(j, 0, n1 - 1), clampi(k, 0, n2 - 1), c0, c1, c2).ap;
   };
-  // Inner tile: every odd cell of the 1-cell-halo tile is a real cell off
-  // the j and k faces, so its j/k coefficients are c1, c1, c2, c2 and only
-  // the plane decides aw, ae and ap (uniform over the workgroup).
-  const bool inner = j0 >= 2 && j0 + TJ <= n1 - 2 && k0 >= 2 && k0 + TK <= n2 - 2;
+  // odd cell l = tid + 256 e of the 1-halo tile: row jr, packed column m;
+  // on parity class s its k is k0 - 1 + 2m + ph, ph = (s + jr + 1) & 1
+  auto odd_cell = [&](int t, int e, int r, int& j, int& k) {
+    const int l = min(t + 256 * e, OJ * OH - 1);
+    const int jr = l / OH, m = l % OH;
+    j = j0 - 1 + jr;
+    k = k0 - 1 + 2 * m + ((sb ^ r) + jr + 1) % 2;
+    return t + 256 * e < OJ * OH && in_jk(j, k);
+  };
+  // pair p = tid + 256 e: row jr, k_a = k0 + 2 mm; the even cell is k_a + kx
+  auto pair_cell = [&](int t, int e, int r, int& j, int& ke, int& ko) {
+    const int p = t + 256 * e;
+    const int jr = p / HK, mm = p % HK;
+    const int kx = ((sb ^ r) + jr) % 2;
+    j = j0 + jr;
+    ke = k0 + 2 * mm + kx;
+    ko = k0 + 2 * mm + 1 - kx;
+  };
+  // Every tile cell is a real cell off the j and k faces: no j/k ring images.
+  const bool wfast = j0 >= 1 && j0 + TJ <= n1 - 1 && k0 >= 1 && k0 + TK <= n2 - 1;
 
-  // pin-plane loads: l = tid + 256 e covers the (TJ+4) x (TK+4) halo tile
-  int poff[NP], pl_[NP];
+  // pin cells: LDS word w = tid + 256 e = row jr, packed column h; on class s
+  // the cell is (j0 - 2 + jr, k0 - 2 + 2h + ((s + jr) & 1))
+  uint32_t gofs[2][NPE];
 #pragma unroll
-  for (int e = 0; e < NP; ++e) {
-    const int l = min(tid + 256 * e, PN - 1);
-    pl_[e] = tid + 256 * e;
-    poff[e] = plane_off(j0 - 2 + l / PK, k0 - 2 + l % PK);
+  for (int e = 0; e < NPE; ++e) {
+    const int w = tid + 256 * e;
+    const int wc = min(w, PJ * PH - 1);
+    const int jr = wc / PH, h = wc % PH;
+    const int j = j0 - 2 + jr;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int k = k0 - 2 + 2 * h + ((sb ^ r) + jr) % 2;
+      gofs[r][e] = w < PJ * PH && in_jk(j, k) ? plane_boff(j, k) : SWEEP_DEAD;
+    }
   }
-  // odd cells of the 1-cell-halo tile, per plane parity par = q & 1
-  int ooff[2][NO], oP[2][NO], oO[2][NO], oj[NO], ok[2][NO];
-  bool olive[NO], oin[2][NO];
+  // odd cells: rhs offsets, LDS columns (k neighbours at a0, a0+1 of pin row
+  // jr+1; j and i neighbours at column aj), interior-plane 1/ap (0 if dead)
+  uint32_t oofs[2][NO];
+  int a0[NO], aj[2][NO];
+  T rapo[2][NO];
 #pragma unroll
   for (int e = 0; e < NO; ++e) {
-    const int l = tid + 256 * e;
-    olive[e] = l < OJ * OH;
-    const int lc = olive[e] ? l : 0;
-    const int jj = lc / OH, m = lc % OH;
-    oj[e] = j0 - 1 + jj;
+    const int l = min(tid + 256 * e, OJ * OH - 1);
+    const int jr = l / OH, m = l % OH;
+    a0[e] = (jr + 1) * PH + m;
 #pragma unroll
-    for (int par = 0; par < 2; ++par) {
-      const int kk = 2 * m + ((par + oj[e] + k0) & 1);
-      ok[par][e] = k0 - 1 + kk;
-      oin[par][e] = oj[e] >= 0 && oj[e] < n1 && ok[par][e] >= 0 && ok[par][e] < n2;
-      ooff[par][e] = plane_off(oj[e], ok[par][e]);
-      oP[par][e] = (jj + 1) * PK + (kk + 1);
-      oO[par][e] = jj * OK + kk;
+    for (int r = 0; r < 2; ++r) {
+      int j, k;
+      const bool live = odd_cell(tid, e, r, j, k);
+      oofs[r][e] = live ? plane_boff(j, k) : SWEEP_DEAD;
+      aj[r][e] = a0[e] + ((sb ^ r) + jr + 1) % 2;
+      rapo[r][e] = live ? rdiag(imid, j, k) : T(0);
     }
   }
-  // this thread's (even, odd) pairs p = tid + 256 e: k_a = k0 + 2m,
-  // k_b = k_a + 1; which one is even alternates with the plane parity
-  int ej[NPAIR], ka[NPAIR], erow[NPAIR], eoff[NPAIR];
-  bool ej_in[NPAIR];
+  // pairs: offsets of the even and the odd cell (rhs and output), LDS columns
+  // (k neighbours of the even cell at b0, b0+1 of odd row jr+1, j and i
+  // neighbours at column bj, the odd partner at bo), interior-plane 1/ap
+  // Face flags of the pair's cells a = k_a and b = k_a + 1 (set for live
+  // cells only): FL_JLO/FL_JHI j == 0 / n1-1, FL_ALO/FL_AHI k_a == 0 / n2-1,
+  // FL_BHI k_a + 1 == n2-1; FL_X: the even cell is b on class 0.
+  constexpr int FL_JLO = 1, FL_JHI = 2, FL_ALO = 4, FL_AHI = 8, FL_BHI = 16, FL_AL = 32,
+                FL_BL = 64, FL_X = 128;
+  uint32_t eofs[2][NPAIR], pofs[2][NPAIR];
+  int b0[NPAIR], bj[2][NPAIR], bo[2][NPAIR], fl[NPAIR];
+  T rape[2][NPAIR];
 #pragma unroll
   for (int e = 0; e < NPAIR; ++e) {
-    const int pr = tid + 256 * e;
-    ej[e] = j0 + pr / HK;
-    ka[e] = k0 + 2 * (pr % HK);
-    erow[e] = (pr / HK + 1) * OK + (ka[e] - k0 + 1);  // O index of k_a
-    eoff[e] = plane_off(ej[e], ka[e]);                 // plane offset of k_a
-    ej_in[e] = ej[e] < n1;
+    const int p = tid + 256 * e;
+    const int jr = p / HK, mm = p % HK;
+    b0[e] = (jr + 1) * OH + mm;
+    {
+      const int j = j0 + jr, ka = k0 + 2 * mm;
+      const bool la = j < n1 && ka < n2, lb = j < n1 && ka + 1 < n2;
+      fl[e] = (la || lb ? (j == 0 ? FL_JLO : 0) | (j == n1 - 1 ? FL_JHI : 0) : 0) |
+              (la ? (ka == 0 ? FL_ALO : 0) | (ka == n2 - 1 ? FL_AHI : 0) | FL_AL : 0) |
+              (lb ? (ka + 1 == n2 - 1 ? FL_BHI : 0) | FL_BL : 0) |
+              ((sb + jr) % 2 ? FL_X : 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      int j, ke, ko;
+      pair_cell(tid, e, r, j, ke, ko);
+      const int kx = ke - (k0 + 2 * mm);
+      eofs[r][e] = j < n1 && ke < n2 ? plane_boff(j, ke) : SWEEP_DEAD;
+      pofs[r][e] = j < n1 && ko < n2 ? plane_boff(j, ko) : SWEEP_DEAD;
+      bj[r][e] = b0[e] + kx;
+      bo[r][e] = b0[e] + 1 - kx;
+      rape[r][e] = rdiag(imid, j, min(ke, n2 - 1));
+    }
   }
 
-  // coefficients of plane q for an inner tile, in mg_coef's order
-  auto plane_coef = [&](int q) {
-    Coef<T> cf;
-    cf.aw = q == 0 ? T(0) : c0;
-    cf.ae = q == n0 - 1 ? T(0) : c0;
-    cf.as = c1;
-    cf.an = c1;
-    cf.ab = c2;
-    cf.at = c2;
-    cf.ap = -(cf.aw + cf.ae + cf.as + cf.an + cf.ab + cf.at);
-    return cf;
+  // The rare i-face paths recompute their cells from an opaque copy of the
+  // thread index, so the compiler keeps none of that work live across the
+  // loop.
+  auto opaque_tid = [&]() {
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    return t;
+  };
+  // Descriptors start at array plane i0 (real plane i0-1); real plane x >= i0-1
+  // sits at soffset (x + 1 - i0) * pb.
+  const int pb = (int)(d.s0 * (int64_t)sizeof(T));
+  auto chunk_rsrc = [&](const T* a) {
+    return buf_rsrc(reinterpret_cast<const char*>(a) + (int64_t)i0 * pb, 0x7fffffffu);
+  };
+  const auto dpin = chunk_rsrc(pin), drhs = chunk_rsrc(rhs), dout = chunk_rsrc(pout);
+  const auto dnone = buf_rsrc(pout, 0u);  // every store dropped
+  auto soff = [&](int x) { return (x + 1 - i0) * pb; };
+
+  // Loads run two planes ahead of their use.  pre[c] holds the next pin
+  // plane of class c; rodd[x] and reven[x] the odd- and even-cell rhs of the
+  // plane whose LDS slot is x (three consecutive planes).
+  T pre[2][NPE], rodd[3][NO], reven[3][NPAIR];
+  auto load_pin = [&](auto Rc, int x) {  // pin plane x (array plane in range) of class r
+    constexpr int r = decltype(Rc)::value;
+#pragma unroll
+    for (int e = 0; e < NPE; ++e) pre[r][e] = buf_ld<T>(dpin, gofs[r][e], soff(x));
+  };
+  // pre[r] (plane x) to LDS; the ring planes -1 and n0 enter as zero
+  auto store_pin = [&](auto Rc, T* dst, int x) {
+    constexpr int r = decltype(Rc)::value;
+    const bool z = x < 0 || x >= n0;
+#pragma unroll
+    for (int e = 0; e < NPE; ++e) dst[tid + 256 * e] = z ? T(0) : pre[r][e];
+  };
+  auto load_rhs_odd = [&](auto Rc, auto Xc, int x) {  // plane x of class r into slot X
+    constexpr int r = decltype(Rc)::value, X = decltype(Xc)::value;
+#pragma unroll
+    for (int e = 0; e < NO; ++e) rodd[X][e] = buf_ld<T>(drhs, oofs[r][e], soff(x));
+  };
+  auto load_rhs_even = [&](auto Rc, auto Xc, int x) {
+    constexpr int r = decltype(Rc)::value, X = decltype(Xc)::value;
+#pragma unroll
+    for (int e = 0; e < NPAIR; ++e) reven[X][e] = buf_ld<T>(drhs, eofs[r][e], soff(x));
   };
 
-  // Loads run one plane ahead of their use: pre holds pin plane q+1,
-  // rodd/rodd_n the odd-cell rhs of planes q, q+1, and reven/reven_n the
-  // even-cell rhs of planes q-1, q.
-  T pre[NP], rodd[NO], rodd_n[NO];
-  T reven[NPAIR], reven_n[NPAIR];
-#pragma unroll
-  for (int e = 0; e < NPAIR; ++e) reven[e] = reven_n[e] = T(0);
-  auto gload_plane = [&](int q, T (&dst)[NP]) {
-    const T* b = plane_ptr(pin, q);
-#pragma unroll
-    for (int e = 0; e < NP; ++e) dst[e] = b[poff[e]];
-  };
-  auto store_plane = [&](T* dst, const T (&src)[NP]) {
-#pragma unroll
-    for (int e = 0; e < NP; ++e)
-      if (pl_[e] < PN) dst[pl_[e]] = src[e];
-  };
-  auto gload_rhs_odd = [&](int q, T (&dst)[NO]) {
-    const T* b = plane_ptr(rhs, q);
-    const int par = q & 1;
-#pragma unroll
-    for (int e = 0; e < NO; ++e) dst[e] = b[ooff[par][e]];
-  };
-  auto gload_rhs_even = [&](int pl, int e) {
-    const int kx = ((pl + ej[e] + k0) & 1) ? 1 : 0;  // k_a even -> 0, else k_b
-    return plane_ptr(rhs, pl)[eoff[e] + kx];
-  };
-
-  // LDS slot of plane q is (q - i0 + 2) mod 3; iteration t handles q = i0-1+t,
-  // so with R = t mod 3 every slot below is a compile-time constant.
-  auto step = [&](auto Rc, int q) {
+  // Iteration t handles q = i0-1+t.  Plane x sits in LDS slot
+  // (x - i0 + 2) mod 3 and has parity class sb ^ ((x - i0 + 1) mod 2); for
+  // q-1, q, q+1 these are compile-time constants here.  Planes past what the
+  // chunk needs are clamped to one it may read and their values ignored.
+  const int xpin = min(i1 + 1, n0);  // last pin plane read (n0: the ring plane)
+  auto step = [&](auto Rc, auto Cc, int q) {
     constexpr int R = decltype(Rc)::value;
+    constexpr int r = decltype(Cc)::value, rn = r ^ 1;  // classes of planes q, q +- 1
     constexpr int SM = R, S0 = (R + 1) % 3, SP = (R + 2) % 3;  // planes q-1, q, q+1
-    store_plane(P[SP], pre);  // plane q+1
-#pragma unroll
-    for (int e = 0; e < NO; ++e) rodd[e] = rodd_n[e];
-#pragma unroll
-    for (int e = 0; e < NPAIR; ++e) reven[e] = reven_n[e];
-    if (q + 2 <= i1 + 1) gload_plane(q + 2, pre);
-    if (q + 1 <= i1) gload_rhs_odd(q + 1, rodd_n);
-    if (q >= i0 && q + 1 <= i1) {
-#pragma unroll
-      for (int e = 0; e < NPAIR; ++e) reven_n[e] = gload_rhs_even(q, e);
-    }
+    using Crn = std::integral_constant<int, rn>;
+    store_pin(Crn{}, P[SP], q + 1);
+    load_pin(Crn{}, min(q + 3, xpin));
+    load_rhs_odd(std::integral_constant<int, r>{}, std::integral_constant<int, SM>{},
+                 min(q + 2, i1));
+    load_rhs_even(Crn{}, std::integral_constant<int, SP>{}, min(q + 1, i1));
     __syncthreads();
     {  // odd values of plane q over the halo tile
-      const int par = q & 1;
-      const bool qin = q >= 0 && q < n0;
-      const T* pm = P[SM];
-      const T* p0 = P[S0];
-      const T* pp = P[SP];
-      T* o = O[S0];
-      const Coef<T> cq = plane_coef(clampi(q, 0, n0 - 1));
-      const T rq = T(1) / cq.ap;
+      T rp[NO];
+#pragma unroll
+      for (int e = 0; e < NO; ++e) rp[e] = rapo[r][e];
+      if (q <= 0 || q >= n0 - 1) {  // i faces (and the ring planes: all dead)
+        const int t = opaque_tid();
+        const bool qin = q >= 0 && q < n0;
+#pragma unroll
+        for (int e = 0; e < NO; ++e) {
+          int j, k;
+          const bool live = odd_cell(t, e, r, j, k) && qin;
+          rp[e] = live ? rdiag(qin ? q : 0, j, k) : T(0);
+        }
+      }
 #pragma unroll
       for (int e = 0; e < NO; ++e) {
-        const int c = oP[par][e];
-        const Coef<T> cf =
-            inner ? cq
-                  : mg_coef(d, clampi(q, 0, n0 - 1), clampi(oj[e], 0, n1 - 1),
-                            clampi(ok[par][e], 0, n2 - 1), c0, c1, c2);
-        const T rap = inner ? rq : T(1) / cf.ap;
-        const T sum = cf.aw * pm[c] + cf.ae * pp[c] + cf.as * p0[c - PK] +
-                      cf.an * p0[c + PK] + cf.ab * p0[c - 1] + cf.at * p0[c + 1];
-        const T v = (rodd[e] - sum) * rap;
-        if (olive[e]) o[oO[par][e]] = (qin && (inner || oin[par][e])) ? v : T(0);
+        const int a = a0[e], c = aj[r][e];
+        const T sum = c0 * P[SM][c] + c0 * P[SP][c] + c1 * P[S0][c - PH] +
+                      c1 * P[S0][c + PH] + c2 * P[S0][a] + c2 * P[S0][a + 1];
+        O[S0][tid + 256 * e] = (rodd[S0][e] - sum) * rp[e];
       }
     }
     __syncthreads();
+    // Even values of plane pl (class rn) and the write of plane pl.  The
+    // first two iterations (pl < i0) compute too, into dropped stores.
     const int pl = q - 1;
+    const bool own = pl >= i0;
+    T re[NPAIR];
+#pragma unroll
+    for (int e = 0; e < NPAIR; ++e) re[e] = rape[rn][e];
+    if (pl == 0 || pl == n0 - 1) {
+      const int t = opaque_tid();
+#pragma unroll
+      for (int e = 0; e < NPAIR; ++e) {
+        int j, ke, ko;
+        pair_cell(t, e, rn, j, ke, ko);
+        re[e] = rdiag(pl, j, min(ke, n2 - 1));
+      }
+    }
+    const auto dst = own ? dout : dnone;
+    const int so = own ? soff(pl) : 0;
 #pragma unroll
     for (int e = 0; e < NPAIR; ++e) {
-      if (!(pl >= i0 && ej_in[e])) continue;  // even values of plane pl, write plane pl
-      const T* om = O[SP];  // plane pl-1
-      const T* o0 = O[SM];  // plane pl
-      const T* op = O[S0];  // plane pl+1
-      const int kx = ((pl + ej[e] + k0) & 1) ? 1 : 0;  // offset of the even cell
-      const int ke = ka[e] + kx, ko = ka[e] + 1 - kx;
-      const int c = erow[e] + kx;
-      const Coef<T> cf =
-          inner ? plane_coef(pl) : mg_coef(d, pl, ej[e], min(ke, n2 - 1), c0, c1, c2);
-      const T sum = cf.aw * om[c] + cf.ae * op[c] + cf.as * o0[c - OK] + cf.an * o0[c + OK] +
-                    cf.ab * o0[c - 1] + cf.at * o0[c + 1];
-      const T ve = (reven[e] - sum) * (T(1) / cf.ap);
-      const T vo = o0[erow[e] + 1 - kx];
-      T* out = pout + (int64_t)(pl + 1) * d.s0;
-      const int eo = eoff[e];
-      if (inner) {  // only the i faces can have ring images
-        out[eo + kx] = ve;
-        out[eo + 1 - kx] = vo;
+      const int b = b0[e], c = bj[rn][e];
+      const T sum = c0 * O[SP][c] + c0 * O[S0][c] + c1 * O[SM][c - OH] + c1 * O[SM][c + OH] +
+                    c2 * O[SM][b] + c2 * O[SM][b + 1];
+      const T ve = (reven[SM][e] - sum) * re[e];
+      const T vo = O[SM][bo[rn][e]];
+      const uint32_t oe = eofs[rn][e], oo = pofs[rn][e];
+      buf_st(dst, oe, so, ve);
+      buf_st(dst, oo, so, vo);
+      if (own && (pl == 0 || pl == n0 - 1)) {  // i-face ring images
         if (pl == 0) {
-          out[eo + kx - d.s0] = ve;
-          out[eo + 1 - kx - d.s0] = vo;
+          buf_st(dout, oe, soff(-1), ve);
+          buf_st(dout, oo, soff(-1), vo);
         }
         if (pl == n0 - 1) {
-          out[eo + kx + d.s0] = ve;
-          out[eo + 1 - kx + d.s0] = vo;
+          buf_st(dout, oe, soff(n0), ve);
+          buf_st(dout, oo, soff(n0), vo);
         }
-      } else {
-        if (ke < n2) {
-          out[eo + kx] = ve;
-          ring_images(pout, d, pl, ej[e], ke, mg_at(d, pl, ej[e], ke), ve);
+      }
+      if (!wfast) {  // j and k face-ring images of cells a and b (dst: dropped if !own)
+        const int f = fl[e];
+        const bool xb = ((f & FL_X) != 0) != (rn != 0);  // the even cell is b
+        const uint32_t oa = xb ? oo : oe, ob = xb ? oe : oo;
+        const T va = xb ? vo : ve, vb = xb ? ve : vo;
+        const uint32_t sj = (uint32_t)(s1 * (int)sizeof(T)), sk = (uint32_t)sizeof(T);
+        const uint32_t jd = (f & FL_JLO) ? 0u - sj : sj;
+        const bool jf = (f & (FL_JLO | FL_JHI)) != 0;
+        buf_st(dst, jf && (f & FL_AL) ? oa + jd : SWEEP_DEAD, so, va);
+        buf_st(dst, jf && (f & FL_BL) ? ob + jd : SWEEP_DEAD, so, vb);
+        buf_st(dst, (f & FL_ALO) ? oa - sk : (f & FL_AHI) ? oa + sk : SWEEP_DEAD, so, va);
+        buf_st(dst, (f & FL_BHI) ? ob + sk : SWEEP_DEAD, so, vb);
+        if (n1 == 1) {  // both j faces: the high images too
+          buf_st(dst, (f & FL_AL) ? oa + sj : SWEEP_DEAD, so, va);
+          buf_st(dst, (f & FL_BL) ? ob + sj : SWEEP_DEAD, so, vb);
         }
-        if (ko < n2) {
-          out[eo + 1 - kx] = vo;
-          ring_images(pout, d, pl, ej[e], ko, mg_at(d, pl, ej[e], ko), vo);
-        }
+        if (n2 == 1) buf_st(dst, (f & FL_AL) ? oa + sk : SWEEP_DEAD, so, va);
       }
     }
   };
 
-  gload_plane(i0 - 2, pre);
-  store_plane(P[0], pre);
-  gload_plane(i0 - 1, pre);
-  store_plane(P[1], pre);
-  gload_plane(i0, pre);
-  gload_rhs_odd(i0 - 1, rodd_n);
-  const int cnt = i1 - i0 + 2;  // q = i0-1 .. i1
-  for (int t = 0; t < cnt; t += 3) {
-    step(std::integral_constant<int, 0>{}, i0 - 1 + t);
-    if (t + 1 < cnt) step(std::integral_constant<int, 1>{}, i0 + t);
-    if (t + 2 < cnt) step(std::integral_constant<int, 2>{}, i0 + 1 + t);
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  if (i0 >= 2) {  // pin plane i0-2 lies before the descriptors' base
+    const auto dp = buf_rsrc(reinterpret_cast<const char*>(pin) + (int64_t)(i0 - 1) * pb,
+                             0x7fffffffu);
+#pragma unroll
+    for (int e = 0; e < NPE; ++e) pre[1][e] = buf_ld<T>(dp, gofs[1][e], 0);
+  }
+  store_pin(I1{}, P[0], i0 >= 2 ? i0 - 2 : -1);
+  load_pin(I0{}, i0 - 1);
+  store_pin(I0{}, P[1], i0 - 1);
+  load_pin(I1{}, i0);
+  load_pin(I0{}, min(i0 + 1, xpin));
+  load_rhs_odd(I0{}, I1{}, i0 - 1);  // slot 1
+  load_rhs_odd(I1{}, I2{}, i0);      // slot 2
+  const int cnt = i1 - i0 + 2;  // q = i0-1 .. i1, cnt >= 3
+  for (int t = 0;; t += 6) {
+    step(I0{}, I0{}, i0 - 1 + t);
+    if (t + 1 >= cnt) break;
+    step(I1{}, I1{}, i0 + t);
+    if (t + 2 >= cnt) break;
+    step(I2{}, I0{}, i0 + 1 + t);
+    if (t + 3 >= cnt) break;
+    step(I0{}, I1{}, i0 + 2 + t);
+    if (t + 4 >= cnt) break;
+    step(I1{}, I0{}, i0 + 3 + t);
+    if (t + 5 >= cnt) break;
+    step(I2{}, I1{}, i0 + 4 + t);
+    if (t + 6 >= cnt) break;
   }
 }
 template <class T, int TJ, int TK>
@@ -853,8 +983,12 @@ int launch_sweep(const wf_mg_level& L, const void* pin, void* pout, hipStream_t 
   const int64_t tiles = (int64_t)kt * jt;
   const int tuned = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP_ICH_F64 : WF_TUNE_SWEEP_ICH_F32,
                              d.n0, d.n1, d.n2);
-  const int ich = tuned > 0 ? tuned
-                            : (int)std::max<int64_t>(4, std::min<int64_t>(16, tiles * d.n0 / 2048));
+  int ich = tuned > 0 ? tuned
+                      : (int)std::max<int64_t>(4, std::min<int64_t>(16, tiles * d.n0 / 2048));
+  // the kernel's plane offsets within a chunk ((ich + 3) planes) stay below 2^30
+  const int64_t pb = d.s0 * (int64_t)sizeof(T);
+  if ((int64_t)4 * pb >= ((int64_t)1 << 30)) return WF_EINVAL;
+  ich = (int)std::min<int64_t>(ich, ((int64_t)1 << 30) / pb - 3);
   const dim3 grid(kt, jt, div_up(d.n0, ich));
   hipLaunchKernelGGL((sweep_kernel<T, TJ, TK>), grid, dim3(256), 0, s, (const T*)pin, (T*)pout,
                      (const T*)L.rhs, d, c0, c1, c2, ich);
