@@ -95,7 +95,7 @@ def run_form(a, precision):
     sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
     tuned = None
     if not a.no_autotune:  # launch shapes for this grid and form, untimed, arrays restored
-        tuned = [[k, list(n3), v] for k, n3, v in autotune.tune_solver(sol, reps=5)["entries"]]
+        tuned = [[k, list(n3), v] for k, n3, v in autotune.tune_solver(sol, reps=10)["entries"]]
     for _ in range(a.warmup):
         sol.step()
     synchronize()

@@ -203,6 +203,13 @@ int wf_bc_const(double* fld, wf_grid g, int axis, int high, int g0, double val,
  * per cell (each side operation only touches its own column). */
 int wf_bc_fill(double* fld, wf_grid g, int which, double param, void* stream);
 
+/* apply_u(u, lid velocity ulid), apply_v(v), apply_w(w) (bcs.py:201-265) --
+ * the three fills solver.py issues back to back (boundu, boundv, boundw) --
+ * on three fields of one geometry, as three launches (one per axis) that
+ * each cover the three fields.  The fields are independent, so the result is
+ * that of three wf_bc_fill calls. */
+int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream);
+
 /* ------------------------------------------------------------------------
  * Multigrid (warpflow/multigrid.py).  Level arrays follow wf_mg_level.
  * ---------------------------------------------------------------------- */

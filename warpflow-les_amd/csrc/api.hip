@@ -1,6 +1,7 @@
 // Runtime side of the C ABI: device selection, device memory, copies,
 // fills, streams and events.  Fills use a kernel rather than
 // hipMemsetAsync so that the same calls are safe inside captured graphs.
+#include <cstring>
 #include <string.h>
 
 #include "wf_common.h"
@@ -131,9 +132,23 @@ int wf_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
   return (int)hipStreamSynchronize((hipStream_t)stream);
 }
 
+// Small read-backs (reduction results, a few scalars per step) go through a
+// pinned bounce buffer held per host thread for the life of the process:
+// the copy then lands in host memory directly instead of through the
+// runtime's pageable staging path.
 int wf_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
-  WF_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
-  return (int)hipStreamSynchronize((hipStream_t)stream);
+  constexpr size_t SMALL = 4096;
+  thread_local void* pinned = nullptr;
+  const hipStream_t s = (hipStream_t)stream;
+  if (bytes <= SMALL) {
+    if (!pinned) WF_TRY(hipHostMalloc(&pinned, SMALL, hipHostMallocDefault));
+    WF_TRY(hipMemcpyAsync(pinned, src, bytes, hipMemcpyDeviceToHost, s));
+    WF_TRY(hipStreamSynchronize(s));
+    std::memcpy(dst, pinned, bytes);
+    return 0;
+  }
+  WF_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+  return (int)hipStreamSynchronize(s);
 }
 
 int wf_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {

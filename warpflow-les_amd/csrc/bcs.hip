@@ -8,6 +8,8 @@
 // operations of its (a, b) line, which is exactly the reference's per-cell
 // order.  Different axes stay in separate launches because the y fill reads
 // ghosts written by the x fill at other (a, b) positions, and so on.
+#include <algorithm>
+
 #include "batch.h"
 #include "wf_common.h"
 
@@ -61,6 +63,23 @@ bc_axis_kernel(double* __restrict__ f, wf_grid g, int axis, int a0, int na, int 
   const int aa = blockIdx.y * blockDim.y + threadIdx.y;
   if (aa >= na || bb >= nb) return;
   bc_line(f, g, axis, a0 + aa, b0 + bb, ops);
+}
+
+// The same axis launch for the u, v and w fills at once (blockIdx.z picks
+// the field): the three fields are independent, so each keeps its own
+// per-cell order of operations.
+struct Bc3 {
+  double* f[3];
+  int32_t a0[3], na[3], b0[3], nb[3];
+  BcOps ops[3];
+};
+
+__global__ void __launch_bounds__(256) bc_axis3_kernel(Bc3 A, wf_grid g, int axis) {
+  const int z = blockIdx.z;
+  const int bb = blockIdx.x * blockDim.x + threadIdx.x;
+  const int aa = blockIdx.y * blockDim.y + threadIdx.y;
+  if (aa >= A.na[z] || bb >= A.nb[z]) return;
+  bc_line(A.f[z], g, axis, A.a0[z] + aa, A.b0[z] + bb, A.ops[z]);
 }
 
 }  // namespace wf
@@ -283,6 +302,38 @@ int wf_bc_fill(double* f, wf_grid g, int which, double param, void* stream) {
     if (!bc_axis_ops(which, axis, g.npl, g.ni, g.nj, g.nk, param, a0, na, b0, nb, o))
       return WF_EINVAL;
     WF_TRY((hipError_t)launch_axis(f, g, axis, a0, na, b0, nb, o, s));
+  }
+  return 0;
+}
+
+// apply_u(u), apply_v(v), apply_w(w) (bcs.py:201-265) on three fields of
+// one geometry: per axis one launch covering the three fields.
+int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream) {
+  const hipStream_t s = (hipStream_t)stream;
+  if (g.npl < 1 || !u || !v || !w) return WF_EINVAL;
+  for (int axis = 0; axis < 3; ++axis) {
+    Bc3 A{};
+    A.f[0] = u;
+    A.f[1] = v;
+    A.f[2] = w;
+    int namax = 0, nbmax = 0;
+    for (int z = 0; z < 3; ++z) {
+      int a0, na, b0, nb;
+      if (!bc_axis_ops(z, axis, g.npl, g.ni, g.nj, g.nk, z == 0 ? ulid : 0.0, a0, na, b0, nb,
+                       A.ops[z]) ||
+          !ops_ok(g, axis, a0, na, b0, nb, A.ops[z]))
+        return WF_EINVAL;
+      A.a0[z] = a0;
+      A.na[z] = na;
+      A.b0[z] = b0;
+      A.nb[z] = nb;
+      namax = std::max(namax, na);
+      nbmax = std::max(nbmax, nb);
+    }
+    if (namax == 0 || nbmax == 0) continue;
+    hipLaunchKernelGGL(bc_axis3_kernel, dim3(div_up(nbmax, 64), div_up(namax, 4), 3),
+                       dim3(64, 4), 0, s, A, g, axis);
+    WF_LAUNCHED();
   }
   return 0;
 }

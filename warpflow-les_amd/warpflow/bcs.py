@@ -71,6 +71,16 @@ class BoundaryKernels:
         """bcs.py:246-265"""
         self._fill(fld, "w", 0.0, n)
 
+    def apply_uvw(self, u, v, w, n, ulid):
+        """apply_u, apply_v, apply_w (bcs.py:201-265) back to back, as one
+        launch per axis covering the three fields."""
+        g = _check_n(u, n)
+        key = lambda x: (x.ni, x.nj, x.nk, x.npl, x.sj, x.si)  # noqa: E731
+        for f in (v, w):
+            if key(grid_of(f, NPL)) != key(g) or f.lead != u.lead:
+                raise ValueError("u, v and w must share one geometry")
+        _hip.call("wf_bc_fill_uvw", u.ptr, v.ptr, w.ptr, g, float(ulid), current_stream())
+
     def apply_vis(self, fld, n, rrey):
         """bcs.py:267-293"""
         self._fill(fld, "vis", rrey, n)

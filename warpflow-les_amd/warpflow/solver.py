@@ -60,7 +60,12 @@ class FlowSolver:
         self.ulid = b.lid_velocity
 
         self.mg = self._make_mg()
-        self._buf = zeros(4, np.float64, device)
+        # reduction scratch: [0:4] maxabs / calmas, [4:8] checkdt; one read-back
+        # returns both
+        self._buf = zeros(8, np.float64, device)
+        self._rbuf = self._buf.view(0, (4,))
+        self._dbuf = self._buf.view(4, (4,))
+        self._dt_pending = False
 
         self.dt = cfg.time.dt_init
         self.ctime = 0.0
@@ -87,6 +92,15 @@ class FlowSolver:
     # ------------------------------------------------------------------ #
     # ghost fills                                                        #
     # ------------------------------------------------------------------ #
+    def fill_bc_uvw(self, alfabc=1):
+        """fill_bc of u, v, w back to back (solver.py:88-97 three times): the
+        three halo exchanges, then one fill launch per axis for the three
+        fields (independent of each other, so the order within each field is
+        the reference's)."""
+        for name in ("u", "v", "w"):
+            self.halo.exchange(self.f[name])
+        self.bcs.apply_uvw(self.f["u"], self.f["v"], self.f["w"], self.ncells[0], self.ulid)
+
     def fill_bc(self, name, alfabc=1):
         """solver.py:88-97: one boundu/boundv/boundw call for one field."""
         self.halo.exchange(self.f[name])
@@ -114,26 +128,39 @@ class FlowSolver:
     # ------------------------------------------------------------------ #
     def _maxabs(self, name):
         """solver.py:115-120"""
-        self._buf.zero_()
-        K.maxabs_reduce(self.ncells, self.f[name], self._buf, NPL)
-        loc = float(self._buf.numpy()[0])
+        self._rbuf.zero_()
+        K.maxabs_reduce(self.ncells, self.f[name], self._rbuf, NPL)
+        loc = float(self._rbuf.numpy()[0])
         return self.topo.comm.allreduce(loc)
 
     def _reductions(self):
         if not self.fused:
             umax, vmax, wmax = (self._maxabs(c) for c in ("u", "v", "w"))
-            self._buf.fill_(1.0e10)
+            self._rbuf.fill_(1.0e10)
             K.dtvisc_reduce(self.ncells, self.f["u"], self.f["v"], self.f["w"],
-                            self.f["vis"], self._buf, self.dx, self.dy, self.dz, NPL)
-            return umax, vmax, wmax, float(self._buf.numpy()[0])
-        K.checkdt(self.ncells, self.f["u"], self.f["v"], self.f["w"], self.f["vis"],
-                  self._buf, self.dx, self.dy, self.dz, NPL)
-        r = self._buf.numpy()
+                            self.f["vis"], self._rbuf, self.dx, self.dy, self.dz, NPL)
+            return umax, vmax, wmax, float(self._rbuf.numpy()[0])
+        self._launch_checkdt()
+        r = self._dbuf.numpy()
         return float(r[0]), float(r[1]), float(r[2]), float(r[3])
 
-    def compute_dt(self):
-        """solver.py:122-150"""
-        umax, vmax, wmax, dtvisc = self._reductions()
+    def _launch_checkdt(self):
+        K.checkdt(self.ncells, self.f["u"], self.f["v"], self.f["w"], self.f["vis"],
+                  self._dbuf, self.dx, self.dy, self.dz, NPL)
+
+    def compute_dt(self, defer=False):
+        """solver.py:122-150.  With a fixed dt nothing in a step needs the
+        reductions before the projection, so step() (defer=True, fused path)
+        only launches checkdt here and finishes (cfl, the collapse check)
+        when the first calmas result is read back in the same step -- one
+        host round trip fewer per step, same values."""
+        if defer and self.fused and not self.cfg.time.variable_dt:
+            self._launch_checkdt()
+            self._dt_pending = True
+            return
+        self._finish_dt(*self._reductions())
+
+    def _finish_dt(self, umax, vmax, wmax, dtvisc):
         dtvisc = self.topo.comm.allreduce(dtvisc)
         big = 1.0e10
         dtcfl = big
@@ -170,10 +197,14 @@ class FlowSolver:
 
     def _calmas(self):
         """solver.py:168-176"""
-        self._buf.zero_()
+        self._rbuf.zero_()
         K.calmas(self.ncells, self.f["u"], self.f["v"], self.f["w"], self.f["su"],
-                 self._buf, self.dx, self.dy, self.dz, self.dt, NPL)
-        loc = float(self._buf.numpy()[0])
+                 self._rbuf, self.dx, self.dy, self.dz, self.dt, NPL)
+        r = self._buf.numpy()
+        if self._dt_pending:
+            self._dt_pending = False
+            self._finish_dt(float(r[4]), float(r[5]), float(r[6]), float(r[7]))
+        loc = float(r[0])
         self.rmax = self.topo.comm.allreduce(loc)
 
     def _project(self):
@@ -203,8 +234,7 @@ class FlowSolver:
                 print("warning: projection did not converge, rmax = %.6e after %d "
                       "iterations" % (self.rmax, maxcy), flush=True)
         self.outer_iters = iters
-        for name in ("u", "v", "w"):
-            self.fill_bc(name, 0)
+        self.fill_bc_uvw(0)
 
     # ------------------------------------------------------------------ #
     # one time step                                                      #
@@ -235,8 +265,7 @@ class FlowSolver:
                 self.fill_bc(c, alfabc)
                 kcd(self._op_dim(c), f["u"], f["v"], f["w"], f[c + "oo"], f["vis"],
                     f[c + "star"], *d3, dt, alpha, NPL)
-            for c, _, _, _ in comps:
-                self.fill_bc(c, alfabc)
+            self.fill_bc_uvw(alfabc)
             return
         for c, kconv, _, _ in comps:
             self.fill_bc(c, alfabc)
@@ -249,7 +278,7 @@ class FlowSolver:
     def step(self):
         """solver.py:217-271"""
         self.itime += 1
-        self.compute_dt()
+        self.compute_dt(defer=True)
         self.ctime += self.dt
         f = self.f
         copy(f["uoo"], f["u"])
@@ -261,5 +290,9 @@ class FlowSolver:
             if stage + 1 < len(ALFARK):
                 self._calvel(alpha)
         self._project()
+        if self._dt_pending:  # no calmas ran (max_outer_iters < 1)
+            self._dt_pending = False
+            r = self._dbuf.numpy()
+            self._finish_dt(float(r[0]), float(r[1]), float(r[2]), float(r[3]))
         return {"ctime": self.ctime, "dt": self.dt, "rmax": self.rmax,
                 "iters": self.outer_iters, "cfl": self.cfl}
