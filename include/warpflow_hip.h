@@ -396,11 +396,19 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  * shape it names -- the real cells (n0, n1, n2) of the field (convdiff) or of
  * the multigrid level (sweeps) -- so tuning one grid never changes another.
  * value 0 removes the entry (the built-in choice); results never depend on
- * any setting.  At most 64 entries.
+ * any setting.
  *   WF_TUNE_CD_ICH              i-planes per workgroup of wf_convdiff4th
  *   WF_TUNE_SWEEP_TILE_F64/F32  fused RBGS sweep tile (j x k): 1 = 8 x 64,
  *                               2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32
  *   WF_TUNE_SWEEP_ICH_F64/F32   i-planes per workgroup of the fused sweep
+ *   WF_TUNE_SWEEP2_TILE_F64/F32 tile of the two-sweep kernel (pairs of sweeps
+ *                               in a V-cycle): 1 = 8 x 64, 2 = 16 x 32
+ *   WF_TUNE_SWEEP2_ICH_F64/F32  i-planes per workgroup of the two-sweep kernel
+ *   WF_TUNE_SWEEP_PAIRS         pairs of sweeps of this level as 1 = two
+ *                               one-sweep launches, 2 = one two-sweep launch
+ *                               (built-in: 2 for FP64 levels of >= 2^23
+ *                               cells, else 1)
+ * At most 128 entries.
  * ---------------------------------------------------------------------- */
 enum {
   WF_TUNE_CD_ICH = 0,
@@ -408,7 +416,12 @@ enum {
   WF_TUNE_SWEEP_TILE_F32 = 2,
   WF_TUNE_SWEEP_ICH_F64 = 3,
   WF_TUNE_SWEEP_ICH_F32 = 4,
-  WF_TUNE_COUNT = 5
+  WF_TUNE_SWEEP2_TILE_F64 = 5,
+  WF_TUNE_SWEEP2_TILE_F32 = 6,
+  WF_TUNE_SWEEP2_ICH_F64 = 7,
+  WF_TUNE_SWEEP2_ICH_F32 = 8,
+  WF_TUNE_SWEEP_PAIRS = 9,
+  WF_TUNE_COUNT = 10
 };
 int wf_tune_set(int knob, int n0, int n1, int n2, int value);
 int wf_tune_get(int knob, int n0, int n1, int n2, int* value);

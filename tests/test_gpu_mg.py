@@ -201,6 +201,7 @@ def test_sweep_tiles_agree_bitwise(gpu, prec, tile):
     lev.rhs.assign(rhs)
     autotune.reset()
     try:
+        autotune.set_shape(autotune.SWEEP_PAIRS, n3, 1)  # one-sweep launches only
         lev.phi.assign(phi0)
         _hip.call("wf_mg_relax", C.byref(d), 4, None)
         want = lev.phi.numpy()
@@ -212,3 +213,80 @@ def test_sweep_tiles_agree_bitwise(gpu, prec, tile):
             assert np.array_equal(lev.phi.numpy(), want), (tile, ich)
     finally:
         autotune.reset()
+
+
+@pytest.mark.parametrize("prec", [np.float64, np.float32])
+@pytest.mark.parametrize("tile", [1, 2])
+def test_two_sweep_launches_agree_bitwise(gpu, prec, tile):
+    """Pairs of sweeps as one two-sweep launch (4 lagged stages, halo
+    recomputation) give the one-sweep launches' result bit for bit, for both
+    tiles and several i-chunks (the first and last chunks touch the i faces),
+    on a level whose extents are multiples of none of them."""
+    import ctypes as C
+    from warpflow import _hip, autotune
+    from warpflow.multigrid import _Level
+    n3 = (37, 42, 70)
+    h = tuple(1.0 / n for n in n3)
+    lev = _Level(n3, h, None, prec)
+    rng = np.random.default_rng(12)
+    rhs = rng.standard_normal(lev.shape)
+    phi0 = rng.standard_normal(lev.shape)
+    d = lev.desc()
+    knob_t = autotune.SWEEP2_TILE_F64 if prec == np.float64 else autotune.SWEEP2_TILE_F32
+    knob_i = autotune.SWEEP2_ICH_F64 if prec == np.float64 else autotune.SWEEP2_ICH_F32
+    lev.rhs.assign(rhs)
+    autotune.reset()
+    try:
+        autotune.set_shape(autotune.SWEEP_PAIRS, n3, 1)
+        lev.phi.assign(phi0)
+        _hip.call("wf_mg_relax", C.byref(d), 4, None)
+        want = lev.phi.numpy()
+        autotune.set_shape(autotune.SWEEP_PAIRS, n3, 2)
+        for ich in (1, 3, 16, 40):
+            autotune.set_shape(knob_t, n3, tile)
+            autotune.set_shape(knob_i, n3, ich)
+            lev.phi.assign(phi0)
+            _hip.call("wf_mg_relax", C.byref(d), 4, None)
+            assert np.array_equal(lev.phi.numpy(), want), (tile, ich)
+    finally:
+        autotune.reset()
+
+
+@pytest.mark.parametrize("npre,npost", [(6, 6), (2, 4), (4, 2)])
+def test_vcycle_two_sweep_parity(gpu, npre, npost):
+    """A V-cycle relaxing with two-sweep launches equals one with one-sweep
+    launches bit for bit on every level's phi -- including an odd number of
+    two-sweep launches in one smoothing phase, which leaves the iterate in
+    tmp and swaps the level's roles until the cycle ends.  The ring's edge
+    and corner cells are zero, as the reference leaves them (prolongation
+    reads them; no kernel writes them, in phi or tmp)."""
+    from warpflow import _hip, autotune
+    from warpflow.device import current_stream
+    n3 = (48, 40, 64)
+    mg, _ = build(n3, 3)
+    load(mg, _, seed=9)
+    phis = []
+    for lev in mg.levels:
+        p = lev.phi.numpy()
+        ring = np.zeros(p.shape, int)
+        for ax in range(3):
+            sl = [slice(None)] * 3
+            sl[ax] = [0, -1]
+            ring[tuple(sl)] += 1
+        p[ring >= 2] = 0.0
+        phis.append(p)
+    descs = mg._descs()
+    res = []
+    autotune.reset()
+    try:
+        for pairs in (1, 2):
+            for lev in mg.levels:
+                autotune.set_shape(autotune.SWEEP_PAIRS, lev.n3, pairs)
+            for lev, p in zip(mg.levels, phis):
+                lev.phi.assign(p)
+            _hip.call("wf_mg_vcycle", descs, len(mg.levels), npre, npost, 8, current_stream())
+            res.append([lev.phi.numpy() for lev in mg.levels])
+    finally:
+        autotune.reset()
+    for l, (a, b) in enumerate(zip(*res)):
+        assert np.array_equal(a, b), (npre, npost, l, float(np.abs(a - b).max()))

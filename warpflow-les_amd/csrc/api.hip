@@ -62,7 +62,7 @@ int copy(void* d, const void* s, size_t bytes, hipStream_t st) {
 struct TuneEntry {
   int knob, n0, n1, n2, value;
 };
-constexpr int TUNE_MAX = 64;
+constexpr int TUNE_MAX = 128;
 TuneEntry g_tune[TUNE_MAX];
 int g_ntune = 0;
 
@@ -233,6 +233,9 @@ int wf_tune_set(int knob, int n0, int n1, int n2, int value) {
     return WF_EINVAL;
   if ((knob == WF_TUNE_SWEEP_TILE_F64 || knob == WF_TUNE_SWEEP_TILE_F32) && value > 4)
     return WF_EINVAL;
+  if ((knob == WF_TUNE_SWEEP2_TILE_F64 || knob == WF_TUNE_SWEEP2_TILE_F32) && value > 2)
+    return WF_EINVAL;
+  if (knob == WF_TUNE_SWEEP_PAIRS && value > 2) return WF_EINVAL;
   const int q = tune_find(knob, n0, n1, n2);
   if (value == 0) {  // remove
     if (q >= 0) g_tune[q] = g_tune[--g_ntune];

@@ -366,48 +366,59 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   const auto dnone = buf_rsrc(pout, 0u);  // every store dropped
   auto soff = [&](int x) { return (x + 1 - i0) * pb; };
 
-  // Loads run two planes ahead of their use.  pre[c] holds the next pin
-  // plane of class c; rodd[x] and reven[x] the odd- and even-cell rhs of the
-  // plane whose LDS slot is x (three consecutive planes).
-  T pre[2][NPE], rodd[3][NO], reven[3][NPAIR];
-  auto load_pin = [&](auto Rc, int x) {  // pin plane x (array plane in range) of class r
-    constexpr int r = decltype(Rc)::value;
+  // Loads run three planes ahead of their use: pre[x] holds the pin plane
+  // to be stored into LDS slot x; rodd[u] and reven[u] the odd-cell rhs of
+  // the plane iteration t = u (mod 6) relaxes and the even-cell rhs of the
+  // plane it writes.
+  T pre[3][NPE], rodd[6][NO], reven[6][NPAIR];
 #pragma unroll
-    for (int e = 0; e < NPE; ++e) pre[r][e] = buf_ld<T>(dpin, gofs[r][e], soff(x));
+  for (int u = 0; u < 6; ++u) {
+#pragma unroll
+    for (int e = 0; e < NO; ++e) rodd[u][e] = T(0);
+#pragma unroll
+    for (int e = 0; e < NPAIR; ++e) reven[u][e] = T(0);
+  }
+  // pin plane x (class r) into set X
+  auto load_pin = [&](auto Rc, auto Xc, int x) {
+    constexpr int r = decltype(Rc)::value, X = decltype(Xc)::value;
+#pragma unroll
+    for (int e = 0; e < NPE; ++e) pre[X][e] = buf_ld<T>(dpin, gofs[r][e], soff(x));
   };
-  // pre[r] (plane x) to LDS; the ring planes -1 and n0 enter as zero
-  auto store_pin = [&](auto Rc, T* dst, int x) {
-    constexpr int r = decltype(Rc)::value;
+  // set X (plane x) to LDS; the ring planes -1 and n0 enter as zero
+  auto store_pin = [&](auto Xc, T* dst, int x) {
+    constexpr int X = decltype(Xc)::value;
     const bool z = x < 0 || x >= n0;
 #pragma unroll
-    for (int e = 0; e < NPE; ++e) dst[tid + 256 * e] = z ? T(0) : pre[r][e];
+    for (int e = 0; e < NPE; ++e) dst[tid + 256 * e] = z ? T(0) : pre[X][e];
   };
-  auto load_rhs_odd = [&](auto Rc, auto Xc, int x) {  // plane x of class r into slot X
-    constexpr int r = decltype(Rc)::value, X = decltype(Xc)::value;
+  auto load_rhs_odd = [&](auto Rc, auto Uc, int x) {  // plane x of class r into set U
+    constexpr int r = decltype(Rc)::value, U = decltype(Uc)::value;
 #pragma unroll
-    for (int e = 0; e < NO; ++e) rodd[X][e] = buf_ld<T>(drhs, oofs[r][e], soff(x));
+    for (int e = 0; e < NO; ++e) rodd[U][e] = buf_ld<T>(drhs, oofs[r][e], soff(x));
   };
-  auto load_rhs_even = [&](auto Rc, auto Xc, int x) {
-    constexpr int r = decltype(Rc)::value, X = decltype(Xc)::value;
+  auto load_rhs_even = [&](auto Rc, auto Uc, int x) {
+    constexpr int r = decltype(Rc)::value, U = decltype(Uc)::value;
 #pragma unroll
-    for (int e = 0; e < NPAIR; ++e) reven[X][e] = buf_ld<T>(drhs, eofs[r][e], soff(x));
+    for (int e = 0; e < NPAIR; ++e) reven[U][e] = buf_ld<T>(drhs, eofs[r][e], soff(x));
   };
 
   // Iteration t handles q = i0-1+t.  Plane x sits in LDS slot
-  // (x - i0 + 2) mod 3 and has parity class sb ^ ((x - i0 + 1) mod 2); for
-  // q-1, q, q+1 these are compile-time constants here.  Planes past what the
+  // (x - i0 + 2) mod 3 and has parity class sb ^ ((x - i0 + 1) mod 2); with
+  // u = t mod 6 these are compile-time constants here.  Planes past what the
   // chunk needs are clamped to one it may read and their values ignored.
   const int xpin = min(i1 + 1, n0);  // last pin plane read (n0: the ring plane)
-  auto step = [&](auto Rc, auto Cc, int q) {
-    constexpr int R = decltype(Rc)::value;
-    constexpr int r = decltype(Cc)::value, rn = r ^ 1;  // classes of planes q, q +- 1
+  auto step = [&](auto Uc, int q) {
+    constexpr int U = decltype(Uc)::value;
+    constexpr int R = U % 3, r = U & 1, rn = r ^ 1;  // classes of planes q, q +- 1
     constexpr int SM = R, S0 = (R + 1) % 3, SP = (R + 2) % 3;  // planes q-1, q, q+1
-    using Crn = std::integral_constant<int, rn>;
-    store_pin(Crn{}, P[SP], q + 1);
-    load_pin(Crn{}, min(q + 3, xpin));
-    load_rhs_odd(std::integral_constant<int, r>{}, std::integral_constant<int, SM>{},
-                 min(q + 2, i1));
-    load_rhs_even(Crn{}, std::integral_constant<int, SP>{}, min(q + 1, i1));
+    constexpr int U3 = (U + 3) % 6;
+    store_pin(std::integral_constant<int, SP>{}, P[SP], q + 1);
+    load_pin(std::integral_constant<int, r>{}, std::integral_constant<int, SP>{},
+             min(q + 4, xpin));
+    load_rhs_odd(std::integral_constant<int, rn>{}, std::integral_constant<int, U3>{},
+                 min(q + 3, i1));
+    load_rhs_even(std::integral_constant<int, r>{}, std::integral_constant<int, U3>{},
+                  min(q + 2, i1));
     __syncthreads();
     {  // odd values of plane q over the halo tile
       T rp[NO];
@@ -428,7 +439,7 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
         const int a = a0[e], c = aj[r][e];
         const T sum = c0 * P[SM][c] + c0 * P[SP][c] + c1 * P[S0][c - PH] +
                       c1 * P[S0][c + PH] + c2 * P[S0][a] + c2 * P[S0][a + 1];
-        O[S0][tid + 256 * e] = (rodd[S0][e] - sum) * rp[e];
+        O[S0][tid + 256 * e] = (rodd[U][e] - sum) * rp[e];
       }
     }
     __syncthreads();
@@ -455,7 +466,7 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
       const int b = b0[e], c = bj[rn][e];
       const T sum = c0 * O[SP][c] + c0 * O[S0][c] + c1 * O[SM][c - OH] + c1 * O[SM][c + OH] +
                     c2 * O[SM][b] + c2 * O[SM][b + 1];
-      const T ve = (reven[SM][e] - sum) * re[e];
+      const T ve = (reven[U][e] - sum) * re[e];
       const T vo = O[SM][bo[rn][e]];
       const uint32_t oe = eofs[rn][e], oo = pofs[rn][e];
       buf_st(dst, oe, so, ve);
@@ -494,32 +505,39 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  using I5 = std::integral_constant<int, 5>;
   if (i0 >= 2) {  // pin plane i0-2 lies before the descriptors' base
     const auto dp = buf_rsrc(reinterpret_cast<const char*>(pin) + (int64_t)(i0 - 1) * pb,
                              0x7fffffffu);
 #pragma unroll
-    for (int e = 0; e < NPE; ++e) pre[1][e] = buf_ld<T>(dp, gofs[1][e], 0);
+    for (int e = 0; e < NPE; ++e) pre[0][e] = buf_ld<T>(dp, gofs[1][e], 0);
   }
-  store_pin(I1{}, P[0], i0 >= 2 ? i0 - 2 : -1);
-  load_pin(I0{}, i0 - 1);
-  store_pin(I0{}, P[1], i0 - 1);
-  load_pin(I1{}, i0);
-  load_pin(I0{}, min(i0 + 1, xpin));
-  load_rhs_odd(I0{}, I1{}, i0 - 1);  // slot 1
-  load_rhs_odd(I1{}, I2{}, i0);      // slot 2
+  store_pin(I0{}, P[0], i0 >= 2 ? i0 - 2 : -1);
+  load_pin(I0{}, I1{}, i0 - 1);
+  store_pin(I1{}, P[1], i0 - 1);
+  // planes i0, i0+1, i0+2 go to slots 2, 0, 1 at iterations 0, 1, 2
+  load_pin(I1{}, I2{}, i0);
+  load_pin(I0{}, I0{}, min(i0 + 1, xpin));
+  load_pin(I1{}, I1{}, min(i0 + 2, xpin));
+  load_rhs_odd(I0{}, I0{}, i0 - 1);
+  load_rhs_odd(I1{}, I1{}, i0);
+  load_rhs_odd(I0{}, I2{}, min(i0 + 1, i1));
+  load_rhs_even(I1{}, I2{}, i0);  // written at iteration 2 (plane i0)
   const int cnt = i1 - i0 + 2;  // q = i0-1 .. i1, cnt >= 3
   for (int t = 0;; t += 6) {
-    step(I0{}, I0{}, i0 - 1 + t);
+    step(I0{}, i0 - 1 + t);
     if (t + 1 >= cnt) break;
-    step(I1{}, I1{}, i0 + t);
+    step(I1{}, i0 + t);
     if (t + 2 >= cnt) break;
-    step(I2{}, I0{}, i0 + 1 + t);
+    step(I2{}, i0 + 1 + t);
     if (t + 3 >= cnt) break;
-    step(I0{}, I1{}, i0 + 2 + t);
+    step(I3{}, i0 + 2 + t);
     if (t + 4 >= cnt) break;
-    step(I1{}, I0{}, i0 + 3 + t);
+    step(I4{}, i0 + 3 + t);
     if (t + 5 >= cnt) break;
-    step(I2{}, I1{}, i0 + 4 + t);
+    step(I5{}, i0 + 4 + t);
     if (t + 6 >= cnt) break;
   }
 }
@@ -528,6 +546,359 @@ __global__ void __launch_bounds__(256)
 sweep_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
              MgDev d, T c0, T c1, T c2, int ich) {
   sweep_body<T, TJ, TK>(pin, pout, rhs, d, c0, c1, c2, ich, this_blk());
+}
+
+// Two full RBGS sweeps fused in one pass: pout = S(S(pin)).  The workgroup
+// streams its (j, k) tile along i through four lagged stages, one plane
+// apart in pairs: at plane q it computes
+//   A-odd  (first sweep, odd cells)  at q      over the tile + 3-cell halo,
+//   A-even (first sweep, even cells) at q - 2  over the tile + 2-cell halo,
+//   B-odd  (second sweep)            at q - 4  over the tile + 1-cell halo,
+//   B-even (second sweep)            at q - 6  over the tile, and writes it
+// with the B-odd values of that plane to pout.  Each stage reads only values
+// the previous stage produced in earlier iterations (pin's even cells over
+// the tile + 4-cell halo for A-odd), so one barrier per plane separates
+// every producer from its consumers; each LDS array keeps four planes.  A
+// stage is the single sweep's packed-checkerboard update: destination cell
+// (row jr, packed column m) of the halo-h layout reads source row jr + 1 of
+// the halo-(h+1) layout at columns m and m + 1 (k neighbours) and m + ph (j
+// and i neighbours), ph the destination colour's phase in its row.  Halo and
+// out-of-domain cells are masked as in the single sweep (zero values, zero
+// reciprocal diagonal), so every value, halo copies included, is the single
+// sweep's bit for bit.  HBM traffic per two sweeps: one read of pin and rhs,
+// one write of pout; the halo recomputation costs arithmetic only.
+template <class T, int TJ, int TK>
+__device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ pout,
+                                   const T* __restrict__ rhs, const MgDev& d, T c0, T c1, T c2,
+                                   int ich, Blk B) {
+  constexpr int NPAIR = TJ * TK / 512;  // output pairs per thread
+  static_assert(TJ * TK == 512 * NPAIR && (NPAIR == 1 || NPAIR == 2), "256 threads, 1-2 pairs");
+  static_assert(TK % 2 == 0, "pairs along k");
+  // packed halo-h layout: TJ + 2h rows of W_h = (TK + 2h) / 2 cells of one colour
+  constexpr int W0 = TK / 2, W1 = W0 + 1, W2 = W0 + 2, W3 = W0 + 3, W4 = W0 + 4;
+  constexpr int C1 = (TJ + 2) * W1, C2 = (TJ + 4) * W2, C3 = (TJ + 6) * W3, C4 = (TJ + 8) * W4;
+  constexpr int N1 = (C1 + 255) / 256, N2 = (C2 + 255) / 256, N3 = (C3 + 255) / 256,
+                N4 = (C4 + 255) / 256;  // cells per thread
+  // a plane slot holds the cells and one spare word that dead lanes write
+  constexpr int Z1 = (C1 + 2) & ~1, Z2 = (C2 + 2) & ~1, Z3 = (C3 + 2) & ~1, Z4 = (C4 + 2) & ~1;
+  __shared__ T P[4][Z4];   // pin, even cells, halo 4
+  __shared__ T OA[4][Z3];  // first sweep, odd cells, halo 3
+  __shared__ T EA[4][Z2];  // first sweep, even cells, halo 2
+  __shared__ T OB[4][Z1];  // second sweep, odd cells, halo 1
+  const int tid = threadIdx.x;
+  const int j0 = B.y * TJ, k0 = B.x * TK;
+  const int i0 = B.z * ich, i1 = min(i0 + ich, d.n0);
+  const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
+  const int s1 = (int)d.s1;
+  const int sb = (i0 - 3 + j0 + k0) & 1;  // parity class of plane i0-3; index r: sb ^ r
+  const int imid = n0 >= 3 ? 1 : 0;
+  auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  auto plane_boff = [&](int j, int k) {
+    return (uint32_t)(((j + 1) * s1 + (k + 1)) * (int)sizeof(T));
+  };
+  auto in_jk = [&](int j, int k) { return j >= 0 && j < n1 && k >= 0 && k < n2; };
+  auto rdiag = [&](int i, int j, int k) {
+    return T(1) / mg_coef(d, i, clampi(j, 0, n1 - 1), clampi(k, 0, n2 - 1), c0, c1, c2).ap;
+  };
+  // cell e of a halo-h stage on class r: (j, k) and liveness; odd: the
+  // colour of i + j + k odd
+  auto stage_cell = [&](int t, int e, int r, int h, int W, int C, bool odd, int& j, int& k,
+                        int& ph) {
+    const int l = min(t + 256 * e, C - 1);
+    const int jr = l / W, m = l % W;
+    ph = ((sb ^ r) + jr + (odd ? 1 : 0)) & 1;
+    j = j0 - h + jr;
+    k = k0 - h + 2 * m + ph;
+    return t + 256 * e < C && in_jk(j, k);
+  };
+  const bool wfast = j0 >= 1 && j0 + TJ <= n1 - 1 && k0 >= 1 && k0 + TK <= n2 - 1;
+
+  // pin even cells (halo 4), offsets per class (SWEEP_DEAD outside the domain)
+  uint32_t gofs[2][N4];
+#pragma unroll
+  for (int e = 0; e < N4; ++e)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      int j, k, ph;
+      const bool live = stage_cell(tid, e, r, 4, W4, C4, false, j, k, ph);
+      gofs[r][e] = live ? plane_boff(j, k) : SWEEP_DEAD;
+    }
+  // stage tables: source columns (a0: k neighbours a0, a0+1 of source row jr+1;
+  // aj[r]: j and i neighbours), rhs offsets and interior-plane 1/ap per class
+#define WF_STAGE_TABLES(NAME, N, H, W, C, WS, ODD)                                   \
+  int NAME##_a0[N], NAME##_aj[2][N];                                                 \
+  uint32_t NAME##_ro[2][N];                                                          \
+  T NAME##_rp[2][N];                                                                 \
+  _Pragma("unroll") for (int e = 0; e < N; ++e) {                                    \
+    const int l = min(tid + 256 * e, C - 1);                                         \
+    NAME##_a0[e] = (l / W + 1) * WS + l % W;                                         \
+    _Pragma("unroll") for (int r = 0; r < 2; ++r) {                                  \
+      int j, k, ph;                                                                  \
+      const bool live = stage_cell(tid, e, r, H, W, C, ODD, j, k, ph);               \
+      NAME##_aj[r][e] = NAME##_a0[e] + ph;                                           \
+      NAME##_ro[r][e] = live ? plane_boff(j, k) : SWEEP_DEAD;                        \
+      NAME##_rp[r][e] = live ? rdiag(imid, j, k) : T(0);                             \
+    }                                                                                \
+  }
+  WF_STAGE_TABLES(ao, N3, 3, W3, C3, W4, true)
+  WF_STAGE_TABLES(ae, N2, 2, W2, C2, W3, false)
+  WF_STAGE_TABLES(bo, N1, 1, W1, C1, W2, true)
+  WF_STAGE_TABLES(be, NPAIR, 0, W0, TJ * W0, W1, false)
+#undef WF_STAGE_TABLES
+  // output: the even cell's offset is be_ro; the odd partner's offset and
+  // OB column, and face flags of the pair's cells a = k_a, b = k_a + 1
+  constexpr int FL_JLO = 1, FL_JHI = 2, FL_ALO = 4, FL_AHI = 8, FL_BHI = 16, FL_AL = 32,
+                FL_BL = 64, FL_X = 128;
+  uint32_t pofs[2][NPAIR];
+  int pcol[2][NPAIR], fl[NPAIR];
+#pragma unroll
+  for (int e = 0; e < NPAIR; ++e) {
+    const int p = tid + 256 * e;
+    const int jr = p / W0, mm = p % W0;
+    const int j = j0 + jr, ka = k0 + 2 * mm;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int kx = ((sb ^ r) + jr) & 1;  // the even cell is k_a + kx
+      const int ko = ka + 1 - kx;
+      pofs[r][e] = j < n1 && ko < n2 ? plane_boff(j, ko) : SWEEP_DEAD;
+      pcol[r][e] = (jr + 1) * W1 + mm + 1 - kx;
+    }
+    const bool la = j < n1 && ka < n2, lb = j < n1 && ka + 1 < n2;
+    fl[e] = (la || lb ? (j == 0 ? FL_JLO : 0) | (j == n1 - 1 ? FL_JHI : 0) : 0) |
+            (la ? (ka == 0 ? FL_ALO : 0) | (ka == n2 - 1 ? FL_AHI : 0) | FL_AL : 0) |
+            (lb ? (ka + 1 == n2 - 1 ? FL_BHI : 0) | FL_BL : 0) | (((sb + jr) & 1) ? FL_X : 0);
+  }
+
+  auto opaque_tid = [&]() {
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    return t;
+  };
+  // Descriptors start at the array plane of real plane max(i0-4, -1); every
+  // plane a chunk touches lies within (ich + 8) planes of it.
+  const int pb = (int)(d.s0 * (int64_t)sizeof(T));
+  const int xb = max(i0 - 4, -1);
+  auto chunk_rsrc = [&](const T* a) {
+    return buf_rsrc(reinterpret_cast<const char*>(a) + (int64_t)(xb + 1) * pb, 0x7fffffffu);
+  };
+  const auto dpin = chunk_rsrc(pin), drhs = chunk_rsrc(rhs), dout = chunk_rsrc(pout);
+  const auto dnone = buf_rsrc(pout, 0u);
+  // soffset of real plane x, clamped to the planes [lo, hi] a stage needs
+  // (values of other planes are never used) and to the array
+  auto soff = [&](int x, int lo, int hi) {
+    return (clampi(clampi(x, lo, hi), -1, n0) - xb) * pb;
+  };
+
+  // zero every slot once, so no value a stage reads is ever uninitialised
+  for (int w = tid; w < 4 * Z4; w += 256) (&P[0][0])[w] = T(0);
+  for (int w = tid; w < 4 * Z3; w += 256) (&OA[0][0])[w] = T(0);
+  for (int w = tid; w < 4 * Z2; w += 256) (&EA[0][0])[w] = T(0);
+  for (int w = tid; w < 4 * Z1; w += 256) (&OB[0][0])[w] = T(0);
+  __syncthreads();
+
+  // Prefetch: pin two planes ahead (pre[c]: the next plane of class c); the
+  // first sweep's rhs two planes ahead (4 sets), the second's one plane
+  // ahead (2 sets; those planes were read a few planes earlier).
+  T pre[2][N4], rao[4][N3], rae[4][N2], rbo[2][N1], rbe[2][NPAIR];
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+#pragma unroll
+    for (int e = 0; e < N3; ++e) rao[x][e] = T(0);
+#pragma unroll
+    for (int e = 0; e < N2; ++e) rae[x][e] = T(0);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+#pragma unroll
+    for (int e = 0; e < N1; ++e) rbo[x][e] = T(0);
+#pragma unroll
+    for (int e = 0; e < NPAIR; ++e) rbe[x][e] = T(0);
+  }
+  const int pin_hi = min(i1 + 3, n0);  // last pin plane needed (n0: ring, zero)
+  auto load_pin = [&](auto Rc, int x) {
+    constexpr int r = decltype(Rc)::value;
+    const int so = soff(x, i0 - 4, pin_hi);
+#pragma unroll
+    for (int e = 0; e < N4; ++e) pre[r][e] = buf_ld<T>(dpin, gofs[r][e], so);
+  };
+  auto store_pin = [&](auto Rc, T* dst, int x) {  // ring planes and beyond enter as zero
+    constexpr int r = decltype(Rc)::value;
+    const bool z = x < 0 || x >= n0;
+#pragma unroll
+    for (int e = 0; e < N4; ++e) dst[tid + 256 * e < C4 ? tid + 256 * e : C4] = z ? T(0) : pre[r][e];
+  };
+
+  // one stage's cells: v = (rhs - sum) * 1/ap into dst (dead lanes: spare word)
+  auto relax = [&](auto Nc, auto WSc, auto Cc, const T* sm, const T* s0, const T* sp,
+                   const int* a0, const int* aj, const T* rv, const T* rp, T* dst) {
+    constexpr int N = decltype(Nc)::value, WS = decltype(WSc)::value, C = decltype(Cc)::value;
+#pragma unroll
+    for (int e = 0; e < N; ++e) {
+      const int a = a0[e], c = aj[e];
+      const T sum = c0 * sm[c] + c0 * sp[c] + c1 * s0[c - WS] + c1 * s0[c + WS] +
+                    c2 * s0[a] + c2 * s0[a + 1];
+      const int l = tid + 256 * e;
+      dst[(e + 1) * 256 <= C ? l : (l < C ? l : C)] = (rv[e] - sum) * rp[e];
+    }
+  };
+  // 1/ap of a stage's cells on an i-face (or ring) plane x
+  auto face_rp = [&](auto Nc, int x, int r, int h, int W, int C, bool odd, T* rp) {
+    constexpr int N = decltype(Nc)::value;
+    const int t = opaque_tid();
+    const bool xin = x >= 0 && x < n0;
+#pragma unroll
+    for (int e = 0; e < N; ++e) {
+      int j, k, ph;
+      const bool live = stage_cell(t, e, r, h, W, C, odd, j, k, ph) && xin;
+      rp[e] = live ? rdiag(xin ? x : 0, j, k) : T(0);
+    }
+  };
+
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  // Iteration t handles q = i0-3+t; plane x sits in LDS slot (x - i0 + 3) & 3
+  // and has class index (x - i0 + 3) & 1.
+  auto step = [&](auto Uc, int q) {
+    constexpr int U = decltype(Uc)::value;
+    constexpr int r = U & 1, rn = r ^ 1;
+    constexpr int Sm1 = (U + 3) & 3, S0 = U, Sp1 = (U + 1) & 3, Sm2 = (U + 2) & 3;
+    using Cr = std::integral_constant<int, r>;
+    using Crn = std::integral_constant<int, rn>;
+    store_pin(Crn{}, P[Sp1], q + 1);
+    load_pin(Crn{}, q + 3);
+    {  // rhs: A stages of iteration t+2 (class r), B stages of t+1 (class rn)
+      constexpr int XA = (U + 2) & 3, XB = (U + 1) & 1;
+      const int sao = soff(q + 2, i0 - 3, i1 + 2), sae = soff(q, i0 - 2, i1 + 1);
+      const int sbo = soff(q - 3, i0 - 1, i1), sbe = soff(q - 5, i0, i1 - 1);
+#pragma unroll
+      for (int e = 0; e < N3; ++e) rao[XA][e] = buf_ld<T>(drhs, ao_ro[r][e], sao);
+#pragma unroll
+      for (int e = 0; e < N2; ++e) rae[XA][e] = buf_ld<T>(drhs, ae_ro[r][e], sae);
+#pragma unroll
+      for (int e = 0; e < N1; ++e) rbo[XB][e] = buf_ld<T>(drhs, bo_ro[rn][e], sbo);
+#pragma unroll
+      for (int e = 0; e < NPAIR; ++e) rbe[XB][e] = buf_ld<T>(drhs, be_ro[rn][e], sbe);
+    }
+    __syncthreads();
+    constexpr int XA = U, XB = U & 1;
+    {  // A-odd at q: pin planes q-1, q, q+1 -> OA[q]
+      T rp[N3];
+#pragma unroll
+      for (int e = 0; e < N3; ++e) rp[e] = ao_rp[r][e];
+      if (q <= 0 || q >= n0 - 1) face_rp(std::integral_constant<int, N3>{}, q, r, 3, W3, C3, true, rp);
+      relax(std::integral_constant<int, N3>{}, std::integral_constant<int, W4>{},
+            std::integral_constant<int, C3>{}, P[Sm1], P[S0], P[Sp1], ao_a0, ao_aj[r],
+            rao[XA], rp, OA[S0]);
+    }
+    {  // A-even at q-2: OA planes q-3, q-2, q-1 -> EA[q-2]
+      const int x = q - 2;
+      T rp[N2];
+#pragma unroll
+      for (int e = 0; e < N2; ++e) rp[e] = ae_rp[r][e];
+      if (x <= 0 || x >= n0 - 1) face_rp(std::integral_constant<int, N2>{}, x, r, 2, W2, C2, false, rp);
+      relax(std::integral_constant<int, N2>{}, std::integral_constant<int, W3>{},
+            std::integral_constant<int, C2>{}, OA[(U + 1) & 3], OA[Sm2], OA[Sm1], ae_a0,
+            ae_aj[r], rae[XA], rp, EA[Sm2]);
+    }
+    {  // B-odd at q-4: EA planes q-5, q-4, q-3 -> OB[q-4]
+      const int x = q - 4;
+      T rp[N1];
+#pragma unroll
+      for (int e = 0; e < N1; ++e) rp[e] = bo_rp[r][e];
+      if (x <= 0 || x >= n0 - 1) face_rp(std::integral_constant<int, N1>{}, x, r, 1, W1, C1, true, rp);
+      relax(std::integral_constant<int, N1>{}, std::integral_constant<int, W2>{},
+            std::integral_constant<int, C1>{}, EA[Sm1], EA[S0], EA[Sp1], bo_a0, bo_aj[r],
+            rbo[XB], rp, OB[S0]);
+    }
+    {  // B-even at q-6: OB planes q-7, q-6, q-5 -> pout plane q-6 (with B-odd)
+      const int x = q - 6;
+      const bool own = x >= i0 && x < i1;
+      T rp[NPAIR];
+#pragma unroll
+      for (int e = 0; e < NPAIR; ++e) rp[e] = be_rp[r][e];
+      if (x <= 0 || x >= n0 - 1) face_rp(std::integral_constant<int, NPAIR>{}, x, r, 0, W0, TJ * W0, false, rp);
+      const T* om = OB[Sp1];  // plane q-7
+      const T* o0 = OB[Sm2];  // plane q-6
+      const T* op = OB[Sm1];  // plane q-5
+      const auto dst = own ? dout : dnone;
+      const int so = own ? (x - xb) * pb : 0;
+#pragma unroll
+      for (int e = 0; e < NPAIR; ++e) {
+        const int a = be_a0[e], c = be_aj[r][e];
+        const T sum = c0 * om[c] + c0 * op[c] + c1 * o0[c - W1] + c1 * o0[c + W1] +
+                      c2 * o0[a] + c2 * o0[a + 1];
+        const T ve = (rbe[XB][e] - sum) * rp[e];
+        const T vo = o0[pcol[r][e]];
+        const uint32_t oe = be_ro[r][e], oo = pofs[r][e];
+        buf_st(dst, oe, so, ve);
+        buf_st(dst, oo, so, vo);
+        if (own && (x == 0 || x == n0 - 1)) {  // i-face ring images
+          if (x == 0) {
+            buf_st(dout, oe, (-1 - xb) * pb, ve);
+            buf_st(dout, oo, (-1 - xb) * pb, vo);
+          }
+          if (x == n0 - 1) {
+            buf_st(dout, oe, (n0 - xb) * pb, ve);
+            buf_st(dout, oo, (n0 - xb) * pb, vo);
+          }
+        }
+        if (!wfast) {  // j and k face-ring images (dst drops them if !own)
+          const int f = fl[e];
+          const bool xbf = ((f & FL_X) != 0) != (r != 0);  // the even cell is b
+          const uint32_t oa = xbf ? oo : oe, ob = xbf ? oe : oo;
+          const T va = xbf ? vo : ve, vb = xbf ? ve : vo;
+          const uint32_t sj = (uint32_t)(s1 * (int)sizeof(T)), sk = (uint32_t)sizeof(T);
+          const uint32_t jd = (f & FL_JLO) ? 0u - sj : sj;
+          const bool jf = (f & (FL_JLO | FL_JHI)) != 0;
+          buf_st(dst, jf && (f & FL_AL) ? oa + jd : SWEEP_DEAD, so, va);
+          buf_st(dst, jf && (f & FL_BL) ? ob + jd : SWEEP_DEAD, so, vb);
+          buf_st(dst, (f & FL_ALO) ? oa - sk : (f & FL_AHI) ? oa + sk : SWEEP_DEAD, so, va);
+          buf_st(dst, (f & FL_BHI) ? ob + sk : SWEEP_DEAD, so, vb);
+          if (n1 == 1) {
+            buf_st(dst, (f & FL_AL) ? oa + sj : SWEEP_DEAD, so, va);
+            buf_st(dst, (f & FL_BL) ? ob + sj : SWEEP_DEAD, so, vb);
+          }
+          if (n2 == 1) buf_st(dst, (f & FL_AL) ? oa + sk : SWEEP_DEAD, so, va);
+        }
+      }
+    }
+  };
+
+  // prologue: pin planes i0-4 and i0-3 to LDS, i0-2 and i0-1 to registers;
+  // the first sweep's odd rhs of planes i0-3 and i0-2
+  load_pin(I1{}, i0 - 4);
+  store_pin(I1{}, P[3], i0 - 4);
+  load_pin(I0{}, i0 - 3);
+  store_pin(I0{}, P[0], i0 - 3);
+  load_pin(I1{}, i0 - 2);
+  load_pin(I0{}, i0 - 1);
+  {
+    const int s0x = soff(i0 - 3, i0 - 3, i1 + 2), s1x = soff(i0 - 2, i0 - 3, i1 + 2);
+#pragma unroll
+    for (int e = 0; e < N3; ++e) {
+      rao[0][e] = buf_ld<T>(drhs, ao_ro[0][e], s0x);
+      rao[1][e] = buf_ld<T>(drhs, ao_ro[1][e], s1x);
+    }
+  }
+  const int cnt = i1 - i0 + 9;  // q = i0-3 .. i1+5
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  for (int t = 0;; t += 4) {
+    step(I0{}, i0 - 3 + t);
+    if (t + 1 >= cnt) break;
+    step(I1{}, i0 - 2 + t);
+    if (t + 2 >= cnt) break;
+    step(I2{}, i0 - 1 + t);
+    if (t + 3 >= cnt) break;
+    step(I3{}, i0 + t);
+    if (t + 4 >= cnt) break;
+  }
+}
+template <class T, int TJ, int TK>
+__global__ void __launch_bounds__(256)
+sweep2_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
+              MgDev d, T c0, T c1, T c2, int ich) {
+  sweep2_body<T, TJ, TK>(pin, pout, rhs, d, c0, c1, c2, ich, this_blk());
 }
 
 // n sweeps of RBGS on one small level held in LDS by one workgroup
@@ -1015,8 +1386,45 @@ int sweep_t(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
   }
 }
 
+template <class T, int TJ, int TK>
+int launch_sweep2(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
+  const MgDev d = mg_dev(L);
+  T c0, c1, c2;
+  inv_h2(L, c0, c1, c2);
+  const int kt = div_up(d.n2, TK), jt = div_up(d.n1, TJ);
+  const int64_t tiles = (int64_t)kt * jt;
+  const int tuned = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP2_ICH_F64 : WF_TUNE_SWEEP2_ICH_F32,
+                             d.n0, d.n1, d.n2);
+  int ich = tuned > 0 ? tuned
+                      : (int)std::max<int64_t>(4, std::min<int64_t>(32, tiles * d.n0 / 1024));
+  // the kernel's plane offsets within a chunk ((ich + 8) planes) stay below 2^30
+  const int64_t pb = d.s0 * (int64_t)sizeof(T);
+  if ((int64_t)9 * pb >= ((int64_t)1 << 30)) return WF_EINVAL;
+  ich = (int)std::min<int64_t>(ich, ((int64_t)1 << 30) / pb - 8);
+  const dim3 grid(kt, jt, div_up(d.n0, ich));
+  hipLaunchKernelGGL((sweep2_kernel<T, TJ, TK>), grid, dim3(256), 0, s, (const T*)pin, (T*)pout,
+                     (const T*)L.rhs, d, c0, c1, c2, ich);
+  WF_LAUNCHED();
+  return 0;
+}
+
 template <class T>
-int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s) {
+int sweep2_t(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
+  const int tile = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP2_TILE_F64 : WF_TUNE_SWEEP2_TILE_F32,
+                            L.n0, L.n1, L.n2);
+  return tile == 1 ? launch_sweep2<T, 8, 64>(L, pin, pout, s)
+                   : launch_sweep2<T, 16, 32>(L, pin, pout, s);
+}
+
+// nsweeps RBGS sweeps of L.  Pairs of sweeps run as two-sweep launches
+// (phi -> tmp, then tmp -> phi, ...); with allow_swap an odd number of them
+// may leave the result in tmp, which *swapped reports (the caller then
+// treats tmp as the level's phi); without it the pairs go in even numbers
+// and the rest as one-sweep launches, so the result is in phi.
+template <class T>
+int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s, bool allow_swap = false,
+            bool* swapped = nullptr) {
+  if (swapped) *swapped = false;
   if (nsweeps <= 0) return 0;
   const size_t cells = (size_t)L.n0 * L.n1 * L.n2;
   const size_t lds = (size_t)max_lds();
@@ -1027,17 +1435,30 @@ int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s) {
     return launch_relax_lds<T, true>(L, nsweeps, 2 * cells * sizeof(T), s);
   if (cells <= LDS_RELAX_MAX_CELLS && cells * sizeof(T) <= lds)
     return launch_relax_lds<T, false>(L, nsweeps, cells * sizeof(T), s);
+  wf_mg_level C = L;  // C.phi holds the current iterate
   int q = 0;
-  if (L.tmp) {
+  const int pairs = tune_get(WF_TUNE_SWEEP_PAIRS, L.n0, L.n1, L.n2);
+  // built-in: two-sweep launches where they measured faster (FP64, >= 2^23 cells)
+  if (C.tmp && (pairs == 2 || (pairs == 0 && sizeof(T) == 8 && cells >= ((size_t)1 << 23)))) {
+    int npair = nsweeps / 2;
+    if (!allow_swap) npair &= ~1;
+    for (int f = 0; f < npair; ++f) {
+      WF_TRY((hipError_t)sweep2_t<T>(C, C.phi, C.tmp, s));
+      std::swap(C.phi, C.tmp);
+    }
+    q = 2 * npair;
+  }
+  if (C.tmp) {
     for (; q + 1 < nsweeps; q += 2) {
-      WF_TRY((hipError_t)sweep_t<T>(L, L.phi, L.tmp, s));
-      WF_TRY((hipError_t)sweep_t<T>(L, L.tmp, L.phi, s));
+      WF_TRY((hipError_t)sweep_t<T>(C, C.phi, C.tmp, s));
+      WF_TRY((hipError_t)sweep_t<T>(C, C.tmp, C.phi, s));
     }
   }
   for (; q < nsweeps; ++q) {
-    WF_TRY((hipError_t)launch_rbgs<T>(L, 0, 1, s));
-    WF_TRY((hipError_t)launch_rbgs<T>(L, 1, 1, s));
+    WF_TRY((hipError_t)launch_rbgs<T>(C, 0, 1, s));
+    WF_TRY((hipError_t)launch_rbgs<T>(C, 1, 1, s));
   }
+  if (swapped) *swapped = C.phi != L.phi;
   return 0;
 }
 
@@ -1077,8 +1498,10 @@ bool pair_ok(const wf_mg_level* F, const wf_mg_level* C) {
          F->n1 == 2 * C->n1 && F->n2 == 2 * C->n2;
 }
 
-int relax_any(const wf_mg_level& L, int n, hipStream_t s) {
-  return L.prec == 8 ? relax_t<double>(L, n, s) : relax_t<float>(L, n, s);
+int relax_any(const wf_mg_level& L, int n, hipStream_t s, bool allow_swap = false,
+              bool* swapped = nullptr) {
+  return L.prec == 8 ? relax_t<double>(L, n, s, allow_swap, swapped)
+                     : relax_t<float>(L, n, s, allow_swap, swapped);
 }
 
 constexpr int COARSE_OP_MAX_CELLS = 4096;
@@ -1126,8 +1549,9 @@ int coarse_op_build_t(const wf_mg_level& L, int nsweeps, void* op, hipStream_t s
   return 0;
 }
 
-int vcycle_any(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse,
-               hipStream_t s) {
+template <class Relax>
+int vcycle_levels(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse,
+                  hipStream_t s, Relax&& relax_l) {
   for (int l = 0; l < nlev; ++l) {
     if (l > 0) {
       const int r = lv[l].prec == 8 ? restrict_t<double>(lv[l - 1], lv[l], s)
@@ -1140,16 +1564,39 @@ int vcycle_any(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse
       if (r) return r;
       continue;
     }
-    const int r = relax_any(lv[l], l < nlev - 1 ? npre : ncoarse, s);
+    const int r = relax_l(l, l < nlev - 1 ? npre : ncoarse);
     if (r) return r;
   }
   for (int l = nlev - 2; l >= 0; --l) {
     const int r = lv[l].prec == 8 ? prolong_t<double>(lv[l + 1], lv[l], s)
                                   : prolong_t<float>(lv[l + 1], lv[l], s);
     if (r) return r;
-    const int r2 = relax_any(lv[l], npost, s);
+    const int r2 = relax_l(l, npost);
     if (r2) return r2;
   }
+  return 0;
+}
+
+// One V-cycle.  The levels are worked on through a local view whose phi and
+// tmp swap when a relax leaves its result in tmp (an odd number of two-sweep
+// launches); a level still swapped at the end is copied back, so the caller
+// sees every level's iterate in its phi (with V(6,6): 3 + 3 launches, none).
+int vcycle_any(const wf_mg_level* lv0, int nlev, int npre, int npost, int ncoarse,
+               hipStream_t s) {
+  wf_mg_level lv[16];
+  for (int l = 0; l < nlev; ++l) lv[l] = lv0[l];
+  auto relax = [&](int l, int n) -> int {
+    bool sw = false;
+    const int r = relax_any(lv[l], n, s, true, &sw);
+    if (sw) std::swap(lv[l].phi, lv[l].tmp);
+    return r;
+  };
+  const int r = vcycle_levels(lv, nlev, npre, npost, ncoarse, s, relax);
+  if (r) return r;
+  for (int l = 0; l < nlev; ++l)
+    if (lv[l].phi != lv0[l].phi)
+      WF_TRY(hipMemcpyAsync(lv0[l].phi, lv[l].phi, (size_t)level_elems(lv0[l]) * lv0[l].prec,
+                            hipMemcpyDeviceToDevice, s));
   return 0;
 }
 

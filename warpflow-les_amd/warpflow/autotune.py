@@ -1,15 +1,18 @@
 """Launch-shape autotuner for the hot kernels.
 
-The fused momentum kernel (``wf_convdiff4th``) and the fused RBGS sweep have
-launch-shape parameters that change no result: the i-planes each workgroup
-streams through (``cd_ich``, ``sweep_ich``) and the sweep's (j, k) tile
-(``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32).  The library keeps one setting per
-(parameter, launch shape); ``tune_solver`` times every candidate with HIP
-events on a solver's own arrays -- the momentum kernel on the solver's grid,
-the sweep on each multigrid level that uses it -- checks that each
-candidate's output equals the built-in choice bit for bit, restores the
-arrays and records the fastest.  Results can be cached in a JSON file keyed
-by device and grid.
+The fused momentum kernel (``wf_convdiff4th``) and the RBGS sweep kernels
+have launch-shape parameters that change no result: the i-planes each
+workgroup streams through (``cd_ich``, ``sweep_ich``, ``sweep2_ich``) and the
+(j, k) tile of the one-sweep kernel (``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32,
+3 = 16 x 64, 4 = 32 x 32) and of the two-sweep kernel (``sweep2_tile``: 1 =
+8 x 64, 2 = 16 x 32); ``sweep_pairs`` picks, per level, one-sweep launches
+(1) or two-sweep launches (2) for pairs of sweeps.  The library keeps one setting per (parameter,
+launch shape); ``tune_solver`` times every candidate with HIP events on a
+solver's own arrays -- the momentum kernel on the solver's grid, the sweeps
+on each multigrid level that uses them -- checks that each candidate's
+output equals the built-in choice bit for bit (two-sweep launches against
+one-sweep launches), restores the arrays and records the fastest.  Results
+can be cached in a JSON file keyed by device and grid.
 """
 from __future__ import annotations
 
@@ -23,10 +26,13 @@ from . import _hip
 from .device import copy, current_stream, zeros, zeros_like
 
 # knob ids (include/warpflow_hip.h)
-CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32 = range(5)
+(CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32, SWEEP2_TILE_F64,
+ SWEEP2_TILE_F32, SWEEP2_ICH_F64, SWEEP2_ICH_F32, SWEEP_PAIRS) = range(10)
 CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
 SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
 SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32)
+SWEEP2_TILE_CANDIDATES = (1, 2)
+SWEEP2_ICH_CANDIDATES = (0, 2, 4, 8, 16, 32, 64)
 SWEEP_MIN_CELLS = 4096  # smaller levels relax in one LDS-resident workgroup
 
 
@@ -127,6 +133,7 @@ def _tune_level(lev, T, reps, out):
         _hip.call("wf_mg_relax", C.byref(d), 2, current_stream())
     set_shape(ktile, n3, 0)
     set_shape(kich, n3, 0)
+    set_shape(SWEEP_PAIRS, n3, 0)
     relax()
     want = lev.phi.numpy()
     t_copy = T.us(lambda: copy(lev.phi, keep_phi), reps)
@@ -145,6 +152,46 @@ def _tune_level(lev, T, reps, out):
     set_shape(ktile, n3, best[1])
     set_shape(kich, n3, best[2])
     out["entries"] += [(ktile, tuple(n3), best[1]), (kich, tuple(n3), best[2])]
+
+    # four sweeps: two one-sweep pairs against two two-sweep launches
+    k2tile = SWEEP2_TILE_F64 if f64 else SWEEP2_TILE_F32
+    k2ich = SWEEP2_ICH_F64 if f64 else SWEEP2_ICH_F32
+
+    def relax4():
+        copy(lev.phi, keep_phi)
+        _hip.call("wf_mg_relax", C.byref(d), 4, current_stream())
+    set_shape(SWEEP_PAIRS, n3, 1)
+    relax4()
+    want4 = lev.phi.numpy()
+    t_single = T.us(relax4, reps) - t_copy
+    out["timings_us"]["4 sweeps %s %s one-sweep launches" % (
+        "x".join(map(str, n3)), "f64" if f64 else "f32")] = round(t_single, 2)
+    set_shape(SWEEP_PAIRS, n3, 2)
+    best2 = None
+    try:
+        for tile in SWEEP2_TILE_CANDIDATES:
+            for ich in SWEEP2_ICH_CANDIDATES:
+                set_shape(k2tile, n3, tile)
+                set_shape(k2ich, n3, ich)
+                t = T.us(relax4, reps) - t_copy
+                if not np.array_equal(lev.phi.numpy(), want4):
+                    raise RuntimeError("two-sweep result differs from one-sweep launches "
+                                       "(tile=%d ich=%d)" % (tile, ich))
+                out["timings_us"]["4 sweeps %s %s two-sweep tile=%d ich=%d" % (
+                    "x".join(map(str, n3)), "f64" if f64 else "f32", tile, ich)] = round(t, 2)
+                if best2 is None or t < best2[0]:
+                    best2 = (t, tile, ich)
+    finally:
+        set_shape(k2tile, n3, 0)
+        set_shape(k2ich, n3, 0)
+    if best2[0] < t_single:
+        set_shape(k2tile, n3, best2[1])
+        set_shape(k2ich, n3, best2[2])
+        out["entries"] += [(k2tile, tuple(n3), best2[1]), (k2ich, tuple(n3), best2[2]),
+                           (SWEEP_PAIRS, tuple(n3), 2)]
+    else:
+        set_shape(SWEEP_PAIRS, n3, 1)
+        out["entries"].append((SWEEP_PAIRS, tuple(n3), 1))
     copy(lev.phi, keep_phi)
     copy(lev.tmp, keep_tmp)
 
