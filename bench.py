@@ -61,6 +61,10 @@ def parse(argv=None):
                     help="ensemble64: timed steps of the sequential single-cavity baselines")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in launch shapes instead of tuning them first")
+    ap.add_argument("--tune-cache", default=None,
+                    help="JSON file of tuned launch shapes: read if it holds this grid and "
+                         "device, else tuned and written (e.g. to profile a run with the "
+                         "shapes an unprofiled run chose)")
     ap.add_argument("--probe", action="store_true",
                     help="only launch each hot kernel --reps times (hardware-counter runs)")
     ap.add_argument("--mixed-cycles", type=int, default=2,
@@ -95,7 +99,8 @@ def run_form(a, precision):
     sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
     tuned = None
     if not a.no_autotune:  # launch shapes for this grid and form, untimed, arrays restored
-        tuned = [[k, list(n3), v] for k, n3, v in autotune.tune_solver(sol, reps=10)["entries"]]
+        tuned = [[k, list(n3), v] for k, n3, v in
+                 autotune.tune_solver(sol, reps=10, cache=a.tune_cache)["entries"]]
     for _ in range(a.warmup):
         sol.step()
     synchronize()
