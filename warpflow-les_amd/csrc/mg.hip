@@ -960,6 +960,98 @@ relax_lds_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, 
   relax_lds_body<T, RHS_LDS>(phi, rhs, d, c0, c1, c2, nsweeps, reinterpret_cast<T*>(smem_raw));
 }
 
+// n sweeps of RBGS on one level held in LDS by one workgroup of 1024 threads,
+// for levels whose zero-ringed copy fits LDS and whose cells of one colour
+// number at most 4 per thread (e.g. 16^3).
+// LDS holds the (n0+2) x (n1+2) x (n2+2) padded level split by colour: the
+// cells of one colour of a padded row packed into W = ceil((n2+2)/2) words,
+// so cell (I, J, K) (padded coordinates) of colour a = (I+J+K) & 1 sits at
+// word ((I*(n1+2) + J)*W + K/2) of array a, its i and j neighbours at that
+// word -/+ (n1+2)*W and -/+ W of the other array, and its k neighbours at
+// km and km + 1 with km = word - 1 + (K & 1): neighbouring lanes read
+// neighbouring words.  The ring is zero, so every cell uses the uniform
+// coefficients c0, c1, c2 in _mgcoeff's order of terms and only 1/ap is per
+// cell -- the arithmetic of the sweep kernels, bit for bit.  Each thread keeps
+// its cells' words, 1/ap and rhs in registers (CPT cells per colour); dead
+// slots point at a ring (or spare) word of their own array with 1/ap = 0,
+// which keeps the ring zero.
+template <class T, int CPT>
+__device__ inline void relax_lds2_body(T* __restrict__ phi, const T* __restrict__ rhs,
+                                       const MgDev& d, T c0, T c1, T c2, int nsweeps, T* sm) {
+  const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
+  const int W = (n2 + 3) / 2;          // packed words per padded row and colour
+  const int R1 = n1 + 2;               // padded rows per padded plane
+  const int PW = (n0 + 2) * R1 * W;    // words per colour array
+  const int SI = R1 * W;               // i stride in words
+  T* A[2] = {sm, sm + PW};
+  const int nt = blockDim.x, tid = threadIdx.x;
+  const int ncell = n0 * n1 * n2;
+  auto word = [&](int i, int j, int k) {  // real cell (i, j, k)
+    return ((i + 1) * R1 + (j + 1)) * W + ((k + 1) >> 1);
+  };
+  for (int w = tid; w < 2 * PW; w += nt) sm[w] = T(0);
+  __syncthreads();
+  for (int l = tid; l < ncell; l += nt) {
+    const int k = l % n2, j = (l / n2) % n1, i = l / (n1 * n2);
+    A[(i + j + k + 3) & 1][word(i, j, k)] = phi[mg_at(d, i, j, k)];
+  }
+  // this thread's cells of colour c: k = 2m + ((i + j + c + 1) & 1) in row
+  // (i, j), slot q = (i*n1 + j)*np + m; word*2 + (K & 1) packed in pk
+  const int np = (n2 + 1) / 2, nslot = n0 * n1 * np;
+  int pk[2][CPT];
+  T rp[2][CPT], rv[2][CPT];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int e = 0; e < CPT; ++e) {
+      const int q = tid + nt * e;
+      const int qc = q < nslot ? q : nslot - 1;
+      const int m = qc % np, r = qc / np;
+      const int j = r % n1, i = r / n1;
+      const int k = 2 * m + ((i + j + c + 1) & 1);
+      const bool live = q < nslot && k < n2;
+      if (live) {
+        pk[c][e] = word(i, j, k) * 2 + ((k + 1) & 1);
+      } else {  // a ring (or spare) word of array c in row (i+1, j+1): K = 0 or n2+1
+        const int sp = (c + i + j) & 1;  // array c holds K = 2t + sp in this row
+        const int t = sp == 0 ? 0 : W - 1;
+        pk[c][e] = (((i + 1) * R1 + (j + 1)) * W + t) * 2 + sp;
+      }
+      rp[c][e] = live ? T(1) / mg_coef(d, i, j, k, c0, c1, c2).ap : T(0);
+      rv[c][e] = live ? rhs[mg_at(d, i, j, k)] : T(0);
+    }
+  __syncthreads();
+  for (int s = 0; s < nsweeps; ++s) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const T* src = A[c ^ 1];
+      T* dst = A[c];
+#pragma unroll
+      for (int e = 0; e < CPT; ++e) {
+        const int wd = pk[c][e] >> 1, km = wd - 1 + (pk[c][e] & 1);
+        const T sum = c0 * src[wd - SI] + c0 * src[wd + SI] + c1 * src[wd - W] +
+                      c1 * src[wd + W] + c2 * src[km] + c2 * src[km + 1];
+        dst[wd] = (rv[c][e] - sum) * rp[c][e];
+      }
+      __syncthreads();
+    }
+  }
+  for (int l = tid; l < ncell; l += nt) {
+    const int k = l % n2, j = (l / n2) % n1, i = l / (n1 * n2);
+    const int64_t x = mg_at(d, i, j, k);
+    const T v = A[(i + j + k + 3) & 1][word(i, j, k)];
+    phi[x] = v;
+    if (nsweeps > 0) ring_images(phi, d, i, j, k, x, v);
+  }
+}
+template <class T, int CPT>
+__global__ void __launch_bounds__(1024)
+relax_lds2_kernel(T* __restrict__ phi, const T* __restrict__ rhs, MgDev d, T c0, T c1, T c2,
+                  int nsweeps) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  relax_lds2_body<T, CPT>(phi, rhs, d, c0, c1, c2, nsweeps, reinterpret_cast<T*>(smem_raw));
+}
+
 // Column `col` (= workgroup) of a small level's coarse operator: nsweeps RBGS
 // sweeps in LDS from phi = 0 with rhs = e_col, the relax_lds_kernel
 // arithmetic; M is column-major (M[col * ncell + row]).
@@ -1319,6 +1411,46 @@ int launch_rbgs(const wf_mg_level& L, int color, int write_ring, hipStream_t s) 
   return 0;
 }
 
+// Packed single-workgroup relax: bytes of LDS and cells per thread per colour
+// (0 when the level does not fit).
+inline size_t relax_lds2_bytes(const wf_mg_level& L, size_t elem) {
+  return 2 * (size_t)(L.n0 + 2) * (L.n1 + 2) * ((L.n2 + 3) / 2) * elem;
+}
+inline int relax_lds2_cpt(const wf_mg_level& L) {
+  const int64_t nslot = (int64_t)L.n0 * L.n1 * ((L.n2 + 1) / 2);
+  for (int cpt = 1; cpt <= 4; cpt *= 2)  // more cells per thread spill registers
+    if (nslot <= (int64_t)cpt * 1024) return cpt;
+  return 0;
+}
+
+template <class T, int CPT>
+int launch_relax_lds2_t(const wf_mg_level& L, int nsweeps, size_t bytes, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    WF_TRY(hipFuncSetAttribute((const void*)relax_lds2_kernel<T, CPT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds()));
+    attr_set = true;
+  }
+  const MgDev d = mg_dev(L);
+  T c0, c1, c2;
+  inv_h2(L, c0, c1, c2);
+  hipLaunchKernelGGL((relax_lds2_kernel<T, CPT>), dim3(1), dim3(1024), bytes, s, (T*)L.phi,
+                     (const T*)L.rhs, d, c0, c1, c2, nsweeps);
+  WF_LAUNCHED();
+  return 0;
+}
+
+template <class T>
+int launch_relax_lds2(const wf_mg_level& L, int nsweeps, hipStream_t s) {
+  const size_t bytes = relax_lds2_bytes(L, sizeof(T));
+  switch (relax_lds2_cpt(L)) {
+    case 1: return launch_relax_lds2_t<T, 1>(L, nsweeps, bytes, s);
+    case 2: return launch_relax_lds2_t<T, 2>(L, nsweeps, bytes, s);
+    case 4: return launch_relax_lds2_t<T, 4>(L, nsweeps, bytes, s);
+    default: return WF_EINVAL;
+  }
+}
+
 template <class T, bool RHS_LDS>
 int launch_relax_lds(const wf_mg_level& L, int nsweeps, size_t bytes, hipStream_t s) {
   static bool attr_set = false;
@@ -1428,6 +1560,9 @@ int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s, bool allow_swap = 
   if (nsweeps <= 0) return 0;
   const size_t cells = (size_t)L.n0 * L.n1 * L.n2;
   const size_t lds = (size_t)max_lds();
+  // levels whose zero-ringed copy fits LDS: all sweeps in one workgroup
+  if (relax_lds2_cpt(L) > 0 && relax_lds2_bytes(L, sizeof(T)) <= lds)
+    return launch_relax_lds2<T>(L, nsweeps, s);
   // many sweeps in one workgroup pay off only with rhs in LDS too: with rhs
   // read from global memory every colour pass waits on its latency
   if ((cells <= LDS_RELAX_MAX_CELLS || nsweeps >= LDS_RELAX_MIN_SWEEPS) &&
