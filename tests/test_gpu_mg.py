@@ -91,12 +91,14 @@ def test_rbgs_refill_restrict_prolong(gpu, n3):
 @pytest.mark.parametrize("n3,nsweeps", [((8, 8, 8), 300), ((20, 20, 20), 300),
                                         ((20, 20, 20), 6), ((40, 40, 40), 3),
                                         ((16, 8, 24), 5), ((32, 32, 32), 4),
-                                        ((64, 48, 40), 6), ((24, 72, 100), 2)])
+                                        ((64, 48, 40), 6), ((24, 72, 100), 2),
+                                        ((9, 7, 11), 6), ((5, 13, 3), 7)])
 def test_relax_paths(gpu, n3, nsweeps):
     """_relax: small levels (and long coarsest solves) run in one LDS-resident
-    workgroup, larger ones as fused out-of-place full sweeps (ping-pong, both
-    tile shapes, partial tiles), an odd leftover sweep as two half-sweep
-    launches; all must match the oracle incl. the ring."""
+    workgroup (colour-packed, zero-ringed; odd extents leave spare words),
+    larger ones as out-of-place sweep launches (ping-pong, tile shapes,
+    partial tiles, two-sweep launches), an odd leftover sweep as two
+    half-sweep launches; all must match the oracle incl. the ring."""
     mg, ref = build(n3, 1)
     load(mg, ref, seed=4)
     ref._relax(0, nsweeps)
