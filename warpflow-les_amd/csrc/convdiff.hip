@@ -20,9 +20,13 @@
 //     I4y(u)(p) (I4z for w) at planes i+1 .. i-2; Q of the newest plane is
 //     formed from plane i+1 of u staged in LDS and rolled through registers.
 //     The remaining cross term reads plane i of the other component from LDS.
-// LDS is double-buffered by plane parity, so one barrier per plane orders
-// the staging of plane i against the reads of plane i-2; every global load
-// for plane i+1 is issued before the compute of plane i.
+// The cross-quantity LDS is double-buffered by plane parity, so one barrier
+// per plane orders the staging of plane i against the reads of plane i-2;
+// every global load for plane i+1 is issued before the compute of plane i.
+// The own-component plane has four slots: a thread's halo value of plane
+// i+3 is loaded during plane i and stored into its slot during plane i+1, so
+// that neighbouring tiles read a halo line close to when its owner tile
+// fetched it for its register window (the line is still in L2).
 //
 // Arithmetic.  Each interpolant, product, D4 and Laplacian is the reference's
 // expression term for term (summation order included), except that the
@@ -181,7 +185,7 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
                                      const Coefs& K) {
   using RA = typename Lay<C>::A;
   using RB = typename Lay<C>::B;
-  __shared__ double Ls[2][OwnR::N];
+  __shared__ double Ls[4][OwnR::N];
   __shared__ double La[2][RA::N];
   __shared__ double Lb[2][RB::N];
 
@@ -223,7 +227,12 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
   for (int q = 1; q < 7; ++q) f[q] = F[G.plane(i0 - 4 + q) + own.off];
   f[0] = 0.0;
   double fnew = F[G.plane(i0 + 3) + own.off];
-  double fhalo = hs.on ? F[G.plane(i0) + hs.off] : 0.0;
+  // halo of planes i0, i0+1 straight into their slots; plane i0+2 in flight
+  if (hs.on) {
+    Ls[0][hs.lds] = F[G.plane(i0) + hs.off];
+    Ls[1][hs.lds] = F[G.plane(i0 + 1) + hs.off];
+  }
+  double fhalo = hs.on ? F[G.plane(i0 + 2) + hs.off] : 0.0;
   double cnew = coo[G.plane(i0) + own.off];
   double vnew = vis[G.plane(i0) + visoff];
 
@@ -281,17 +290,18 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
   }
 
   for (int i = i0; i < i1; ++i) {
-    const int buf = (i - i0) & 1;
+    const int rel = i - i0, buf = rel & 1;
     // shift the windows to planes i-3 .. i+3 (own) and i-1 .. i+2 (C = 0)
 #pragma unroll
     for (int q = 0; q < 6; ++q) f[q] = f[q + 1];
     f[6] = fnew;
     const double cur_coo = cnew, cur_vis = vnew;
-    double* __restrict__ S = Ls[buf];
+    double* __restrict__ S = Ls[rel & 3];
     double* __restrict__ A = La[buf];
     double* __restrict__ B = Lb[buf];
     S[own.lds] = f[3];
-    if (hs.on) S[hs.lds] = fhalo;
+    // halo of plane i+2 into the slot last read at plane i-2
+    if (hs.on) Ls[(rel + 2) & 3][hs.lds] = fhalo;
     if constexpr (C == 0) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -318,7 +328,7 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
     // loads for plane i+1
     if (i + 1 < i1) {
       fnew = F[G.plane(i + 4) + own.off];
-      if (hs.on) fhalo = F[G.plane(i + 1) + hs.off];
+      if (hs.on) fhalo = F[G.plane(i + 3) + hs.off];
       cnew = coo[G.plane(i + 1) + own.off];
       vnew = vis[G.plane(i + 1) + visoff];
       if constexpr (C == 0) {

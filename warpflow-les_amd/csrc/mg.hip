@@ -894,8 +894,13 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
     if (t + 4 >= cnt) break;
   }
 }
+// The FP32 two-sweep kernel is held to 128 VGPRs so that four workgroups
+// share a CU (it fits without spilling); the FP64 one keeps the compiler's
+// allocation.
+template <class T>
+constexpr int sweep2_min_blocks() { return sizeof(T) == 4 ? 4 : 1; }
 template <class T, int TJ, int TK>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, (sweep2_min_blocks<T>()))
 sweep2_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
               MgDev d, T c0, T c1, T c2, int ich) {
   sweep2_body<T, TJ, TK>(pin, pout, rhs, d, c0, c1, c2, ich, this_blk());
