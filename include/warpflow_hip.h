@@ -142,6 +142,18 @@ int wf_calmas(const double* u, const double* v, const double* w, double* su,
               double* rmax_dev, wf_grid g, int d0, int d1, int d2, double dx,
               double dy, double dz, double dt, void* stream);
 
+/* wf_calvel3 followed by wf_calmas over the real cells, in one pass: the
+ * pair _project launches back to back (solver.py:186-197, calvel_{u,v,w}
+ * kernels.py:232-264 then calmas kernels.py:271-289; the halo exchange
+ * between them is a no-op on one domain).  Writes u, v, w, su and
+ * *rmax_dev = max(*rmax_dev, max|div*dx*dy*dz|) bit for bit as the two
+ * entries do; calmas no longer re-reads u, v, w from memory.  No single
+ * reference kernel corresponds to it. */
+int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* wstar,
+                      const double* p, double* u, double* v, double* w, double* su,
+                      double* rmax_dev, wf_grid g, double dx, double dy, double dz,
+                      double dt, double alpha, void* stream);
+
 /* wale_viscosity, kernels.py:297-367: vis = rrey + nut (WALE, cw), incl. the
  * reference's g2zx/g2zy terms (kernels.py:336, :338).  delta is
  * (dx*dy*dz)^(1/3) (kernels.py:360), computed once by the caller. */

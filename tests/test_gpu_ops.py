@@ -92,6 +92,40 @@ def test_calvel_calvel3_calmas(gpu, n3):
     assert abs(rbuf.numpy()[0] - rmax_ref) <= TOL * rmax_ref
 
 
+@pytest.mark.parametrize("n3", SHAPES + [(20, 19, 37), (9, 17, 70), (33, 8, 33)])
+def test_calvel3_calmas_one_pass(gpu, n3):
+    """The fused calvel/calmas pass equals calvel3 then calmas bit for bit (u, v,
+    w incl. ghosts, su, rmax), on random fields with random ghosts, with
+    partial (j, k) tiles, and matches the oracle's calvel + calmas."""
+    from warpflow import kernels as K
+    from warpflow.device import zeros
+    h = (0.031, 0.027, 0.019)
+    dt = 7e-4
+    f = rand_fields(n3, ["us", "vs", "ws", "p", "u", "v", "w", "su"], seed=5)
+    a = {k: up(v) for k, v in f.items()}
+    b = {k: up(v) for k, v in f.items()}
+    ra, rb = zeros(1), zeros(1)
+    K.calvel3(n3, a["us"], a["vs"], a["ws"], a["p"], a["u"], a["v"], a["w"], *h, dt, 1.0, N)
+    K.calmas(n3, a["u"], a["v"], a["w"], a["su"], ra, *h, dt, N)
+    K.calvel3_calmas(n3, b["us"], b["vs"], b["ws"], b["p"], b["u"], b["v"], b["w"], b["su"],
+                     rb, *h, dt, 1.0, N)
+    for nm in ("u", "v", "w", "su"):
+        assert np.array_equal(a[nm].numpy(), b[nm].numpy()), nm
+    assert ra.numpy()[0] == rb.numpy()[0] > 0.0
+    # oracle
+    ref = {c: f[c].copy() for c in "uvw"}
+    ops = {"u": (n3[0] - 1, n3[1], n3[2]), "v": (n3[0], n3[1] - 1, n3[2]),
+           "w": (n3[0], n3[1], n3[2] - 1)}
+    for c, ax in (("u", 0), ("v", 1), ("w", 2)):
+        getattr(O, "calvel_" + c)(f[c + "s"], f["p"], ref[c], h[ax], dt, 1.0, N, ops[c])
+    su_ref = f["su"].copy()
+    rmax_ref = O.calmas(ref["u"], ref["v"], ref["w"], su_ref, *h, dt, N, n3)
+    for c in "uvw":
+        assert rel_err(b[c].numpy() - f[c + "s"], ref[c] - f[c + "s"]) < TOL, c
+    assert rel_err(b["su"].numpy(), su_ref) < TOL
+    assert abs(rb.numpy()[0] - rmax_ref) <= TOL * rmax_ref
+
+
 @pytest.mark.parametrize("n3", SHAPES)
 def test_wale_and_checkdt(gpu, n3):
     from warpflow import kernels as K

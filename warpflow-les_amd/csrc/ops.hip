@@ -120,6 +120,30 @@ diff4th_kernel(const double* __restrict__ c, const double* __restrict__ vis,
   });
 }
 
+// The pressure correction of one velocity value, c = cstar - dt*alpha*dp/h
+// (kernels.py:239, :250, :261), with 1/h formed on the host.  The rounding
+// is fixed here (one product, one fused multiply-add) rather than left to
+// the compiler, so that every kernel computing a calvel value -- the three
+// calvel launches, calvel3 and the fused calvel/calmas pass, single-domain
+// or batched -- produces the same bits.
+__device__ inline double calvel_val(double cstar, double pn, double p0, double dta, double rh) {
+#pragma clang fp contract(off)
+  return __builtin_fma(-rh, dta * (pn - p0), cstar);
+}
+// The bracket of the staggered 4-point derivative, -s[+] + 27 s[0] - 27 s[-]
+// + s[--] (kernels.py:281-286), and the divergence dudx + dvdy + dwdz from
+// the three brackets and 1/(24 h): fixed roundings, shared by calmas and the
+// fused calvel/calmas pass.
+__device__ inline double d4_bracket(double sp, double sn, double sm, double smm) {
+#pragma clang fp contract(off)
+  return __builtin_fma(-27.0, sm, __builtin_fma(27.0, sn, -sp)) + smm;
+}
+__device__ inline double div4(double tx, double ty, double tz, double r24x, double r24y,
+                              double r24z) {
+#pragma clang fp contract(off)
+  return __builtin_fma(tz, r24z, __builtin_fma(ty, r24y, tx * r24x));
+}
+
 __global__ void __launch_bounds__(256)
 calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
               double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double rh, double dta,
@@ -128,7 +152,7 @@ calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
     int i, j, k;
     if (!box_thread(b, B, i, j, k)) return;
     const int64_t x = at(g, i, j, k);
-    c[x] = cstar[x] - dta * (p[x + s] - p[x]) * rh;
+    c[x] = calvel_val(cstar[x], p[x + s], p[x], dta, rh);
   });
 }
 
@@ -143,9 +167,9 @@ __device__ inline void calvel3_body(const double* __restrict__ us, const double*
   if (!box_thread(b, B, i, j, k)) return;
   const int64_t x = at(g, i, j, k);
   const double p0 = p[x];
-  if (i < g.npl + g.ni - 1) u[x] = us[x] - dta * (p[x + g.si] - p0) * rdx;
-  if (j < g.npl + g.nj - 1) v[x] = vs[x] - dta * (p[x + g.sj] - p0) * rdy;
-  if (k < g.npl + g.nk - 1) w[x] = ws[x] - dta * (p[x + 1] - p0) * rdz;
+  if (i < g.npl + g.ni - 1) u[x] = calvel_val(us[x], p[x + g.si], p0, dta, rdx);
+  if (j < g.npl + g.nj - 1) v[x] = calvel_val(vs[x], p[x + g.sj], p0, dta, rdy);
+  if (k < g.npl + g.nk - 1) w[x] = calvel_val(ws[x], p[x + 1], p0, dta, rdz);
 }
 
 __global__ void __launch_bounds__(256)
@@ -219,10 +243,11 @@ __device__ inline void calmas_body(const double* __restrict__ u, const double* _
     const int64_t sx = g.si, sy = g.sj;
     for (int ii = ibeg; ii < iend; ++ii) {
       const int64_t x = at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k);
-      const double dudx = D4r(u[x + sx], u[x], u[x - sx], u[x - 2 * sx], r24x);
-      const double dvdy = D4r(v[x + sy], v[x], v[x - sy], v[x - 2 * sy], r24y);
-      const double dwdz = D4r(w[x + 1], w[x], w[x - 1], w[x - 2], r24z);
-      const double fact = dudx + dvdy + dwdz;
+      // dudx + dvdy + dwdz (kernels.py:281-287)
+      const double fact = div4(d4_bracket(u[x + sx], u[x], u[x - sx], u[x - 2 * sx]),
+                               d4_bracket(v[x + sy], v[x], v[x - sy], v[x - 2 * sy]),
+                               d4_bracket(w[x + 1], w[x], w[x - 1], w[x - 2]), r24x, r24y,
+                               r24z);
       su[x] = fact * rdt;
       val = nan_max(val, fabs(fact * dx * dy * dz));
     }
@@ -236,6 +261,184 @@ calmas_kernel(const double* __restrict__ u, const double* __restrict__ v,
               const double* __restrict__ w, double* __restrict__ su, wf_grid g, Box3 b,
               int ich, double dx, double dy, double dz, double dt) {
   calmas_body(u, v, w, su, g, b, this_blk(), ich, dx, dy, dz, dt, g_red_part, red_block());
+}
+
+// calvel3 followed by calmas in one pass.  The projection loop always runs
+// calmas right after a calvel of all three components (solver.py:186-197;
+// the halo exchange between them does nothing on one domain), and calmas
+// then re-reads the u, v, w calvel has just written.  Here a workgroup owns
+// an 8 x 32 (j, k) tile of the real cells and streams a chunk of i-planes:
+//   * u is corrected one plane ahead in the thread's own column and kept in
+//     a four-plane register window (the x-derivative needs planes i+1 .. i-2);
+//   * v and w of plane i are corrected at the tile cells and at the halo the
+//     y- and z-derivatives reach (rows -2, -1, +8 of v; columns -2, -1, +32
+//     of w), staged in LDS (double-buffered by plane parity, one barrier per
+//     plane), and the divergence of plane i is formed from them.
+// A halo value is recomputed with the same expression as the tile that owns
+// it, and a position outside calvel's write range (walls, ghosts; calvel3_body
+// above) contributes its value in memory, as calmas would read it, so u, v,
+// w, su and rmax equal calvel3 + calmas bit for bit.  Each value is written
+// once, by the workgroup owning its cell and plane.  HBM traffic falls from
+// 88 to 64 bytes per cell (calmas no longer reads u, v, w).
+namespace pj {
+constexpr int TJ = 8, TK = 32, NT = TJ * TK;
+constexpr int VW = TK, VR = TJ + 3;  // v: rows -2 .. TJ, columns 0 .. TK-1
+constexpr int WW = TK + 3;           // w: rows 0 .. TJ-1, columns -2 .. TK
+constexpr int NVH = 3 * TK, NWH = 3 * TJ;
+static_assert(NVH <= NT && NWH <= NT, "one halo position of each kind per thread");
+__device__ inline int vix(int r, int c) { return (r + 2) * VW + c; }
+__device__ inline int wix(int r, int c) { return r * WW + (c + 2); }
+}  // namespace pj
+
+__global__ void __launch_bounds__(256)
+calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ vs,
+                      const double* __restrict__ ws, const double* __restrict__ p,
+                      double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+                      double* __restrict__ su, wf_grid g, int ich, int kt, int jt, double rdx,
+                      double rdy, double rdz, double dta, double r24x, double r24y,
+                      double r24z, double rdt, double dx, double dy, double dz) {
+  using namespace pj;
+  __shared__ double Vs[2][VR * VW];
+  __shared__ double Ws[2][TJ * WW];
+  const int nb = kt * jt * (int)gridDim.y;
+  const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
+  const int ktile = L % kt, jtile = (L / kt) % jt, chunk = L / (kt * jt);
+  const int tid = threadIdx.x, tj = tid / TK, tk = tid % TK;
+  const int npl = g.npl;
+  const int NI = g.ni + 2 * npl, NJ = g.nj + 2 * npl, NK = g.nk + 2 * npl;
+  const int J0 = npl + jtile * TJ, K0 = npl + ktile * TK;
+  const int I0 = npl + chunk * ich, I1 = npl + min(chunk * ich + ich, g.ni);
+  const int J = J0 + tj, K = K0 + tk;
+  const bool live = J < npl + g.nj && K < npl + g.nk;
+  // in-plane offset of (j, k), clamped into the array (clamped positions are
+  // never read by a live cell)
+  auto at2 = [=](int j, int k) -> int64_t {
+    return (int64_t)min(max(j, 0), NJ - 1) * g.sj + min(max(k, 0), NK - 1);
+  };
+  auto plane = [=](int i) -> int64_t { return (int64_t)min(max(i, 0), NI - 1) * g.si; };
+  // calvel's write ranges (calvel3_body)
+  auto u_in = [=](int i) { return i >= npl && i < npl + g.ni - 1; };
+  auto v_in = [=](int j, int k) {
+    return j >= npl && j < npl + g.nj - 1 && k >= npl && k < npl + g.nk;
+  };
+  auto w_in = [=](int j, int k) {
+    return k >= npl && k < npl + g.nk - 1 && j >= npl && j < npl + g.nj;
+  };
+
+  const int64_t xo = at2(J, K), xoj = at2(J + 1, K), xok = at2(J, K + 1);
+  const bool vo_in = v_in(J, K), wo_in = w_in(J, K);
+  const bool hv = tid < NVH, hw = tid < NWH;
+  int vr = 0, vc = 0, wr = 0, wc = 0;
+  if (hv) {
+    const int q = tid / TK;
+    vr = q < 2 ? q - 2 : TJ;
+    vc = tid % TK;
+  }
+  if (hw) {
+    const int q = tid / TJ;
+    wr = tid % TJ;
+    wc = q < 2 ? q - 2 : TK;
+  }
+  const int64_t xv = at2(J0 + vr, K0 + vc), xv1 = at2(J0 + vr + 1, K0 + vc);
+  const int64_t xw = at2(J0 + wr, K0 + wc), xw1 = at2(J0 + wr, K0 + wc + 1);
+  const bool vh_in = hv && v_in(J0 + vr, K0 + vc), wh_in = hw && w_in(J0 + wr, K0 + wc);
+  const double* __restrict__ vos = vo_in ? vs : v;  // source of the own v value
+  const double* __restrict__ wos = wo_in ? ws : w;
+  const double* __restrict__ vhs = vh_in ? vs : v;
+  const double* __restrict__ whs = wh_in ? ws : w;
+
+  // u at plane i of the own column: corrected inside calvel's range, else
+  // the value in memory
+  auto u_at = [=](int i, double a, double pn, double p0) {
+    return u_in(i) ? calvel_val(a, pn, p0, dta, rdx) : a;
+  };
+  auto u_src = [=](int i) { return u_in(i) ? us : u; };
+
+  // prologue: u(I0), u(I0-1), u(I0-2) in uw[0..2]; at the top of plane i the
+  // window shifts and uw[0..3] = u at planes i+1, i, i-1, i-2
+  double uw[4];
+  {
+    double pq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pq[q] = p[plane(I0 - 2 + q) + xo];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = I0 - 2 + q;
+      uw[2 - q] = u_at(i, u_src(i)[plane(i) + xo], pq[q + 1], pq[q]);
+    }
+    if (live && u_in(I0)) u[plane(I0) + xo] = uw[0];
+    uw[3] = 0.0;
+  }
+  double p0 = p[plane(I0) + xo], p1 = p[plane(I0 + 1) + xo];
+
+  // loads of plane i (issued during plane i-1)
+  struct Grp {
+    double ua, p2, va, pv, wa, pw, vh, vh0, vh1, wh, wh0, wh1;
+  };
+  auto issue = [&](Grp& G, int i) {
+    i = min(i, I1 - 1);
+    const int64_t pi = plane(i);
+    G.ua = u_src(i + 1)[plane(i + 1) + xo];
+    G.p2 = p[plane(i + 2) + xo];
+    G.va = vos[pi + xo];
+    G.pv = p[pi + xoj];
+    G.wa = wos[pi + xo];
+    G.pw = p[pi + xok];
+    if (hv) {
+      G.vh = vhs[pi + xv];
+      G.vh0 = p[pi + xv];
+      G.vh1 = p[pi + xv1];
+    }
+    if (hw) {
+      G.wh = whs[pi + xw];
+      G.wh0 = p[pi + xw];
+      G.wh1 = p[pi + xw1];
+    }
+  };
+  Grp N;
+  N.vh = N.vh0 = N.vh1 = N.wh = N.wh0 = N.wh1 = 0.0;
+  issue(N, I0);
+
+  double val = 0.0;
+  for (int i = I0; i < I1; ++i) {
+    const int buf = (i - I0) & 1;
+    // u of plane i+1 into the window
+    uw[3] = uw[2];
+    uw[2] = uw[1];
+    uw[1] = uw[0];
+    uw[0] = u_at(i + 1, N.ua, N.p2, p1);
+    const double vo = vo_in ? calvel_val(N.va, N.pv, p0, dta, rdy) : N.va;
+    const double wo = wo_in ? calvel_val(N.wa, N.pw, p0, dta, rdz) : N.wa;
+    double* __restrict__ V = Vs[buf];
+    double* __restrict__ W = Ws[buf];
+    V[vix(tj, tk)] = vo;
+    W[wix(tj, tk)] = wo;
+    if (hv) V[vix(vr, vc)] = vh_in ? calvel_val(N.vh, N.vh1, N.vh0, dta, rdy) : N.vh;
+    if (hw) W[wix(wr, wc)] = wh_in ? calvel_val(N.wh, N.wh1, N.wh0, dta, rdz) : N.wh;
+    if (live) {
+      if (u_in(i + 1) && i + 1 < I1) u[plane(i + 1) + xo] = uw[0];
+      if (vo_in) v[plane(i) + xo] = vo;
+      if (wo_in) w[plane(i) + xo] = wo;
+    }
+    p0 = p1;
+    p1 = N.p2;
+    issue(N, i + 1);
+    __syncthreads();
+    if (live) {
+      // dudx + dvdy + dwdz at (i, J, K) (kernels.py:281-287)
+      const double fact =
+          div4(d4_bracket(uw[0], uw[1], uw[2], uw[3]),
+               d4_bracket(V[vix(tj + 1, tk)], V[vix(tj, tk)], V[vix(tj - 1, tk)],
+                          V[vix(tj - 2, tk)]),
+               d4_bracket(W[wix(tj, tk + 1)], W[wix(tj, tk)], W[wix(tj, tk - 1)],
+                          W[wix(tj, tk - 2)]),
+               r24x, r24y, r24z);
+      su[plane(i) + xo] = fact * rdt;
+      val = nan_max(val, fabs(fact * dx * dy * dz));
+    }
+  }
+  const double r = block_max(val);
+  if (tid == 0) g_red_part[blockIdx.x + blockIdx.y * gridDim.x] = r;
 }
 
 // Fold the partials of nq quantities: bit q of min_mask selects min (else
@@ -872,6 +1075,36 @@ int wf_calmas(const double* u, const double* v, const double* w, double* su,
   if (!red_geom(b, r)) return WF_EINVAL;
   hipLaunchKernelGGL(calmas_kernel, r.grid, box_block(), 0, S_(stream), u, v, w, su, g, b,
                      r.ich, dx, dy, dz, dt);
+  WF_LAUNCHED();
+  return fold(rmax_dev, r, 1, 0, 1, S_(stream));
+}
+
+int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* wstar,
+                      const double* p, double* u, double* v, double* w, double* su,
+                      double* rmax_dev, wf_grid g, double dx, double dy, double dz, double dt,
+                      double alpha, void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 2, 1) || !rmax_dev) return WF_EINVAL;
+  if (empty(b)) return 0;
+  const int kt = div_up(g.nk, pj::TK), jt = div_up(g.nj, pj::TJ);
+  const int64_t tiles = (int64_t)kt * jt;
+  if (tiles > RED_MAX_BLOCKS) {  // more tiles than partial slots: the two passes
+    const int e = wf_calvel3(ustar, vstar, wstar, p, u, v, w, g, dx, dy, dz, dt, alpha, stream);
+    if (e) return e;
+    return wf_calmas(u, v, w, su, rmax_dev, g, g.ni, g.nj, g.nk, dx, dy, dz, dt, stream);
+  }
+  // >= 2048 workgroups where the grid allows, at most RED_MAX_BLOCKS
+  int ich = (int)std::max<int64_t>(8, (tiles * g.ni + 2047) / 2048);
+  ich = std::max(ich, div_up(g.ni, (int)(RED_MAX_BLOCKS / tiles)));
+  const int nch = div_up(g.ni, ich);
+  RedGeom r;
+  r.grid = dim3((unsigned)tiles, (unsigned)nch);
+  r.ich = ich;
+  r.nblk = (int)(tiles * nch);
+  hipLaunchKernelGGL(calvel3_calmas_kernel, r.grid, dim3(pj::NT), 0, S_(stream), ustar, vstar,
+                     wstar, p, u, v, w, su, g, ich, kt, jt, 1.0 / dx, 1.0 / dy, 1.0 / dz,
+                     dt * alpha, 1.0 / (24.0 * dx), 1.0 / (24.0 * dy), 1.0 / (24.0 * dz),
+                     1.0 / dt, dx, dy, dz);
   WF_LAUNCHED();
   return fold(rmax_dev, r, 1, 0, 1, S_(stream));
 }

@@ -103,6 +103,18 @@ def _calmas(dim, u, v, w, su, rmax, dx, dy, dz, dt, npl):
               dx, dy, dz, dt, current_stream())
 
 
+def _calvel3_calmas(dim, ustar, vstar, wstar, p, u, v, w, su, rmax, dx, dy, dz, dt, alpha,
+                    npl):
+    """calvel3 then calmas over the real cells in one pass (``_project`` runs
+    them back to back, solver.py:186-197); same results as the two kernels."""
+    _same(ustar, vstar, wstar, p, u, v, w, su)
+    g = grid_of(u, npl)
+    if dim != (g.ni, g.nj, g.nk):
+        raise ValueError("calvel3_calmas runs over the real cells (dim = ncells)")
+    _hip.call("wf_calvel3_calmas", ustar.ptr, vstar.ptr, wstar.ptr, p.ptr, u.ptr, v.ptr,
+              w.ptr, su.ptr, rmax.ptr, g, dx, dy, dz, dt, alpha, current_stream())
+
+
 def wale_delta(dx, dy, dz):
     """(dx*dy*dz)^(1/3) (kernels.py:360)."""
     return (dx * dy * dz) ** (1.0 / 3.0)
@@ -166,6 +178,7 @@ calvel_v = Kernel("calvel_v", _calvel(1))
 calvel_w = Kernel("calvel_w", _calvel(2))
 calvel3 = Kernel("calvel3", _calvel3)
 calmas = Kernel("calmas", _calmas)
+calvel3_calmas = Kernel("calvel3_calmas", _calvel3_calmas)
 wale_viscosity = Kernel("wale_viscosity", _wale)
 maxabs_reduce = Kernel("maxabs_reduce", _maxabs)
 dtvisc_reduce = Kernel("dtvisc_reduce", _dtvisc)

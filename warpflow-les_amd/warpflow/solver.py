@@ -200,6 +200,10 @@ class FlowSolver:
         self._rbuf.zero_()
         K.calmas(self.ncells, self.f["u"], self.f["v"], self.f["w"], self.f["su"],
                  self._rbuf, self.dx, self.dy, self.dz, self.dt, NPL)
+        self._read_calmas()
+
+    def _read_calmas(self):
+        """Read back rmax (and the deferred checkdt values with it)."""
         r = self._buf.numpy()
         if self._dt_pending:
             self._dt_pending = False
@@ -207,17 +211,38 @@ class FlowSolver:
         loc = float(r[0])
         self.rmax = self.topo.comm.allreduce(loc)
 
+    def _calvel_calmas(self):
+        """``_calvel(1.0)`` then ``_calmas()`` in one pass (solver.py:155-176 as
+        _project runs them back to back): calvel3_calmas writes u, v, w, su
+        and rmax as the two kernels do; the halo exchange between them does
+        nothing on one domain and is issued afterwards."""
+        f = self.f
+        self._rbuf.zero_()
+        K.calvel3_calmas(self.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"], f["u"],
+                         f["v"], f["w"], f["su"], self._rbuf, self.dx, self.dy, self.dz,
+                         self.dt, 1.0, NPL)
+        for name in ("u", "v", "w"):
+            self.halo.exchange(f[name])
+        self._read_calmas()
+
     def _project(self):
-        """solver.py:178-212: outer Poisson/projection iterations."""
+        """solver.py:178-212: outer Poisson/projection iterations.  With the
+        fused kernels each iteration opens with the calvel/calmas pair in one
+        pass, so the calvel that closes an iteration runs separately only when
+        the loop ends without its break."""
         f = self.f
         eps = self.cfg.numerics.div_tolerance
         maxcy = self.cfg.numerics.max_outer_iters
-        self._calvel(1.0)
+        if not self.fused:
+            self._calvel(1.0)
         iters = 0
         self.rmax_history = []
         for it in range(1, maxcy + 1):
             iters = it
-            self._calmas()
+            if self.fused:
+                self._calvel_calmas()
+            else:
+                self._calmas()
             self.rmax_history.append(self.rmax)
             if self.rmax < eps and it > 1:
                 break
@@ -228,8 +253,11 @@ class FlowSolver:
             K.add_inplace(tuple(n + 2 for n in self.ncells), f["p"], f["pp"], NPL)
             self.halo.exchange(f["p"])
             self.bcs.apply_pbound(f["p"], self.ncells[0])
-            self._calvel(1.0)
+            if not self.fused:
+                self._calvel(1.0)
         else:
+            if self.fused:
+                self._calvel(1.0)
             if self.topo.rank == 0:
                 print("warning: projection did not converge, rmax = %.6e after %d "
                       "iterations" % (self.rmax, maxcy), flush=True)
