@@ -160,9 +160,13 @@ def hot_kernels(a, s64, smx):
          48 * opu[0] * opu[1] * opu[2], 6, 6, 1),
         ("calvel3", lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
                                       f["u"], f["v"], f["w"], *h, dt, 1.0, NPL),
-         56 * cells, 3, 3, 1),
+         56 * cells, 1, 1, 1),
+        ("calvel3+calmas", lambda: K.calvel3_calmas(s64.ncells, f["ustar"], f["vstar"],
+                                                    f["wstar"], f["p"], f["u"], f["v"], f["w"],
+                                                    f["su"], s64._buf, *h, dt, 1.0, NPL),
+         64 * cells, 2, 2, 1),
         ("calmas", lambda: K.calmas(s64.ncells, f["u"], f["v"], f["w"], f["su"], s64._buf,
-                                    *h, dt, NPL), 32 * cells, 2, 2, 1),
+                                    *h, dt, NPL), 32 * cells, 0, 0, 1),
         ("checkdt", lambda: K.checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"], s64._buf,
                                       *h, NPL), 32 * cells, 1, 1, 1),
         ("wale", lambda: K.wale_viscosity(s64.ncells, f["u"], f["v"], f["w"], f["vis"], *h,

@@ -420,6 +420,9 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  *                               one-sweep launches, 2 = one two-sweep launch
  *                               (built-in: 2 for FP64 levels of >= 2^23
  *                               cells, else 1)
+ *   WF_TUNE_PROJ_ICH            i-planes per workgroup of wf_calvel3_calmas
+ *                               (raised where needed to keep the workgroup
+ *                               count within the reduction's partial slots)
  * At most 128 entries.
  * ---------------------------------------------------------------------- */
 enum {
@@ -433,7 +436,8 @@ enum {
   WF_TUNE_SWEEP2_ICH_F64 = 7,
   WF_TUNE_SWEEP2_ICH_F32 = 8,
   WF_TUNE_SWEEP_PAIRS = 9,
-  WF_TUNE_COUNT = 10
+  WF_TUNE_PROJ_ICH = 10,
+  WF_TUNE_COUNT = 11
 };
 int wf_tune_set(int knob, int n0, int n1, int n2, int value);
 int wf_tune_get(int knob, int n0, int n1, int n2, int* value);

@@ -32,7 +32,7 @@ def test_tuned_shapes_change_no_result(gpu, precision):
         phi = a.mg.levels[0].phi.numpy()
         res = autotune.tune_solver(a, reps=2)
         knobs = {k for k, _, _ in res["entries"]}
-        assert autotune.CD_ICH in knobs
+        assert autotune.CD_ICH in knobs and autotune.PROJ_ICH in knobs
         for k, n3, v in res["entries"]:
             assert autotune.get_shape(k, n3) == v
             assert autotune.get_shape(k, (n3[0] + 1, n3[1], n3[2])) == 0  # other shapes untouched

@@ -1093,8 +1093,9 @@ int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* ws
     if (e) return e;
     return wf_calmas(u, v, w, su, rmax_dev, g, g.ni, g.nj, g.nk, dx, dy, dz, dt, stream);
   }
-  // >= 2048 workgroups where the grid allows, at most RED_MAX_BLOCKS
-  int ich = (int)std::max<int64_t>(8, (tiles * g.ni + 2047) / 2048);
+  // built-in: >= 2048 workgroups where the grid allows; at most RED_MAX_BLOCKS
+  int ich = tune_get(WF_TUNE_PROJ_ICH, g.ni, g.nj, g.nk);
+  if (ich <= 0) ich = (int)std::max<int64_t>(8, (tiles * g.ni + 2047) / 2048);
   ich = std::max(ich, div_up(g.ni, (int)(RED_MAX_BLOCKS / tiles)));
   const int nch = div_up(g.ni, ich);
   RedGeom r;

@@ -1,8 +1,9 @@
 """Launch-shape autotuner for the hot kernels.
 
-The fused momentum kernel (``wf_convdiff4th``) and the RBGS sweep kernels
-have launch-shape parameters that change no result: the i-planes each
-workgroup streams through (``cd_ich``, ``sweep_ich``, ``sweep2_ich``) and the
+The fused momentum kernel (``wf_convdiff4th``), the fused calvel/calmas pass
+(``wf_calvel3_calmas``) and the RBGS sweep kernels have launch-shape
+parameters that change no result: the i-planes each workgroup streams
+through (``cd_ich``, ``proj_ich``, ``sweep_ich``, ``sweep2_ich``) and the
 (j, k) tile of the one-sweep kernel (``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32,
 3 = 16 x 64, 4 = 32 x 32) and of the two-sweep kernel (``sweep2_tile``: 1 =
 8 x 64, 2 = 16 x 32); ``sweep_pairs`` picks, per level, one-sweep launches
@@ -27,12 +28,13 @@ from .device import copy, current_stream, zeros, zeros_like
 
 # knob ids (include/warpflow_hip.h)
 (CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32, SWEEP2_TILE_F64,
- SWEEP2_TILE_F32, SWEEP2_ICH_F64, SWEEP2_ICH_F32, SWEEP_PAIRS) = range(10)
+ SWEEP2_TILE_F32, SWEEP2_ICH_F64, SWEEP2_ICH_F32, SWEEP_PAIRS, PROJ_ICH) = range(11)
 CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
 SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
 SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32)
 SWEEP2_TILE_CANDIDATES = (1, 2)
 SWEEP2_ICH_CANDIDATES = (0, 2, 4, 8, 16, 32, 64)
+PROJ_ICH_CANDIDATES = (0, 32, 64, 86, 128, 172, 256)
 SWEEP_MIN_CELLS = 4096  # smaller levels relax in one LDS-resident workgroup
 
 
@@ -116,6 +118,42 @@ def _tune_convdiff(sol, T, reps, out):
     set_shape(CD_ICH, n3, best[1])
     out["entries"].append((CD_ICH, tuple(n3), best[1]))
     copy(f["ustar"], saved)
+
+
+def _tune_proj(sol, T, reps, out):
+    from . import kernels as K
+    from .topology import NPL
+    f = sol.f
+    n3 = sol.ncells
+    names = ("u", "v", "w", "su")
+    saved = {nm: zeros_like(f[nm]) for nm in names}
+    for nm in names:
+        copy(saved[nm], f[nm])
+
+    def run():
+        for nm in ("u", "v", "w"):
+            copy(f[nm], saved[nm])
+        sol._rbuf.zero_()
+        K.calvel3_calmas(n3, f["ustar"], f["vstar"], f["wstar"], f["p"], f["u"], f["v"],
+                         f["w"], f["su"], sol._rbuf, sol.dx, sol.dy, sol.dz, sol.dt, 1.0, NPL)
+    t_copy = T.us(lambda: [copy(f[nm], saved[nm]) for nm in ("u", "v", "w")], reps)
+    set_shape(PROJ_ICH, n3, 0)
+    run()
+    want = [f[nm].numpy() for nm in names] + [sol._rbuf.numpy()]
+    best = None
+    for ich in PROJ_ICH_CANDIDATES:
+        set_shape(PROJ_ICH, n3, ich)
+        t = T.us(run, reps) - t_copy
+        got = [f[nm].numpy() for nm in names] + [sol._rbuf.numpy()]
+        if not all(np.array_equal(a, b) for a, b in zip(got, want)):
+            raise RuntimeError("calvel3_calmas result depends on proj_ich=%d" % ich)
+        out["timings_us"]["calvel3_calmas %s ich=%d" % ("x".join(map(str, n3)), ich)] = round(t, 2)
+        if best is None or t < best[0]:
+            best = (t, ich)
+    set_shape(PROJ_ICH, n3, best[1])
+    out["entries"].append((PROJ_ICH, tuple(n3), best[1]))
+    for nm in names:
+        copy(f[nm], saved[nm])
 
 
 def _tune_level(lev, T, reps, out):
@@ -211,6 +249,7 @@ def tune_solver(sol, reps=10, cache=None):
     T = _Timer()
     try:
         _tune_convdiff(sol, T, reps, out)
+        _tune_proj(sol, T, reps, out)
         for lev in sol.mg.levels:
             if lev.tmp is not None and int(np.prod(lev.n3)) > SWEEP_MIN_CELLS:
                 _tune_level(lev, T, reps, out)
