@@ -310,8 +310,10 @@ int wf_mg_mixed_workspace_bytes(wf_grid g, size_t* bytes);
  * max_cycles cycles, no host synchronisation.  rtol > 0: stop once
  * max|r| <= rtol*max|su| (one host read per cycle), at most max_cycles.
  * flags & 2: subtract the FP64 mean of pp at the end (pressure gauge).  The
- * first ghost layer of pp gets the Neumann face mirror; every other ghost
- * cell is zero.  *cycles_out / *res_out (nullable) report the V-cycles run
+ * first ghost layer of pp gets the Neumann face mirror on its faces and zero
+ * on its edges and corners; the deeper ghost layers are not written (the
+ * reference's solve leaves them zero: keep them zero, as a freshly allocated
+ * field is).  *cycles_out / *res_out (nullable) report the V-cycles run
  * and, in tolerance mode, the last relative residual.  work: device buffer
  * of wf_mg_mixed_workspace_bytes. */
 int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su,

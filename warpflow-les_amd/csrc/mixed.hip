@@ -159,9 +159,33 @@ __device__ inline void update_body(double* __restrict__ x, const wf_grid& g,
   const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
   x[q] = x[q] + (double)e[mg_at(d, i, j, k)];
 }
+// The same on the solve's own field: assign != 0 takes x = 0 (the first
+// cycle; 0.0 + e keeps the bits of adding e to a zeroed x, signed zeros
+// included) without reading x; partials != nullptr also stores the block
+// sums of the new x, in the block shape and order of field_partials_body.
 __global__ void __launch_bounds__(256)
-update_kernel(double* __restrict__ x, wf_grid g, const float* __restrict__ e, MgDev d, int ich) {
-  for_planes(g.ni, ich, [&](Blk B) { update_body(x, g, e, d, B); });
+update_kernel(double* __restrict__ x, wf_grid g, const float* __restrict__ e, MgDev d, int ich,
+              int assign, double* __restrict__ partials) {
+  if (!partials) {
+    for_planes(g.ni, ich, [&](Blk B) {
+      const int k = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y, i = B.z;
+      if (k >= g.nk || j >= g.nj) return;
+      const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
+      x[q] = (assign ? 0.0 : x[q]) + (double)e[mg_at(d, i, j, k)];
+    });
+    return;
+  }
+  for_planes<true>(g.ni, ich, [&](Blk B) {
+    const int k = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y, i = B.z;
+    double v = 0.0;
+    if (k < g.nk && j < g.nj) {
+      const int64_t q = at(g, i + g.npl, j + g.npl, k + g.npl);
+      v = (assign ? 0.0 : x[q]) + (double)e[mg_at(d, i, j, k)];
+      x[q] = v;
+    }
+    const double s = block_sum_fixed(v);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partials[block_linear(g, B)] = s;
+  });
 }
 
 __device__ inline void shift_body(double* __restrict__ x, const wf_grid& g,
@@ -177,7 +201,11 @@ shift_kernel(double* __restrict__ x, wf_grid g, const double* __restrict__ mean,
 }
 
 // First ghost layer of the field: faces = mirror of the adjacent real cell,
-// edges/corners untouched (the ring state of the plain solve's copy-out).
+// edges and corners = 0 (the ring state of the plain solve's copy-out, whose
+// level ring keeps zero edges and corners).  Thread t < field_ring_cells(g).
+__host__ __device__ inline int field_ring_cells(const wf_grid& g) {
+  return 2 * (g.nj * g.nk + g.ni * g.nk + g.ni * g.nj) + 4 * (g.ni + g.nj + g.nk) + 8;
+}
 __device__ inline void field_ring_body(double* __restrict__ x, const wf_grid& g, int t) {
   const int n0 = g.ni, n1 = g.nj, n2 = g.nk, P = g.npl;
   const int fx = n1 * n2, fy = n0 * n2, fz = n0 * n1;
@@ -206,7 +234,29 @@ __device__ inline void field_ring_body(double* __restrict__ x, const wf_grid& g,
     const int i = q / n1, j = q % n1, k = hi ? n2 - 1 : 0;
     const int64_t p = at(g, i + P, j + P, k + P);
     x[p + (hi ? 1 : -1)] = x[p];
+    return;
   }
+  q -= 2 * fz;
+  // edges along k, j, i (ring coordinate -1 or n on the other two axes), corners
+  if (q < 4 * n2) {
+    const int e = q / n2;
+    x[at(g, (e & 1 ? n0 : -1) + P, (e & 2 ? n1 : -1) + P, q % n2 + P)] = 0.0;
+    return;
+  }
+  q -= 4 * n2;
+  if (q < 4 * n1) {
+    const int e = q / n1;
+    x[at(g, (e & 1 ? n0 : -1) + P, q % n1 + P, (e & 2 ? n2 : -1) + P)] = 0.0;
+    return;
+  }
+  q -= 4 * n1;
+  if (q < 4 * n0) {
+    const int e = q / n0;
+    x[at(g, q % n0 + P, (e & 1 ? n1 : -1) + P, (e & 2 ? n2 : -1) + P)] = 0.0;
+    return;
+  }
+  q -= 4 * n0;
+  if (q < 8) x[at(g, (q & 1 ? n0 : -1) + P, (q & 2 ? n1 : -1) + P, (q & 4 ? n2 : -1) + P)] = 0.0;
 }
 __global__ void __launch_bounds__(256) field_ring_kernel(double* __restrict__ x, wf_grid g) {
   field_ring_body(x, g, blockIdx.x * blockDim.x + threadIdx.x);
@@ -304,8 +354,8 @@ batch_field_ring_kernel(BView V, double* pp, const int* active) {
   if (active && !active[m]) return;
   const MemDev& M = V.mem[m];
   const wf_grid& g = M.g;
-  const int nface = 2 * (g.nj * g.nk + g.ni * g.nk + g.ni * g.nj);
-  for (int t = threadIdx.x; t < nface; t += blockDim.x) field_ring_body(pp + M.off, g, t);
+  const int nring = field_ring_cells(g);
+  for (int t = threadIdx.x; t < nring; t += blockDim.x) field_ring_body(pp + M.off, g, t);
 }
 
 __global__ void __launch_bounds__(256) batch_fill_kernel(double* p, int64_t n, double v) {
@@ -437,14 +487,20 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
   double* rmax = work + 1;
   double* xmean = work + 2;
 
-  int r = mg_zero<double>(pp, (int64_t)(g.ni + 2 * g.npl) * g.si, s);
-  if (r) return r;
+  // x = 0 is implicit: the first defect reads no x and the first update
+  // assigns x = e, so the real cells are never read before being written;
+  // the ring is rewritten at the end.  Only a solve that runs no cycle
+  // zeroes x explicitly.
+  int r = 0;
   if (flags & MIX_DEFECT_MEAN) {
     r = field_sum(su, g, 1.0 / ncell, work, mean, s);
     if (r) return r;
   }
   double bmax = 0.0, res = 0.0;
   int cycles = 0;
+  // with a fixed cycle count the last update also forms the block sums of
+  // x that the result mean needs (the same sums field_partials would form)
+  bool summed = false;
   for (int c = 0; c <= max_cycles; ++c) {
     const bool last_check = c == max_cycles;
     if (last_check && !tol_mode) break;
@@ -468,20 +524,32 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
     if (r) return r;
     r = mg_vcycle_levels(levels, nlev, n_pre, n_post, n_coarse, s);
     if (r) return r;
+    summed = (flags & MIX_RESULT_MEAN) && !tol_mode && c == max_cycles - 1;
     hipLaunchKernelGGL(update_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g,
-                       (const float*)L0.phi, d0, field_ich(g));
+                       (const float*)L0.phi, d0, field_ich(g), c == 0 ? 1 : 0,
+                       summed ? work + WORK_HEAD : nullptr);
     WF_LAUNCHED();
     ++cycles;
   }
-  if (flags & MIX_RESULT_MEAN) {
-    r = field_sum(pp, g, 1.0 / ncell, work, xmean, s);
+  if (cycles == 0) {
+    r = mg_zero<double>(pp, (int64_t)(g.ni + 2 * g.npl) * g.si, s);
     if (r) return r;
+  }
+  if (flags & MIX_RESULT_MEAN) {
+    if (summed) {
+      hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(1024), 0, s, work + WORK_HEAD,
+                         field_blocks(g), 1.0 / ncell, xmean);
+      WF_LAUNCHED();
+    } else {
+      r = field_sum(pp, g, 1.0 / ncell, work, xmean, s);
+      if (r) return r;
+    }
     hipLaunchKernelGGL(shift_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g, xmean,
                        field_ich(g));
     WF_LAUNCHED();
   }
-  const int nface = 2 * (g.nj * g.nk + g.ni * g.nk + g.ni * g.nj);
-  hipLaunchKernelGGL(field_ring_kernel, dim3(div_up(nface, 256)), dim3(256), 0, s, pp, g);
+  hipLaunchKernelGGL(field_ring_kernel, dim3(div_up(field_ring_cells(g), 256)), dim3(256), 0, s,
+                     pp, g);
   WF_LAUNCHED();
   if (cycles_out) *cycles_out = cycles;
   if (res_out) *res_out = res;
