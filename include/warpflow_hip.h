@@ -386,6 +386,10 @@ int wf_batch_calvel3(wf_batch b, const double* us, const double* vs, const doubl
 /* wf_calmas: rmax[m] of member m. */
 int wf_batch_calmas(wf_batch b, const double* u, const double* v, const double* w, double* su,
                     double* rmax, const int* active, void* stream);
+/* wf_calvel3_calmas: calvel3 then calmas (rmax[m] of member m) in one pass. */
+int wf_batch_calvel3_calmas(wf_batch b, const double* us, const double* vs, const double* ws,
+                            const double* p, double* u, double* v, double* w, double* su,
+                            double* rmax, double alpha, const int* active, void* stream);
 /* wf_add_ring. */
 int wf_batch_add_ring(wf_batch b, double* p, const double* pp, const int* active,
                       void* stream);

@@ -146,6 +146,9 @@ int batch_calvel3(const Batch& B, const double* us, const double* vs, const doub
                   const int* active, hipStream_t s);
 int batch_calmas(const Batch& B, const double* u, const double* v, const double* w, double* su,
                  double* rmax, const int* active, hipStream_t s);
+int batch_calvel3_calmas(const Batch& B, const double* us, const double* vs, const double* ws,
+                         const double* p, double* u, double* v, double* w, double* su,
+                         double* rmax, double alpha, const int* active, hipStream_t s);
 int batch_add_ring(const Batch& B, double* p, const double* pp, const int* active,
                    hipStream_t s);
 int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,

@@ -292,6 +292,14 @@ int wf_batch_calmas(wf_batch b, const double* u, const double* v, const double* 
   return batch_calmas(B_(b), u, v, w, su, rmax, active, (hipStream_t)stream);
 }
 
+int wf_batch_calvel3_calmas(wf_batch b, const double* us, const double* vs, const double* ws,
+                            const double* p, double* u, double* v, double* w, double* su,
+                            double* rmax, double alpha, const int* active, void* stream) {
+  if (!ok(b) || !us || !vs || !ws || !p || !u || !v || !w || !su || !rmax) return WF_EINVAL;
+  return batch_calvel3_calmas(B_(b), us, vs, ws, p, u, v, w, su, rmax, alpha, active,
+                              (hipStream_t)stream);
+}
+
 int wf_batch_add_ring(wf_batch b, double* p, const double* pp, const int* active,
                       void* stream) {
   if (!ok(b) || !p || !pp) return WF_EINVAL;

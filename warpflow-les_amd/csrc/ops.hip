@@ -281,28 +281,32 @@ calmas_kernel(const double* __restrict__ u, const double* __restrict__ v,
 // once, by the workgroup owning its cell and plane.  HBM traffic falls from
 // 88 to 64 bytes per cell (calmas no longer reads u, v, w).
 namespace pj {
-constexpr int TJ = 8, TK = 32, NT = TJ * TK;
-constexpr int VW = TK, VR = TJ + 3;  // v: rows -2 .. TJ, columns 0 .. TK-1
-constexpr int WW = TK + 3;           // w: rows 0 .. TJ-1, columns -2 .. TK
-constexpr int NVH = 3 * TK, NWH = 3 * TJ;
-static_assert(NVH <= NT && NWH <= NT, "one halo position of each kind per thread");
-__device__ inline int vix(int r, int c) { return (r + 2) * VW + c; }
-__device__ inline int wix(int r, int c) { return r * WW + (c + 2); }
+// v: rows -2 .. TJ, columns 0 .. TK-1; w: rows 0 .. TJ-1, columns -2 .. TK
+template <int TJ, int TK>
+struct Tile {
+  static constexpr int NT = TJ * TK, VW = TK, VR = TJ + 3, WW = TK + 3;
+  static constexpr int NVH = 3 * TK, NWH = 3 * TJ;
+  static_assert(NVH <= NT && NWH <= NT, "one halo position of each kind per thread");
+  __device__ static int vix(int r, int c) { return (r + 2) * VW + c; }
+  __device__ static int wix(int r, int c) { return r * WW + (c + 2); }
+};
+struct Consts {
+  double rdx, rdy, rdz, dta, r24x, r24y, r24z, rdt, dx, dy, dz;
+};
 }  // namespace pj
 
-__global__ void __launch_bounds__(256)
-calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ vs,
-                      const double* __restrict__ ws, const double* __restrict__ p,
-                      double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-                      double* __restrict__ su, wf_grid g, int ich, int kt, int jt, double rdx,
-                      double rdy, double rdz, double dta, double r24x, double r24y,
-                      double r24z, double rdt, double dx, double dy, double dz) {
-  using namespace pj;
-  __shared__ double Vs[2][VR * VW];
-  __shared__ double Ws[2][TJ * WW];
-  const int nb = kt * jt * (int)gridDim.y;
-  const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
-  const int ktile = L % kt, jtile = (L / kt) % jt, chunk = L / (kt * jt);
+// One (k tile, j tile, i chunk) work item of the fused pass on a TJ x TK
+// tile (TJ * TK threads, one per tile cell); returns the item's max|div dV|
+// to every thread.
+template <int TJ, int TK>
+__device__ inline double calvel3_calmas_body(
+    const double* __restrict__ us, const double* __restrict__ vs,
+    const double* __restrict__ ws, const double* __restrict__ p, double* __restrict__ u,
+    double* __restrict__ v, double* __restrict__ w, double* __restrict__ su, const wf_grid& g,
+    int ich, int ktile, int jtile, int chunk, const pj::Consts& C) {
+  using T = pj::Tile<TJ, TK>;
+  __shared__ double Vs[2][T::VR * T::VW];
+  __shared__ double Ws[2][TJ * T::WW];
   const int tid = threadIdx.x, tj = tid / TK, tk = tid % TK;
   const int npl = g.npl;
   const int NI = g.ni + 2 * npl, NJ = g.nj + 2 * npl, NK = g.nk + 2 * npl;
@@ -310,6 +314,7 @@ calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ 
   const int I0 = npl + chunk * ich, I1 = npl + min(chunk * ich + ich, g.ni);
   const int J = J0 + tj, K = K0 + tk;
   const bool live = J < npl + g.nj && K < npl + g.nk;
+  const double dta = C.dta;
   // in-plane offset of (j, k), clamped into the array (clamped positions are
   // never read by a live cell)
   auto at2 = [=](int j, int k) -> int64_t {
@@ -327,7 +332,7 @@ calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ 
 
   const int64_t xo = at2(J, K), xoj = at2(J + 1, K), xok = at2(J, K + 1);
   const bool vo_in = v_in(J, K), wo_in = w_in(J, K);
-  const bool hv = tid < NVH, hw = tid < NWH;
+  const bool hv = tid < T::NVH, hw = tid < T::NWH;
   int vr = 0, vc = 0, wr = 0, wc = 0;
   if (hv) {
     const int q = tid / TK;
@@ -350,7 +355,7 @@ calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ 
   // u at plane i of the own column: corrected inside calvel's range, else
   // the value in memory
   auto u_at = [=](int i, double a, double pn, double p0) {
-    return u_in(i) ? calvel_val(a, pn, p0, dta, rdx) : a;
+    return u_in(i) ? calvel_val(a, pn, p0, dta, C.rdx) : a;
   };
   auto u_src = [=](int i) { return u_in(i) ? us : u; };
 
@@ -407,14 +412,14 @@ calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ 
     uw[2] = uw[1];
     uw[1] = uw[0];
     uw[0] = u_at(i + 1, N.ua, N.p2, p1);
-    const double vo = vo_in ? calvel_val(N.va, N.pv, p0, dta, rdy) : N.va;
-    const double wo = wo_in ? calvel_val(N.wa, N.pw, p0, dta, rdz) : N.wa;
+    const double vo = vo_in ? calvel_val(N.va, N.pv, p0, dta, C.rdy) : N.va;
+    const double wo = wo_in ? calvel_val(N.wa, N.pw, p0, dta, C.rdz) : N.wa;
     double* __restrict__ V = Vs[buf];
     double* __restrict__ W = Ws[buf];
-    V[vix(tj, tk)] = vo;
-    W[wix(tj, tk)] = wo;
-    if (hv) V[vix(vr, vc)] = vh_in ? calvel_val(N.vh, N.vh1, N.vh0, dta, rdy) : N.vh;
-    if (hw) W[wix(wr, wc)] = wh_in ? calvel_val(N.wh, N.wh1, N.wh0, dta, rdz) : N.wh;
+    V[T::vix(tj, tk)] = vo;
+    W[T::wix(tj, tk)] = wo;
+    if (hv) V[T::vix(vr, vc)] = vh_in ? calvel_val(N.vh, N.vh1, N.vh0, dta, C.rdy) : N.vh;
+    if (hw) W[T::wix(wr, wc)] = wh_in ? calvel_val(N.wh, N.wh1, N.wh0, dta, C.rdz) : N.wh;
     if (live) {
       if (u_in(i + 1) && i + 1 < I1) u[plane(i + 1) + xo] = uw[0];
       if (vo_in) v[plane(i) + xo] = vo;
@@ -428,17 +433,31 @@ calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ 
       // dudx + dvdy + dwdz at (i, J, K) (kernels.py:281-287)
       const double fact =
           div4(d4_bracket(uw[0], uw[1], uw[2], uw[3]),
-               d4_bracket(V[vix(tj + 1, tk)], V[vix(tj, tk)], V[vix(tj - 1, tk)],
-                          V[vix(tj - 2, tk)]),
-               d4_bracket(W[wix(tj, tk + 1)], W[wix(tj, tk)], W[wix(tj, tk - 1)],
-                          W[wix(tj, tk - 2)]),
-               r24x, r24y, r24z);
-      su[plane(i) + xo] = fact * rdt;
-      val = nan_max(val, fabs(fact * dx * dy * dz));
+               d4_bracket(V[T::vix(tj + 1, tk)], V[T::vix(tj, tk)], V[T::vix(tj - 1, tk)],
+                          V[T::vix(tj - 2, tk)]),
+               d4_bracket(W[T::wix(tj, tk + 1)], W[T::wix(tj, tk)], W[T::wix(tj, tk - 1)],
+                          W[T::wix(tj, tk - 2)]),
+               C.r24x, C.r24y, C.r24z);
+      su[plane(i) + xo] = fact * C.rdt;
+      val = nan_max(val, fabs(fact * C.dx * C.dy * C.dz));
     }
   }
-  const double r = block_max(val);
-  if (tid == 0) g_red_part[blockIdx.x + blockIdx.y * gridDim.x] = r;
+  return block_max(val);
+}
+
+// Single domain: 8 x 32 tiles; the workgroup ids are remapped over the XCDs
+// (k tile fastest, then j tile, then i-chunk).
+constexpr int PJ_TJ = 8, PJ_TK = 32;
+__global__ void __launch_bounds__(256)
+calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ vs,
+                      const double* __restrict__ ws, const double* __restrict__ p,
+                      double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+                      double* __restrict__ su, wf_grid g, int ich, int kt, int jt, pj::Consts K) {
+  const int nb = kt * jt * (int)gridDim.y;
+  const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
+  const double r = calvel3_calmas_body<PJ_TJ, PJ_TK>(us, vs, ws, p, u, v, w, su, g, ich, L % kt,
+                                                      (L / kt) % jt, L / (kt * jt), K);
+  if (threadIdx.x == 0) g_red_part[blockIdx.x + blockIdx.y * gridDim.x] = r;
 }
 
 // Fold the partials of nq quantities: bit q of min_mask selects min (else
@@ -822,6 +841,26 @@ batch_calmas_kernel(BView V, Plan P, const double* u, const double* v, const dou
   });
 }
 
+// The fused calvel/calmas pass per member on the reduction items (64 x 4
+// (k, j) tiles x BATCH_RED_ICH planes), so its partials fill the slots
+// batch_red_fold_kernel folds, as batch_calmas_kernel's do.
+__global__ void __launch_bounds__(256)
+batch_calvel3_calmas_kernel(BView V, Plan P, const double* us, const double* vs,
+                            const double* ws, const double* p, double* u, double* v, double* w,
+                            double* su, double* part, double alpha, const int* active) {
+  for_items<true>(P, V, active, [&](int, int q, const MemDev& M) {
+    const wf_grid& g = M.g;
+    const int kt = (g.nk + 63) / 64, jt = (g.nj + 3) / 4;
+    const pj::Consts K{1.0 / M.dx, 1.0 / M.dy, 1.0 / M.dz, M.dt * alpha,
+                       1.0 / (24.0 * M.dx), 1.0 / (24.0 * M.dy), 1.0 / (24.0 * M.dz),
+                       1.0 / M.dt, M.dx, M.dy, M.dz};
+    const double r = calvel3_calmas_body<4, 64>(
+        us + M.off, vs + M.off, ws + M.off, p + M.off, u + M.off, v + M.off, w + M.off,
+        su + M.off, g, BATCH_RED_ICH, q % kt, (q / kt) % jt, q / (kt * jt), K);
+    if (threadIdx.x == 0) part[M.red0 + q] = r;
+  });
+}
+
 __global__ void __launch_bounds__(256)
 batch_checkdt_kernel(BView V, Plan P, const double* u, const double* v, const double* w,
                      const double* vis, double* part, int stride) {
@@ -910,6 +949,19 @@ int batch_calmas(const Batch& B, const double* u, const double* v, const double*
   const Plan& P = B.plan[PK_RED];
   hipLaunchKernelGGL(batch_calmas_kernel, dim3(P.grid), box_block(), 0, s, B.view(), P, u, v, w,
                      su, B.red_part, active);
+  WF_LAUNCHED();
+  hipLaunchKernelGGL(batch_red_fold_kernel, dim3(B.nmem), dim3(1024), 0, s, B.view(),
+                     (const double*)B.red_part, B.nred, 1, 0, rmax, active);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_calvel3_calmas(const Batch& B, const double* us, const double* vs, const double* ws,
+                         const double* p, double* u, double* v, double* w, double* su,
+                         double* rmax, double alpha, const int* active, hipStream_t s) {
+  const Plan& P = B.plan[PK_RED];
+  hipLaunchKernelGGL(batch_calvel3_calmas_kernel, dim3(P.grid), dim3(256), 0, s, B.view(), P, us,
+                     vs, ws, p, u, v, w, su, B.red_part, alpha, active);
   WF_LAUNCHED();
   hipLaunchKernelGGL(batch_red_fold_kernel, dim3(B.nmem), dim3(1024), 0, s, B.view(),
                      (const double*)B.red_part, B.nred, 1, 0, rmax, active);
@@ -1086,7 +1138,7 @@ int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* ws
   const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
   if (!box_ok(g, b, 2, 1) || !rmax_dev) return WF_EINVAL;
   if (empty(b)) return 0;
-  const int kt = div_up(g.nk, pj::TK), jt = div_up(g.nj, pj::TJ);
+  const int kt = div_up(g.nk, PJ_TK), jt = div_up(g.nj, PJ_TJ);
   const int64_t tiles = (int64_t)kt * jt;
   if (tiles > RED_MAX_BLOCKS) {  // more tiles than partial slots: the two passes
     const int e = wf_calvel3(ustar, vstar, wstar, p, u, v, w, g, dx, dy, dz, dt, alpha, stream);
@@ -1102,10 +1154,11 @@ int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* ws
   r.grid = dim3((unsigned)tiles, (unsigned)nch);
   r.ich = ich;
   r.nblk = (int)(tiles * nch);
-  hipLaunchKernelGGL(calvel3_calmas_kernel, r.grid, dim3(pj::NT), 0, S_(stream), ustar, vstar,
-                     wstar, p, u, v, w, su, g, ich, kt, jt, 1.0 / dx, 1.0 / dy, 1.0 / dz,
-                     dt * alpha, 1.0 / (24.0 * dx), 1.0 / (24.0 * dy), 1.0 / (24.0 * dz),
-                     1.0 / dt, dx, dy, dz);
+  const pj::Consts K{1.0 / dx, 1.0 / dy, 1.0 / dz, dt * alpha,
+                     1.0 / (24.0 * dx), 1.0 / (24.0 * dy), 1.0 / (24.0 * dz),
+                     1.0 / dt, dx, dy, dz};
+  hipLaunchKernelGGL(calvel3_calmas_kernel, r.grid, dim3(PJ_TJ * PJ_TK), 0, S_(stream), ustar,
+                     vstar, wstar, p, u, v, w, su, g, ich, kt, jt, K);
   WF_LAUNCHED();
   return fold(rmax_dev, r, 1, 0, 1, S_(stream));
 }

@@ -74,6 +74,7 @@ _SIGS = {
     "wf_batch_convdiff": [_P, _I, _P, _P, _P, _P, _P, _P, C.c_double, _P],
     "wf_batch_calvel3": [_P, _P, _P, _P, _P, _P, _P, _P, C.c_double, _P, _P],
     "wf_batch_calmas": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "wf_batch_calvel3_calmas": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_double, _P, _P],
     "wf_batch_add_ring": [_P, _P, _P, _P, _P],
     "wf_batch_mg_solve_mixed": [_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
     "wf_batch_project_mask": [_P, _P, C.c_double, _I, _P, _P, _P, _P],

@@ -221,21 +221,28 @@ class Ensemble:
                 self._fill(c)
             if stage + 1 < len(ALFARK):
                 self._calvel(alpha)
-        # projection: calvel, reset the per-member loop state
-        self._calvel(1.0)
+        # projection: reset the per-member loop state; every outer test opens
+        # with the calvel/calmas pair in one pass, so a calvel of its own runs
+        # here only when no test does (solver.py:178-212)
+        if self.maxcy < 1:
+            self._calvel(1.0)
         _hip.call("wf_batch_set_int", self.active.ptr, self.nmem, 1, s)
         _hip.call("wf_batch_set_int", self.iters.ptr, self.nmem, 0, s)
 
     def _outer_test(self, it):
-        """calmas + the break test of outer iteration `it` (solver.py:185-197)."""
+        """calvel + calmas in one pass and the break test of outer iteration
+        `it` (solver.py:185-197) for the active members."""
         P = self.pool
-        self._call("wf_batch_calmas", P["u"].ptr, P["v"].ptr, P["w"].ptr, P["su"].ptr,
-                   self.rmax.ptr, self.active.ptr, self.stream)
+        self._call("wf_batch_calvel3_calmas", P["ustar"].ptr, P["vstar"].ptr, P["wstar"].ptr,
+                   P["p"].ptr, P["u"].ptr, P["v"].ptr, P["w"].ptr, P["su"].ptr, self.rmax.ptr,
+                   1.0, self.active.ptr, self.stream)
         self._call("wf_batch_project_mask", self.rmax.ptr, self.eps, int(it), self.active.ptr,
                    self.iters.ptr, self.status.ptr, self.stream)
 
-    def _outer_rest(self):
-        """MG solve, p += pp, p ghosts, calvel for the active members."""
+    def _outer_rest(self, it):
+        """MG solve, p += pp, p ghosts for the active members; after the last
+        allowed iteration `it` also the calvel that closes it (the next test
+        would otherwise run it)."""
         P, s, a = self.pool, self.stream, self.active.ptr
         self._call("wf_batch_mg_solve_mixed", self._lptr["phi"], self._lptr["rhs"],
                    self._lptr["tmp"], self.cop.ptr if self.use_cop else None, P["su"].ptr,
@@ -243,13 +250,14 @@ class Ensemble:
                    a, s)
         self._call("wf_batch_add_ring", P["p"].ptr, P["pp"].ptr, a, s)
         self._fill("p", "pbound", self.active)
-        self._calvel(1.0, self.active)
+        if it == self.maxcy:
+            self._calvel(1.0, self.active)
 
     def _part_a(self):
         self._part_a_head()
         for it in range(1, min(self.graph_iters, self.maxcy) + 1):
             self._outer_test(it)
-            self._outer_rest()
+            self._outer_rest(it)
         if self.graph_iters + 1 <= self.maxcy:
             self._outer_test(self.graph_iters + 1)
 
@@ -284,7 +292,7 @@ class Ensemble:
                 break
             if not self.active.numpy().any():
                 break
-            self._outer_rest()
+            self._outer_rest(it)
             if it + 1 > self.maxcy:
                 break
             it += 1
