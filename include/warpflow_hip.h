@@ -269,8 +269,12 @@ int wf_mg_copy_out_ring(const wf_mg_level* lev, double* dst, wf_grid g,
 int wf_mg_vcycle(const wf_mg_level* levels, int nlev, int n_pre, int n_post,
                  int n_coarse, void* stream);
 
-/* Multigrid.solve, multigrid.py:324-337: copy-in, zero pp and phi0,
- * n_vcycles V-cycles, copy-out incl. ring.  All levels float64. */
+/* Multigrid.solve, multigrid.py:324-337: copy-in, zero phi0, n_vcycles
+ * V-cycles, copy-out incl. ring.  All levels float64.  The reference also
+ * zeroes pp first; here pp's real cells and whole ring are written by the
+ * copy-out and its deeper ghost layers are left as they are (keep them zero,
+ * as a freshly allocated field is, for the reference's pp).  phi0 is not
+ * zeroed when the first sweep can take the zero iterate unread. */
 int wf_mg_solve(const wf_mg_level* levels, int nlev, const double* su,
                 double* pp, wf_grid g, int n_pre, int n_post, int n_coarse,
                 int n_vcycles, void* stream);

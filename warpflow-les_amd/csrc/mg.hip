@@ -362,7 +362,10 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   auto chunk_rsrc = [&](const T* a) {
     return buf_rsrc(reinterpret_cast<const char*>(a) + (int64_t)i0 * pb, 0x7fffffffu);
   };
-  const auto dpin = chunk_rsrc(pin), drhs = chunk_rsrc(rhs), dout = chunk_rsrc(pout);
+  // pin == nullptr: an iterate of zeros (a V-cycle's first sweep), read
+  // through an empty descriptor -- every load returns 0, nothing is fetched
+  const auto dpin = pin ? chunk_rsrc(pin) : buf_rsrc(rhs, 0u);
+  const auto drhs = chunk_rsrc(rhs), dout = chunk_rsrc(pout);
   const auto dnone = buf_rsrc(pout, 0u);  // every store dropped
   auto soff = [&](int x) { return (x + 1 - i0) * pb; };
 
@@ -509,8 +512,9 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   using I4 = std::integral_constant<int, 4>;
   using I5 = std::integral_constant<int, 5>;
   if (i0 >= 2) {  // pin plane i0-2 lies before the descriptors' base
-    const auto dp = buf_rsrc(reinterpret_cast<const char*>(pin) + (int64_t)(i0 - 1) * pb,
-                             0x7fffffffu);
+    const auto dp =
+        pin ? buf_rsrc(reinterpret_cast<const char*>(pin) + (int64_t)(i0 - 1) * pb, 0x7fffffffu)
+            : buf_rsrc(rhs, 0u);
 #pragma unroll
     for (int e = 0; e < NPE; ++e) pre[0][e] = buf_ld<T>(dp, gofs[1][e], 0);
   }
@@ -681,7 +685,9 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
   auto chunk_rsrc = [&](const T* a) {
     return buf_rsrc(reinterpret_cast<const char*>(a) + (int64_t)(xb + 1) * pb, 0x7fffffffu);
   };
-  const auto dpin = chunk_rsrc(pin), drhs = chunk_rsrc(rhs), dout = chunk_rsrc(pout);
+  // pin == nullptr: an iterate of zeros, read through an empty descriptor
+  const auto dpin = pin ? chunk_rsrc(pin) : buf_rsrc(rhs, 0u);
+  const auto drhs = chunk_rsrc(rhs), dout = chunk_rsrc(pout);
   const auto dnone = buf_rsrc(pout, 0u);
   // soffset of real plane x, clamped to the planes [lo, hi] a stage needs
   // (values of other planes are never used) and to the array
@@ -1558,24 +1564,55 @@ int sweep2_t(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
 // may leave the result in tmp, which *swapped reports (the caller then
 // treats tmp as the level's phi); without it the pairs go in even numbers
 // and the rest as one-sweep launches, so the result is in phi.
+// Which smoother relax_t runs for nsweeps sweeps of level L.
+enum class RelaxPath { LDS2, LDS_RHS, LDS, SWEEPS };
 template <class T>
-int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s, bool allow_swap = false,
-            bool* swapped = nullptr) {
-  if (swapped) *swapped = false;
-  if (nsweeps <= 0) return 0;
+RelaxPath relax_path(const wf_mg_level& L, int nsweeps) {
   const size_t cells = (size_t)L.n0 * L.n1 * L.n2;
   const size_t lds = (size_t)max_lds();
   // levels whose zero-ringed copy fits LDS: all sweeps in one workgroup
-  if (relax_lds2_cpt(L) > 0 && relax_lds2_bytes(L, sizeof(T)) <= lds)
-    return launch_relax_lds2<T>(L, nsweeps, s);
+  if (relax_lds2_cpt(L) > 0 && relax_lds2_bytes(L, sizeof(T)) <= lds) return RelaxPath::LDS2;
   // many sweeps in one workgroup pay off only with rhs in LDS too: with rhs
   // read from global memory every colour pass waits on its latency
   if ((cells <= LDS_RELAX_MAX_CELLS || nsweeps >= LDS_RELAX_MIN_SWEEPS) &&
       2 * cells * sizeof(T) <= lds)
-    return launch_relax_lds<T, true>(L, nsweeps, 2 * cells * sizeof(T), s);
-  if (cells <= LDS_RELAX_MAX_CELLS && cells * sizeof(T) <= lds)
-    return launch_relax_lds<T, false>(L, nsweeps, cells * sizeof(T), s);
+    return RelaxPath::LDS_RHS;
+  if (cells <= LDS_RELAX_MAX_CELLS && cells * sizeof(T) <= lds) return RelaxPath::LDS;
+  return RelaxPath::SWEEPS;
+}
+// True when relax_t's first launch for nsweeps sweeps of L is a full-sweep
+// kernel, which can take a zero iterate without reading it (zero_start).
+template <class T>
+bool relax_zero_start_ok(const wf_mg_level& L, int nsweeps) {
+  return nsweeps >= 2 && L.tmp && relax_path<T>(L, nsweeps) == RelaxPath::SWEEPS;
+}
+
+// zero_start: the iterate in L.phi is to be taken as zero (the caller has
+// not zeroed it); only valid when relax_zero_start_ok.
+template <class T>
+int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s, bool allow_swap = false,
+            bool* swapped = nullptr, bool zero_start = false) {
+  if (swapped) *swapped = false;
+  if (nsweeps <= 0) return 0;
+  const size_t cells = (size_t)L.n0 * L.n1 * L.n2;
+  switch (relax_path<T>(L, nsweeps)) {
+    case RelaxPath::LDS2:
+      return launch_relax_lds2<T>(L, nsweeps, s);
+    case RelaxPath::LDS_RHS:
+      return launch_relax_lds<T, true>(L, nsweeps, 2 * cells * sizeof(T), s);
+    case RelaxPath::LDS:
+      return launch_relax_lds<T, false>(L, nsweeps, cells * sizeof(T), s);
+    case RelaxPath::SWEEPS:
+      break;
+  }
+  if (zero_start && !relax_zero_start_ok<T>(L, nsweeps)) return WF_EINVAL;
   wf_mg_level C = L;  // C.phi holds the current iterate
+  // the first launch reads the zero iterate through an empty descriptor
+  auto src = [&](const void* p) -> const void* {
+    const void* r = zero_start ? nullptr : p;
+    zero_start = false;
+    return r;
+  };
   int q = 0;
   const int pairs = tune_get(WF_TUNE_SWEEP_PAIRS, L.n0, L.n1, L.n2);
   // built-in: two-sweep launches where they measured faster (FP64, >= 2^23 cells)
@@ -1583,14 +1620,14 @@ int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s, bool allow_swap = 
     int npair = nsweeps / 2;
     if (!allow_swap) npair &= ~1;
     for (int f = 0; f < npair; ++f) {
-      WF_TRY((hipError_t)sweep2_t<T>(C, C.phi, C.tmp, s));
+      WF_TRY((hipError_t)sweep2_t<T>(C, src(C.phi), C.tmp, s));
       std::swap(C.phi, C.tmp);
     }
     q = 2 * npair;
   }
   if (C.tmp) {
     for (; q + 1 < nsweeps; q += 2) {
-      WF_TRY((hipError_t)sweep_t<T>(C, C.phi, C.tmp, s));
+      WF_TRY((hipError_t)sweep_t<T>(C, src(C.phi), C.tmp, s));
       WF_TRY((hipError_t)sweep_t<T>(C, C.tmp, C.phi, s));
     }
   }
@@ -1639,9 +1676,12 @@ bool pair_ok(const wf_mg_level* F, const wf_mg_level* C) {
 }
 
 int relax_any(const wf_mg_level& L, int n, hipStream_t s, bool allow_swap = false,
-              bool* swapped = nullptr) {
-  return L.prec == 8 ? relax_t<double>(L, n, s, allow_swap, swapped)
-                     : relax_t<float>(L, n, s, allow_swap, swapped);
+              bool* swapped = nullptr, bool zero_start = false) {
+  return L.prec == 8 ? relax_t<double>(L, n, s, allow_swap, swapped, zero_start)
+                     : relax_t<float>(L, n, s, allow_swap, swapped, zero_start);
+}
+bool relax_zero_start_any(const wf_mg_level& L, int n) {
+  return L.prec == 8 ? relax_zero_start_ok<double>(L, n) : relax_zero_start_ok<float>(L, n);
 }
 
 constexpr int COARSE_OP_MAX_CELLS = 4096;
@@ -1691,7 +1731,7 @@ int coarse_op_build_t(const wf_mg_level& L, int nsweeps, void* op, hipStream_t s
 
 template <class Relax>
 int vcycle_levels(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse,
-                  hipStream_t s, Relax&& relax_l) {
+                  hipStream_t s, Relax&& relax_l, bool zero0 = false) {
   for (int l = 0; l < nlev; ++l) {
     if (l > 0) {
       const int r = lv[l].prec == 8 ? restrict_t<double>(lv[l - 1], lv[l], s)
@@ -1704,14 +1744,14 @@ int vcycle_levels(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoa
       if (r) return r;
       continue;
     }
-    const int r = relax_l(l, l < nlev - 1 ? npre : ncoarse);
+    const int r = relax_l(l, l < nlev - 1 ? npre : ncoarse, l == 0 && zero0);
     if (r) return r;
   }
   for (int l = nlev - 2; l >= 0; --l) {
     const int r = lv[l].prec == 8 ? prolong_t<double>(lv[l + 1], lv[l], s)
                                   : prolong_t<float>(lv[l + 1], lv[l], s);
     if (r) return r;
-    const int r2 = relax_l(l, npost);
+    const int r2 = relax_l(l, npost, false);
     if (r2) return r2;
   }
   return 0;
@@ -1721,17 +1761,19 @@ int vcycle_levels(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoa
 // tmp swap when a relax leaves its result in tmp (an odd number of two-sweep
 // launches); a level still swapped at the end is copied back, so the caller
 // sees every level's iterate in its phi (with V(6,6): 3 + 3 launches, none).
+// zero0: level 0's iterate is to be taken as zero without reading it (the
+// caller checked relax_zero_start_any(lv0[0], npre) and skipped its zeroing).
 int vcycle_any(const wf_mg_level* lv0, int nlev, int npre, int npost, int ncoarse,
-               hipStream_t s) {
+               hipStream_t s, bool zero0 = false) {
   wf_mg_level lv[16];
   for (int l = 0; l < nlev; ++l) lv[l] = lv0[l];
-  auto relax = [&](int l, int n) -> int {
+  auto relax = [&](int l, int n, bool zs) -> int {
     bool sw = false;
-    const int r = relax_any(lv[l], n, s, true, &sw);
+    const int r = relax_any(lv[l], n, s, true, &sw, zs);
     if (sw) std::swap(lv[l].phi, lv[l].tmp);
     return r;
   };
-  const int r = vcycle_levels(lv, nlev, npre, npost, ncoarse, s, relax);
+  const int r = vcycle_levels(lv, nlev, npre, npost, ncoarse, s, relax, zero0);
   if (r) return r;
   for (int l = 0; l < nlev; ++l)
     if (lv[l].phi != lv0[l].phi)
@@ -1761,8 +1803,11 @@ bool mg_level_ok(const wf_mg_level* L) { return level_ok(L); }
 int64_t mg_level_elems(const wf_mg_level& L) { return level_elems(L); }
 dim3 mg_cell_grid(int n0, int n1, int n2) { return cell_grid(n0, n1, n2); }
 int mg_vcycle_levels(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse,
-                     hipStream_t s) {
-  return vcycle_any(lv, nlev, npre, npost, ncoarse, s);
+                     hipStream_t s, bool zero0) {
+  return vcycle_any(lv, nlev, npre, npost, ncoarse, s, zero0);
+}
+bool mg_relax_zero_start_ok(const wf_mg_level& L, int nsweeps) {
+  return relax_zero_start_any(L, nsweeps);
 }
 bool mg_hier_ok(const wf_mg_level* lv, int nlev) { return hier_ok(lv, nlev); }
 bool mg_field_grid_ok(const wf_grid& g) { return field_grid_ok(g); }
@@ -1890,12 +1935,18 @@ int wf_mg_solve(const wf_mg_level* levels, int nlev, const double* su, double* p
   const hipStream_t s = S_(stream);
   int r = wf_mg_copy_in(su, g, &levels[0], stream);
   if (r) return r;
-  r = zero_t<double>(pp, (int64_t)(g.ni + 2 * g.npl) * g.si, s);
-  if (r) return r;
-  r = zero_t<double>(levels[0].phi, level_elems(levels[0]), s);
-  if (r) return r;
+  // pp: the copy-out below writes its real cells and its whole ring; its
+  // deeper ghost layers are not written (they stay zero, as the reference's
+  // zeroed pp leaves them).  phi = 0: the first sweep of the first V-cycle
+  // takes the zero iterate without reading it where its smoother allows.
+  const bool zs =
+      n_vcycles > 0 && relax_zero_start_any(levels[0], nlev > 1 ? n_pre : n_coarse);
+  if (!zs) {
+    r = zero_t<double>(levels[0].phi, level_elems(levels[0]), s);
+    if (r) return r;
+  }
   for (int c = 0; c < n_vcycles; ++c) {
-    r = vcycle_any(levels, nlev, n_pre, n_post, n_coarse, s);
+    r = vcycle_any(levels, nlev, n_pre, n_post, n_coarse, s, c == 0 && zs);
     if (r) return r;
   }
   return wf_mg_copy_out_ring(&levels[0], pp, g, stream);

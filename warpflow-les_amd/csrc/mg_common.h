@@ -71,8 +71,11 @@ namespace wf {
 bool mg_level_ok(const wf_mg_level* L);
 int64_t mg_level_elems(const wf_mg_level& L);
 dim3 mg_cell_grid(int n0, int n1, int n2);
+// zero0: level 0's iterate is taken as zero without being read or zeroed;
+// only when mg_relax_zero_start_ok(lv[0], first sweep count of level 0).
 int mg_vcycle_levels(const wf_mg_level* lv, int nlev, int npre, int npost, int ncoarse,
-                     hipStream_t s);
+                     hipStream_t s, bool zero0 = false);
+bool mg_relax_zero_start_ok(const wf_mg_level& L, int nsweeps);
 bool mg_hier_ok(const wf_mg_level* lv, int nlev);
 bool mg_field_grid_ok(const wf_grid& g);
 template <class T>

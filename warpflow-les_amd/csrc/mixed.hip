@@ -520,9 +520,14 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
       res = bmax > 0.0 ? h / bmax : 0.0;
       if (res <= rtol || last_check) break;
     }
-    r = mg_zero<float>(L0.phi, mg_level_elems(L0), s);
-    if (r) return r;
-    r = mg_vcycle_levels(levels, nlev, n_pre, n_post, n_coarse, s);
+    // the V-cycle starts from e = 0: the first sweep of level 0 takes it
+    // without reading phi where its smoother allows, else phi is zeroed
+    const bool zs = mg_relax_zero_start_ok(L0, nlev > 1 ? n_pre : n_coarse);
+    if (!zs) {
+      r = mg_zero<float>(L0.phi, mg_level_elems(L0), s);
+      if (r) return r;
+    }
+    r = mg_vcycle_levels(levels, nlev, n_pre, n_post, n_coarse, s, zs);
     if (r) return r;
     summed = (flags & MIX_RESULT_MEAN) && !tol_mode && c == max_cycles - 1;
     hipLaunchKernelGGL(update_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g,
