@@ -121,6 +121,17 @@ int wf_convdiff4th(int comp, const double* u, const double* v, const double* w,
                    wf_grid g, int d0, int d1, int d2, double dx, double dy,
                    double dz, double dt, double alpha, void* stream);
 
+/* wf_convdiff4th for the first RK stage, where the base value c_oo is the
+ * component itself (solver.py:223-225 copies u, v, w to uoo, voo, woo just
+ * before): the base is taken from the component, and written to coo_save on
+ * the operator box -- the cells the second stage reads as c_oo -- so no
+ * separate full-field copy is needed.  coo_save's other cells are not
+ * written.  No single reference kernel corresponds to it. */
+int wf_convdiff4th_save(int comp, const double* u, const double* v, const double* w,
+                        double* coo_save, const double* vis, double* cstar, wf_grid g,
+                        int d0, int d1, int d2, double dx, double dy, double dz, double dt,
+                        double alpha, void* stream);
+
 /* calvel_u / _v / _w, kernels.py:232-264: c = cstar - dt*alpha*(p[+1]-p)/h
  * along the component's axis. */
 int wf_calvel(int comp, const double* cstar, const double* p, double* c,

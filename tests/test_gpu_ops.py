@@ -92,6 +92,32 @@ def test_calvel_calvel3_calmas(gpu, n3):
     assert abs(rbuf.numpy()[0] - rmax_ref) <= TOL * rmax_ref
 
 
+@pytest.mark.parametrize("n3", SHAPES + [(21, 10, 70)])
+@pytest.mark.parametrize("comp", ["u", "v", "w"])
+def test_convdiff_first_stage_saves_the_base(gpu, n3, comp):
+    """First-stage conv+diff: the base value is the component itself and is
+    stored into c_oo on the operator box -- the same star bits as the plain
+    kernel given c_oo = c, c_oo = c on the box, every other cell untouched."""
+    from warpflow import kernels as K
+    h = tuple(1.0 / n for n in n3)
+    f = rand_fields(n3, ["u", "v", "w", "coo", "vis", "star0"], seed=7)
+    f["vis"] = 0.0025 + 0.001 * np.abs(f["vis"])
+    dims = {"u": (n3[0] - 1, n3[1], n3[2]), "v": (n3[0], n3[1] - 1, n3[2]),
+            "w": (n3[0], n3[1], n3[2] - 1)}[comp]
+    d = {k: up(v) for k, v in f.items()}
+    plain = up(f["star0"])
+    getattr(K, "convdiff4th_" + comp)(dims, d["u"], d["v"], d["w"], d[comp], d["vis"], plain,
+                                      *h, 3e-4, 0.5, N)
+    star = up(f["star0"])
+    getattr(K, "convdiff4th_save_" + comp)(dims, d["u"], d["v"], d["w"], d["coo"], d["vis"],
+                                           star, *h, 3e-4, 0.5, N)
+    assert np.array_equal(star.numpy(), plain.numpy())
+    box = tuple(slice(N, N + x) for x in dims)
+    want = f["coo"].copy()
+    want[box] = f[comp][box]
+    assert np.array_equal(d["coo"].numpy(), want)
+
+
 @pytest.mark.parametrize("n3", SHAPES + [(20, 19, 37), (9, 17, 70), (33, 8, 33)])
 def test_calvel3_calmas_one_pass(gpu, n3):
     """The fused calvel/calmas pass equals calvel3 then calmas bit for bit (u, v,

@@ -174,15 +174,18 @@ struct Lay {  // regions of the two staged cross quantities
 };
 
 // One (k tile, j tile, i chunk) work item; every thread of the workgroup
-// calls it with the same item.
-template <int C>
+// calls it with the same item.  SAVE (the first RK stage): the step's base
+// value c_oo is the component itself, taken from the thread's register
+// window instead of being loaded, and written to coo_save on the box -- the
+// cells the second stage reads as c_oo -- in place of a full-field copy.
+template <int C, bool SAVE = false>
 __device__ inline void convdiff_body(const double* __restrict__ u, const double* __restrict__ v,
                                      const double* __restrict__ w,
                                      const double* __restrict__ coo,
                                      const double* __restrict__ vis,
                                      double* __restrict__ cstar, const wf_grid& g,
                                      const Box3& b, int ich, int ktile, int jtile, int chunk,
-                                     const Coefs& K) {
+                                     const Coefs& K, double* __restrict__ coo_save = nullptr) {
   using RA = typename Lay<C>::A;
   using RB = typename Lay<C>::B;
   __shared__ double Ls[4][OwnR::N];
@@ -233,7 +236,8 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
     Ls[1][hs.lds] = F[G.plane(i0 + 1) + hs.off];
   }
   double fhalo = hs.on ? F[G.plane(i0 + 2) + hs.off] : 0.0;
-  double cnew = coo[G.plane(i0) + own.off];
+  double cnew = 0.0;
+  if constexpr (!SAVE) cnew = coo[G.plane(i0) + own.off];
   double vnew = vis[G.plane(i0) + visoff];
 
   // C = 0: windows of v and w at the tile cell and at the halo position,
@@ -295,7 +299,7 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
 #pragma unroll
     for (int q = 0; q < 6; ++q) f[q] = f[q + 1];
     f[6] = fnew;
-    const double cur_coo = cnew, cur_vis = vnew;
+    const double cur_coo = SAVE ? f[3] : cnew, cur_vis = vnew;
     double* __restrict__ S = Ls[rel & 3];
     double* __restrict__ A = La[buf];
     double* __restrict__ B = Lb[buf];
@@ -329,7 +333,7 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
     if (i + 1 < i1) {
       fnew = F[G.plane(i + 4) + own.off];
       if (hs.on) fhalo = F[G.plane(i + 3) + hs.off];
-      cnew = coo[G.plane(i + 1) + own.off];
+      if constexpr (!SAVE) cnew = coo[G.plane(i + 1) + own.off];
       vnew = vis[G.plane(i + 1) + visoff];
       if constexpr (C == 0) {
         const int64_t pn = G.plane(i + 3);
@@ -383,6 +387,7 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
         double t = cur_coo - K.dta * nsum;
         t = t + K.dta * (cur_vis * lap);
         cstar[G.plane(i) + own.off] = t;
+        if constexpr (SAVE) coo_save[G.plane(i) + own.off] = cur_coo;
       }
     } else {
       // x: D4 over Q(i+1), Q(i), Q(i-1), Q(i-2); Q(i+1) from u at plane i+1
@@ -436,20 +441,22 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
         double t = cur_coo - K.dta * nsum;
         t = t + K.dta * (cur_vis * lap);
         cstar[G.plane(i) + own.off] = t;
+        if constexpr (SAVE) coo_save[G.plane(i) + own.off] = cur_coo;
       }
     }
   }
 }
 
-template <int C>
+template <int C, bool SAVE>
 __global__ void __launch_bounds__(256)
 convdiff_kernel(const double* __restrict__ u, const double* __restrict__ v,
                 const double* __restrict__ w, const double* __restrict__ coo,
                 const double* __restrict__ vis, double* __restrict__ cstar, wf_grid g, Box3 b,
-                int ich, int kt, int jt, Coefs K) {
+                int ich, int kt, int jt, Coefs K, double* __restrict__ coo_save) {
   const int nb = kt * jt * gridDim.y;
   const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
-  convdiff_body<C>(u, v, w, coo, vis, cstar, g, b, ich, L % kt, (L / kt) % jt, L / (kt * jt), K);
+  convdiff_body<C, SAVE>(u, v, w, coo, vis, cstar, g, b, ich, L % kt, (L / kt) % jt,
+                         L / (kt * jt), K, coo_save);
 }
 
 // Ensemble form: a persistent grid over the (member, tile, chunk) items of
@@ -507,7 +514,7 @@ int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,
 int convdiff_launch(int comp, const double* u, const double* v, const double* w,
                     const double* coo, const double* vis, double* cstar, const wf_grid& g,
                     const Box3& b, double dx, double dy, double dz, double dta, int ich,
-                    hipStream_t s) {
+                    hipStream_t s, double* coo_save) {
   using namespace cdv;
   const int kt = div_up(b.d2, TK), jt = div_up(b.d1, TJ);
   if (ich <= 0) ich = tune_get(WF_TUNE_CD_ICH, g.ni, g.nj, g.nk);
@@ -528,15 +535,25 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
   K.dta = dta;
   // 1-D id over (k tile, j tile) per grid.x row, one row per i-chunk
   const dim3 grid((unsigned)(kt * jt), (unsigned)nch);
-  if (comp == 0)
-    hipLaunchKernelGGL(convdiff_kernel<0>, grid, dim3(NT), 0, s, u, v, w, coo, vis, cstar, g,
-                       b, ich, kt, jt, K);
-  else if (comp == 1)
-    hipLaunchKernelGGL(convdiff_kernel<1>, grid, dim3(NT), 0, s, u, v, w, coo, vis, cstar, g,
-                       b, ich, kt, jt, K);
-  else
-    hipLaunchKernelGGL(convdiff_kernel<2>, grid, dim3(NT), 0, s, u, v, w, coo, vis, cstar, g,
-                       b, ich, kt, jt, K);
+#define CD_LAUNCH(C_, S_)                                                                   \
+  hipLaunchKernelGGL((convdiff_kernel<C_, S_>), grid, dim3(NT), 0, s, u, v, w, coo, vis, cstar, \
+                     g, b, ich, kt, jt, K, coo_save)
+  if (coo_save) {
+    if (comp == 0)
+      CD_LAUNCH(0, true);
+    else if (comp == 1)
+      CD_LAUNCH(1, true);
+    else
+      CD_LAUNCH(2, true);
+  } else {
+    if (comp == 0)
+      CD_LAUNCH(0, false);
+    else if (comp == 1)
+      CD_LAUNCH(1, false);
+    else
+      CD_LAUNCH(2, false);
+  }
+#undef CD_LAUNCH
   WF_LAUNCHED();
   return 0;
 }

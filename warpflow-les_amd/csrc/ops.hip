@@ -1094,6 +1094,17 @@ int wf_convdiff4th(int comp, const double* u, const double* v, const double* w,
                          S_(stream));
 }
 
+int wf_convdiff4th_save(int comp, const double* u, const double* v, const double* w,
+                        double* coo_save, const double* vis, double* cstar, wf_grid g, int d0,
+                        int d1, int d2, double dx, double dy, double dz, double dt,
+                        double alpha, void* stream) {
+  const Box3 b = npl_box(g, d0, d1, d2);
+  if (comp < 0 || comp > 2 || !box_ok(g, b, 3, 3) || !coo_save) return WF_EINVAL;
+  if (empty(b)) return 0;
+  return convdiff_launch(comp, u, v, w, nullptr, vis, cstar, g, b, dx, dy, dz, dt * alpha, 0,
+                         S_(stream), coo_save);
+}
+
 int wf_calvel(int comp, const double* cstar, const double* p, double* c, wf_grid g, int d0,
               int d1, int d2, double h, double dt, double alpha, void* stream) {
   const Box3 b = npl_box(g, d0, d1, d2);

@@ -134,10 +134,12 @@ __device__ inline double wave_sum(double v) {
 int tune_get(int knob, int n0, int n1, int n2);
 
 // Fused conv4th + diff4th of component comp over box b (convdiff.hip); ich =
-// i-planes per workgroup, 0 picks it from the box size.
+// i-planes per workgroup, 0 picks it from the box size.  coo_save != nullptr:
+// the base value is the component itself (coo unused), written to coo_save
+// on the box.
 int convdiff_launch(int comp, const double* u, const double* v, const double* w,
                     const double* coo, const double* vis, double* cstar, const wf_grid& g,
                     const Box3& b, double dx, double dy, double dz, double dta, int ich,
-                    hipStream_t s);
+                    hipStream_t s, double* coo_save = nullptr);
 
 }  // namespace wf

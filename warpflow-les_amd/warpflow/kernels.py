@@ -80,6 +80,18 @@ def _convdiff(comp):
     return f
 
 
+def _convdiff_save(comp):
+    def f(dim, u, v, w, coo_save, vis, cstar, dx, dy, dz, dt, alpha, npl):
+        """convdiff4th_c of the first RK stage: the base value is the component
+        itself, written to coo_save on the operator box (the copy of
+        solver.py:223-225 restricted to the cells the second stage reads)."""
+        _same(u, v, w, coo_save, vis, cstar)
+        _hip.call("wf_convdiff4th_save", comp, u.ptr, v.ptr, w.ptr, coo_save.ptr, vis.ptr,
+                  cstar.ptr, grid_of(cstar, npl), *dim, dx, dy, dz, dt, alpha,
+                  current_stream())
+    return f
+
+
 def _calvel(comp):
     def f(dim, cstar, p, c, h, dt, alpha, npl):
         _same(cstar, p, c)
@@ -173,6 +185,9 @@ diff4th_w = Kernel("diff4th_w", _diff(2))
 convdiff4th_u = Kernel("convdiff4th_u", _convdiff(0))
 convdiff4th_v = Kernel("convdiff4th_v", _convdiff(1))
 convdiff4th_w = Kernel("convdiff4th_w", _convdiff(2))
+convdiff4th_save_u = Kernel("convdiff4th_save_u", _convdiff_save(0))
+convdiff4th_save_v = Kernel("convdiff4th_save_v", _convdiff_save(1))
+convdiff4th_save_w = Kernel("convdiff4th_save_w", _convdiff_save(2))
 calvel_u = Kernel("calvel_u", _calvel(0))
 calvel_v = Kernel("calvel_v", _calvel(1))
 calvel_w = Kernel("calvel_w", _calvel(2))

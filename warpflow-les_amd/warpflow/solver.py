@@ -15,8 +15,11 @@ Every operator runs on the GPU through the HIP library; the host keeps only
 the scalars the reference keeps (dt, ctime, rmax, cfl) and reads back the
 reductions it branches on, as the reference does.  ``fused=True`` (default)
 runs each component's conv4th+diff4th pair and the three calvel launches as
-single passes; ``fused=False`` replays the reference's launches one for one.
-Both give the same fields.
+single passes, opens each outer iteration with calvel and calmas in one pass,
+and has the first stage's conv+diff store uoo, voo, woo on the operator boxes
+instead of copying the whole fields; ``fused=False`` replays the reference's
+launches one for one.  Both give the same fields (uoo, voo, woo agree on the
+operator boxes, the only cells any operator reads).
 """
 from __future__ import annotations
 
@@ -288,11 +291,18 @@ class FlowSolver:
                  ("w", K.conv4th_w, K.diff4th_w, K.convdiff4th_w))
         if self.fused:
             # conv+diff of component c right after its first fill; the second
-            # fills still run, in order, for their effect on the ghosts.
-            for c, _, _, kcd in comps:
+            # fills still run, in order, for their effect on the ghosts.  In
+            # the first stage c_oo is c itself (step() skips the copies):
+            # the kernel takes it from c and stores it into c_oo on its box.
+            save = (K.convdiff4th_save_u, K.convdiff4th_save_v, K.convdiff4th_save_w)
+            for q, (c, _, _, kcd) in enumerate(comps):
                 self.fill_bc(c, alfabc)
-                kcd(self._op_dim(c), f["u"], f["v"], f["w"], f[c + "oo"], f["vis"],
-                    f[c + "star"], *d3, dt, alpha, NPL)
+                if stage == 0:
+                    save[q](self._op_dim(c), f["u"], f["v"], f["w"], f[c + "oo"], f["vis"],
+                            f[c + "star"], *d3, dt, alpha, NPL)
+                else:
+                    kcd(self._op_dim(c), f["u"], f["v"], f["w"], f[c + "oo"], f["vis"],
+                        f[c + "star"], *d3, dt, alpha, NPL)
             self.fill_bc_uvw(alfabc)
             return
         for c, kconv, _, _ in comps:
@@ -309,9 +319,14 @@ class FlowSolver:
         self.compute_dt(defer=True)
         self.ctime += self.dt
         f = self.f
-        copy(f["uoo"], f["u"])
-        copy(f["voo"], f["v"])
-        copy(f["woo"], f["w"])
+        if not self.fused:
+            copy(f["uoo"], f["u"])
+            copy(f["voo"], f["v"])
+            copy(f["woo"], f["w"])
+        # fused: the first stage's conv+diff stores u, v, w into uoo, voo, woo
+        # on the operator boxes, the only cells the second stage reads (the
+        # copies' other cells are not refreshed); the SGS and fills before it
+        # leave those cells of u, v, w as they are here
         self._sgs()
         for stage, alpha in enumerate(ALFARK):
             self._rk_stage(stage, alpha)
