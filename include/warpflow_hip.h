@@ -394,6 +394,11 @@ int wf_batch_bc_fill(wf_batch b, double* f, int which, const int* active, void* 
 int wf_batch_convdiff(wf_batch b, int comp, const double* u, const double* v, const double* w,
                       const double* coo, const double* vis, double* cstar, double alpha,
                       void* stream);
+/* wf_convdiff4th_save: the first stage's form, base value from the component,
+ * stored into coo_save on the operator box. */
+int wf_batch_convdiff_save(wf_batch b, int comp, const double* u, const double* v,
+                           const double* w, double* coo_save, const double* vis, double* cstar,
+                           double alpha, void* stream);
 /* wf_calvel3. */
 int wf_batch_calvel3(wf_batch b, const double* us, const double* vs, const double* ws,
                      const double* p, double* u, double* v, double* w, double alpha,

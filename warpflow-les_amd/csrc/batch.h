@@ -153,7 +153,7 @@ int batch_add_ring(const Batch& B, double* p, const double* pp, const int* activ
                    hipStream_t s);
 int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,
                    const double* w, const double* coo, const double* vis, double* cstar,
-                   double alpha, hipStream_t s);
+                   double alpha, hipStream_t s, double* coo_save = nullptr);
 int batch_bc_fill(const Batch& B, double* f, int which, const int* active, hipStream_t s);
 int batch_vcycle(const Batch& B, float* const* phi, float* const* rhs, float* const* tmp,
                  const float* cop, int npre, int npost, int ncoarse, const int* active,

@@ -279,6 +279,15 @@ int wf_batch_convdiff(wf_batch b, int comp, const double* u, const double* v, co
   return batch_convdiff(B_(b), comp, u, v, w, coo, vis, cstar, alpha, (hipStream_t)stream);
 }
 
+int wf_batch_convdiff_save(wf_batch b, int comp, const double* u, const double* v,
+                           const double* w, double* coo_save, const double* vis, double* cstar,
+                           double alpha, void* stream) {
+  if (!ok(b) || comp < 0 || comp > 2 || !u || !v || !w || !coo_save || !vis || !cstar)
+    return WF_EINVAL;
+  return batch_convdiff(B_(b), comp, u, v, w, nullptr, vis, cstar, alpha, (hipStream_t)stream,
+                        coo_save);
+}
+
 int wf_batch_calvel3(wf_batch b, const double* us, const double* vs, const double* ws,
                      const double* p, double* u, double* v, double* w, double alpha,
                      const int* active, void* stream) {

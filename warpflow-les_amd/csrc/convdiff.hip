@@ -461,12 +461,12 @@ convdiff_kernel(const double* __restrict__ u, const double* __restrict__ v,
 
 // Ensemble form: a persistent grid over the (member, tile, chunk) items of
 // component C, chunks of BATCH_CD_ICH planes.
-template <int C>
+template <int C, bool SAVE>
 __global__ void __launch_bounds__(256)
 batch_convdiff_kernel(BView V, Plan P, const double* __restrict__ u,
                       const double* __restrict__ v, const double* __restrict__ w,
                       const double* __restrict__ coo, const double* __restrict__ vis,
-                      double* __restrict__ cstar, double alpha) {
+                      double* __restrict__ cstar, double alpha, double* __restrict__ coo_save) {
   for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
     const int m = member_of(P.start, V.nmem, it);
     const int q = it - P.start[m];
@@ -484,8 +484,9 @@ batch_convdiff_kernel(BView V, Plan P, const double* __restrict__ u,
     K.ir12y = 1.0 / (12.0 * M.dy * M.dy);
     K.ir12z = 1.0 / (12.0 * M.dz * M.dz);
     K.dta = M.dt * alpha;
-    convdiff_body<C>(u + M.off, v + M.off, w + M.off, coo + M.off, vis + M.off, cstar + M.off,
-                     g, b, BATCH_CD_ICH, q % kt, (q / kt) % jt, q / (kt * jt), K);
+    convdiff_body<C, SAVE>(u + M.off, v + M.off, w + M.off, SAVE ? nullptr : coo + M.off,
+                           vis + M.off, cstar + M.off, g, b, BATCH_CD_ICH, q % kt,
+                           (q / kt) % jt, q / (kt * jt), K, SAVE ? coo_save + M.off : nullptr);
     __syncthreads();  // LDS of this item is free for the next
   }
 }
@@ -494,18 +495,28 @@ batch_convdiff_kernel(BView V, Plan P, const double* __restrict__ u,
 
 int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,
                    const double* w, const double* coo, const double* vis, double* cstar,
-                   double alpha, hipStream_t s) {
+                   double alpha, hipStream_t s, double* coo_save) {
   using namespace cdv;
   const Plan& P = B.plan[PK_CD0 + comp];
-  if (comp == 0)
-    hipLaunchKernelGGL(batch_convdiff_kernel<0>, dim3(P.grid), dim3(NT), 0, s, B.view(), P, u,
-                       v, w, coo, vis, cstar, alpha);
-  else if (comp == 1)
-    hipLaunchKernelGGL(batch_convdiff_kernel<1>, dim3(P.grid), dim3(NT), 0, s, B.view(), P, u,
-                       v, w, coo, vis, cstar, alpha);
-  else
-    hipLaunchKernelGGL(batch_convdiff_kernel<2>, dim3(P.grid), dim3(NT), 0, s, B.view(), P, u,
-                       v, w, coo, vis, cstar, alpha);
+#define BCD_LAUNCH(C_, S_)                                                                    \
+  hipLaunchKernelGGL((batch_convdiff_kernel<C_, S_>), dim3(P.grid), dim3(NT), 0, s, B.view(), \
+                     P, u, v, w, coo, vis, cstar, alpha, coo_save)
+  if (coo_save) {
+    if (comp == 0)
+      BCD_LAUNCH(0, true);
+    else if (comp == 1)
+      BCD_LAUNCH(1, true);
+    else
+      BCD_LAUNCH(2, true);
+  } else {
+    if (comp == 0)
+      BCD_LAUNCH(0, false);
+    else if (comp == 1)
+      BCD_LAUNCH(1, false);
+    else
+      BCD_LAUNCH(2, false);
+  }
+#undef BCD_LAUNCH
   WF_LAUNCHED();
   return 0;
 }

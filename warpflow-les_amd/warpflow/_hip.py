@@ -72,6 +72,7 @@ _SIGS = {
     "wf_batch_checkdt": [_P, _P, _P, _P, _P, _P, _P],
     "wf_batch_bc_fill": [_P, _P, _I, _P, _P],
     "wf_batch_convdiff": [_P, _I, _P, _P, _P, _P, _P, _P, C.c_double, _P],
+    "wf_batch_convdiff_save": [_P, _I, _P, _P, _P, _P, _P, _P, C.c_double, _P],
     "wf_batch_calvel3": [_P, _P, _P, _P, _P, _P, _P, _P, C.c_double, _P, _P],
     "wf_batch_calmas": [_P, _P, _P, _P, _P, _P, _P, _P],
     "wf_batch_calvel3_calmas": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_double, _P, _P],

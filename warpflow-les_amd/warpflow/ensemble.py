@@ -201,12 +201,12 @@ class Ensemble:
                    active.ptr if active is not None else None, self.stream)
 
     def _part_a_head(self):
-        """compute_dt, uoo copies, SGS and both RK2 stages."""
+        """compute_dt, SGS and both RK2 stages (the first storing uoo, voo, woo)."""
         P, s = self.pool, self.stream
         self._call("wf_batch_checkdt", P["u"].ptr, P["v"].ptr, P["w"].ptr, P["vis"].ptr,
                    self.out4.ptr, s)
-        for c in "uvw":
-            _hip.call("wf_copy", P[c + "oo"].ptr, P[c].ptr, P[c].nbytes, s)
+        # no copies u -> uoo: the first stage's conv+diff stores them on the
+        # operator boxes (the cells the second stage reads)
         if self.sgs:
             self._call("wf_batch_wale", P["u"].ptr, P["v"].ptr, P["w"].ptr, P["vis"].ptr, s)
             self._fill("vis")
@@ -215,8 +215,9 @@ class Ensemble:
         for stage, alpha in enumerate(ALFARK):
             for comp, c in enumerate("uvw"):
                 self._fill(c)
-                self._call("wf_batch_convdiff", comp, P["u"].ptr, P["v"].ptr, P["w"].ptr,
-                           P[c + "oo"].ptr, P["vis"].ptr, P[c + "star"].ptr, float(alpha), s)
+                self._call("wf_batch_convdiff_save" if stage == 0 else "wf_batch_convdiff",
+                           comp, P["u"].ptr, P["v"].ptr, P["w"].ptr, P[c + "oo"].ptr,
+                           P["vis"].ptr, P[c + "star"].ptr, float(alpha), s)
             for c in "uvw":
                 self._fill(c)
             if stage + 1 < len(ALFARK):
