@@ -170,9 +170,12 @@ def hot_kernels(a, s64, smx):
         ("calmas", lambda: K.calmas(s64.ncells, f["u"], f["v"], f["w"], f["su"], s64._buf,
                                     *h, dt, NPL), 32 * cells, 0, 0, 1),
         ("checkdt", lambda: K.checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"], s64._buf,
-                                      *h, NPL), 32 * cells, 1, 1, 1),
+                                      *h, NPL), 32 * cells, 0, 0, 1),
         ("wale", lambda: K.wale_viscosity(s64.ncells, f["u"], f["v"], f["w"], f["vis"], *h,
-                                          s64.rey, 0.46, NPL), 32 * cells, 1, 1, 1),
+                                          s64.rey, 0.46, NPL), 32 * cells, 0, 0, 1),
+        ("wale+checkdt", lambda: K.wale_checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"],
+                                                s64._buf, *h, s64.rey, 0.46, NPL),
+         40 * cells, 1, 1, 1),
         ("add_ring", lambda: K.add_inplace(tuple(x + 2 for x in s64.ncells), f["p"], f["pp"],
                                            NPL), 24 * (n + 2) ** 3, 1, 1, 1),
         # the plain copy: the step no longer copies u -> uoo (the first stage's

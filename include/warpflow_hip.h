@@ -172,6 +172,16 @@ int wf_wale(const double* u, const double* v, const double* w, double* vis,
             wf_grid g, int d0, int d1, int d2, double dx, double dy, double dz,
             double rrey, double cw, double delta, void* stream);
 
+/* wf_checkdt then wf_wale in one pass over the real cells: the reductions of
+ * compute_dt (solver.py:115-150) and the WALE viscosity that follows them in
+ * a step (solver.py:227-237; nothing in between changes u, v, w or vis).
+ * out4 and vis are written bit for bit as the two entries write them (each
+ * cell's vis is read before it is overwritten).  No single reference kernel
+ * corresponds to it. */
+int wf_wale_checkdt(const double* u, const double* v, const double* w, double* vis,
+                    double* out4_dev, wf_grid g, double dx, double dy, double dz,
+                    double rrey, double cw, double delta, void* stream);
+
 /* maxabs_reduce, kernels.py:374-381: *out = max(*out, max|f|). */
 int wf_maxabs(const double* f, double* out_dev, wf_grid g, int d0, int d1,
               int d2, void* stream);

@@ -152,6 +152,26 @@ def test_calvel3_calmas_one_pass(gpu, n3):
     assert abs(rb.numpy()[0] - rmax_ref) <= TOL * rmax_ref
 
 
+@pytest.mark.parametrize("n3", SHAPES + [(20, 19, 37), (9, 17, 70)])
+def test_wale_checkdt_one_pass(gpu, n3):
+    """checkdt's reductions and WALE in one pass equal checkdt then WALE bit for
+    bit (out4 and vis incl. ghosts), with the vis each cell had before."""
+    from warpflow import kernels as K
+    from warpflow.device import zeros
+    h = (0.031, 0.027, 0.019)
+    f = rand_fields(n3, ["u", "v", "w", "vis"], seed=9, scale=0.3)
+    f["vis"] = 0.0025 + np.abs(f["vis"]) * 0.01
+    a = {k: up(v) for k, v in f.items()}
+    b = {k: up(v) for k, v in f.items()}
+    oa, ob = zeros(4), zeros(4)
+    K.checkdt(n3, a["u"], a["v"], a["w"], a["vis"], oa, *h, N)
+    K.wale_viscosity(n3, a["u"], a["v"], a["w"], a["vis"], *h, 1 / 400, 0.46, N)
+    K.wale_checkdt(n3, b["u"], b["v"], b["w"], b["vis"], ob, *h, 1 / 400, 0.46, N)
+    assert np.array_equal(oa.numpy(), ob.numpy())
+    assert np.array_equal(a["vis"].numpy(), b["vis"].numpy())
+    assert ob.numpy()[3] < 1e10 and ob.numpy()[0] > 0.0
+
+
 @pytest.mark.parametrize("n3", SHAPES)
 def test_wale_and_checkdt(gpu, n3):
     from warpflow import kernels as K

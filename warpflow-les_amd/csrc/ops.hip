@@ -144,6 +144,19 @@ __device__ inline double div4(double tx, double ty, double tz, double r24x, doub
   return __builtin_fma(tz, r24z, __builtin_fma(ty, r24y, tx * r24x));
 }
 
+// The viscous/advective time-step bound of one cell (kernels.py:384-402) from
+// the velocities at the cell and its low neighbours and the cell's vis, with
+// fixed roundings (no contraction) so that checkdt and the fused WALE pass
+// agree bit for bit.
+__device__ inline double dtvisc_val(double u0, double um, double v0, double vm, double w0,
+                                    double wm, double vis, double dx, double dy, double dz,
+                                    double rsum) {
+#pragma clang fp contract(off)
+  const double uc = 0.5 * (u0 + um);
+  const double vc = 0.5 * (v0 + vm);
+  const double wc = 0.5 * (w0 + wm);
+  return 1.0 / (fabs(uc) / dx + fabs(vc) / dy + fabs(wc) / dz + 2.0 * vis * rsum + 1.0e-30);
+}
 __global__ void __launch_bounds__(256)
 calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
               double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double rh, double dta,
@@ -491,6 +504,10 @@ red_fold_kernel(double* __restrict__ out, int nblk, int nq, int min_mask, int co
 template <class FU, class FV, class FW>
 __device__ inline double wale_value(const FU& U, const FV& V, const FW& W, double rdx,
                                     double rdy, double rdz, double rrey, double ls) {
+  // Roundings are fixed here -- each sum of products accumulates with fused
+  // multiply-adds left to right, nothing else is contracted -- so that every
+  // kernel inlining this (WALE alone, with checkdt, batched) gives the same bits.
+#pragma clang fp contract(off)
   // kernels.py:311-327
   const double dudx = (U(0, 0, 0) - U(-1, 0, 0)) * rdx;
   const double dudy = (0.25 * (U(0, 1, 0) + U(-1, 1, 0) - U(0, -1, 0) - U(-1, -1, 0))) * rdy;
@@ -501,29 +518,33 @@ __device__ inline double wale_value(const FU& U, const FV& V, const FW& W, doubl
   const double dwdx = (0.25 * (W(1, 0, 0) + W(1, 0, -1) - W(-1, 0, 0) - W(-1, 0, -1))) * rdx;
   const double dwdy = (0.25 * (W(0, 1, 0) + W(0, 1, -1) - W(0, -1, 0) - W(0, -1, -1))) * rdy;
   const double dwdz = (W(0, 0, 0) - W(0, 0, -1)) * rdz;
+  // a*b + c*d + e*f, left to right
+  auto dot3 = [](double a, double b, double c, double d, double e, double f) {
+    return __builtin_fma(e, f, __builtin_fma(c, d, a * b));
+  };
   // kernels.py:330-338, g2zx and g2zy with the reference's last terms
-  const double g2xx = dudx * dudx + dudy * dvdx + dudz * dwdx;
-  const double g2yy = dvdx * dudy + dvdy * dvdy + dvdz * dwdy;
-  const double g2zz = dwdx * dudz + dwdy * dvdz + dwdz * dwdz;
-  const double g2xy = dudx * dudy + dudy * dvdy + dudz * dwdy;
-  const double g2yx = dvdx * dudx + dvdy * dvdx + dvdz * dwdx;
-  const double g2xz = dudx * dudz + dudy * dvdz + dudz * dwdz;
-  const double g2zx = dwdx * dudx + dwdy * dvdx + dwdz * dudz;
-  const double g2yz = dvdx * dudz + dvdy * dvdz + dvdz * dwdz;
-  const double g2zy = dwdx * dudy + dwdy * dvdy + dwdz * dudy;
+  const double g2xx = dot3(dudx, dudx, dudy, dvdx, dudz, dwdx);
+  const double g2yy = dot3(dvdx, dudy, dvdy, dvdy, dvdz, dwdy);
+  const double g2zz = dot3(dwdx, dudz, dwdy, dvdz, dwdz, dwdz);
+  const double g2xy = dot3(dudx, dudy, dudy, dvdy, dudz, dwdy);
+  const double g2yx = dot3(dvdx, dudx, dvdy, dvdx, dvdz, dwdx);
+  const double g2xz = dot3(dudx, dudz, dudy, dvdz, dudz, dwdz);
+  const double g2zx = dot3(dwdx, dudx, dwdy, dvdx, dwdz, dudz);
+  const double g2yz = dot3(dvdx, dudz, dvdy, dvdz, dvdz, dwdz);
+  const double g2zy = dot3(dwdx, dudy, dwdy, dvdy, dwdz, dudy);
   // kernels.py:340-358
   const double tr = (g2xx + g2yy + g2zz) / 3.0;
   const double sdxx = g2xx - tr, sdyy = g2yy - tr, sdzz = g2zz - tr;
   const double sdxy = 0.5 * (g2xy + g2yx);
   const double sdxz = 0.5 * (g2xz + g2zx);
   const double sdyz = 0.5 * (g2yz + g2zy);
-  const double sdsd = sdxx * sdxx + sdyy * sdyy + sdzz * sdzz +
-                      2.0 * (sdxy * sdxy + sdxz * sdxz + sdyz * sdyz);
+  const double sdsd = __builtin_fma(2.0, dot3(sdxy, sdxy, sdxz, sdxz, sdyz, sdyz),
+                                    dot3(sdxx, sdxx, sdyy, sdyy, sdzz, sdzz));
   const double sxy = 0.5 * (dudy + dvdx);
   const double sxz = 0.5 * (dudz + dwdx);
   const double syz = 0.5 * (dvdz + dwdy);
-  const double ss = dudx * dudx + dvdy * dvdy + dwdz * dwdz +
-                    2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+  const double ss = __builtin_fma(2.0, dot3(sxy, sxy, sxz, sxz, syz, syz),
+                                  dot3(dudx, dudx, dvdy, dvdy, dwdz, dwdz));
   // kernels.py:360-367 with x^1.5 = x*sqrt(x), |s|^2.5 = s*s*sqrt(|s|),
   // x^1.25 = x*sqrt(sqrt(x)) (exact identities; a few ulps from pow)
   double nut = 0.0;
@@ -531,7 +552,7 @@ __device__ inline double wale_value(const FU& U, const FV& V, const FW& W, doubl
     const double rs = sqrt(sdsd);
     const double sdsd15 = sdsd * rs;
     const double ass = fabs(ss);
-    const double denom = ass * ass * sqrt(ass) + sdsd * sqrt(rs);
+    const double denom = __builtin_fma(sdsd, sqrt(rs), ass * ass * sqrt(ass));
     nut = ls * ls * sdsd15 / denom;
   }
   return rrey + nut;
@@ -544,18 +565,36 @@ __device__ inline double wale_value(const FU& U, const FV& V, const FW& W, doubl
 constexpr int WS_TJ = 8, WS_TK = 32, WS_RK = WS_TK + 2, WS_RN = (WS_TJ + 2) * WS_RK;
 constexpr int WS_NL = (WS_RN + 255) / 256;  // staged values per thread per field
 
+// CHK: the checkdt reductions of the same cells in the same pass (max |u|,
+// |v|, |w| and min dtvisc with the vis read before it is overwritten), from
+// the values staged for WALE, into part[q * RED_MAX_BLOCKS + slot].
+struct ChkArgs {
+  double dx, dy, dz, rsum;
+  double* part;
+  int slot;
+};
+template <bool CHK = false>
 __device__ inline void wale_stream_body(const double* __restrict__ u,
                                         const double* __restrict__ v,
                                         const double* __restrict__ w, double* __restrict__ vis,
                                         const wf_grid& g, const Box3& b, int ich, int ktile,
                                         int jtile, int chunk, double rdx, double rdy, double rdz,
-                                        double rrey, double ls) {
+                                        double rrey, double ls, ChkArgs ck = ChkArgs{}) {
   __shared__ double S[4][3][WS_RN];
   const int tid = threadIdx.x;
   const int tj = tid / WS_TK, tk = tid % WS_TK;
   const int j0 = jtile * WS_TJ, k0 = ktile * WS_TK;
   const int i0 = chunk * ich, i1 = min(i0 + ich, b.d0);
-  if (i0 >= i1) return;
+  if (i0 >= i1) {  // uniform per workgroup
+    if (CHK && tid == 0) {
+      ck.part[0 * RED_MAX_BLOCKS + ck.slot] = 0.0;
+      ck.part[1 * RED_MAX_BLOCKS + ck.slot] = 0.0;
+      ck.part[2 * RED_MAX_BLOCKS + ck.slot] = 0.0;
+      ck.part[3 * RED_MAX_BLOCKS + ck.slot] = 1.0e10;
+    }
+    return;
+  }
+  double mu = 0.0, mv = 0.0, mw = 0.0, dtv = 1.0e10;
   const int nI = g.ni + 2 * g.npl, nJ = g.nj + 2 * g.npl, nK = g.nk + 2 * g.npl;
   int64_t off[WS_NL];
   int lds[WS_NL];
@@ -598,11 +637,14 @@ __device__ inline void wale_stream_body(const double* __restrict__ u,
   load(i0);
   store(1);
   load(i0 + 1);
+  const int64_t xrow = (int64_t)(b.lo1 + j0 + tj) * g.sj + (b.lo2 + k0 + tk);
   auto step = [&](auto Rc, int i) {
     constexpr int R = decltype(Rc)::value;  // (i - i0) mod 4
     constexpr int SPv = R, S0v = (R + 1) % 4, SNv = (R + 2) % 4;
     store(SNv);  // plane i+1
     if (i + 2 <= i1) load(i + 2);
+    double vold = 0.0;  // this step's vis of the cell, before WALE overwrites it
+    if (CHK && live) vold = vis[plane(i) + xrow];
     __syncthreads();
     if (live) {
       auto F = [&](int f, int di, int dj, int dk) {
@@ -612,8 +654,15 @@ __device__ inline void wale_stream_body(const double* __restrict__ u,
       auto U = [&](int di, int dj, int dk) { return F(0, di, dj, dk); };
       auto V = [&](int di, int dj, int dk) { return F(1, di, dj, dk); };
       auto W = [&](int di, int dj, int dk) { return F(2, di, dj, dk); };
-      vis[plane(i) + (int64_t)(b.lo1 + j0 + tj) * g.sj + (b.lo2 + k0 + tk)] =
-          wale_value(U, V, W, rdx, rdy, rdz, rrey, ls);
+      if constexpr (CHK) {  // checkdt_body's quantities of cell (i, j, k)
+        mu = nan_max(mu, fabs(U(0, 0, 0)));
+        mv = nan_max(mv, fabs(V(0, 0, 0)));
+        mw = nan_max(mw, fabs(W(0, 0, 0)));
+        dtv = nan_min(dtv, dtvisc_val(U(0, 0, 0), U(-1, 0, 0), V(0, 0, 0), V(0, -1, 0),
+                                      W(0, 0, 0), W(0, 0, -1), vold, ck.dx, ck.dy, ck.dz,
+                                      ck.rsum));
+      }
+      vis[plane(i) + xrow] = wale_value(U, V, W, rdx, rdy, rdz, rrey, ls);
     }
   };
   for (int t = 0; t < i1 - i0; t += 4) {
@@ -621,6 +670,22 @@ __device__ inline void wale_stream_body(const double* __restrict__ u,
     if (t + 1 < i1 - i0) step(std::integral_constant<int, 1>{}, i0 + t + 1);
     if (t + 2 < i1 - i0) step(std::integral_constant<int, 2>{}, i0 + t + 2);
     if (t + 3 < i1 - i0) step(std::integral_constant<int, 3>{}, i0 + t + 3);
+  }
+  if constexpr (CHK) {
+    __syncthreads();  // block_max reuses its own LDS; the planes are done with
+    mu = block_max(mu);
+    __syncthreads();
+    mv = block_max(mv);
+    __syncthreads();
+    mw = block_max(mw);
+    __syncthreads();
+    dtv = block_min(dtv);
+    if (tid == 0) {
+      ck.part[0 * RED_MAX_BLOCKS + ck.slot] = mu;
+      ck.part[1 * RED_MAX_BLOCKS + ck.slot] = mv;
+      ck.part[2 * RED_MAX_BLOCKS + ck.slot] = mw;
+      ck.part[3 * RED_MAX_BLOCKS + ck.slot] = dtv;
+    }
   }
 }
 
@@ -633,6 +698,20 @@ wale_stream_kernel(const double* __restrict__ u, const double* __restrict__ v,
   const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
   wale_stream_body(u, v, w, vis, g, b, ich, L % kt, (L / kt) % jt, L / (kt * jt), rdx, rdy, rdz,
                    rrey, ls);
+}
+
+// WALE with the checkdt reductions of the same step in one pass.
+__global__ void __launch_bounds__(256)
+wale_checkdt_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                    const double* __restrict__ w, double* __restrict__ vis, wf_grid g, Box3 b,
+                    int ich, int kt, int jt, double rdx, double rdy, double rdz, double rrey,
+                    double ls, double dx, double dy, double dz, double rsum) {
+  const int nb = kt * jt * gridDim.y;
+  const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
+  wale_stream_body<true>(u, v, w, vis, g, b, ich, L % kt, (L / kt) % jt, L / (kt * jt), rdx,
+                         rdy, rdz, rrey, ls,
+                         ChkArgs{dx, dy, dz, rsum, g_red_part,
+                                 (int)(blockIdx.x + blockIdx.y * gridDim.x)});
 }
 
 __global__ void __launch_bounds__(256)
@@ -654,11 +733,8 @@ __device__ inline double dtvisc_cell(const double* __restrict__ u,
                                      const double* __restrict__ vis, int64_t x,
                                      const wf_grid& g, double dx, double dy, double dz,
                                      double rsum) {
-  const double uc = 0.5 * (u[x] + u[x - g.si]);
-  const double vc = 0.5 * (v[x] + v[x - g.sj]);
-  const double wc = 0.5 * (w[x] + w[x - 1]);
-  return 1.0 / (fabs(uc) / dx + fabs(vc) / dy + fabs(wc) / dz + 2.0 * vis[x] * rsum +
-                1.0e-30);
+  return dtvisc_val(u[x], u[x - g.si], v[x], v[x - g.sj], w[x], w[x - 1], vis[x], dx, dy, dz,
+                    rsum);
 }
 
 __global__ void __launch_bounds__(256)
@@ -1226,6 +1302,32 @@ int wf_checkdt(const double* u, const double* v, const double* w, const double* 
   if (!box_ok(g, b, 1, 0) || !out4_dev || !red_geom(b, r)) return WF_EINVAL;
   hipLaunchKernelGGL(checkdt_kernel, r.grid, box_block(), 0, S_(stream), u, v, w, vis, g, b,
                      r.ich, dx, dy, dz, inv_sq_sum(dx, dy, dz));
+  WF_LAUNCHED();
+  return fold(out4_dev, r, 4, 8, 0, S_(stream));
+}
+
+int wf_wale_checkdt(const double* u, const double* v, const double* w, double* vis,
+                    double* out4_dev, wf_grid g, double dx, double dy, double dz, double rrey,
+                    double cw, double delta, void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 1, 1) || !out4_dev) return WF_EINVAL;
+  if (empty(b)) return 0;
+  const int kt = div_up(b.d2, WS_TK), jt = div_up(b.d1, WS_TJ);
+  const int64_t tiles = (int64_t)kt * jt;
+  int ich = (int)std::max<int64_t>(16, (tiles * b.d0 + 2047) / 2048);
+  if (tiles > RED_MAX_BLOCKS) {  // more tiles than partial slots: the two passes
+    const int e = wf_checkdt(u, v, w, vis, out4_dev, g, dx, dy, dz, stream);
+    if (e) return e;
+    return wf_wale(u, v, w, vis, g, g.ni, g.nj, g.nk, dx, dy, dz, rrey, cw, delta, stream);
+  }
+  ich = std::max(ich, div_up(b.d0, (int)(RED_MAX_BLOCKS / tiles)));
+  RedGeom r;
+  r.grid = dim3((unsigned)tiles, (unsigned)div_up(b.d0, ich));
+  r.ich = ich;
+  r.nblk = (int)(tiles * r.grid.y);
+  hipLaunchKernelGGL(wale_checkdt_kernel, r.grid, dim3(256), 0, S_(stream), u, v, w, vis, g, b,
+                     ich, kt, jt, 1.0 / dx, 1.0 / dy, 1.0 / dz, rrey, cw * delta, dx, dy, dz,
+                     inv_sq_sum(dx, dy, dz));
   WF_LAUNCHED();
   return fold(out4_dev, r, 4, 8, 0, S_(stream));
 }

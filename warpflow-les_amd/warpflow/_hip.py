@@ -104,6 +104,7 @@ _SIGS = {
     "wf_calmas": [_P, _P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _P],
     "wf_calvel3_calmas": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _P],
     "wf_wale": [_P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _D, _D, _P],
+    "wf_wale_checkdt": [_P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _D, _P],
     "wf_maxabs": [_P, _P, _G, _I, _I, _I, _P],
     "wf_dtvisc": [_P, _P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _P],
     "wf_checkdt": [_P, _P, _P, _P, _P, _G, _D, _D, _D, _P],

@@ -138,6 +138,17 @@ def _wale(dim, u, v, w, vis, dx, dy, dz, rrey, cw, npl):
               rrey, cw, wale_delta(dx, dy, dz), current_stream())
 
 
+def _wale_checkdt(dim, u, v, w, vis, out4, dx, dy, dz, rrey, cw, npl):
+    """checkdt's four reductions into out4, then WALE into vis, in one pass
+    over the real cells (same values as the two kernels)."""
+    _same(u, v, w, vis)
+    g = grid_of(vis, npl)
+    if dim != (g.ni, g.nj, g.nk) or out4.size < 4:
+        raise ValueError("wale_checkdt runs over the real cells into a 4-element buffer")
+    _hip.call("wf_wale_checkdt", u.ptr, v.ptr, w.ptr, vis.ptr, out4.ptr, g, dx, dy, dz, rrey,
+              cw, wale_delta(dx, dy, dz), current_stream())
+
+
 def _maxabs(dim, f, out, npl):
     _hip.call("wf_maxabs", f.ptr, out.ptr, grid_of(f, npl), *dim, current_stream())
 
@@ -195,6 +206,7 @@ calvel3 = Kernel("calvel3", _calvel3)
 calmas = Kernel("calmas", _calmas)
 calvel3_calmas = Kernel("calvel3_calmas", _calvel3_calmas)
 wale_viscosity = Kernel("wale_viscosity", _wale)
+wale_checkdt = Kernel("wale_checkdt", _wale_checkdt)
 maxabs_reduce = Kernel("maxabs_reduce", _maxabs)
 dtvisc_reduce = Kernel("dtvisc_reduce", _dtvisc)
 checkdt = Kernel("checkdt", _checkdt)

@@ -280,6 +280,20 @@ class FlowSolver:
         else:
             K.fill_const(self.ncells, f["vis"], self.rey, NPL)
 
+    def _sgs_checkdt(self):
+        """compute_dt (solver.py:122-150) and the WALE SGS step
+        (solver.py:227-237) in one pass; dt is finished as compute_dt does:
+        at once with a variable dt, else with the first calmas read-back."""
+        f = self.f
+        K.wale_checkdt(self.ncells, f["u"], f["v"], f["w"], f["vis"], self._dbuf, self.dx,
+                       self.dy, self.dz, self.rey, self.cfg.physics.sgs_cw, NPL)
+        self.bcs.apply_vis(f["vis"], self.ncells[0], self.rey)
+        if self.cfg.time.variable_dt:
+            r = self._dbuf.numpy()
+            self._finish_dt(float(r[0]), float(r[1]), float(r[2]), float(r[3]))
+        else:
+            self._dt_pending = True
+
     def _rk_stage(self, stage, alpha):
         """solver.py:249-265"""
         f = self.f
@@ -316,9 +330,16 @@ class FlowSolver:
     def step(self):
         """solver.py:217-271"""
         self.itime += 1
-        self.compute_dt(defer=True)
-        self.ctime += self.dt
         f = self.f
+        # fused, with WALE on: checkdt's reductions come out of the WALE pass
+        # (nothing between them changes u, v, w or vis; each cell's vis is
+        # read before WALE overwrites it)
+        sgs_chk = self.fused and self.cfg.physics.sgs_enabled
+        if sgs_chk:
+            self._sgs_checkdt()
+        else:
+            self.compute_dt(defer=True)
+        self.ctime += self.dt
         if not self.fused:
             copy(f["uoo"], f["u"])
             copy(f["voo"], f["v"])
@@ -327,7 +348,8 @@ class FlowSolver:
         # on the operator boxes, the only cells the second stage reads (the
         # copies' other cells are not refreshed); the SGS and fills before it
         # leave those cells of u, v, w as they are here
-        self._sgs()
+        if not sgs_chk:
+            self._sgs()
         for stage, alpha in enumerate(ALFARK):
             self._rk_stage(stage, alpha)
             if stage + 1 < len(ALFARK):
