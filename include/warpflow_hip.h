@@ -202,6 +202,15 @@ int wf_checkdt(const double* u, const double* v, const double* w,
  * dst += src over the real cells plus one ghost ring. */
 int wf_add_ring(double* dst, const double* src, wf_grid g, void* stream);
 
+/* wf_add_ring with the mixed solve's deferred pressure gauge (flags & 4 of
+ * wf_mg_solve_mixed leaves the mean of pp in work[2] unsubtracted): dst +=
+ * src - *shift_dev on the real cells and the faces of the first ghost layer,
+ * dst += src on its edges and corners -- the same bits as subtracting the mean
+ * in the solve and then wf_add_ring.  No single reference kernel corresponds
+ * to it. */
+int wf_add_ring_shift(double* dst, const double* src, const double* shift_dev, wf_grid g,
+                      void* stream);
+
 /* fill_const, kernels.py:442-449: f = val over the launch box. */
 int wf_fill_interior(double* f, double val, wf_grid g, int d0, int d1, int d2,
                      void* stream);
@@ -334,7 +343,9 @@ int wf_mg_mixed_workspace_bytes(wf_grid g, size_t* bytes);
  * V-cycle from zero on r, pp += correction in FP64.  rtol <= 0: exactly
  * max_cycles cycles, no host synchronisation.  rtol > 0: stop once
  * max|r| <= rtol*max|su| (one host read per cycle), at most max_cycles.
- * flags & 2: subtract the FP64 mean of pp at the end (pressure gauge).  The
+ * flags & 2: subtract the FP64 mean of pp at the end (pressure gauge); with
+ * flags & 4 as well, only form that mean in work[2] and leave pp unshifted
+ * (for wf_add_ring_shift).  The
  * first ghost layer of pp gets the Neumann face mirror on its faces and zero
  * on its edges and corners; the deeper ghost layers are not written (the
  * reference's solve leaves them zero: keep them zero, as a freshly allocated

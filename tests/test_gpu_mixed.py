@@ -141,3 +141,28 @@ def test_solve_takes_the_zero_start_unread(gpu, mixed):
     pp2 = array(dirty)
     mg.solve(array(su), pp2, 3)
     assert np.array_equal(pp2.numpy(), want)
+
+
+def test_deferred_gauge_mean_gives_the_same_p(gpu):
+    """solve(defer_mean=True) + add_result (the shift of pp applied inside
+    p += pp) leaves p bit for bit as solve + add_inplace of the shifted pp."""
+    from warpflow import kernels as K
+    from warpflow.device import array
+    from warpflow.mixed import MixedMultigrid
+    n3 = (32, 24, 40)
+    h = tuple(1.0 / n for n in n3)
+    shape = tuple(n + 6 for n in n3)
+    rng = np.random.default_rng(8)
+    su = rng.standard_normal(shape)
+    p0 = rng.standard_normal(shape)
+    mg = MixedMultigrid(_Cfg(3), topo_for(n3), None, *h, max_cycles=2)
+    assert mg.result_mean
+    pp, p = array(np.zeros(shape)), array(p0)
+    mg.solve(array(su), pp, 3)
+    K.add_inplace(tuple(n + 2 for n in n3), p, pp, 3)
+    want = p.numpy()
+    pp2, p2 = array(np.zeros(shape)), array(p0)
+    mg.solve(array(su), pp2, 3, defer_mean=True)
+    mg.add_result(p2, pp2, 3)
+    assert np.array_equal(p2.numpy(), want)
+    assert not np.array_equal(pp2.numpy(), pp.numpy())  # pp itself unshifted

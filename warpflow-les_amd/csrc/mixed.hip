@@ -26,7 +26,9 @@
 
 namespace wf {
 
-enum : int { MIX_DEFECT_MEAN = 1, MIX_RESULT_MEAN = 2 };
+// MIX_MEAN_DEFERRED (with MIX_RESULT_MEAN): the mean of x is formed into
+// work[2] but not subtracted; the caller applies it (wf_add_ring_shift).
+enum : int { MIX_DEFECT_MEAN = 1, MIX_RESULT_MEAN = 2, MIX_MEAN_DEFERRED = 4 };
 
 constexpr int WORK_HEAD = 4;  // work[0] mean, work[1] max|r|, work[2] sum, work[3] spare
 
@@ -549,9 +551,11 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
       r = field_sum(pp, g, 1.0 / ncell, work, xmean, s);
       if (r) return r;
     }
-    hipLaunchKernelGGL(shift_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g, xmean,
-                       field_ich(g));
-    WF_LAUNCHED();
+    if (!(flags & MIX_MEAN_DEFERRED)) {
+      hipLaunchKernelGGL(shift_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g, xmean,
+                         field_ich(g));
+      WF_LAUNCHED();
+    }
   }
   hipLaunchKernelGGL(field_ring_kernel, dim3(div_up(field_ring_cells(g), 256)), dim3(256), 0, s,
                      pp, g);

@@ -798,6 +798,29 @@ __device__ inline void add_body(double* __restrict__ dst, const double* __restri
   const int64_t x = at(g, i, j, k);
   dst[x] = dst[x] + src[x];
 }
+// dst += src - *shift on the real cells and the faces of the first ghost
+// layer, dst += src on its edges and corners: p += pp with the mixed
+// solve's pressure-gauge shift of pp applied on the fly (the solve leaves its
+// edges and corners zero and mirrors the shifted cells onto the faces, so
+// this equals shifting pp and then adding, bit for bit).
+__device__ inline void add_shift_body(double* __restrict__ dst, const double* __restrict__ src,
+                                      const double* __restrict__ shift, const wf_grid& g,
+                                      const Box3& b, Blk B) {
+  int i, j, k;
+  if (!box_thread(b, B, i, j, k)) return;
+  const int64_t x = at(g, i, j, k);
+  const int lo = g.npl - 1;
+  const int nring = (i == lo || i == g.npl + g.ni) + (j == lo || j == g.npl + g.nj) +
+                    (k == lo || k == g.npl + g.nk);
+  const double v = src[x];
+  dst[x] = dst[x] + (nring <= 1 ? v - *shift : v);
+}
+__global__ void __launch_bounds__(256)
+add_shift_kernel(double* __restrict__ dst, const double* __restrict__ src,
+                 const double* __restrict__ shift, wf_grid g, Box3 b, int ich) {
+  for_planes(b.d0, ich, [&](Blk B) { add_shift_body(dst, src, shift, g, b, B); });
+}
+
 __global__ void __launch_bounds__(256)
 add_kernel(double* __restrict__ dst, const double* __restrict__ src, wf_grid g, Box3 b, int ich) {
   for_planes(b.d0, ich, [&](Blk B) { add_body(dst, src, g, b, B); });
@@ -1337,6 +1360,17 @@ int wf_add_ring(double* dst, const double* src, wf_grid g, void* stream) {
   const Box3 b{g.npl - 1, g.npl - 1, g.npl - 1, g.ni + 2, g.nj + 2, g.nk + 2};
   if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
   hipLaunchKernelGGL(add_kernel, CG(b), box_block(), 0, S_(stream), dst, src, g, b, CH(b));
+  WF_LAUNCHED();
+  return 0;
+}
+
+int wf_add_ring_shift(double* dst, const double* src, const double* shift_dev, wf_grid g,
+                      void* stream) {
+  if (g.npl < 1 || !shift_dev) return WF_EINVAL;
+  const Box3 b{g.npl - 1, g.npl - 1, g.npl - 1, g.ni + 2, g.nj + 2, g.nk + 2};
+  if (!box_ok(g, b, 0, 0)) return WF_EINVAL;
+  hipLaunchKernelGGL(add_shift_kernel, CG(b), box_block(), 0, S_(stream), dst, src, shift_dev,
+                     g, b, CH(b));
   WF_LAUNCHED();
   return 0;
 }

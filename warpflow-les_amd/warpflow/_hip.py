@@ -109,6 +109,7 @@ _SIGS = {
     "wf_dtvisc": [_P, _P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _P],
     "wf_checkdt": [_P, _P, _P, _P, _P, _G, _D, _D, _D, _P],
     "wf_add_ring": [_P, _P, _G, _P],
+    "wf_add_ring_shift": [_P, _P, _P, _G, _P],
     "wf_fill_interior": [_P, _D, _G, _I, _I, _I, _P],
     "wf_init_noise": [_P, _P, _P, _D, C.c_int32, _G, _I, _I, _I, _P],
     # bcs.py

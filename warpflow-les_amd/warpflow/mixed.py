@@ -44,6 +44,17 @@ class MixedMultigrid(Multigrid):
         self.last_cycles = 0
         self.last_residual = 0.0
 
+    def add_result(self, p3: DeviceArray, pp3: DeviceArray, npl: int):
+        """p += pp over the real cells and the first ghost layer (add_inplace),
+        with the gauge mean of a ``solve(defer_mean=True)`` subtracted on the
+        way: the same p as a demeaned pp added, without rewriting pp."""
+        g = grid_of(pp3, npl)
+        if not getattr(self, "_deferred", False):
+            _hip.call("wf_add_ring", p3.ptr, pp3.ptr, g, current_stream())
+            return
+        _hip.call("wf_add_ring_shift", p3.ptr, pp3.ptr, self._work.ptr + 2 * 8, g,
+                  current_stream())
+
     def _workspace(self, g):
         nbytes = C.c_size_t(0)
         _hip.call("wf_mg_mixed_workspace_bytes", g, C.byref(nbytes))
@@ -51,7 +62,9 @@ class MixedMultigrid(Multigrid):
             self._work = zeros((nbytes.value + 7) // 8, np.float64, self.device)
         return self._work
 
-    def solve(self, su3: DeviceArray, pp3: DeviceArray, npl: int):
+    def solve(self, su3: DeviceArray, pp3: DeviceArray, npl: int, defer_mean: bool = False):
+        """pp3 = solution for su3.  defer_mean (with result_mean): leave the
+        gauge mean unsubtracted and apply it in ``add_result`` instead."""
         g = grid_of(pp3, npl)
         if su3.shape != pp3.shape or su3.pitch != pp3.pitch or su3.lead != pp3.lead:
             raise ValueError("su and pp must have one shape and memory layout")
@@ -60,6 +73,9 @@ class MixedMultigrid(Multigrid):
         cycles = C.c_int(0)
         res = C.c_double(0.0)
         flags = (1 if self.defect_mean else 0) | (2 if self.result_mean else 0)
+        self._deferred = bool(defer_mean and self.result_mean)
+        if self._deferred:
+            flags |= 4
         ncyc = self.n_vcycles if self.max_cycles is None else int(self.max_cycles)
         _hip.call("wf_mg_solve_mixed", descs, len(self.levels), su3.ptr, pp3.ptr, g,
                   self.n_pre, self.n_post, self.n_coarse, ncyc, self.rtol, flags,

@@ -252,8 +252,14 @@ class FlowSolver:
             if self.rmax > 10.0:
                 raise RuntimeError("projection diverged: rmax = %.6e at itime %d"
                                    % (self.rmax, self.itime))
-            self.mg.solve(f["su"], f["pp"], NPL)
-            K.add_inplace(tuple(n + 2 for n in self.ncells), f["p"], f["pp"], NPL)
+            if self.fused and hasattr(self.mg, "add_result"):
+                # mixed form: the pressure-gauge shift of pp is applied inside
+                # p += pp (pp itself is left unshifted; p gets the same bits)
+                self.mg.solve(f["su"], f["pp"], NPL, defer_mean=True)
+                self.mg.add_result(f["p"], f["pp"], NPL)
+            else:
+                self.mg.solve(f["su"], f["pp"], NPL)
+                K.add_inplace(tuple(n + 2 for n in self.ncells), f["p"], f["pp"], NPL)
             self.halo.exchange(f["p"])
             self.bcs.apply_pbound(f["p"], self.ncells[0])
             if not self.fused:
