@@ -80,7 +80,17 @@ __device__ inline void fold_body(const double* __restrict__ partials, int64_t n,
   __shared__ double s[1024];
   const int t = threadIdx.x;
   double acc = 0.0;
-  for (int64_t q = t; q < n; q += 1024) acc += partials[q];
+  // the same additions in the same order as one load per step, with eight
+  // loads issued ahead (the fold was bound by one load latency per partial)
+  int64_t q = t;
+  for (; q + 7 * 1024 < n; q += 8 * 1024) {
+    double a[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) a[e] = partials[q + e * 1024];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc += a[e];
+  }
+  for (; q < n; q += 1024) acc += partials[q];
   s[t] = acc;
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
