@@ -158,12 +158,15 @@ int wf_calmas(const double* u, const double* v, const double* w, double* su,
  * kernels.py:232-264 then calmas kernels.py:271-289; the halo exchange
  * between them is a no-op on one domain).  Writes u, v, w, su and
  * *rmax_dev = max(*rmax_dev, max|div*dx*dy*dz|) bit for bit as the two
- * entries do; calmas no longer re-reads u, v, w from memory.  No single
- * reference kernel corresponds to it. */
+ * entries do; calmas no longer re-reads u, v, w from memory.  rhs32
+ * (nullable): an FP32 multigrid level array of the real cells ((n+2)^3,
+ * one-cell ring) that also receives (float)su on its real cells -- the
+ * first defect of wf_mg_solve_mixed (flags & 8).  No single reference
+ * kernel corresponds to it. */
 int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* wstar,
                       const double* p, double* u, double* v, double* w, double* su,
-                      double* rmax_dev, wf_grid g, double dx, double dy, double dz,
-                      double dt, double alpha, void* stream);
+                      double* rmax_dev, float* rhs32, wf_grid g, double dx, double dy,
+                      double dz, double dt, double alpha, void* stream);
 
 /* wale_viscosity, kernels.py:297-367: vis = rrey + nut (WALE, cw), incl. the
  * reference's g2zx/g2zy terms (kernels.py:336, :338).  delta is
@@ -345,7 +348,9 @@ int wf_mg_mixed_workspace_bytes(wf_grid g, size_t* bytes);
  * max|r| <= rtol*max|su| (one host read per cycle), at most max_cycles.
  * flags & 2: subtract the FP64 mean of pp at the end (pressure gauge); with
  * flags & 4 as well, only form that mean in work[2] and leave pp unshifted
- * (for wf_add_ring_shift).  The
+ * (for wf_add_ring_shift).  flags & 8: levels[0].rhs already holds (float)su
+ * on the real cells (wf_calvel3_calmas' rhs32), so the first defect is not
+ * formed again -- honoured with a fixed cycle count and no defect mean.  The
  * first ghost layer of pp gets the Neumann face mirror on its faces and zero
  * on its edges and corners; the deeper ghost layers are not written (the
  * reference's solve leaves them zero: keep them zero, as a freshly allocated

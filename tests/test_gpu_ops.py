@@ -152,6 +152,32 @@ def test_calvel3_calmas_one_pass(gpu, n3):
     assert abs(rb.numpy()[0] - rmax_ref) <= TOL * rmax_ref
 
 
+@pytest.mark.parametrize("n3", [(20, 19, 37), (32, 32, 32)])
+def test_calvel3_calmas_writes_the_fp32_rhs(gpu, n3):
+    """With an FP32 level array the fused pass also stores (float)su on its real
+    cells (the mixed solve's first defect) and leaves its ring alone; u, v, w,
+    su and rmax are unchanged by it."""
+    from warpflow import kernels as K
+    from warpflow.device import array, zeros
+    h = (0.031, 0.027, 0.019)
+    f = rand_fields(n3, ["us", "vs", "ws", "p", "u", "v", "w", "su"], seed=11)
+    a = {k: up(v) for k, v in f.items()}
+    b = {k: up(v) for k, v in f.items()}
+    r0 = np.random.default_rng(3).standard_normal(tuple(n + 2 for n in n3)).astype(np.float32)
+    rhs = array(r0, np.float32)
+    ra, rb = zeros(1), zeros(1)
+    K.calvel3_calmas(n3, a["us"], a["vs"], a["ws"], a["p"], a["u"], a["v"], a["w"], a["su"],
+                     ra, *h, 7e-4, 1.0, N)
+    K.calvel3_calmas(n3, b["us"], b["vs"], b["ws"], b["p"], b["u"], b["v"], b["w"], b["su"],
+                     rb, *h, 7e-4, 1.0, N, rhs)
+    for nm in ("u", "v", "w", "su"):
+        assert np.array_equal(a[nm].numpy(), b[nm].numpy()), nm
+    assert ra.numpy()[0] == rb.numpy()[0]
+    want = r0.copy()
+    want[1:-1, 1:-1, 1:-1] = b["su"].numpy()[N:-N, N:-N, N:-N].astype(np.float32)
+    assert np.array_equal(rhs.numpy(), want)
+
+
 @pytest.mark.parametrize("n3", SHAPES + [(20, 19, 37), (9, 17, 70)])
 def test_wale_checkdt_one_pass(gpu, n3):
     """checkdt's reductions and WALE in one pass equal checkdt then WALE bit for

@@ -28,7 +28,9 @@ namespace wf {
 
 // MIX_MEAN_DEFERRED (with MIX_RESULT_MEAN): the mean of x is formed into
 // work[2] but not subtracted; the caller applies it (wf_add_ring_shift).
-enum : int { MIX_DEFECT_MEAN = 1, MIX_RESULT_MEAN = 2, MIX_MEAN_DEFERRED = 4 };
+// MIX_RHS_READY: levels[0].rhs already holds (float)b on the real cells.
+enum : int { MIX_DEFECT_MEAN = 1, MIX_RESULT_MEAN = 2, MIX_MEAN_DEFERRED = 4,
+             MIX_RHS_READY = 8 };
 
 constexpr int WORK_HEAD = 4;  // work[0] mean, work[1] max|r|, work[2] sum, work[3] spare
 
@@ -516,11 +518,15 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
   for (int c = 0; c <= max_cycles; ++c) {
     const bool last_check = c == max_cycles;
     if (last_check && !tol_mode) break;
-    hipLaunchKernelGGL(defect_kernel, field_cgrid(g), dim3(64, 4), 0, s, su, c == 0 ? nullptr : pp,
-                       g, (float*)L0.rhs, d0, c0, c1, c2,
-                       (flags & MIX_DEFECT_MEAN) ? mean : nullptr,
-                       tol_mode ? work + WORK_HEAD : nullptr, field_ich(g));
-    WF_LAUNCHED();
+    const bool rhs_ready =
+        c == 0 && (flags & MIX_RHS_READY) && !tol_mode && !(flags & MIX_DEFECT_MEAN);
+    if (!rhs_ready) {
+      hipLaunchKernelGGL(defect_kernel, field_cgrid(g), dim3(64, 4), 0, s, su,
+                         c == 0 ? nullptr : pp, g, (float*)L0.rhs, d0, c0, c1, c2,
+                         (flags & MIX_DEFECT_MEAN) ? mean : nullptr,
+                         tol_mode ? work + WORK_HEAD : nullptr, field_ich(g));
+      WF_LAUNCHED();
+    }
     if (tol_mode) {
       hipLaunchKernelGGL(max_fold_kernel, dim3(1), dim3(1024), 0, s, work + WORK_HEAD,
                          field_blocks(g), rmax);

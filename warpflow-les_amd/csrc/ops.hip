@@ -316,7 +316,8 @@ __device__ inline double calvel3_calmas_body(
     const double* __restrict__ us, const double* __restrict__ vs,
     const double* __restrict__ ws, const double* __restrict__ p, double* __restrict__ u,
     double* __restrict__ v, double* __restrict__ w, double* __restrict__ su, const wf_grid& g,
-    int ich, int ktile, int jtile, int chunk, const pj::Consts& C) {
+    int ich, int ktile, int jtile, int chunk, const pj::Consts& C,
+    float* __restrict__ r32 = nullptr) {
   using T = pj::Tile<TJ, TK>;
   __shared__ double Vs[2][T::VR * T::VW];
   __shared__ double Ws[2][TJ * T::WW];
@@ -451,7 +452,11 @@ __device__ inline double calvel3_calmas_body(
                d4_bracket(W[T::wix(tj, tk + 1)], W[T::wix(tj, tk)], W[T::wix(tj, tk - 1)],
                           W[T::wix(tj, tk - 2)]),
                C.r24x, C.r24y, C.r24z);
-      su[plane(i) + xo] = fact * C.rdt;
+      const double sv = fact * C.rdt;
+      su[plane(i) + xo] = sv;
+      if (r32)  // (n+2)^3 level array, one-cell ring: real cell (i, J, K) - npl + 1
+        r32[((int64_t)(i - npl + 1) * (g.nj + 2) + (J - npl + 1)) * (g.nk + 2) + (K - npl + 1)] =
+            (float)sv;
       val = nan_max(val, fabs(fact * C.dx * C.dy * C.dz));
     }
   }
@@ -465,11 +470,12 @@ __global__ void __launch_bounds__(256)
 calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ vs,
                       const double* __restrict__ ws, const double* __restrict__ p,
                       double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-                      double* __restrict__ su, wf_grid g, int ich, int kt, int jt, pj::Consts K) {
+                      double* __restrict__ su, wf_grid g, int ich, int kt, int jt, pj::Consts K,
+                      float* __restrict__ r32) {
   const int nb = kt * jt * (int)gridDim.y;
   const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
   const double r = calvel3_calmas_body<PJ_TJ, PJ_TK>(us, vs, ws, p, u, v, w, su, g, ich, L % kt,
-                                                      (L / kt) % jt, L / (kt * jt), K);
+                                                      (L / kt) % jt, L / (kt * jt), K, r32);
   if (threadIdx.x == 0) g_red_part[blockIdx.x + blockIdx.y * gridDim.x] = r;
 }
 
@@ -1243,14 +1249,15 @@ int wf_calmas(const double* u, const double* v, const double* w, double* su,
 
 int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* wstar,
                       const double* p, double* u, double* v, double* w, double* su,
-                      double* rmax_dev, wf_grid g, double dx, double dy, double dz, double dt,
-                      double alpha, void* stream) {
+                      double* rmax_dev, float* rhs32, wf_grid g, double dx, double dy,
+                      double dz, double dt, double alpha, void* stream) {
   const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
   if (!box_ok(g, b, 2, 1) || !rmax_dev) return WF_EINVAL;
   if (empty(b)) return 0;
   const int kt = div_up(g.nk, PJ_TK), jt = div_up(g.nj, PJ_TJ);
   const int64_t tiles = (int64_t)kt * jt;
   if (tiles > RED_MAX_BLOCKS) {  // more tiles than partial slots: the two passes
+    if (rhs32) return WF_EINVAL;  // the two-pass form has no FP32 rhs output
     const int e = wf_calvel3(ustar, vstar, wstar, p, u, v, w, g, dx, dy, dz, dt, alpha, stream);
     if (e) return e;
     return wf_calmas(u, v, w, su, rmax_dev, g, g.ni, g.nj, g.nk, dx, dy, dz, dt, stream);
@@ -1268,7 +1275,7 @@ int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* ws
                      1.0 / (24.0 * dx), 1.0 / (24.0 * dy), 1.0 / (24.0 * dz),
                      1.0 / dt, dx, dy, dz};
   hipLaunchKernelGGL(calvel3_calmas_kernel, r.grid, dim3(PJ_TJ * PJ_TK), 0, S_(stream), ustar,
-                     vstar, wstar, p, u, v, w, su, g, ich, kt, jt, K);
+                     vstar, wstar, p, u, v, w, su, g, ich, kt, jt, K, rhs32);
   WF_LAUNCHED();
   return fold(rmax_dev, r, 1, 0, 1, S_(stream));
 }

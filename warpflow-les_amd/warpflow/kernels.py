@@ -16,6 +16,8 @@ four reductions of ``FlowSolver.compute_dt``).
 """
 from __future__ import annotations
 
+import numpy as np
+
 from . import _hip
 from .device import DeviceArray, current_stream, grid_of
 
@@ -116,15 +118,22 @@ def _calmas(dim, u, v, w, su, rmax, dx, dy, dz, dt, npl):
 
 
 def _calvel3_calmas(dim, ustar, vstar, wstar, p, u, v, w, su, rmax, dx, dy, dz, dt, alpha,
-                    npl):
+                    npl, rhs32=None):
     """calvel3 then calmas over the real cells in one pass (``_project`` runs
-    them back to back, solver.py:186-197); same results as the two kernels."""
+    them back to back, solver.py:186-197); same results as the two kernels.
+    rhs32: an FP32 (n+2)^3 multigrid level array that also receives (float)su
+    on its real cells."""
     _same(ustar, vstar, wstar, p, u, v, w, su)
     g = grid_of(u, npl)
     if dim != (g.ni, g.nj, g.nk):
         raise ValueError("calvel3_calmas runs over the real cells (dim = ncells)")
+    if rhs32 is not None and (rhs32.shape != tuple(n + 2 for n in dim)
+                              or np.dtype(rhs32.dtype) != np.float32 or rhs32.pitch
+                              != rhs32.shape[2] or rhs32.lead != 0):
+        raise ValueError("rhs32 must be a dense FP32 (n+2)^3 level array")
     _hip.call("wf_calvel3_calmas", ustar.ptr, vstar.ptr, wstar.ptr, p.ptr, u.ptr, v.ptr,
-              w.ptr, su.ptr, rmax.ptr, g, dx, dy, dz, dt, alpha, current_stream())
+              w.ptr, su.ptr, rmax.ptr, rhs32.ptr if rhs32 is not None else None, g, dx, dy,
+              dz, dt, alpha, current_stream())
 
 
 def wale_delta(dx, dy, dz):

@@ -62,9 +62,12 @@ class MixedMultigrid(Multigrid):
             self._work = zeros((nbytes.value + 7) // 8, np.float64, self.device)
         return self._work
 
-    def solve(self, su3: DeviceArray, pp3: DeviceArray, npl: int, defer_mean: bool = False):
+    def solve(self, su3: DeviceArray, pp3: DeviceArray, npl: int, defer_mean: bool = False,
+              rhs_ready: bool = False):
         """pp3 = solution for su3.  defer_mean (with result_mean): leave the
-        gauge mean unsubtracted and apply it in ``add_result`` instead."""
+        gauge mean unsubtracted and apply it in ``add_result`` instead.
+        rhs_ready: levels[0].rhs already holds (float)su3 on the real cells
+        (written by calvel3_calmas), so the first defect is not formed again."""
         g = grid_of(pp3, npl)
         if su3.shape != pp3.shape or su3.pitch != pp3.pitch or su3.lead != pp3.lead:
             raise ValueError("su and pp must have one shape and memory layout")
@@ -76,6 +79,8 @@ class MixedMultigrid(Multigrid):
         self._deferred = bool(defer_mean and self.result_mean)
         if self._deferred:
             flags |= 4
+        if rhs_ready:
+            flags |= 8
         ncyc = self.n_vcycles if self.max_cycles is None else int(self.max_cycles)
         _hip.call("wf_mg_solve_mixed", descs, len(self.levels), su3.ptr, pp3.ptr, g,
                   self.n_pre, self.n_post, self.n_coarse, ncyc, self.rtol, flags,

@@ -221,9 +221,12 @@ class FlowSolver:
         nothing on one domain and is issued afterwards."""
         f = self.f
         self._rbuf.zero_()
+        # mixed form: the pass also writes (float)su into the FP32 level-0 rhs,
+        # the first defect of the solve that may follow
+        rhs32 = self.mg.levels[0].rhs if hasattr(self.mg, "add_result") else None
         K.calvel3_calmas(self.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"], f["u"],
                          f["v"], f["w"], f["su"], self._rbuf, self.dx, self.dy, self.dz,
-                         self.dt, 1.0, NPL)
+                         self.dt, 1.0, NPL, rhs32)
         for name in ("u", "v", "w"):
             self.halo.exchange(f[name])
         self._read_calmas()
@@ -255,7 +258,7 @@ class FlowSolver:
             if self.fused and hasattr(self.mg, "add_result"):
                 # mixed form: the pressure-gauge shift of pp is applied inside
                 # p += pp (pp itself is left unshifted; p gets the same bits)
-                self.mg.solve(f["su"], f["pp"], NPL, defer_mean=True)
+                self.mg.solve(f["su"], f["pp"], NPL, defer_mean=True, rhs_ready=True)
                 self.mg.add_result(f["p"], f["pp"], NPL)
             else:
                 self.mg.solve(f["su"], f["pp"], NPL)
