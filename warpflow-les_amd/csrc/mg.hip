@@ -1107,6 +1107,11 @@ coarse_op_build_kernel(T* __restrict__ M, MgDev d, T c0, T c1, T c2, int nsweeps
 }
 
 // phi = M rhs on the real cells of a small level, plus face-ring images.
+// acc + m * r with one rounding, as both coarse-operator kernels accumulate
+__device__ inline float cop_fma(float m, float r, float acc) { return __builtin_fmaf(m, r, acc); }
+__device__ inline double cop_fma(double m, double r, double acc) {
+  return __builtin_fma(m, r, acc);
+}
 template <class T>
 __device__ inline void coarse_op_apply_body(const T* __restrict__ M, T* __restrict__ phi,
                                             const T* __restrict__ rhs, const MgDev& d, T* r) {
@@ -1128,21 +1133,48 @@ __device__ inline void coarse_op_apply_body(const T* __restrict__ M, T* __restri
 #pragma unroll
       for (int q = 0; q < 16; ++q) m[q] = M[(size_t)(col + q) * ncell + row];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) acc = acc + m[q] * r[col + q];
+      for (int q = 0; q < 16; ++q) acc = cop_fma(m[q], r[col + q], acc);
     }
-    for (; col < ncell; ++col) acc = acc + M[(size_t)col * ncell + row] * r[col];
+    for (; col < ncell; ++col) acc = cop_fma(M[(size_t)col * ncell + row], r[col], acc);
     const int k = row % n2, j = (row / n2) % n1, i = row / (n1 * n2);
     const int64_t x = mg_at(d, i, j, k);
     phi[x] = acc;
     ring_images(phi, d, i, j, k, x, acc);
   }
 }
+// The rows of coarse_op_apply_body spread over one 64-thread workgroup per 64 rows, so that the
+// matrix streams through many CUs instead of one; each row is still one
+// thread's serial sum in column order (the bits of coarse_op_apply_body).
 template <class T>
-__global__ void __launch_bounds__(1024)
-coarse_op_apply_kernel(const T* __restrict__ M, T* __restrict__ phi, const T* __restrict__ rhs,
-                       MgDev d) {
+__global__ void __launch_bounds__(64)
+coarse_op_apply_rows_kernel(const T* __restrict__ M, T* __restrict__ phi,
+                            const T* __restrict__ rhs, MgDev d) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  coarse_op_apply_body<T>(M, phi, rhs, d, reinterpret_cast<T*>(smem_raw));
+  T* r = reinterpret_cast<T*>(smem_raw);
+  const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
+  const int ncell = n0 * n1 * n2;
+  const int tid = threadIdx.x;
+  for (int l = tid; l < ncell; l += 64) {
+    const int k = l % n2, j = (l / n2) % n1, i = l / (n1 * n2);
+    r[l] = rhs[mg_at(d, i, j, k)];
+  }
+  __syncthreads();
+  const int row = blockIdx.x * 64 + tid;
+  if (row >= ncell) return;
+  T acc = T(0);
+  int col = 0;
+  for (; col + 16 <= ncell; col += 16) {
+    T m[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) m[q] = M[(size_t)(col + q) * ncell + row];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc = cop_fma(m[q], r[col + q], acc);
+  }
+  for (; col < ncell; ++col) acc = cop_fma(M[(size_t)col * ncell + row], r[col], acc);
+  const int k = row % n2, j = (row / n2) % n1, i = row / (n1 * n2);
+  const int64_t x = mg_at(d, i, j, k);
+  phi[x] = acc;
+  ring_images(phi, d, i, j, k, x, acc);
 }
 
 __global__ void __launch_bounds__(256)
@@ -1694,13 +1726,12 @@ int coarse_op_apply_t(const wf_mg_level& L, hipStream_t s) {
   if (ncell > COARSE_OP_MAX_CELLS || bytes > (size_t)max_lds()) return WF_EINVAL;
   static bool attr_set = false;
   if (!attr_set) {
-    WF_TRY(hipFuncSetAttribute((const void*)coarse_op_apply_kernel<T>,
+    WF_TRY(hipFuncSetAttribute((const void*)coarse_op_apply_rows_kernel<T>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, max_lds()));
     attr_set = true;
   }
-  const int nt = ncell >= 1024 ? 1024 : ((ncell + 63) / 64) * 64;
-  hipLaunchKernelGGL(coarse_op_apply_kernel<T>, dim3(1), dim3(nt), bytes, s, (const T*)L.cop,
-                     (T*)L.phi, (const T*)L.rhs, d);
+  hipLaunchKernelGGL(coarse_op_apply_rows_kernel<T>, dim3((unsigned)div_up(ncell, 64)),
+                     dim3(64), bytes, s, (const T*)L.cop, (T*)L.phi, (const T*)L.rhs, d);
   WF_LAUNCHED();
   return 0;
 }
