@@ -188,3 +188,28 @@ def test_solve_with_the_first_defect_supplied(gpu):
     pp2 = array(np.zeros(shape))
     mg.solve(array(su), pp2, 3, rhs_ready=True)
     assert np.array_equal(pp2.numpy(), want)
+
+
+@pytest.mark.parametrize("defect_mean", [False, True])
+def test_unrolled_defect_matches_the_plane_walk(gpu, defect_mean):
+    """The fixed-cycle solve forms its later defects four planes per load
+    group; the tolerance mode forms every defect plane by plane (with the
+    residual maxima).  Run to the same cycle count they give pp bit for bit.
+    172 x 252 x 232 cells: 5-plane chunks (a full and a one-plane load group)
+    and a 2-plane last chunk, the last k tile ragged."""
+    from warpflow.device import array
+    from warpflow.mixed import MixedMultigrid
+    n3 = (172, 252, 232)
+    h = tuple(1.0 / n for n in n3)
+    shape = tuple(n + 6 for n in n3)
+    su = np.random.default_rng(21).standard_normal(shape)
+    kw = dict(defect_mean=defect_mean, result_mean=False)
+    fixed = MixedMultigrid(_Cfg(3), topo_for(n3), None, *h, max_cycles=3, **kw)
+    pp = array(np.zeros(shape))
+    fixed.solve(array(su), pp, 3)
+    assert fixed.last_cycles == 3
+    tol = MixedMultigrid(_Cfg(3), topo_for(n3), None, *h, rtol=1e-300, max_cycles=3, **kw)
+    pp2 = array(np.zeros(shape))
+    tol.solve(array(su), pp2, 3)
+    assert tol.last_cycles == 3
+    assert np.array_equal(pp2.numpy(), pp.numpy())

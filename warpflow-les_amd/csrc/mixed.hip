@@ -108,6 +108,22 @@ fold_kernel(const double* __restrict__ partials, int64_t n, double scale, double
 
 __global__ void set_scalar_kernel(double* p, double v) { *p = v; }
 
+// b - A x at one cell from x's centre value and its six neighbours, with
+// fixed roundings (no contraction), so that the plane-walking and the
+// unrolled defect kernels, single-domain or batched, agree bit for
+// bit.
+__device__ inline double defect_val(double b, const Coef<double>& c, double xc, double xw,
+                                    double xe, double xs, double xn, double xb, double xt) {
+#pragma clang fp contract(off)
+  double nb = c.aw * xw;
+  nb = __builtin_fma(c.ae, xe, nb);
+  nb = __builtin_fma(c.as, xs, nb);
+  nb = __builtin_fma(c.an, xn, nb);
+  nb = __builtin_fma(c.ab, xb, nb);
+  nb = __builtin_fma(c.at, xt, nb);
+  return b - __builtin_fma(c.ap, xc, nb);
+}
+
 // r = b - A x (FP64; x == nullptr means x = 0), minus *mean if given, cast to
 // FP32 into the level-0 rhs; rmax (nullable): each workgroup's max|b - A x|
 // into rmax[workgroup].
@@ -123,9 +139,8 @@ __device__ inline void defect_body(const double* __restrict__ b, const double* _
     double r = b[q];
     if (x) {
       const Coef<double> c = mg_coef(d, i, j, k, c0, c1, c2);
-      const double nb = c.aw * x[q - g.si] + c.ae * x[q + g.si] + c.as * x[q - g.sj] +
-                        c.an * x[q + g.sj] + c.ab * x[q - 1] + c.at * x[q + 1];
-      r = r - (nb + c.ap * x[q]);
+      r = defect_val(r, c, x[q], x[q - g.si], x[q + g.si], x[q - g.sj], x[q + g.sj], x[q - 1],
+                     x[q + 1]);
     }
     av = fabs(r);
     if (mean) r = r - *mean;
@@ -146,6 +161,52 @@ defect_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_gri
               const double* __restrict__ mean, double* __restrict__ rmax, int ich) {
   for_planes<true>(g.ni, ich,
                    [&](Blk B) { defect_body(b, x, g, r32, d, c0, c1, c2, mean, rmax, B); });
+}
+
+// The same defect (x given, no rmax), DEFECT_U planes at a time: a thread
+// issues every load of its (j, k) column for U planes -- x at planes i0-1 ..
+// i0+U, b and the four in-plane neighbours of x at each plane -- before the
+// first use, so U planes of HBM reads are in flight per wait (the
+// plane-at-a-time walk waits on each plane's loads in turn).  Plane indices
+// past the chunk are clamped onto its last plane (x: onto plane z1, real or
+// the first ghost plane), so every load stays inside the field; their values
+// are not used.  Same addresses and the same bits as defect_kernel.
+constexpr int DEFECT_U = 4;
+__global__ void __launch_bounds__(256)
+defect_stream_kernel(const double* __restrict__ b, const double* __restrict__ x, wf_grid g,
+                     float* __restrict__ r32, MgDev d, double c0, double c1, double c2,
+                     const double* __restrict__ mean, int ich) {
+  const Blk C = this_blk();
+  const int k = C.x * 64 + threadIdx.x, j = C.y * 4 + threadIdx.y;
+  if (k >= g.nk || j >= g.nj) return;  // no barriers below
+  const int z0 = C.z * ich, z1 = min(z0 + ich, g.ni);
+  const int64_t q0 = at(g, g.npl, j + g.npl, k + g.npl);  // real plane 0 of the column
+  for (int i0 = z0; i0 < z1; i0 += DEFECT_U) {
+    double xc[DEFECT_U + 2], bv[DEFECT_U], xs[DEFECT_U], xn[DEFECT_U], xb[DEFECT_U],
+        xt[DEFECT_U];
+#pragma unroll
+    for (int u = 0; u < DEFECT_U + 2; ++u)
+      xc[u] = x[q0 + (int64_t)min(i0 - 1 + u, z1) * g.si];
+#pragma unroll
+    for (int u = 0; u < DEFECT_U; ++u) {
+      const int64_t q = q0 + (int64_t)min(i0 + u, z1 - 1) * g.si;
+      bv[u] = b[q];
+      xs[u] = x[q - g.sj];
+      xn[u] = x[q + g.sj];
+      xb[u] = x[q - 1];
+      xt[u] = x[q + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < DEFECT_U; ++u) {
+      const int i = i0 + u;
+      if (i < z1) {
+        const Coef<double> c = mg_coef(d, i, j, k, c0, c1, c2);
+        double r = defect_val(bv[u], c, xc[u + 1], xc[u], xc[u + 2], xs[u], xn[u], xb[u], xt[u]);
+        if (mean) r = r - *mean;
+        r32[mg_at(d, i, j, k)] = (float)r;
+      }
+    }
+  }
 }
 
 // out = max(partials[0..n)), one 1024-thread workgroup, fixed order.
@@ -521,10 +582,14 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
     const bool rhs_ready =
         c == 0 && (flags & MIX_RHS_READY) && !tol_mode && !(flags & MIX_DEFECT_MEAN);
     if (!rhs_ready) {
-      hipLaunchKernelGGL(defect_kernel, field_cgrid(g), dim3(64, 4), 0, s, su,
-                         c == 0 ? nullptr : pp, g, (float*)L0.rhs, d0, c0, c1, c2,
-                         (flags & MIX_DEFECT_MEAN) ? mean : nullptr,
-                         tol_mode ? work + WORK_HEAD : nullptr, field_ich(g));
+      const double* dmean = (flags & MIX_DEFECT_MEAN) ? mean : nullptr;
+      if (c > 0 && !tol_mode)
+        hipLaunchKernelGGL(defect_stream_kernel, field_cgrid(g), dim3(64, 4), 0, s, su, pp, g,
+                           (float*)L0.rhs, d0, c0, c1, c2, dmean, field_ich(g));
+      else
+        hipLaunchKernelGGL(defect_kernel, field_cgrid(g), dim3(64, 4), 0, s, su,
+                           c == 0 ? nullptr : pp, g, (float*)L0.rhs, d0, c0, c1, c2, dmean,
+                           tol_mode ? work + WORK_HEAD : nullptr, field_ich(g));
       WF_LAUNCHED();
     }
     if (tol_mode) {
