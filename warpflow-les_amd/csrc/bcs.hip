@@ -28,7 +28,33 @@ struct BcOps {
   int32_t n;
 };
 
-// The ops of one (a, b) line along `axis`, low side then high side.
+// Whether every source cell of the op list lies outside the ghost cells
+// written by the same or an earlier op (and every op has at most 3 layers):
+// then all sources can be loaded before the first store and the stores keep
+// their order, which leaves every cell with the value of the op-by-op walk.
+// Uniform over a launch (it depends on the ops only).
+__host__ __device__ inline bool bc_ops_hoistable(const BcOps& ops) {
+  if (ops.n > 4) return false;
+  for (int q = 0; q < ops.n; ++q) {
+    const BcOp& o = ops.op[q];
+    if (o.nlay > 3) return false;
+    if (o.kind == 0 || o.nlay <= 0) continue;
+    const int slo = o.high ? o.s0 - o.nlay + 1 : o.s0, shi = o.high ? o.s0 : o.s0 + o.nlay - 1;
+    for (int p = 0; p <= q; ++p) {
+      const BcOp& w = ops.op[p];
+      if (w.nlay <= 0) continue;
+      const int glo = w.high ? w.g0 : w.g0 - w.nlay + 1, ghi = w.high ? w.g0 + w.nlay - 1 : w.g0;
+      if (slo <= ghi && glo <= shi) return false;
+    }
+  }
+  return true;
+}
+
+// The ops of one (a, b) line along `axis`, low side then high side.  When
+// the ops allow it (bc_ops_hoistable) a thread issues all its loads before
+// its first store, so a line costs one memory round trip instead of one per
+// layer and side (a load after a store to the same array cannot be moved
+// ahead of it by the compiler).
 __device__ inline void bc_line(double* __restrict__ f, const wf_grid& g, int axis, int a, int b,
                                const BcOps& ops) {
   int64_t base, s;
@@ -41,6 +67,36 @@ __device__ inline void bc_line(double* __restrict__ f, const wf_grid& g, int axi
   } else {
     base = at(g, a, b, 0);
     s = 1;
+  }
+  if (bc_ops_hoistable(ops)) {
+    // every op field, destination and source value is in registers before
+    // the first store (a kernel-argument read placed after a store would
+    // wait for that store to complete)
+    double v[4][3];
+    int64_t dst[4][3];
+    bool on[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const BcOp o = ops.op[q];
+      const bool used = q < ops.n;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        on[q][m] = used && m < o.nlay;
+        dst[q][m] = base + (int64_t)(o.high ? o.g0 + m : o.g0 - m) * s;
+        // unconditional load (cell 0 of the line when no source is read), so
+        // no branch separates the loads
+        const bool src = on[q][m] && o.kind != 0;
+        const double x = f[base + (int64_t)(src ? (o.high ? o.s0 - m : o.s0 + m) : 0) * s];
+        v[q][m] = src ? o.val * x : o.val;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        if (on[q][m]) f[dst[q][m]] = v[q][m];
+    }
+    return;
   }
   for (int q = 0; q < ops.n; ++q) {
     const BcOp o = ops.op[q];
