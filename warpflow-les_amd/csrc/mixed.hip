@@ -209,6 +209,54 @@ defect_stream_kernel(const double* __restrict__ b, const double* __restrict__ x,
   }
 }
 
+// The first cycle's update fused with the second cycle's defect: x = 0.0 + e
+// on the real cells (update_kernel's assign) and r = b - A x from those same
+// values, formed from e directly, so x is written and never read.  Across
+// the domain faces the stencil takes e's ring in place of x's ghosts; both
+// enter with a zero coefficient and are finite, so r has the bits of the
+// separate update and defect.  Loads are grouped DEFECT_U planes at a time as
+// in defect_stream_kernel (plane indices past the chunk clamped onto plane
+// z1, which the (n+2)^3 level array holds).
+__global__ void __launch_bounds__(256)
+assign_defect_kernel(const double* __restrict__ b, double* __restrict__ x, wf_grid g,
+                     const float* __restrict__ e, float* __restrict__ r32, MgDev d, double c0,
+                     double c1, double c2, const double* __restrict__ mean, int ich) {
+  const Blk C = this_blk();
+  const int k = C.x * 64 + threadIdx.x, j = C.y * 4 + threadIdx.y;
+  if (k >= g.nk || j >= g.nj) return;  // no barriers below
+  const int z0 = C.z * ich, z1 = min(z0 + ich, g.ni);
+  const int64_t q0 = at(g, g.npl, j + g.npl, k + g.npl);  // real plane 0 of the column
+  const int64_t m0 = mg_at(d, 0, j, k);
+  for (int i0 = z0; i0 < z1; i0 += DEFECT_U) {
+    double xc[DEFECT_U + 2], bv[DEFECT_U], xs[DEFECT_U], xn[DEFECT_U], xb[DEFECT_U],
+        xt[DEFECT_U];
+#pragma unroll
+    for (int u = 0; u < DEFECT_U + 2; ++u)
+      xc[u] = 0.0 + (double)e[m0 + (int64_t)min(i0 - 1 + u, z1) * d.s0];
+#pragma unroll
+    for (int u = 0; u < DEFECT_U; ++u) {
+      const int p = min(i0 + u, z1 - 1);
+      const int64_t mq = m0 + (int64_t)p * d.s0;
+      bv[u] = b[q0 + (int64_t)p * g.si];
+      xs[u] = 0.0 + (double)e[mq - d.s1];
+      xn[u] = 0.0 + (double)e[mq + d.s1];
+      xb[u] = 0.0 + (double)e[mq - 1];
+      xt[u] = 0.0 + (double)e[mq + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < DEFECT_U; ++u) {
+      const int i = i0 + u;
+      if (i < z1) {
+        x[q0 + (int64_t)i * g.si] = xc[u + 1];
+        const Coef<double> c = mg_coef(d, i, j, k, c0, c1, c2);
+        double r = defect_val(bv[u], c, xc[u + 1], xc[u], xc[u + 2], xs[u], xn[u], xb[u], xt[u]);
+        if (mean) r = r - *mean;
+        r32[m0 + (int64_t)i * d.s0] = (float)r;
+      }
+    }
+  }
+}
+
 // out = max(partials[0..n)), one 1024-thread workgroup, fixed order.
 __global__ void __launch_bounds__(1024)
 max_fold_kernel(const double* __restrict__ partials, int64_t n, double* __restrict__ out) {
@@ -576,13 +624,16 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
   // with a fixed cycle count the last update also forms the block sums of
   // x that the result mean needs (the same sums field_partials would form)
   bool summed = false;
+  const double* dmean = (flags & MIX_DEFECT_MEAN) ? mean : nullptr;
+  bool defect_formed = false;  // the fused first update formed this cycle's defect
   for (int c = 0; c <= max_cycles; ++c) {
     const bool last_check = c == max_cycles;
     if (last_check && !tol_mode) break;
     const bool rhs_ready =
-        c == 0 && (flags & MIX_RHS_READY) && !tol_mode && !(flags & MIX_DEFECT_MEAN);
+        defect_formed ||
+        (c == 0 && (flags & MIX_RHS_READY) && !tol_mode && !(flags & MIX_DEFECT_MEAN));
+    defect_formed = false;
     if (!rhs_ready) {
-      const double* dmean = (flags & MIX_DEFECT_MEAN) ? mean : nullptr;
       if (c > 0 && !tol_mode)
         hipLaunchKernelGGL(defect_stream_kernel, field_cgrid(g), dim3(64, 4), 0, s, su, pp, g,
                            (float*)L0.rhs, d0, c0, c1, c2, dmean, field_ich(g));
@@ -613,9 +664,17 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su, dou
     r = mg_vcycle_levels(levels, nlev, n_pre, n_post, n_coarse, s, zs);
     if (r) return r;
     summed = (flags & MIX_RESULT_MEAN) && !tol_mode && c == max_cycles - 1;
-    hipLaunchKernelGGL(update_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g,
-                       (const float*)L0.phi, d0, field_ich(g), c == 0 ? 1 : 0,
-                       summed ? work + WORK_HEAD : nullptr);
+    if (c == 0 && !tol_mode && c + 1 < max_cycles) {
+      // x = e and the next cycle's defect in one pass
+      hipLaunchKernelGGL(assign_defect_kernel, field_cgrid(g), dim3(64, 4), 0, s, su, pp, g,
+                         (const float*)L0.phi, (float*)L0.rhs, d0, c0, c1, c2, dmean,
+                         field_ich(g));
+      defect_formed = true;
+    } else {
+      hipLaunchKernelGGL(update_kernel, field_cgrid(g), dim3(64, 4), 0, s, pp, g,
+                         (const float*)L0.phi, d0, field_ich(g), c == 0 ? 1 : 0,
+                         summed ? work + WORK_HEAD : nullptr);
+    }
     WF_LAUNCHED();
     ++cycles;
   }
