@@ -440,6 +440,34 @@ batch_defect_kernel(BView V, Plan P, const double* su, const double* pp, float* 
   });
 }
 
+// One plane of assign_defect_kernel's pass (x = 0.0 + e on the real cells,
+// r = b - A x from e), for the batched solve's one-plane work items.
+__device__ inline void assign_defect_body(const double* __restrict__ b, double* __restrict__ x,
+                                          const wf_grid& g, const float* __restrict__ e,
+                                          float* __restrict__ r32, const MgDev& d, double c0,
+                                          double c1, double c2, Blk B) {
+  const int k = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y, i = B.z;
+  if (k >= g.nk || j >= g.nj) return;
+  const int64_t m = mg_at(d, i, j, k);
+  const auto xe = [&](int64_t o) { return 0.0 + (double)e[m + o]; };
+  const double xc = xe(0);
+  x[at(g, i + g.npl, j + g.npl, k + g.npl)] = xc;
+  const Coef<double> c = mg_coef(d, i, j, k, c0, c1, c2);
+  r32[m] = (float)defect_val(b[at(g, i + g.npl, j + g.npl, k + g.npl)], c, xc, xe(-d.s0),
+                             xe(d.s0), xe(-d.s1), xe(d.s1), xe(-1), xe(1));
+}
+
+__global__ void __launch_bounds__(256)
+batch_assign_defect_kernel(BView V, Plan P, const double* su, double* pp, const float* phi0,
+                           float* rhs0, const int* active) {
+  mx_items<false>(P, V, active, [&](int, int q, const MemDev& M) {
+    const wf_grid& g = M.g;
+    assign_defect_body(su + M.off, pp + M.off, g, phi0 + M.loff[0], rhs0 + M.loff[0],
+                       level0_dev(M), 1.0 / (M.dx * M.dx), 1.0 / (M.dy * M.dy),
+                       1.0 / (M.dz * M.dz), cells_blk(q, g.nj, g.nk));
+  });
+}
+
 __global__ void __launch_bounds__(256)
 batch_update_kernel(BView V, Plan P, double* pp, const float* phi0, const int* active) {
   mx_items<false>(P, V, active, [&](int, int q, const MemDev& M) {
@@ -502,17 +530,28 @@ int batch_mg_mixed(const Batch& B, float* const* phi, float* const* rhs, float* 
   const int fb = (int)std::min<int64_t>((B.field_elems + 255) / 256, 8192);
   hipLaunchKernelGGL(batch_fill_kernel, dim3(fb), dim3(256), 0, s, pp, B.field_elems, 0.0);
   WF_LAUNCHED();
+  bool defect_formed = false;  // the fused first update formed this cycle's defect
   for (int c = 0; c < ncycles; ++c) {
-    hipLaunchKernelGGL(batch_defect_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, su,
-                       c == 0 ? (const double*)nullptr : (const double*)pp, rhs[0], active);
-    WF_LAUNCHED();
+    if (!defect_formed) {
+      hipLaunchKernelGGL(batch_defect_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, su,
+                         c == 0 ? (const double*)nullptr : (const double*)pp, rhs[0], active);
+      WF_LAUNCHED();
+    }
+    defect_formed = false;
     const int lb = (int)std::min<int64_t>((B.level_elems[0] + 255) / 256, 8192);
     hipLaunchKernelGGL(batch_fillf_kernel, dim3(lb), dim3(256), 0, s, phi[0], B.level_elems[0]);
     WF_LAUNCHED();
     const int r = batch_vcycle(B, phi, rhs, tmp, cop, npre, npost, ncoarse, active, s);
     if (r) return r;
-    hipLaunchKernelGGL(batch_update_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, pp,
-                       (const float*)phi[0], active);
+    if (c == 0 && ncycles > 1) {
+      // pp = 0 + e and the second cycle's defect in one pass
+      hipLaunchKernelGGL(batch_assign_defect_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, su,
+                         pp, (const float*)phi[0], rhs[0], active);
+      defect_formed = true;
+    } else {
+      hipLaunchKernelGGL(batch_update_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, pp,
+                         (const float*)phi[0], active);
+    }
     WF_LAUNCHED();
   }
   if (flags & MIX_RESULT_MEAN) {
