@@ -786,7 +786,7 @@ FIELDS = ("u", "v", "w", "p", "ustar", "vstar", "wstar",
 
 class Solver:
     """solver.py:29-271  one RK2 fractional step per ``step()``, in the
-    reference's exact launch order."""
+    reference's exact launch order (warpflow/solver.py:217-271)."""
 
     def __init__(self, case: Case, init=True):
         self.case = case
