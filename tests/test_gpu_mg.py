@@ -161,10 +161,14 @@ def test_reference_mg_script_flow(gpu):
 
 
 @pytest.mark.parametrize("n3,prec", [((8, 8, 8), np.float64), ((8, 8, 8), np.float32),
-                                     ((4, 6, 8), np.float64), ((12, 12, 12), np.float64)])
+                                     ((4, 6, 8), np.float64), ((12, 12, 12), np.float64),
+                                     ((16, 16, 16), np.float32), ((20, 20, 20), np.float64),
+                                     ((20, 20, 20), np.float32), ((18, 24, 20), np.float64)])
 def test_coarse_operator_equals_sweeps(gpu, n3, prec):
     """phi = M rhs (M formed by running the sweeps on unit vectors) equals
-    300 RBGS sweeps from phi = 0, ring included (multigrid.py:313-316)."""
+    300 RBGS sweeps from phi = 0, ring included (multigrid.py:313-316).
+    Levels above 4096 cells take the row x column-chunk apply (20^3 is the
+    coarsest level of the 80^3 case on 3 levels)."""
     import ctypes as C
     from warpflow import _hip
     from warpflow.multigrid import _Level

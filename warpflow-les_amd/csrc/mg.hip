@@ -1177,6 +1177,60 @@ coarse_op_apply_rows_kernel(const T* __restrict__ M, T* __restrict__ phi,
   ring_images(phi, d, i, j, k, x, acc);
 }
 
+// Coarse operators of more than COARSE_OP_SERIAL_MAX_CELLS cells (a 20^3
+// coarsest level is 8000 cells: M is 512 MB in FP64) are applied in two
+// passes so that the matrix streams through the whole chip: workgroup
+// (x, y) sums columns [y*clen, (y+1)*clen) of rows [x*256, x*256+256) in
+// column order into part[y][row], and the fold adds a row's chunk sums in
+// chunk order.  Both orders are fixed, so the result is deterministic; it
+// differs from the serial row sum only by rounding.
+constexpr int COP_SPLIT_ROWS = 256;
+constexpr int COP_SPLIT_MAX_CLEN = 512;
+template <class T>
+__global__ void __launch_bounds__(256)
+coarse_op_split_kernel(const T* __restrict__ M, const T* __restrict__ rhs, MgDev d, int clen,
+                       T* __restrict__ part) {
+  __shared__ T r[COP_SPLIT_MAX_CLEN];
+  const int n1 = d.n1, n2 = d.n2;
+  const int ncell = d.n0 * n1 * n2;
+  const int c0 = blockIdx.y * clen;
+  const int nc = min(clen, ncell - c0);
+  for (int l = threadIdx.x; l < nc; l += COP_SPLIT_ROWS) {
+    const int col = c0 + l;
+    const int k = col % n2, j = (col / n2) % n1, i = col / (n1 * n2);
+    r[l] = rhs[mg_at(d, i, j, k)];
+  }
+  __syncthreads();
+  const int row = blockIdx.x * COP_SPLIT_ROWS + threadIdx.x;
+  if (row >= ncell) return;
+  const T* __restrict__ m0 = M + (size_t)c0 * ncell + row;
+  T acc = T(0);
+  int c = 0;
+  for (; c + 16 <= nc; c += 16) {
+    T m[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) m[q] = m0[(size_t)(c + q) * ncell];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc = cop_fma(m[q], r[c + q], acc);
+  }
+  for (; c < nc; ++c) acc = cop_fma(m0[(size_t)c * ncell], r[c], acc);
+  part[(size_t)blockIdx.y * ncell + row] = acc;
+}
+template <class T>
+__global__ void __launch_bounds__(256)
+coarse_op_fold_kernel(const T* __restrict__ part, int nchunk, T* __restrict__ phi, MgDev d) {
+  const int n1 = d.n1, n2 = d.n2;
+  const int ncell = d.n0 * n1 * n2;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= ncell) return;
+  T acc = part[row];
+  for (int c = 1; c < nchunk; ++c) acc += part[(size_t)c * ncell + row];
+  const int k = row % n2, j = (row / n2) % n1, i = row / (n1 * n2);
+  const int64_t x = mg_at(d, i, j, k);
+  phi[x] = acc;
+  ring_images(phi, d, i, j, k, x, acc);
+}
+
 __global__ void __launch_bounds__(256)
 copy_in_kernel(const double* __restrict__ su, wf_grid g, double* __restrict__ rhs, MgDev d) {
   const int k = blockIdx.x * 64 + threadIdx.x;
@@ -1721,14 +1775,48 @@ bool relax_zero_start_any(const wf_mg_level& L, int n) {
   return L.prec == 8 ? relax_zero_start_ok<double>(L, n) : relax_zero_start_ok<float>(L, n);
 }
 
-constexpr int COARSE_OP_MAX_CELLS = 4096;
+// Coarse operators: up to COARSE_OP_SERIAL_MAX_CELLS cells each row is one
+// thread's serial sum (the form the batched solve shares bit for bit); above
+// it, up to COARSE_OP_MAX_CELLS, the split form (coarse_op_split_kernel),
+// whose chunk sums sit in a tail of the operator buffer after the ncell^2
+// matrix entries.
+constexpr int COARSE_OP_SERIAL_MAX_CELLS = 4096;
+constexpr int COARSE_OP_MAX_CELLS = 16384;
+
+// column chunks of the split form: about 2048 workgroups, at most 64 chunks,
+// at most COP_SPLIT_MAX_CLEN columns each
+inline int cop_split_chunks(int ncell) {
+  const int rb = div_up(ncell, COP_SPLIT_ROWS);
+  int nch = std::max(1, std::min(64, div_up(2048, rb)));
+  while (div_up(ncell, nch) > COP_SPLIT_MAX_CLEN) ++nch;
+  return nch;
+}
+inline size_t cop_elems(size_t ncell) {
+  size_t e = ncell * ncell;
+  if (ncell > (size_t)COARSE_OP_SERIAL_MAX_CELLS) e += (size_t)cop_split_chunks((int)ncell) * ncell;
+  return e;
+}
 
 template <class T>
 int coarse_op_apply_t(const wf_mg_level& L, hipStream_t s) {
   const MgDev d = mg_dev(L);
   const int ncell = d.n0 * d.n1 * d.n2;
   const size_t bytes = (size_t)ncell * sizeof(T);
-  if (ncell > COARSE_OP_MAX_CELLS || bytes > (size_t)max_lds()) return WF_EINVAL;
+  if (ncell > COARSE_OP_MAX_CELLS) return WF_EINVAL;
+  if (ncell > COARSE_OP_SERIAL_MAX_CELLS) {
+    const int nch = cop_split_chunks(ncell), clen = div_up(ncell, nch);
+    T* part = (T*)L.cop + (size_t)ncell * ncell;
+    hipLaunchKernelGGL(coarse_op_split_kernel<T>,
+                       dim3((unsigned)div_up(ncell, COP_SPLIT_ROWS), (unsigned)nch),
+                       dim3(COP_SPLIT_ROWS), 0, s, (const T*)L.cop, (const T*)L.rhs, d, clen,
+                       part);
+    WF_LAUNCHED();
+    hipLaunchKernelGGL(coarse_op_fold_kernel<T>, dim3((unsigned)div_up(ncell, 256)), dim3(256),
+                       0, s, (const T*)part, nch, (T*)L.phi, d);
+    WF_LAUNCHED();
+    return 0;
+  }
+  if (bytes > (size_t)max_lds()) return WF_EINVAL;
   static bool attr_set = false;
   if (!attr_set) {
     WF_TRY(hipFuncSetAttribute((const void*)coarse_op_apply_rows_kernel<T>,
@@ -1885,7 +1973,7 @@ int wf_mg_coarse_op_bytes(const wf_mg_level* L, size_t* bytes) {
   const size_t ncell = (size_t)L->n0 * L->n1 * L->n2;
   if (ncell > (size_t)COARSE_OP_MAX_CELLS || ncell * L->prec > (size_t)max_lds())
     return WF_EINVAL;
-  *bytes = ncell * ncell * (size_t)L->prec;
+  *bytes = cop_elems(ncell) * (size_t)L->prec;
   return 0;
 }
 

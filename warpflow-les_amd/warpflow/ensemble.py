@@ -29,7 +29,7 @@ from . import _hip
 from .config import CaseConfig, validate
 from .device import zeros
 from .kernels import wale_delta
-from .multigrid import COARSE_OPERATOR_MAX_CELLS
+from .multigrid import COARSE_OPERATOR_SERIAL_MAX_CELLS
 from .solver import ALFARK, FIELDS
 from .topology import NPL
 
@@ -130,7 +130,10 @@ class Ensemble:
                       for k, v in self.lpool.items()}
         # coarsest-level operators, built per member exactly as Multigrid does
         coarse_cells = [int(np.prod(n)) for n in self.level_n3[-1]]
-        self.use_cop = self.nlev > 1 and max(coarse_cells) <= COARSE_OPERATOR_MAX_CELLS
+        # the batched apply is the serial-row form, so a member whose coarsest
+        # level is larger relaxes it instead (its single-cavity solver would
+        # apply the split form and differ from it by rounding)
+        self.use_cop = self.nlev > 1 and max(coarse_cells) <= COARSE_OPERATOR_SERIAL_MAX_CELLS
         self.cop = zeros(max(cel.value, 1), np.float32, device)
         if self.use_cop:
             for m in range(self.nmem):

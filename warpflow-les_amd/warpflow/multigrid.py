@@ -21,7 +21,11 @@ from .device import DeviceArray, current_stream, grid_of, zeros
 
 NPL_MG = 1
 SIDES = ("west", "east", "south", "north", "bottom", "top")
-COARSE_OPERATOR_MAX_CELLS = 1728
+# coarsest levels solved as phi = M rhs: up to 4096 cells each row of M is
+# summed serially by one thread (the form the batched solve shares); larger
+# ones, up to 16384 cells (M = 2 GiB in FP64), in row x column chunks
+COARSE_OPERATOR_MAX_CELLS = 16384
+COARSE_OPERATOR_SERIAL_MAX_CELLS = 4096
 
 
 class _Level:
@@ -103,8 +107,10 @@ class Multigrid:
         self.n_post = 6
         self.n_coarse = 300
         self.n_vcycles = 4
-        # the coarsest solve as one matrix-vector product when the level is
-        # small (<= 12^3 cells: M <= 24 MB in FP64)
+        # the coarsest solve as a matrix-vector product (M = the n_coarse
+        # sweeps from zero) instead of 2 * n_coarse dependent half-sweeps in
+        # one workgroup: 80^3 on 3 levels has a 20^3 coarsest level, whose 300
+        # sweeps took 0.67 ms per V-cycle against ~0.1 ms for the 512 MB M
         coarse = self.levels[-1]
         if len(self.levels) > 1 and np.prod(coarse.n3) <= COARSE_OPERATOR_MAX_CELLS:
             coarse.build_coarse_operator(self.n_coarse)
