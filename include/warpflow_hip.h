@@ -34,6 +34,11 @@ extern "C" {
 #endif
 
 #define WF_EINVAL 10001
+/* The call needs more workspace than the library's static scratch holds
+ * (wf_mg_solve_mixed_static). */
+#define WF_ESCRATCH 10002
+/* Size of the static device scratch behind wf_mg_solve_mixed_static. */
+#define WF_STATIC_SCRATCH_BYTES ((size_t)1 << 20)
 
 typedef struct wf_grid {
   int32_t ni, nj, nk; /* real cells per axis */
@@ -362,6 +367,24 @@ int wf_mg_solve_mixed(const wf_mg_level* levels, int nlev, const double* su,
                       int n_coarse, int max_cycles, double rtol, int flags,
                       double* work, size_t work_bytes, int* cycles_out,
                       double* res_out, void* stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): wf_mg_solve_mixed with its workspace taken from a static
+ * device array of WF_STATIC_SCRATCH_BYTES (1 MiB) that the library holds on
+ * each GPU it runs on until the process ends, instead of a `work` argument.
+ * Same arguments, results and bits as wf_mg_solve_mixed otherwise; it
+ * allocates nothing.  Limits: the grid's workspace
+ * (wf_mg_mixed_workspace_bytes) must fit WF_STATIC_SCRATCH_BYTES, else
+ * WF_ESCRATCH is returned before anything is launched (up to about 2^17
+ * partial sums: 320^3 fits, 256^3 needs 512 KiB); flags & 4 (deferred gauge
+ * mean) is rejected with WF_EINVAL, since the mean would stay in the
+ * library's scratch.  All calls on one GPU share the one array: calls that
+ * overlap on a GPU (two streams or threads) race. */
+int wf_mg_solve_mixed_static(const wf_mg_level* levels, int nlev, const double* su,
+                             double* pp, wf_grid g, int n_pre, int n_post,
+                             int n_coarse, int max_cycles, double rtol, int flags,
+                             int* cycles_out, double* res_out, void* stream);
 
 /* *out_dev = mean of f over the real cells (deterministic FP64 sum). */
 int wf_field_mean(const double* f, wf_grid g, double* work, size_t work_bytes,

@@ -198,6 +198,8 @@ int wf_event_elapsed_ms(float* ms, void* start, void* stop) {
 
 const char* wf_error_string(int code) {
   if (code == WF_EINVAL) return "warpflow: invalid argument (shape/range check failed)";
+  if (code == WF_ESCRATCH)
+    return "warpflow: workspace exceeds the static scratch (WF_STATIC_SCRATCH_BYTES)";
   return hipGetErrorString((hipError_t)code);
 }
 

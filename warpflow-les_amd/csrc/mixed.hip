@@ -20,6 +20,7 @@
 // Sums are deterministic: fixed-shape block reductions into a partials
 // buffer, then one workgroup folds the partials in a fixed order.
 #include <algorithm>
+#include <mutex>
 
 #include "batch.h"
 #include "mg_common.h"
@@ -753,3 +754,56 @@ int wf_field_mean(const double* f, wf_grid g, double* work, size_t work_bytes, d
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// No-allocation form of wf_mg_solve_mixed: its workspace (four scalars and
+// one partial sum per workgroup of a field pass) lives in `scratch`, a static
+// device array of WF_STATIC_SCRATCH_BYTES that every GPU the library runs on
+// holds until the process ends.  Host code never passes the array's name to
+// a kernel: the name is a host-side stand-in, so the device address of each
+// GPU's copy is looked up once per device with hipGetSymbolAddress and kept.
+// Every kernel writes its partials and scalars before any kernel reads them,
+// so nothing in the scratch needs clearing between calls.  Calls that overlap
+// on one GPU (two streams, two threads) share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[WF_STATIC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[SCRATCH_MAX_DEVICES];  // device address of scratch, per device
+
+int scratch_of_current_device(double** out) {
+  int dev = 0;
+  WF_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return WF_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    WF_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return WF_EINVAL;
+    g_scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int wf_mg_solve_mixed_static(const wf_mg_level* levels, int nlev, const double* su,
+                                        double* pp, wf_grid g, int n_pre, int n_post,
+                                        int n_coarse, int max_cycles, double rtol, int flags,
+                                        int* cycles_out, double* res_out, void* stream) {
+  // the deferred gauge mean would be left inside the library's scratch, where
+  // the caller cannot reach it and the next call overwrites it
+  if (flags & MIX_MEAN_DEFERRED) return WF_EINVAL;
+  size_t need = 0;
+  if (wf_mg_mixed_workspace_bytes(g, &need)) return WF_EINVAL;
+  if (need > WF_STATIC_SCRATCH_BYTES) return WF_ESCRATCH;
+  double* work = nullptr;
+  const int r = scratch_of_current_device(&work);
+  if (r) return r;
+  return wf_mg_solve_mixed(levels, nlev, su, pp, g, n_pre, n_post, n_coarse, max_cycles, rtol,
+                           flags, work, need, cycles_out, res_out, stream);
+}

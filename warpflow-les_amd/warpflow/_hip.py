@@ -14,6 +14,8 @@ _PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = Path(os.environ.get("WARPFLOW_HIP_LIB", _PKG_ROOT / "lib" / "libwarpflow_hip.so"))
 
 WF_EINVAL = 10001
+WF_ESCRATCH = 10002
+WF_STATIC_SCRATCH_BYTES = 1 << 20
 
 
 class WfGrid(C.Structure):
@@ -136,6 +138,7 @@ _SIGS = {
     "wf_mg_mixed_workspace_bytes": [_G, C.POINTER(C.c_size_t)],
     "wf_mg_solve_mixed": [_L, _I, _P, _P, _G, _I, _I, _I, _I, _D, _I, _P, _Z,
                           C.POINTER(C.c_int), C.POINTER(C.c_double), _P],
+    "wf_mg_solve_mixed_static": [_L, _I, _P, _P, _G, _I, _I, _I, _I, _D, _I, _P, _P, _P],
     "wf_field_mean": [_P, _G, _P, _Z, _P, _P],
 }
 
