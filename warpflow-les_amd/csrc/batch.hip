@@ -14,6 +14,9 @@ namespace wf {
 // Projection-loop bookkeeping per member (solver.py:178-212): a member still
 // active at outer iteration `it` records it, then leaves the loop when
 // rmax < eps and it > 1; rmax > 10 marks it diverged (status 1) and stops it.
+// A NaN rmax counts as diverged too: the reference's `rmax > 10.0` test lets a
+// NaN pass, but in a batch such a member would hold every other member in
+// host-synchronised outer iterations up to the limit.
 __global__ void batch_mask_kernel(int nmem, const double* __restrict__ rmax, double eps, int it,
                                   int* __restrict__ active, int* __restrict__ iters,
                                   int* __restrict__ status) {
@@ -23,7 +26,7 @@ __global__ void batch_mask_kernel(int nmem, const double* __restrict__ rmax, dou
   const double r = rmax[m];
   if (r < eps && it > 1) {
     active[m] = 0;
-  } else if (r > 10.0) {
+  } else if (!(r <= 10.0)) {
     active[m] = 0;
     status[m] = 1;
   }

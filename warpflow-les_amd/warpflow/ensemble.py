@@ -290,11 +290,14 @@ class Ensemble:
         else:
             self._part_a()
         self.synchronize()
+        self._check_dt()
         it = min(self.graph_iters, self.maxcy) + 1
+        settled = False  # every member left the loop through its break test
         while it <= self.maxcy:
             if self.status.numpy().any():
                 break
             if not self.active.numpy().any():
+                settled = True
                 break
             self._outer_rest(it)
             if it + 1 > self.maxcy:
@@ -302,10 +305,13 @@ class Ensemble:
             it += 1
             self._outer_test(it)
             self.synchronize()
-        if self.status.numpy().any():
-            bad = [m for m, s in enumerate(self.status.numpy()) if s]
-            raise RuntimeError("projection diverged (rmax > 10) in members %s at itime %d"
-                               % (bad, self.itime))
+        status = self.status.numpy()
+        if status.any():
+            bad = [m for m, s in enumerate(status) if s]
+            raise RuntimeError("projection diverged (rmax > 10 or NaN) in members %s at "
+                               "itime %d" % (bad, self.itime))
+        if not settled:
+            self._warn_unconverged()
         if self.use_graph:
             _hip.call("wf_graph_launch", self._graphs[1], self.stream)
         else:
@@ -316,19 +322,44 @@ class Ensemble:
     def cells(self):
         return int(sum(np.prod(sp.n3) for sp in self.specs))
 
+    def _dtmax(self, r):
+        """dtmax of each member from its checkdt reductions r[m] = (max|u|,
+        max|v|, max|w|, dtvisc), as compute_dt forms it (solver.py:135-141)."""
+        out = []
+        for m, sp in enumerate(self.specs):
+            dtcfl = 1.0e10
+            for n, um in zip(sp.n3, r[m, :3]):
+                if um > 0.0:
+                    dtcfl = min(dtcfl, (1.0 / n) / um)
+            out.append(min(r[m, 3], dtcfl))
+        return out
+
+    def _check_dt(self):
+        """compute_dt's collapse test per member (solver.py:142-144) on this
+        step's checkdt reductions; the batch runs a fixed dt, so nothing else
+        of compute_dt applies.  A NaN field gives NaN maxima and dtvisc."""
+        dtmax = self._dtmax(self.out4.numpy().reshape(self.nmem, 4))
+        bad = [m for m, d in enumerate(dtmax) if d <= 0.0 or not (d < 1.0e10)]
+        if bad:
+            raise RuntimeError("time step collapsed in members %s at itime %d (dtmax %s)"
+                               % (bad, self.itime, ["%.3e" % dtmax[m] for m in bad]))
+
+    def _warn_unconverged(self):
+        """solver.py:205-209 per member: the members still iterating after the
+        last allowed outer iteration."""
+        act = self.active.numpy()
+        left = [m for m in range(self.nmem) if act[m]]
+        if left:
+            rmax = self.rmax.numpy()
+            print("warning: projection did not converge in members %s after %d iterations, "
+                  "rmax = %s" % (left, self.maxcy, ["%.6e" % rmax[m] for m in left]),
+                  flush=True)
+
     def cfl(self):
         """dt / dtmax per member from the last step's checkdt reductions
         (solver.py:122-150)."""
         r = self.out4.numpy().reshape(self.nmem, 4)
-        out = []
-        for m, sp in enumerate(self.specs):
-            d3 = tuple(1.0 / n for n in sp.n3)
-            dtcfl = 1.0e10
-            for d, um in zip(d3, r[m, :3]):
-                if um > 0.0:
-                    dtcfl = min(dtcfl, d / um)
-            out.append(self.dt / min(r[m, 3], dtcfl))
-        return np.array(out)
+        return np.array([self.dt / d for d in self._dtmax(r)])
 
     def close(self):
         if getattr(self, "_graphs", None):
