@@ -48,3 +48,25 @@ def test_restart_from_snapshot_is_exact(gpu, tmp_path):
     for nm in ("u", "v", "w", "p"):
         assert np.array_equal(a.f[nm].numpy(), b.f[nm].numpy()), nm
     assert a.ctime == b.ctime
+
+
+def test_cli_restart_is_exact_with_adaptive_dt(gpu, tmp_path, capsys):
+    """run.py to step 8 in one go, and to step 4 then --restart to step 8
+    (24^3, adaptive dt, WALE): the final snapshots agree bit for bit, because
+    the restart snapshot carries vis (read by the dt limit) and itime."""
+    sys.path.insert(0, ROOT)
+    import run
+    from warpflow.io import read_snapshot
+    case = os.path.join(ROOT, "cases", "smoke24.yaml")
+    a, b, c = (tmp_path / x for x in "abc")
+    assert run.main(["--case", case, "--steps", "8", "--outdir", str(a)]) == 0
+    assert run.main(["--case", case, "--steps", "4", "--outdir", str(b)]) == 0
+    assert run.main(["--case", case, "--steps", "8", "--outdir", str(c),
+                     "--restart", str(b / "snap_000004_r000.npz")]) == 0
+    logs = capsys.readouterr().out
+    assert logs.count("done: itime=8 ") == 2
+    x = read_snapshot(a / "snap_000008_r000.npz")
+    y = read_snapshot(c / "snap_000008_r000.npz")
+    assert int(x["itime"]) == int(y["itime"]) == 8
+    for k in ("u", "v", "w", "p", "vis", "ctime", "dt"):
+        assert np.array_equal(x[k], y[k]), k

@@ -37,3 +37,11 @@ def test_snapshot_layout(tmp_path):
     assert z["ctime"] == 0.125 and z["dt"] == 1e-3
     assert list(z["ncells"]) == [4, 5, 6] and list(z["gshape"]) == [4, 5, 6]
     assert list(z["offsets"]) == [0, 0, 0]
+
+
+def test_restart_snapshot_adds_vis_and_itime(tmp_path):
+    sol = fake_solver(tmp_path)
+    z = read_snapshot(write_snapshot(sol, 7, restart=True))
+    assert set(z) == {"u", "v", "w", "p", "vis", "itime", "ctime", "dt", "offsets", "ncells",
+                      "gshape"}
+    assert np.array_equal(z["vis"], sol.f["vis"].a) and int(z["itime"]) == 7

@@ -1,57 +1,106 @@
-"""CLI entry point (the reference's run.py), on one MI355X.
+"""Run a YAML cavity case on one MI355X.
 
-Usage:
-    python run.py --case cases/cavity_serial.yaml [--device hip:0] [--steps N]
-                  [--outdir DIR]
+    python run.py --case CASE.yaml [--steps N] [--outdir DIR] [--device hip:0]
+                  [--precision fp64|mixed] [--restart SNAPSHOT]
 
-Prints the reference's banner, one log line every output.log_interval steps
-and a closing line, and writes snap_%06d_r000.npz every output.n_out steps
-and after the last step.  A decomposed case (mpi.domains != [1, 1, 1])
-raises: this build runs one domain on one GPU.
+The console output is the reference's (run.py:40-42 banner, :52-54 log line,
+:62-64 closing line), so logs of the two programs can be compared line by
+line.  Snapshots ``snap_%06d_r000.npz`` are written at every ``output.n_out``
+steps and after the last one.  Besides the reference's keys they carry the
+viscosity field and the step number, which make ``--restart`` exact: the run
+continues from the snapshot's step up to the final step number as if it had
+never stopped, adaptive dt included.  One domain on one GPU; a case with
+``mpi.domains`` other than [1, 1, 1] is refused by ``build_topology``.
 """
+from __future__ import annotations
+
 import argparse
 import os
 import sys
 import time
+from dataclasses import dataclass
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
-from warpflow import FlowSolver, build_topology, load_case  # noqa: E402
-from warpflow.io import write_snapshot  # noqa: E402
+from warpflow import FlowSolver, __version__, build_topology, load_case  # noqa: E402
+from warpflow.io import load_into, write_snapshot  # noqa: E402
+
+# console formats kept from the reference (run.py:40-42, :52-54, :62-64)
+BANNER = "warpflow-les %s: case=%s grid=%dx%dx%d domains=%s device=%s"
+LOG_LINE = "it %6d  ctime %.6f  dt %.3e  rmax %.3e  outer %2d  cfl %.2f"
+DONE_LINE = "done: itime=%d ctime=%.6f wall=%.1fs"
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser(description="warpflow-les solver (MI355X)")
-    ap.add_argument("--case", required=True, help="case YAML file")
-    ap.add_argument("--device", default="auto", help="auto | hip:0 | cuda:0")
-    ap.add_argument("--steps", type=int, default=None,
-                    help="override time.n_steps from the case file")
-    ap.add_argument("--outdir", default=None, help="override output.outdir from the case file")
-    args = ap.parse_args(argv)
+@dataclass(frozen=True)
+class Schedule:
+    """Which steps log and which write a snapshot: steps first+1 .. last."""
+    first: int
+    last: int
+    log_every: int
+    snap_every: int
 
-    cfg = load_case(args.case)
-    if args.outdir:
-        cfg.output.outdir = args.outdir
-    nsteps = args.steps if args.steps is not None else cfg.time.n_steps
+    def steps(self):
+        return range(self.first + 1, self.last + 1)
+
+    def logs(self, it: int) -> bool:
+        return it % self.log_every == 0
+
+    def snaps(self, it: int) -> bool:
+        # every n_out steps, and the last step when the interval skips it
+        return it % self.snap_every == 0 or it == self.last
+
+
+def _cli() -> argparse.ArgumentParser:
+    ap = argparse.ArgumentParser(
+        prog="run.py", description="Advance a lid-driven-cavity case on one MI355X GPU.")
+    ap.add_argument("--case", required=True, metavar="YAML", help="the case file to run")
+    ap.add_argument("--steps", type=int, metavar="N",
+                    help="final step number (default: time.n_steps of the case)")
+    ap.add_argument("--outdir", metavar="DIR",
+                    help="snapshot directory (default: output.outdir of the case)")
+    ap.add_argument("--device", default="auto", metavar="DEV",
+                    help="hip:N or cuda:N (the same GPU); auto = hip:0")
+    ap.add_argument("--precision", choices=("fp64", "mixed"),
+                    help="pressure solve: plain FP64 V-cycles, or FP32 V-cycles inside FP64 "
+                         "defect correction (default: numerics.mg_precision of the case)")
+    ap.add_argument("--restart", metavar="SNAPSHOT",
+                    help="continue from a snapshot written by this program")
+    return ap
+
+
+def configure(opts):
+    """The case with the command line's overrides applied."""
+    cfg = load_case(opts.case)
+    if opts.outdir is not None:
+        cfg.output.outdir = opts.outdir
+    if opts.precision is not None:
+        cfg.numerics.mg_precision = opts.precision
+    if opts.steps is not None:
+        cfg.time.n_steps = opts.steps
+    return cfg
+
+
+def main(argv=None) -> int:
+    opts = _cli().parse_args(argv)
+    cfg = configure(opts)
+    device = "hip:0" if opts.device == "auto" else opts.device
     topo = build_topology(cfg)
-    device = "hip:0" if args.device == "auto" else args.device
-    print("warpflow-les %s: case=%s grid=%dx%dx%d domains=%s device=%s"
-          % ("0.1.0", cfg.name, cfg.grid.nx, cfg.grid.ny, cfg.grid.nz, list(cfg.mpi.domains),
-             device), flush=True)
-    sol = FlowSolver(cfg, topo, device=device)
-    t0 = time.time()
-    for itime in range(1, nsteps + 1):
-        info = sol.step()
-        if itime % cfg.output.log_interval == 0:
-            print("it %6d  ctime %.6f  dt %.3e  rmax %.3e  outer %2d  cfl %.2f"
-                  % (itime, info["ctime"], info["dt"], info["rmax"], info["iters"], info["cfl"]),
-                  flush=True)
-        if itime % cfg.output.n_out == 0:
-            write_snapshot(sol, itime)
-    if nsteps % cfg.output.n_out != 0:
-        write_snapshot(sol, nsteps)
-    print("done: itime=%d ctime=%.6f wall=%.1fs" % (nsteps, sol.ctime, time.time() - t0),
+    g = cfg.grid
+    print(BANNER % (__version__, cfg.name, g.nx, g.ny, g.nz, list(cfg.mpi.domains), device),
           flush=True)
+    solver = FlowSolver(cfg, topo, device=device)
+    if opts.restart:
+        load_into(solver, opts.restart)
+    plan = Schedule(solver.itime, cfg.time.n_steps, cfg.output.log_interval, cfg.output.n_out)
+    started = time.time()
+    for it in plan.steps():
+        info = solver.step()
+        if plan.logs(it):
+            print(LOG_LINE % (it, info["ctime"], info["dt"], info["rmax"], info["iters"],
+                              info["cfl"]), flush=True)
+        if plan.snaps(it):
+            write_snapshot(solver, it, restart=True)
+    print(DONE_LINE % (plan.last, solver.ctime, time.time() - started), flush=True)
     return 0
 
 
