@@ -500,3 +500,126 @@ def test_mixed_form_reproduces_fp64_vcycles():
     O.solve_mixed(mg32, su, pp, 3, 4, defect_mean=False, result_mean=False)
     s = (slice(3, -3),) * 3
     assert np.abs(pp[s] - pp64[s]).max() < 1e-5 * np.abs(pp64[s]).max()
+
+
+# --------------------------------------------------------------------------
+# Edge and corner ghosts: every ghost value as a composition of face rules.
+#
+# The Hydro3D rules behind bcs.py:201-293 (its comments state them in
+# Fortran form, e.g. "u(is-1-ly) = 0", "v(is-1-ly) = -v(is+ly)") are written
+# below as a table of face rules per field, in the side order W, E, S, N, B, T.
+# A rule either sets a set of planes to a constant, or mirrors layer m of a
+# side onto the m-th plane inside it with a sign; it covers only its
+# perpendicular index ranges.  A cell's final value is found by pulling, not
+# by replaying the fills on an array: take the LAST rule covering the cell;
+# a constant ends the chain, a mirror multiplies by its sign and moves to the
+# reflected cell, where only rules EARLIER than this one may have written.
+# The chain ends at a constant or at a cell no earlier rule covers, whose
+# value is the field's value before the fill -- edges and corners outside
+# some rule's range keep such earlier (history-dependent) values.  Each
+# ghost value is then +-F[source] or a constant, and must equal the
+# oracle's array-replay fill bit for bit on a field whose ghosts start out
+# random.
+# --------------------------------------------------------------------------
+def _face_rules(name, n3, ulid=0.75, rrey=1.0 / 400.0):
+    ni, nj, nk = n3
+    nn = (ni, nj, nk)
+    full = [(N - 1, N + x) for x in nn]           # first ghost layer .. first ghost layer
+    short = [(N - 1, N + x - 1) for x in nn]      # staggered: up to the wall face
+
+    def planes_lo(first, nl):                     # first, first-1, ..
+        return [first - m for m in range(nl)]
+
+    def planes_hi(first, nl):
+        return [first + m for m in range(nl)]
+
+    def mirror(ax, high, sign, perp, nl=3):
+        # layer m: ghost N-1-m <- N+m (low); N+n+m <- N+n-1-m (high)
+        n = nn[ax]
+        pairs = ([(N + n + m, N + n - 1 - m) for m in range(nl)] if high
+                 else [(N - 1 - m, N + m) for m in range(nl)])
+        return ("mirror", ax, dict(pairs), sign, perp)
+
+    def const(ax, planes, val, perp):
+        return ("const", ax, set(planes), val, perp)
+
+    def perp(ax, r):  # perpendicular ranges, keyed by axis
+        return {a: r[a] for a in range(3) if a != ax}
+
+    if name == "u":
+        # u(is-1-ly) = 0 / u(ie+1+ly) = 0 from the wall face / u(js-1-ly) =
+        # u(js+ly) / u(ks-1-ly) = -u(ks+ly) / u(ke+1+ly) = ulid
+        ry = [short[0], full[1], full[2]]
+        return [const(0, planes_lo(N - 1, 3), 0.0, perp(0, full)),
+                const(0, planes_hi(N + ni - 1, 3), 0.0, perp(0, full)),
+                mirror(1, False, 1.0, perp(1, ry)), mirror(1, True, 1.0, perp(1, ry)),
+                mirror(2, False, -1.0, perp(2, ry)),
+                const(2, planes_hi(N + nk, 3), ulid, perp(2, ry))]
+    if name == "v":
+        rx = [full[0], short[1], full[2]]
+        return [mirror(0, False, -1.0, perp(0, rx)), mirror(0, True, -1.0, perp(0, rx)),
+                const(1, planes_lo(N - 1, 3), 0.0, perp(1, full)),
+                const(1, planes_hi(N + nj - 1, 3), 0.0, perp(1, full)),
+                mirror(2, False, -1.0, perp(2, rx)), mirror(2, True, 1.0, perp(2, rx))]
+    if name == "w":
+        rx = [full[0], full[1], short[2]]
+        return [mirror(0, False, -1.0, perp(0, rx)), mirror(0, True, -1.0, perp(0, rx)),
+                mirror(1, False, 1.0, perp(1, rx)), mirror(1, True, 1.0, perp(1, rx)),
+                const(2, planes_lo(N - 1, 3), 0.0, perp(2, full)),
+                const(2, planes_hi(N + nk - 1, 3), 0.0, perp(2, full))]
+    if name == "vis":
+        # one layer; at the W, E and B walls the first real plane is set to
+        # 1/Re and then mirrored (eddyvis_wale ghost section)
+        p = lambda ax: perp(ax, full)  # noqa: E731
+        return [const(0, [N], rrey, p(0)), mirror(0, False, 1.0, p(0), 1),
+                const(0, [N + ni - 1], rrey, p(0)), mirror(0, True, 1.0, p(0), 1),
+                mirror(1, False, 1.0, p(1), 1), mirror(1, True, 1.0, p(1), 1),
+                const(2, [N], rrey, p(2)), mirror(2, False, 1.0, p(2), 1),
+                mirror(2, True, 1.0, p(2), 1)]
+    raise ValueError(name)
+
+
+def _covers(rule, c):
+    kind, ax, where, _, rng = rule
+    on_plane = c[ax] in where  # const: a plane of the set; mirror: a ghost plane
+    return on_plane and all(lo <= c[a] <= hi for a, (lo, hi) in rng.items())
+
+
+def _compose(rules, c):
+    """(sign, source cell or None, constant): the value at cell c after all
+    rules is sign * F[source], or the constant when source is None."""
+    sign, last = 1.0, len(rules)
+    while True:
+        hit = next((q for q in range(last - 1, -1, -1) if _covers(rules[q], c)), None)
+        if hit is None:
+            return sign, c, None
+        kind, ax, where, val, _ = rules[hit]
+        if kind == "const":
+            return sign, None, val
+        c = tuple(where[c[ax]] if a == ax else c[a] for a in range(3))
+        sign *= val
+        last = hit
+
+
+@pytest.mark.parametrize("n3", [(6, 6, 6), (5, 7, 6)])
+@pytest.mark.parametrize("name", ["u", "v", "w", "vis"])
+def test_edge_and_corner_ghosts_compose_face_rules(name, n3):
+    rules = _face_rules(name, n3)
+    shape = tuple(x + 2 * N for x in n3)
+    f0 = RNG.normal(size=shape)
+    got = f0.copy()
+    {"u": lambda a: O.apply_u(a, n3, 0.75), "v": lambda a: O.apply_v(a, n3),
+     "w": lambda a: O.apply_w(a, n3), "vis": lambda a: O.apply_vis(a, n3, 1.0 / 400.0)}[name](got)
+    exp = np.empty(shape)
+    chains = {0: 0, 1: 0, 2: 0, 3: 0}
+    for c in np.ndindex(*shape):
+        sign, src, val = _compose(rules, c)
+        exp[c] = sign * (val if src is None else f0[src])
+        ghosts = sum(not (N <= c[a] < N + n3[a]) for a in range(3))
+        if ghosts >= 2 and (src is None or src != c):
+            chains[ghosts] += 1
+    # bit for bit, signed zeros included
+    assert np.array_equal(got.view(np.int64), exp.view(np.int64)), \
+        [tuple(int(x) for x in i) for i in np.argwhere(got.view(np.int64) != exp.view(np.int64))[:5]]
+    # the fills do reach edges (two ghost axes) and corners (three)
+    assert chains[2] > 0 and chains[3] > 0, chains
