@@ -5,21 +5,33 @@
 
 Workload: unit cube, 256^3 cells, Re 1000, WALE (cw 0.46), rest + 1% seeded
 noise, fixed dt 1e-4, projection to rmax < 1e-6 (at most 30 outer
-iterations), RBGS V(6,6) multigrid with 300 coarsest sweeps and 4 V-cycles
-per solve on 6 levels (8^3 coarsest).  Two forms run back to back on the same
-GPU from the same initial state:
+iterations), RBGS V(6,6) multigrid with 300 coarsest sweeps on 6 levels
+(8^3 coarsest).  Forms run back to back on the same GPU from the same
+initial state:
 
-  mixed -- FP32 V-cycles inside FP64 defect correction, 2 cycles per solve
-           by default (--mixed-cycles) (the headline value)
-  fp64  -- the reference's plain FP64 V-cycles, 4 per solve
+  mixed  -- FP32 V-cycles inside FP64 defect correction, 2 cycles per solve
+            (--mixed-cycles; the headline value)
+  fp64   -- the reference's plain FP64 V-cycles, 4 per solve
+  mixed4 -- the mixed form at the FP64 form's 4 cycles (the FP32 gain alone)
 
 W untimed steps, then K steps timed between device synchronisations.
 value = mixed-form Mcell/s = nx*ny*nz / (seconds per step) / 1e6.  The JSON
-line also carries the FP64 form's rate, the mixed/FP64 speed-up, the velocity
-rms deviation between the forms, a per-kernel table (HIP-event timings of
-each hot kernel on the solver's own arrays, algorithmic bytes per launch),
-the roofline of the dominant kernel and a CPU baseline (the C oracle, the
-same FP64 schedule, one step of the same 256^3 case on the host cores).
+line also carries the other forms' rates and speed-ups, the velocity rms
+deviation between the forms, a per-kernel table, the roofline of the kernel
+that takes the most time per mixed step, the 80^3 production case
+(cases/cavity_serial.yaml) in both forms, and CPU baselines (the C and NumPy
+oracles: the same FP64 schedule on the host cores).
+
+Kernel table: each hot kernel (or launch group, e.g. four RBGS sweeps as the
+V-cycle issues them) is timed with HIP events on the stream the solver uses,
+after a 1 GiB streaming copy that evicts the 256 MB Infinity Cache and the
+L2 -- inside a step the arrays a kernel reads were last touched one or more
+full-field passes earlier, so back-to-back repetitions on the same arrays
+would time cache-resident data.  Algorithmic bytes per cell follow the
+reference schedule's accounting (distinct arrays read + written x element
+size; an RBGS sweep 32 B/cell in FP64, 16 in FP32; restriction and
+prolongation 17 B per fine cell in FP64); traffic is HBM read + write bytes
+per launch from rocprofv3 counters (profiles/pmc_traffic_cavity256.json).
 """
 from __future__ import annotations
 
@@ -27,11 +39,14 @@ import argparse
 import ctypes as C
 import json
 import os
+import platform
+import statistics
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-for _p in (ROOT, os.path.join(ROOT, "warpflow-les_amd")):
+PKG = os.path.join(ROOT, "warpflow-les_amd")
+for _p in (ROOT, PKG):
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
@@ -39,6 +54,7 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "RK2 step Mcell/s at 256^3, div<1e-6: FP64 vs FP32-MG+FP64 refine"
+REPORT_80_MCELLS = 2.3  # 80^3 FP64 step on an RTX 5060 Laptop GPU (REPORT.md:101-103)
 
 
 def parse(argv=None):
@@ -50,10 +66,11 @@ def parse(argv=None):
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--reynolds", type=float, default=1000.0)
     ap.add_argument("--dt", type=float, default=1e-4)
-    ap.add_argument("--forms", default="mixed,fp64")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--forms", default="mixed,fp64,mixed4")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel table")
-    ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
+    ap.add_argument("--no-80", action="store_true", help="skip the 80^3 production case")
+    ap.add_argument("--reps", type=int, default=10, help="timed launches per kernel")
     ap.add_argument("--workload", default="cavity256", choices=("cavity256", "ensemble64"),
                     help="cavity256: the headline 256^3 step; ensemble64: 64 uneven cavities "
                          "batched (mixed MG, graph-captured step) against sequential solves")
@@ -69,10 +86,16 @@ def parse(argv=None):
                     help="only launch each hot kernel --reps times (hardware-counter runs)")
     ap.add_argument("--mixed-cycles", type=int, default=2,
                     help="FP64-refined FP32 V-cycles per pressure solve in the mixed form")
+    ap.add_argument("--summarize-trace", metavar="DB",
+                    help="print the per-kernel table of a rocprofv3 --kernel-trace database "
+                         "(no GPU needed); --steps: the steps it covers")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("READ_DB", "WRITE_DB", "OUT_JSON"),
+                    help="write HBM bytes per launch from the two rocprofv3 --pmc runs of "
+                         "--probe (no GPU needed)")
     return ap.parse_args(argv)
 
 
-def make_cfg(a, precision):
+def make_cfg(a, precision, cycles=None):
     from warpflow.config import CaseConfig, validate
     c = CaseConfig(name="cavity%d_re%d" % (a.n, a.reynolds))
     c.grid.nx = c.grid.ny = c.grid.nz = a.n
@@ -84,23 +107,31 @@ def make_cfg(a, precision):
     c.numerics.div_tolerance = 1e-6
     c.numerics.max_outer_iters = 30
     c.numerics.mg_precision = precision
-    c.numerics.mg_mixed_cycles = a.mixed_cycles
+    c.numerics.mg_mixed_cycles = a.mixed_cycles if cycles is None else cycles
     c.time.variable_dt = False
     c.time.dt_init = a.dt
     validate(c)
     return c
 
 
-def run_form(a, precision):
-    from warpflow import FlowSolver, build_topology
+FORMS = {"mixed": ("mixed", None), "fp64": ("fp64", None), "mixed4": ("mixed", 4)}
+
+
+def tune(a, sol):
     from warpflow import autotune
+    if a.no_autotune:
+        return None
+    return [[k, list(n3), v] for k, n3, v in
+            autotune.tune_solver(sol, reps=10, cache=a.tune_cache)["entries"]]
+
+
+def run_form(a, form):
+    from warpflow import FlowSolver, build_topology
     from warpflow.device import synchronize
-    cfg = make_cfg(a, precision)
+    precision, cycles = FORMS[form]
+    cfg = make_cfg(a, precision, cycles)
     sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
-    tuned = None
-    if not a.no_autotune:  # launch shapes for this grid and form, untimed, arrays restored
-        tuned = [[k, list(n3), v] for k, n3, v in
-                 autotune.tune_solver(sol, reps=10, cache=a.tune_cache)["entries"]]
+    tuned = tune(a, sol)  # launch shapes for this grid and form, untimed, arrays restored
     for _ in range(a.warmup):
         sol.step()
     synchronize()
@@ -114,7 +145,7 @@ def run_form(a, precision):
 
 class Timer:
     """HIP events on the stream the kernels are launched on (the null
-    stream): average duration of `reps` back-to-back launches."""
+    stream)."""
 
     def __init__(self):
         from warpflow import _hip
@@ -123,27 +154,57 @@ class Timer:
         _hip.call("wf_event_create", C.byref(self.e0))
         _hip.call("wf_event_create", C.byref(self.e1))
 
-    def avg_us(self, fn, reps):
+    def _elapsed_us(self):
+        ms = C.c_float(0.0)  # (waits for e1)
+        self.h.call("wf_event_elapsed_ms", C.byref(ms), self.e0, self.e1)
+        return 1000.0 * ms.value
+
+    def warm_us(self, fn, reps):
+        """Average of `reps` back-to-back calls (cache-warm)."""
         from warpflow.device import current_stream
-        fn()  # warm
+        fn()
         self.h.call("wf_event_record", self.e0, current_stream())
         for _ in range(reps):
             fn()
         self.h.call("wf_event_record", self.e1, current_stream())
-        ms = C.c_float(0.0)
-        self.h.call("wf_event_elapsed_ms", C.byref(ms), self.e0, self.e1)
-        return 1000.0 * ms.value / reps
+        return self._elapsed_us() / reps
+
+    def cold_us(self, fn, reps, flush):
+        """Average of `reps` calls, each timed alone after `flush` has
+        streamed 1 GiB through the memory system."""
+        from warpflow.device import current_stream
+        fn()
+        tot = 0.0
+        for _ in range(reps):
+            flush()
+            self.h.call("wf_event_record", self.e0, current_stream())
+            fn()
+            self.h.call("wf_event_record", self.e1, current_stream())
+            tot += self._elapsed_us()
+        return tot / reps
+
+
+class Flush:
+    """Copy 512 MiB between two device buffers: 1 GiB of HBM traffic, which
+    leaves none of a kernel's arrays in the Infinity Cache or the L2."""
+
+    def __init__(self, nbytes=512 << 20):
+        from warpflow.device import zeros
+        self.a = zeros(nbytes // 8)
+        self.b = zeros(nbytes // 8)
+
+    def __call__(self):
+        from warpflow.device import copy
+        copy(self.b, self.a)
 
 
 def hot_kernels(a, s64, smx):
-    """The hot kernels of one step on the solvers' own arrays, as
-    (name, launch, algorithmic bytes per launch, launches per step FP64,
-    launches per step mixed).  bytes = distinct arrays read + written (RMW
-    twice) x element size x launch cells; launches per step are those of
-    the schedule with 2 outer iterations (one pressure solve: 4 V-cycles in
-    the FP64 form, --mixed-cycles in the mixed form).  The fused RBGS entry
-    is one call of two full sweeps (phi -> tmp -> phi), counted per sweep."""
-    from warpflow import _hip
+    """The hot kernels and launch groups of one step on the solvers' own
+    arrays: dicts with the launch (fn), the kernel it runs, launches and
+    algorithmic bytes per call, and calls per step of each form with two
+    outer iterations (one pressure solve: 4 V-cycles in the FP64 form,
+    --mixed-cycles in the mixed form)."""
+    from warpflow import _hip, autotune
     from warpflow import kernels as K
     from warpflow.device import copy, current_stream
     from warpflow.topology import NPL
@@ -154,58 +215,70 @@ def hot_kernels(a, s64, smx):
     dt = s64.dt
     st = current_stream()
     opu = s64._op_dim("u")
+
+    def row(name, kernel, fn, bytes_per_call, per_step_fp64, per_step_mixed, launches=1):
+        return {"name": name, "kernel": kernel, "fn": fn, "bytes": float(bytes_per_call),
+                "launches": launches, "per_step_fp64": per_step_fp64,
+                "per_step_mixed": per_step_mixed}
+
     out = [
-        # 6 per step: 3 of them the first stage's variant, same bytes (its c_oo
-        # read becomes a c_oo write)
-        ("convdiff4th (u)", lambda: K.convdiff4th_u(opu, f["u"], f["v"], f["w"], f["uoo"],
-                                                    f["vis"], f["ustar"], *h, dt, 0.5, NPL),
-         48 * opu[0] * opu[1] * opu[2], 6, 6, 1),
-        ("calvel3", lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
-                                      f["u"], f["v"], f["w"], *h, dt, 1.0, NPL),
-         56 * cells, 1, 1, 1),
-        ("calvel3+calmas", lambda: K.calvel3_calmas(s64.ncells, f["ustar"], f["vstar"],
-                                                    f["wstar"], f["p"], f["u"], f["v"], f["w"],
-                                                    f["su"], s64._buf, *h, dt, 1.0, NPL),
-         64 * cells, 2, 2, 1),
-        ("calmas", lambda: K.calmas(s64.ncells, f["u"], f["v"], f["w"], f["su"], s64._buf,
-                                    *h, dt, NPL), 32 * cells, 0, 0, 1),
-        ("checkdt", lambda: K.checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"], s64._buf,
-                                      *h, NPL), 32 * cells, 0, 0, 1),
-        ("wale", lambda: K.wale_viscosity(s64.ncells, f["u"], f["v"], f["w"], f["vis"], *h,
-                                          s64.rey, 0.46, NPL), 32 * cells, 0, 0, 1),
-        ("wale+checkdt", lambda: K.wale_checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"],
-                                                s64._buf, *h, s64.rey, 0.46, NPL),
-         40 * cells, 1, 1, 1),
-        ("add_ring", lambda: K.add_inplace(tuple(x + 2 for x in s64.ncells), f["p"], f["pp"],
-                                           NPL), 24 * (n + 2) ** 3, 1, 1, 1),
-        # the plain copy: the step no longer copies u -> uoo (the first stage's
-        # conv+diff stores uoo on its box); kept as the streaming reference
-        ("copy (u -> uoo)", lambda: copy(f["uoo"], f["u"]), 2 * f["u"].phys_nbytes, 0, 0, 1),
-        ("bc_fill (u)", lambda: s64.fill_bc("u"), 0, 15, 15, 1),
+        # 6 per step, 3 of them the first stage's storing form (same bytes: its
+        # c_oo read becomes a c_oo write)
+        row("convdiff4th (u)", "convdiff_kernel<0, *>",
+            lambda: K.convdiff4th_u(opu, f["u"], f["v"], f["w"], f["uoo"], f["vis"], f["ustar"],
+                                    *h, dt, 0.5, NPL),
+            48 * opu[0] * opu[1] * opu[2], 6, 6),
+        row("calvel3", "calvel3_kernel",
+            lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"], f["u"],
+                              f["v"], f["w"], *h, dt, 1.0, NPL), 56 * cells, 1, 1),
+        row("calvel3+calmas", "calvel3_calmas_kernel",
+            lambda: K.calvel3_calmas(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
+                                     f["u"], f["v"], f["w"], f["su"], s64._buf, *h, dt, 1.0,
+                                     NPL), 64 * cells, 2, 2),
+        row("wale+checkdt", "wale_checkdt_kernel",
+            lambda: K.wale_checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"], s64._buf, *h,
+                                   s64.rey, 0.46, NPL), 40 * cells, 1, 1),
+        row("add_ring", "add_kernel",
+            lambda: K.add_inplace(tuple(x + 2 for x in s64.ncells), f["p"], f["pp"], NPL),
+            24 * (n + 2) ** 3, 1, 1),
+        # the plain copy: the step makes no whole-field copy any more (the first
+        # stage's conv+diff stores uoo); kept as the streaming reference
+        row("copy (u -> uoo)", "copy16_kernel", lambda: copy(f["uoo"], f["u"]),
+            2 * f["u"].phys_nbytes, 0, 0),
+        row("bc_fill (u)", "bc_axis_kernel", lambda: s64.fill_bc("u"), 0, 2, 2, 3),
     ]
     nv = s64.mg.n_vcycles
     nm = a.mixed_cycles
-    sweeps = s64.mg.n_pre + s64.mg.n_post
+    nsw = s64.mg.n_pre + s64.mg.n_post  # level-0 sweeps per cycle
     for tag, mg, eb, cyc64, cycmx in (("fp64", s64.mg, 8, nv, 0), ("fp32", smx.mg, 4, 0, nm)):
-        L0, L1 = mg.levels[0].desc(), mg.levels[1].desc()
-        out.append(("rbgs full sweep L0 (%s)" % tag,
-                    lambda L0=L0: _hip.call("wf_mg_relax", C.byref(L0), 2, st),
-                    3 * eb * cells, sweeps * cyc64, sweeps * cycmx, 2))
-        out.append(("restrict L0 (%s)" % tag,
-                    lambda L0=L0, L1=L1: _hip.call("wf_mg_restrict", C.byref(L0), C.byref(L1),
-                                                   st),
-                    (2 * eb + 2 * eb / 8) * cells, cyc64, cycmx, 1))
-        out.append(("prolong L0 (%s)" % tag,
-                    lambda L0=L0, L1=L1: _hip.call("wf_mg_prolong", C.byref(L1), C.byref(L0),
-                                                   st),
-                    (2 * eb + eb / 8) * cells, cyc64, cycmx, 1))
+        lv0, lv1 = mg.levels[0], mg.levels[1]
+        L0, L1 = lv0.desc(), lv1.desc()
+        knob = autotune.SWEEP_PAIRS_F64 if eb == 8 else autotune.SWEEP_PAIRS_F32
+        pairs = autotune.get_shape(knob, lv0.n3)
+        two = pairs == 2 or (pairs == 0 and eb == 8 and cells >= 2 ** 23)
+        ctype = "double" if eb == 8 else "float"
+        kern = ("sweep2_kernel<%s> (two RBGS sweeps per launch)" % ctype if two
+                else "sweep_kernel<%s> (one RBGS sweep per launch)" % ctype)
+        # one cycle's level-0 smoothing (6 pre + 6 post sweeps) as one group of
+        # launches: the first reads arrays another kernel touched long before,
+        # the others read what the previous sweep wrote, as in a V-cycle
+        out.append(row("rbgs %d sweeps L0 (%s)" % (nsw, tag), kern,
+                       lambda L0=L0: _hip.call("wf_mg_relax", C.byref(L0), nsw, st),
+                       nsw * (4 * eb) * cells, cyc64, cycmx, nsw // 2 if two else nsw))
+        out.append(row("restrict L0 (%s)" % tag, "restrict_kernel<%s>" % ctype,
+                       lambda L0=L0, L1=L1: _hip.call("wf_mg_restrict", C.byref(L0),
+                                                      C.byref(L1), st),
+                       (17 * eb / 8) * cells, cyc64, cycmx))
+        out.append(row("prolong L0 (%s)" % tag, "prolong_kernel<%s>" % ctype,
+                       lambda L0=L0, L1=L1: _hip.call("wf_mg_prolong", C.byref(L1), C.byref(L0),
+                                                      st),
+                       (17 * eb / 8) * cells, cyc64, cycmx))
     return out
 
 
 def load_traffic():
     """HBM bytes per launch of each hot kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
-    same counter's reading on the plain copy kernel of known size)."""
+    --pmc summary (read requests by size, write requests by size)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_cavity256.json")
     if not os.path.exists(path):
         return {}
@@ -214,20 +287,25 @@ def load_traffic():
 
 
 def kernel_table(a, s64, smx):
-    """HIP-event timing of every hot kernel (hot_kernels) on the null stream,
-    the stream the solver launches on."""
     T = Timer()
+    flush = Flush()
     traffic = load_traffic()
     rows = []
-    for name, fn, nbytes, ps64, psmx, per_call in hot_kernels(a, s64, smx):
-        us = T.avg_us(fn, a.reps) / per_call
-        row = {"kernel": name, "us": round(us, 2), "bytes": int(nbytes),
-               "GBps": round(nbytes / us / 1e3, 1),
-               "hbm_frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 3),
-               "per_step_fp64": ps64, "per_step_mixed": psmx}
-        if name in traffic:
-            row["traffic"] = traffic[name]["bytes_per_launch"]
-        rows.append(row)
+    for r in hot_kernels(a, s64, smx):
+        fn = r.pop("fn")
+        cold = T.cold_us(fn, a.reps, flush)
+        warm = T.warm_us(fn, a.reps)
+        r.update({"us_cold": round(cold, 2), "us_warm": round(warm, 2),
+                  "us_per_launch": round(cold / r["launches"], 2),
+                  "GBps": round(r["bytes"] / cold / 1e3, 1) if r["bytes"] else None,
+                  "hbm_frac": round(r["bytes"] / cold / 1e3 / HBM_PEAK_GBS, 3)
+                  if r["bytes"] else None,
+                  "us_per_step_mixed": round(cold * r["per_step_mixed"], 1),
+                  "us_per_step_fp64": round(cold * r["per_step_fp64"], 1)})
+        if r["name"] in traffic:
+            r["traffic_per_launch"] = traffic[r["name"]]["bytes_per_launch"]
+        r["bytes"] = int(r["bytes"])
+        rows.append(r)
     return rows
 
 
@@ -238,36 +316,221 @@ def probe(a):
     from warpflow.device import synchronize
     s64 = FlowSolver(make_cfg(a, "fp64"), build_topology(make_cfg(a, "fp64")), "hip:0")
     smx = FlowSolver(make_cfg(a, "mixed"), build_topology(make_cfg(a, "mixed")), "hip:0")
+    tune(a, s64)
+    tune(a, smx)
     synchronize()
-    for name, fn, nbytes, _, _, per_call in hot_kernels(a, s64, smx):
-        print("# probe %s: %d calls, %d algorithmic bytes per kernel launch"
-              % (name, a.reps, nbytes), file=sys.stderr, flush=True)
+    for r in hot_kernels(a, s64, smx):
+        print("# probe %s: %d calls, %d launches per call, %d algorithmic bytes per call"
+              % (r["name"], a.reps, r["launches"], r["bytes"]), file=sys.stderr, flush=True)
         for _ in range(a.reps):
-            fn()
+            r["fn"]()
         synchronize()
     return 0
 
 
-def dominant(rows, key):
-    best = max(rows, key=lambda r: r["us"] * r[key])
-    return best
+# Trace/counter rows: the kernel each table row launches, as rocprofv3 names
+# it (the level-0 launch is the one with the largest grid).
+ROW_KERNELS = {
+    "convdiff4th (u)": ("convdiff_kernel<0, false>",),
+    "calvel3": ("calvel3_kernel(",),
+    "calvel3+calmas": ("calvel3_calmas_kernel(",),
+    "wale+checkdt": ("wale_checkdt_kernel(",),
+    "add_ring": ("add_kernel(",),
+    "copy (u -> uoo)": ("copy16_kernel(",),
+    "bc_fill (u)": ("bc_axis_kernel(",),
+    "rbgs 12 sweeps L0 (fp64)": ("sweep2_kernel<double", "sweep_kernel<double"),
+    "restrict L0 (fp64)": ("restrict_kernel<double>",),
+    "prolong L0 (fp64)": ("prolong_kernel<double>",),
+    "rbgs 12 sweeps L0 (fp32)": ("sweep2_kernel<float", "sweep_kernel<float"),
+    "restrict L0 (fp32)": ("restrict_kernel<float>",),
+    "prolong L0 (fp32)": ("prolong_kernel<float>",),
+}
 
 
-def cpu_baseline(a):
+def _row_dispatches(recs, row):
+    """(kernel name, dispatches) of `row` among recs = [(name, grid, ...)]:
+    the first listed kernel present, at its largest grid."""
+    for pat in ROW_KERNELS[row]:
+        hits = [r for r in recs if pat in r[0]]
+        if hits:
+            g = max(r[1] for r in hits)
+            return hits[0][0], [r for r in hits if r[1] == g]
+    return None, []
+
+
+def summarize_pmc(rd_db, wr_db, out_json):
+    """HBM traffic per launch from two rocprofv3 --pmc runs of --probe:
+    reads = 32, 64 and 128-byte read requests (TCC_EA0_RDREQ_{32B,64B,128B}),
+    writes = 64-byte write requests plus 32 bytes for each other write
+    request (TCC_EA0_WRREQ, TCC_EA0_WRREQ_64B); no calibration factor (the
+    copy kernel reads and writes exactly its array size this way)."""
+    import collections
+    import sqlite3
+
+    def load(db):
+        per = collections.defaultdict(dict)
+        con = sqlite3.connect(db)
+        for d, k, g, cn, v in con.execute(
+                "select dispatch_id, kernel_name, grid_size, counter_name, value "
+                "from counters_collection"):
+            per[d].update({"name": k, "grid": g, cn: v})
+        return per
+    rd, wr = load(rd_db), load(wr_db)
+
+    def recs(per, fn):
+        return [(v["name"], v["grid"], fn(v)) for v in per.values()]
+    rbytes = recs(rd, lambda v: 32 * v.get("TCC_EA0_RDREQ_32B_sum", 0) +
+                  64 * v.get("TCC_EA0_RDREQ_64B_sum", 0) +
+                  128 * v.get("TCC_EA0_RDREQ_128B_sum", 0))
+    wbytes = recs(wr, lambda v: 64 * v.get("TCC_EA0_WRREQ_64B_sum", 0) +
+                  32 * (v.get("TCC_EA0_WRREQ_sum", 0) - v.get("TCC_EA0_WRREQ_64B_sum", 0)))
+    out = {}
+    for row in ROW_KERNELS:
+        kr, hr = _row_dispatches(rbytes, row)
+        kw, hw = _row_dispatches(wbytes, row)
+        if not hr or not hw:
+            continue
+        r = sum(x[2] for x in hr) / len(hr)
+        w = sum(x[2] for x in hw) / len(hw)
+        out[row] = {"kernel": kr.split("(")[0], "grid": hr[0][1], "dispatches": len(hr),
+                    "read_bytes": int(r), "write_bytes": int(w), "bytes_per_launch": int(r + w)}
+    doc = {"counters": ["TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum",
+                        "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum"],
+           "bytes": "read = 32*RDREQ_32B + 64*RDREQ_64B + 128*RDREQ_128B; write = "
+                    "64*WRREQ_64B + 32*(WRREQ - WRREQ_64B); averaged over the dispatches of "
+                    "the row's kernel at its largest grid", "kernels": out}
+    with open(out_json, "w") as fh:
+        json.dump(doc, fh, indent=1)
+    return doc
+
+
+def summarize_trace(db, nsteps):
+    """Per-kernel table of a rocprofv3 --kernel-trace database: launches,
+    total and average duration, share; per step = total / nsteps."""
+    import collections
+    import sqlite3
+    con = sqlite3.connect(db)
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    for name, t0, t1 in con.execute("select name, start, end from kernels"):
+        a = agg[name]
+        a[0] += 1
+        a[1] += (t1 - t0) / 1e3
+    tot = sum(v[1] for v in agg.values())
+    lines = ["| kernel | launches | total us | avg us | per step us | share |",
+             "|---|---:|---:|---:|---:|---:|"]
+    for name, (n, t) in sorted(agg.items(), key=lambda x: -x[1][1]):
+        lines.append("| `%s` | %d | %.1f | %.2f | %.1f | %.1f%% |"
+                     % (name.split("(")[0].replace("void ", ""), n, t, t / n, t / nsteps,
+                        100.0 * t / tot))
+    lines.append("")
+    lines.append("kernel time %.1f us over %d launches; %.1f us per step over %d steps"
+                 % (tot, sum(v[0] for v in agg.values()), tot / nsteps, nsteps))
+    return "\n".join(lines)
+
+
+def cpu_info():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def _time_steps(sol, warm, reps):
+    for _ in range(warm):
+        sol.step()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        sol.step()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
+def cpu_baselines(a):
+    """The C oracle (the reference's FP64 schedule in C, OpenMP) and the NumPy
+    oracle on the host: the headline workload on all cores (median of 3 steps
+    after one) and on one thread (one step), and the 32^3 and 80^3 cases of
+    the reference on one thread and on all cores."""
     from oracle import c_oracle as CO
     from oracle import np_oracle as O
+    from warpflow import load_case
+    model, ncpu = cpu_info()
+    allc = CO.threads()
+    legs = {}
     case = O.Case(n3=(a.n,) * 3, reynolds=a.reynolds, sgs_enabled=True, cw=0.46, noise=0.01,
                   mg_levels=a.levels, variable_dt=False, dt_init=a.dt)
     sol = CO.CSolver(case)
-    sol.step()  # untimed: first touch of every array
-    t0 = time.perf_counter()
-    sol.step()
-    sec = time.perf_counter() - t0
-    return {"value": round(a.n ** 3 / sec / 1e6, 3), "unit": "Mcell/s", "cores": CO.threads(),
+    sec = _time_steps(sol, 1, 3)
+    cells = a.n ** 3
+    base = {"value": round(cells / sec / 1e6, 3), "unit": "Mcell/s", "cores": allc,
             "kind": "port",
-            "sample": "step 2 of the same %d^3 Re-%d cavity (plain FP64 schedule, %d MG levels) "
-                      "in the C oracle (oracle/c/wf_oracle.c), %.2f s"
-                      % (a.n, a.reynolds, a.levels, sec)}
+            "sample": "median of steps 2-4 of the same %d^3 Re-%d cavity (plain FP64 schedule, "
+                      "%d MG levels, 4 V-cycles) in the C oracle (oracle/c/wf_oracle.c), %.3f s "
+                      "per step on %d threads of %s" % (a.n, a.reynolds, a.levels, sec, allc,
+                                                        model)}
+    legs["cavity%d_c_all" % a.n] = {"threads": allc, "ms_per_step": round(1e3 * sec, 1),
+                                    "Mcell_s": base["value"]}
+    CO.set_threads(1)
+    try:
+        sec1 = _time_steps(sol, 0, 1)
+        legs["cavity%d_c_1thread" % a.n] = {"threads": 1, "ms_per_step": round(1e3 * sec1, 1),
+                                            "Mcell_s": round(cells / sec1 / 1e6, 3)}
+        del sol
+        for tag, path, warm, reps in (("cavity32_re100", "cavity_re100_32.yaml", 2, 5),
+                                      ("cavity80_re400", "cavity_serial.yaml", 1, 3)):
+            cfg = load_case(os.path.join(PKG, "cases", path))
+            ncell = cfg.grid.nx * cfg.grid.ny * cfg.grid.nz
+            for thr in (1, allc):
+                CO.set_threads(thr)
+                s = _time_steps(CO.CSolver(O.Case.from_config(cfg)), warm, reps)
+                legs["%s_c_%s" % (tag, "1thread" if thr == 1 else "all")] = {
+                    "threads": thr, "ms_per_step": round(1e3 * s, 2),
+                    "Mcell_s": round(ncell / s / 1e6, 3)}
+            if tag == "cavity32_re100":
+                s = _time_steps(O.Solver(O.Case.from_config(cfg)), 1, 2)
+                legs["cavity32_re100_numpy"] = {"threads": 1, "ms_per_step": round(1e3 * s, 1),
+                                                "Mcell_s": round(ncell / s / 1e6, 4)}
+    finally:
+        CO.set_threads(allc)
+    legs["host"] = {"cpu": model, "logical_cpus": ncpu}
+    return base, legs
+
+
+def cavity80(a):
+    """cases/cavity_serial.yaml (80^3, Re 400, WALE, adaptive dt, 3 levels) in
+    both forms: the only configuration the reference publishes a rate for."""
+    from warpflow import FlowSolver, build_topology, load_case
+    from warpflow.device import synchronize
+    out = {}
+    for form in ("fp64", "mixed"):
+        cfg = load_case(os.path.join(PKG, "cases", "cavity_serial.yaml"))
+        cfg.numerics.mg_precision = form
+        sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
+        tune(a, sol)
+        for _ in range(30):
+            sol.step()
+        synchronize()
+        nsteps = 300
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            info = sol.step()
+        synchronize()
+        sec = (time.perf_counter() - t0) / nsteps
+        out[form] = {"ms_per_step": round(1e3 * sec, 3),
+                     "Mcell_s": round(80 ** 3 / sec / 1e6, 1), "dt": info["dt"],
+                     "iters": info["iters"]}
+        del sol
+    out["reference_Mcell_s"] = REPORT_80_MCELLS
+    out["reference_hardware"] = "RTX 5060 Laptop GPU, FP64 (REPORT.md:101-103)"
+    out["speedup_fp64_vs_reference"] = round(out["fp64"]["Mcell_s"] / REPORT_80_MCELLS, 1)
+    out["steps_timed"] = "steps 31-330 of the case (adaptive dt near its saturated 4.8e-3)"
+    return out
 
 
 def deviation(sa, sb):
@@ -333,7 +596,9 @@ def run_ensemble(a):
             "data": "synthetic (seeded member draw and hash-noise initial states)",
             "config": {"workload": "ensemble64: 64 cavities, n from {32,48,64,96}, Re in "
                                    "[100,1000], WALE, fixed dt 5e-4, 4 MG levels",
-                       "cells": cells, "graph": True},
+                       "cells": cells, "graph": True,
+                       "launches": "one launch per operator for all members (not one "
+                                   "persistent launch per RK stage), graph-captured step"},
             "sequential_fp64_value": round(out["sequential_fp64"], 3),
             "sequential_mixed_value": round(out["sequential_mixed"], 3),
             "speedup_vs_sequential_fp64": round(out["batched"] / out["sequential_fp64"], 3),
@@ -349,6 +614,12 @@ def main(argv=None):
         print("bench.py: this build runs one cavity on one GPU; --gpus %d is not supported "
               "(multi-GPU domain decomposition is out of scope)" % a.gpus, file=sys.stderr)
         return 2
+    if a.summarize_trace:
+        print(summarize_trace(a.summarize_trace, a.steps))
+        return 0
+    if a.summarize_pmc:
+        print(json.dumps(summarize_pmc(*a.summarize_pmc)["kernels"], indent=1))
+        return 0
     if a.probe:
         return probe(a)
     if a.workload == "ensemble64":
@@ -373,29 +644,44 @@ def main(argv=None):
             "config": {"workload": "lid-driven cavity %d^3, Re %g, WALE cw 0.46, fixed dt %g, "
                                    "rmax<1e-6, %s form" % (a.n, a.reynolds, a.dt, head),
                        "n": a.n, "mg_levels": a.levels,
-                       "vcycles_per_solve": {"fp64": 4, "mixed": a.mixed_cycles},
+                       "vcycles_per_solve": {"fp64": 4, "mixed": a.mixed_cycles, "mixed4": 4},
                        "sweeps": "V(6,6)+300 coarsest"}}
     line["config"]["launch_shapes"] = {f: res[f]["tuned"] for f in res}  # [knob, n3, value]
-    if "fp64" in res:
-        line["fp64_value"] = round(res["fp64"]["mcell"], 3)
-        line["fp64_ms_per_step"] = round(res["fp64"]["ms"], 4)
-        if head == "mixed":
-            line["speedup_mixed_vs_fp64"] = round(res["mixed"]["mcell"] / res["fp64"]["mcell"], 4)
-            line["deviation_mixed_vs_fp64"] = deviation(res["mixed"]["sol"], res["fp64"]["sol"])
-            line["rmax_trajectory"] = {f: [float("%.4g" % i["rmax"]) for i in res[f]["infos"]]
-                                       for f in ("mixed", "fp64")}
+    for f in res:
+        if f != head:
+            line["%s_value" % f] = round(res[f]["mcell"], 3)
+            line["%s_ms_per_step" % f] = round(res[f]["ms"], 4)
+    if "fp64" in res and head == "mixed":
+        line["speedup_mixed_vs_fp64"] = round(res["mixed"]["mcell"] / res["fp64"]["mcell"], 4)
+        if "mixed4" in res:
+            line["speedup_mixed_vs_fp64_equal_cycles"] = round(
+                res["mixed4"]["mcell"] / res["fp64"]["mcell"], 4)
+        line["deviation_mixed_vs_fp64"] = deviation(res["mixed"]["sol"], res["fp64"]["sol"])
+        line["rmax_trajectory"] = {f: [float("%.4g" % i["rmax"]) for i in res[f]["infos"]]
+                                   for f in res}
+    for f in list(res):
+        if f not in ("mixed", "fp64"):
+            del res[f]["sol"]
     if not a.no_kernels and "fp64" in res and "mixed" in res:
         rows = kernel_table(a, res["fp64"]["sol"], res["mixed"]["sol"])
         line["kernels"] = rows
-        dom = dominant(rows, "per_step_mixed" if head == "mixed" else "per_step_fp64")
-        achieved = dom["bytes"] / dom["us"] / 1e3
-        line["roofline"] = {"bound": "hbm", "kernel": dom["kernel"],
+        key = "us_per_step_mixed" if head == "mixed" else "us_per_step_fp64"
+        dom = max(rows, key=lambda r: r[key])
+        achieved = dom["bytes"] / dom["us_cold"] / 1e3
+        line["roofline"] = {"bound": "hbm", "kernel": dom["kernel"], "row": dom["name"],
                             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                            "traffic": dom.get("traffic"),
-                            "algorithmic_bytes": dom["bytes"]}
+                            "traffic": dom.get("traffic_per_launch"),
+                            "algorithmic_bytes_per_launch": int(dom["bytes"] / dom["launches"]),
+                            "us_per_launch": dom["us_per_launch"],
+                            "launches_per_step": dom["launches"] * dom[
+                                "per_step_mixed" if head == "mixed" else "per_step_fp64"]}
+    for f in list(res):
+        res[f].pop("sol", None)
+    if not a.no_80:
+        line["cavity80"] = cavity80(a)
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(a)
+        line["cpu_baseline"], line["cpu_legs"] = cpu_baselines(a)
     print(json.dumps(line), flush=True)
     return 0
 

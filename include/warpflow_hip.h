@@ -491,7 +491,7 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  *   WF_TUNE_SWEEP2_TILE_F64/F32 tile of the two-sweep kernel (pairs of sweeps
  *                               in a V-cycle): 1 = 8 x 64, 2 = 16 x 32
  *   WF_TUNE_SWEEP2_ICH_F64/F32  i-planes per workgroup of the two-sweep kernel
- *   WF_TUNE_SWEEP_PAIRS         pairs of sweeps of this level as 1 = two
+ *   WF_TUNE_SWEEP_PAIRS_F64/F32 pairs of sweeps of this level as 1 = two
  *                               one-sweep launches, 2 = one two-sweep launch
  *                               (built-in: 2 for FP64 levels of >= 2^23
  *                               cells, else 1)
@@ -510,9 +510,10 @@ enum {
   WF_TUNE_SWEEP2_TILE_F32 = 6,
   WF_TUNE_SWEEP2_ICH_F64 = 7,
   WF_TUNE_SWEEP2_ICH_F32 = 8,
-  WF_TUNE_SWEEP_PAIRS = 9,
+  WF_TUNE_SWEEP_PAIRS_F64 = 9,
   WF_TUNE_PROJ_ICH = 10,
-  WF_TUNE_COUNT = 11
+  WF_TUNE_SWEEP_PAIRS_F32 = 11,
+  WF_TUNE_COUNT = 12
 };
 int wf_tune_set(int knob, int n0, int n1, int n2, int value);
 int wf_tune_get(int knob, int n0, int n1, int n2, int* value);

@@ -617,3 +617,12 @@ int wfo_threads(void) {
   return 1;
 #endif
 }
+
+/* Threads for the following steps (timing legs: 1 thread, all cores). */
+void wfo_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}

@@ -32,12 +32,18 @@ def lib():
         L.wfo_rmax_history.argtypes = [P, P, I]
         L.wfo_rmax_history.restype = I
         L.wfo_threads.restype = I
+        L.wfo_set_threads.argtypes = [I]
         _lib = L
     return _lib
 
 
 def threads() -> int:
     return lib().wfo_threads()
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the following steps."""
+    lib().wfo_set_threads(int(n))
 
 
 class CSolver:

@@ -204,10 +204,11 @@ def test_sweep_tiles_agree_bitwise(gpu, prec, tile):
     d = lev.desc()
     knob_t = autotune.SWEEP_TILE_F64 if prec == np.float64 else autotune.SWEEP_TILE_F32
     knob_i = autotune.SWEEP_ICH_F64 if prec == np.float64 else autotune.SWEEP_ICH_F32
+    knob_p = autotune.SWEEP_PAIRS_F64 if prec == np.float64 else autotune.SWEEP_PAIRS_F32
     lev.rhs.assign(rhs)
     autotune.reset()
     try:
-        autotune.set_shape(autotune.SWEEP_PAIRS, n3, 1)  # one-sweep launches only
+        autotune.set_shape(knob_p, n3, 1)  # one-sweep launches only
         lev.phi.assign(phi0)
         _hip.call("wf_mg_relax", C.byref(d), 4, None)
         want = lev.phi.numpy()
@@ -240,14 +241,15 @@ def test_two_sweep_launches_agree_bitwise(gpu, prec, tile):
     d = lev.desc()
     knob_t = autotune.SWEEP2_TILE_F64 if prec == np.float64 else autotune.SWEEP2_TILE_F32
     knob_i = autotune.SWEEP2_ICH_F64 if prec == np.float64 else autotune.SWEEP2_ICH_F32
+    knob_p = autotune.SWEEP_PAIRS_F64 if prec == np.float64 else autotune.SWEEP_PAIRS_F32
     lev.rhs.assign(rhs)
     autotune.reset()
     try:
-        autotune.set_shape(autotune.SWEEP_PAIRS, n3, 1)
+        autotune.set_shape(knob_p, n3, 1)
         lev.phi.assign(phi0)
         _hip.call("wf_mg_relax", C.byref(d), 4, None)
         want = lev.phi.numpy()
-        autotune.set_shape(autotune.SWEEP_PAIRS, n3, 2)
+        autotune.set_shape(knob_p, n3, 2)
         for ich in (1, 3, 16, 40):
             autotune.set_shape(knob_t, n3, tile)
             autotune.set_shape(knob_i, n3, ich)
@@ -287,7 +289,7 @@ def test_vcycle_two_sweep_parity(gpu, npre, npost):
     try:
         for pairs in (1, 2):
             for lev in mg.levels:
-                autotune.set_shape(autotune.SWEEP_PAIRS, lev.n3, pairs)
+                autotune.set_shape(autotune.SWEEP_PAIRS_F64, lev.n3, pairs)
             for lev, p in zip(mg.levels, phis):
                 lev.phi.assign(p)
             _hip.call("wf_mg_vcycle", descs, len(mg.levels), npre, npost, 8, current_stream())

@@ -237,7 +237,8 @@ int wf_tune_set(int knob, int n0, int n1, int n2, int value) {
     return WF_EINVAL;
   if ((knob == WF_TUNE_SWEEP2_TILE_F64 || knob == WF_TUNE_SWEEP2_TILE_F32) && value > 2)
     return WF_EINVAL;
-  if (knob == WF_TUNE_SWEEP_PAIRS && value > 2) return WF_EINVAL;
+  if ((knob == WF_TUNE_SWEEP_PAIRS_F64 || knob == WF_TUNE_SWEEP_PAIRS_F32) && value > 2)
+    return WF_EINVAL;
   const int q = tune_find(knob, n0, n1, n2);
   if (value == 0) {  // remove
     if (q >= 0) g_tune[q] = g_tune[--g_ntune];

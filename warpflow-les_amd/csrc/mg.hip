@@ -1700,7 +1700,8 @@ int relax_t(const wf_mg_level& L, int nsweeps, hipStream_t s, bool allow_swap = 
     return r;
   };
   int q = 0;
-  const int pairs = tune_get(WF_TUNE_SWEEP_PAIRS, L.n0, L.n1, L.n2);
+  const int pairs = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP_PAIRS_F64 : WF_TUNE_SWEEP_PAIRS_F32,
+                             L.n0, L.n1, L.n2);
   // built-in: two-sweep launches where they measured faster (FP64, >= 2^23 cells)
   if (C.tmp && (pairs == 2 || (pairs == 0 && sizeof(T) == 8 && cells >= ((size_t)1 << 23)))) {
     int npair = nsweeps / 2;

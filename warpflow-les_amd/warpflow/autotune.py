@@ -6,8 +6,8 @@ parameters that change no result: the i-planes each workgroup streams
 through (``cd_ich``, ``proj_ich``, ``sweep_ich``, ``sweep2_ich``) and the
 (j, k) tile of the one-sweep kernel (``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32,
 3 = 16 x 64, 4 = 32 x 32) and of the two-sweep kernel (``sweep2_tile``: 1 =
-8 x 64, 2 = 16 x 32); ``sweep_pairs`` picks, per level, one-sweep launches
-(1) or two-sweep launches (2) for pairs of sweeps.  The library keeps one setting per (parameter,
+8 x 64, 2 = 16 x 32); ``sweep_pairs`` picks, per level and precision,
+one-sweep launches (1) or two-sweep launches (2) for pairs of sweeps.  The library keeps one setting per (parameter,
 launch shape); ``tune_solver`` times every candidate with HIP events on a
 solver's own arrays -- the momentum kernel on the solver's grid, the sweeps
 on each multigrid level that uses them -- checks that each candidate's
@@ -28,7 +28,8 @@ from .device import copy, current_stream, zeros, zeros_like
 
 # knob ids (include/warpflow_hip.h)
 (CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32, SWEEP2_TILE_F64,
- SWEEP2_TILE_F32, SWEEP2_ICH_F64, SWEEP2_ICH_F32, SWEEP_PAIRS, PROJ_ICH) = range(11)
+ SWEEP2_TILE_F32, SWEEP2_ICH_F64, SWEEP2_ICH_F32, SWEEP_PAIRS_F64, PROJ_ICH,
+ SWEEP_PAIRS_F32) = range(12)
 CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
 SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
 SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32)
@@ -161,6 +162,7 @@ def _tune_level(lev, T, reps, out):
     f64 = np.dtype(lev.dtype) == np.float64
     ktile = SWEEP_TILE_F64 if f64 else SWEEP_TILE_F32
     kich = SWEEP_ICH_F64 if f64 else SWEEP_ICH_F32
+    kpairs = SWEEP_PAIRS_F64 if f64 else SWEEP_PAIRS_F32
     keep_phi, keep_tmp = zeros_like(lev.phi), zeros_like(lev.tmp)
     copy(keep_phi, lev.phi)
     copy(keep_tmp, lev.tmp)
@@ -171,7 +173,7 @@ def _tune_level(lev, T, reps, out):
         _hip.call("wf_mg_relax", C.byref(d), 2, current_stream())
     set_shape(ktile, n3, 0)
     set_shape(kich, n3, 0)
-    set_shape(SWEEP_PAIRS, n3, 0)
+    set_shape(kpairs, n3, 0)
     relax()
     want = lev.phi.numpy()
     t_copy = T.us(lambda: copy(lev.phi, keep_phi), reps)
@@ -198,13 +200,13 @@ def _tune_level(lev, T, reps, out):
     def relax4():
         copy(lev.phi, keep_phi)
         _hip.call("wf_mg_relax", C.byref(d), 4, current_stream())
-    set_shape(SWEEP_PAIRS, n3, 1)
+    set_shape(kpairs, n3, 1)
     relax4()
     want4 = lev.phi.numpy()
     t_single = T.us(relax4, reps) - t_copy
     out["timings_us"]["4 sweeps %s %s one-sweep launches" % (
         "x".join(map(str, n3)), "f64" if f64 else "f32")] = round(t_single, 2)
-    set_shape(SWEEP_PAIRS, n3, 2)
+    set_shape(kpairs, n3, 2)
     best2 = None
     try:
         for tile in SWEEP2_TILE_CANDIDATES:
@@ -226,10 +228,10 @@ def _tune_level(lev, T, reps, out):
         set_shape(k2tile, n3, best2[1])
         set_shape(k2ich, n3, best2[2])
         out["entries"] += [(k2tile, tuple(n3), best2[1]), (k2ich, tuple(n3), best2[2]),
-                           (SWEEP_PAIRS, tuple(n3), 2)]
+                           (kpairs, tuple(n3), 2)]
     else:
-        set_shape(SWEEP_PAIRS, n3, 1)
-        out["entries"].append((SWEEP_PAIRS, tuple(n3), 1))
+        set_shape(kpairs, n3, 1)
+        out["entries"].append((kpairs, tuple(n3), 1))
     copy(lev.phi, keep_phi)
     copy(lev.tmp, keep_tmp)
 
