@@ -61,7 +61,9 @@ CASE32 = os.path.join(PKG, "cases", "cavity_re100_32.yaml")
 def test_32cube_re100_200_steps_vs_c_oracle(gpu):
     """The CPU-runnable reference case (32^3, Re 100, WALE off, fixed dt
     1e-3, 3 levels, 200 steps): identical outer-iteration counts and rmax
-    trajectory, fields to rounding."""
+    trajectory, fields to rounding.  The boundaries are the reference's
+    cavity set (south and north are slip planes), not no-slip on every face:
+    the reference's fills support only that set (bcs.py:166-184)."""
     from warpflow import FlowSolver, build_topology, load_case
     cfg = load_case(CASE32)
     sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
