@@ -489,7 +489,8 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  *                               2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32
  *   WF_TUNE_SWEEP_ICH_F64/F32   i-planes per workgroup of the fused sweep
  *   WF_TUNE_SWEEP2_TILE_F64/F32 tile of the two-sweep kernel (pairs of sweeps
- *                               in a V-cycle): 1 = 8 x 64, 2 = 16 x 32
+ *                               in a V-cycle): 1 = 8 x 64, 2 = 16 x 32; FP32
+ *                               also 3 = 16 x 64, 4 = 32 x 32
  *   WF_TUNE_SWEEP2_ICH_F64/F32  i-planes per workgroup of the two-sweep kernel
  *   WF_TUNE_SWEEP_PAIRS_F64/F32 pairs of sweeps of this level as 1 = two
  *                               one-sweep launches, 2 = one two-sweep launch

@@ -222,13 +222,14 @@ def test_sweep_tiles_agree_bitwise(gpu, prec, tile):
         autotune.reset()
 
 
-@pytest.mark.parametrize("prec", [np.float64, np.float32])
-@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("prec,tile", [(np.float64, 1), (np.float64, 2), (np.float32, 1),
+                                       (np.float32, 2), (np.float32, 3), (np.float32, 4)])
 def test_two_sweep_launches_agree_bitwise(gpu, prec, tile):
     """Pairs of sweeps as one two-sweep launch (4 lagged stages, halo
-    recomputation) give the one-sweep launches' result bit for bit, for both
-    tiles and several i-chunks (the first and last chunks touch the i faces),
-    on a level whose extents are multiples of none of them."""
+    recomputation) give the one-sweep launches' result bit for bit, for every
+    tile (FP32 also with two pairs per thread: 16 x 64, 32 x 32) and several
+    i-chunks (the first and last chunks touch the i faces), on a level whose
+    extents are multiples of none of them."""
     import ctypes as C
     from warpflow import _hip, autotune
     from warpflow.multigrid import _Level

@@ -235,7 +235,8 @@ int wf_tune_set(int knob, int n0, int n1, int n2, int value) {
     return WF_EINVAL;
   if ((knob == WF_TUNE_SWEEP_TILE_F64 || knob == WF_TUNE_SWEEP_TILE_F32) && value > 4)
     return WF_EINVAL;
-  if ((knob == WF_TUNE_SWEEP2_TILE_F64 || knob == WF_TUNE_SWEEP2_TILE_F32) && value > 2)
+  if ((knob == WF_TUNE_SWEEP2_TILE_F64 && value > 2) ||
+      (knob == WF_TUNE_SWEEP2_TILE_F32 && value > 4))
     return WF_EINVAL;
   if ((knob == WF_TUNE_SWEEP_PAIRS_F64 || knob == WF_TUNE_SWEEP_PAIRS_F32) && value > 2)
     return WF_EINVAL;

@@ -903,10 +903,14 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
 // The FP32 two-sweep kernel is held to 128 VGPRs so that four workgroups
 // share a CU (it fits without spilling); the FP64 one keeps the compiler's
 // allocation.
-template <class T>
-constexpr int sweep2_min_blocks() { return sizeof(T) == 4 ? 4 : 1; }
+// The FP32 tiles with two pairs per thread (16 x 64, 32 x 32) carry twice the
+// stage tables and are held to 256 VGPRs (two workgroups per CU).
 template <class T, int TJ, int TK>
-__global__ void __launch_bounds__(256, (sweep2_min_blocks<T>()))
+constexpr int sweep2_min_blocks() {
+  return sizeof(T) == 4 ? (TJ * TK == 512 ? 4 : 2) : 1;
+}
+template <class T, int TJ, int TK>
+__global__ void __launch_bounds__(256, (sweep2_min_blocks<T, TJ, TK>()))
 sweep2_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
               MgDev d, T c0, T c1, T c2, int ich) {
   sweep2_body<T, TJ, TK>(pin, pout, rhs, d, c0, c1, c2, ich, this_blk());
@@ -1641,6 +1645,10 @@ template <class T>
 int sweep2_t(const wf_mg_level& L, const void* pin, void* pout, hipStream_t s) {
   const int tile = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP2_TILE_F64 : WF_TUNE_SWEEP2_TILE_F32,
                             L.n0, L.n1, L.n2);
+  if constexpr (sizeof(T) == 4) {  // two pairs per thread: FP32 only (LDS, registers)
+    if (tile == 3) return launch_sweep2<T, 16, 64>(L, pin, pout, s);
+    if (tile == 4) return launch_sweep2<T, 32, 32>(L, pin, pout, s);
+  }
   return tile == 1 ? launch_sweep2<T, 8, 64>(L, pin, pout, s)
                    : launch_sweep2<T, 16, 32>(L, pin, pout, s);
 }

@@ -6,7 +6,7 @@ parameters that change no result: the i-planes each workgroup streams
 through (``cd_ich``, ``proj_ich``, ``sweep_ich``, ``sweep2_ich``) and the
 (j, k) tile of the one-sweep kernel (``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32,
 3 = 16 x 64, 4 = 32 x 32) and of the two-sweep kernel (``sweep2_tile``: 1 =
-8 x 64, 2 = 16 x 32); ``sweep_pairs`` picks, per level and precision,
+8 x 64, 2 = 16 x 32, and in FP32 also 3 = 16 x 64, 4 = 32 x 32); ``sweep_pairs`` picks, per level and precision,
 one-sweep launches (1) or two-sweep launches (2) for pairs of sweeps.  The library keeps one setting per (parameter,
 launch shape); ``tune_solver`` times every candidate with HIP events on a
 solver's own arrays -- the momentum kernel on the solver's grid, the sweeps
@@ -33,7 +33,7 @@ from .device import copy, current_stream, zeros, zeros_like
 CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
 SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
 SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32)
-SWEEP2_TILE_CANDIDATES = (1, 2)
+SWEEP2_TILE_CANDIDATES = {True: (1, 2), False: (1, 2, 3, 4)}  # by "is FP64"
 SWEEP2_ICH_CANDIDATES = (0, 2, 4, 8, 16, 32, 64)
 PROJ_ICH_CANDIDATES = (0, 32, 64, 86, 128, 172, 256)
 SWEEP_MIN_CELLS = 4096  # smaller levels relax in one LDS-resident workgroup
@@ -209,7 +209,7 @@ def _tune_level(lev, T, reps, out):
     set_shape(kpairs, n3, 2)
     best2 = None
     try:
-        for tile in SWEEP2_TILE_CANDIDATES:
+        for tile in SWEEP2_TILE_CANDIDATES[f64]:
             for ich in SWEEP2_ICH_CANDIDATES:
                 set_shape(k2tile, n3, tile)
                 set_shape(k2ich, n3, ich)
