@@ -70,3 +70,23 @@ def test_cli_restart_is_exact_with_adaptive_dt(gpu, tmp_path, capsys):
     assert int(x["itime"]) == int(y["itime"]) == 8
     for k in ("u", "v", "w", "p", "vis", "ctime", "dt"):
         assert np.array_equal(x[k], y[k]), k
+
+
+def test_h3d_restart_export_of_a_fresh_solver(gpu, tmp_path):
+    """export (tests/h3d_bridge.py:59-115 in the reference): the restart of
+    a freshly initialised cavity holds the solver's real cells of u, v, w, p,
+    Hydro3D's 4 ghost layers and vis = 1/Re."""
+    from warpflow import FlowSolver, build_topology, load_case
+    from warpflow import h3d
+    cfg = load_case(os.path.join(ROOT, "cases", "cavity_serial.yaml"))
+    cfg.grid.nx = cfg.grid.ny = cfg.grid.nz = 16
+    sol = FlowSolver(cfg, build_topology(cfg), "hip:0")
+    path = h3d.export_restart(sol, tmp_path)
+    assert (tmp_path / "final_ctime.dat").read_text().startswith("       0")
+    got = h3d.read_tecbin(path, raw=True)
+    n, pl = 16, h3d.PL_H3D
+    r = slice(pl, pl + n)
+    for k in ("u", "v", "w", "p"):
+        assert np.array_equal(got[k][r, r, r], sol.f[k].numpy()[3:3 + n, 3:3 + n, 3:3 + n]), k
+    assert np.all(got["vis"] == sol.rey)
+    assert np.all(got["u"][r, r, pl + n:] == sol.ulid)  # lid ghosts
