@@ -75,9 +75,12 @@ def parse(argv=None):
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel table")
     ap.add_argument("--no-80", action="store_true", help="skip the 80^3 production case")
     ap.add_argument("--reps", type=int, default=10, help="timed launches per kernel")
-    ap.add_argument("--workload", default="cavity256", choices=("cavity256", "ensemble64"),
+    ap.add_argument("--workload", default="cavity256",
+                    choices=("cavity256", "ensemble64", "poisson256"),
                     help="cavity256: the headline 256^3 step; ensemble64: 64 uneven cavities "
-                         "batched (mixed MG, graph-captured step) against sequential solves")
+                         "batched (mixed MG, graph-captured step) against sequential solves; "
+                         "poisson256: the standalone 256^3 Poisson solve to 1e-10 in both "
+                         "forms with its multigrid kernels' GB/s")
     ap.add_argument("--seq-steps", type=int, default=3,
                     help="ensemble64: timed steps of the sequential single-cavity baselines")
     ap.add_argument("--no-autotune", action="store_true",
@@ -93,6 +96,9 @@ def parse(argv=None):
     ap.add_argument("--summarize-trace", metavar="DB",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace database "
                          "(no GPU needed); --steps: the steps it covers")
+    ap.add_argument("--summarize-sq", nargs="+", metavar="DB",
+                    help="print per-wave SQ counters of the table rows from rocprofv3 --pmc "
+                         "runs of --probe (no GPU needed)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("READ_DB", "WRITE_DB", "OUT_JSON"),
                     help="write HBM bytes per launch from the two rocprofv3 --pmc runs of "
                          "--probe (no GPU needed)")
@@ -465,6 +471,57 @@ def summarize_pmc(rd_db, wr_db, out_json):
     return doc
 
 
+def summarize_sq(dbs):
+    """Per-wave SQ counters of the table rows from rocprofv3 --pmc runs of
+    --probe (one database per pass): counter / SQ_WAVES averaged over the
+    row's dispatches at its largest grid; "of life" = counter / SQ_WAVE_CYCLES
+    (both in the same cycle units)."""
+    import collections
+    import sqlite3
+    per = collections.defaultdict(dict)  # (db index, dispatch) -> counters
+    for q, db in enumerate(dbs):
+        con = sqlite3.connect(db)
+        for d, k, g, cn, v in con.execute(
+                "select dispatch_id, kernel_name, grid_size, counter_name, value "
+                "from counters_collection"):
+            per[(q, d)].update({"name": k, "grid": g, cn: v})
+    recs = [(v["name"], v["grid"], v) for v in per.values()]
+    cols = [("SQ_WAVES", "waves", False), ("SQ_WAVE_CYCLES", "cycles / wave", True),
+            ("SQ_INSTS_VALU", "VALU insts", True), ("SQ_INSTS_SALU", "SALU insts", True),
+            ("SQ_INSTS_LDS", "LDS insts", True), ("SQ_INSTS_VMEM_RD", "VMEM rd", True),
+            ("SQ_INSTS_VMEM_WR", "VMEM wr", True), ("SQ_LDS_BANK_CONFLICT", "LDS conflict cyc",
+                                                    True)]
+    life = [("SQ_ACTIVE_INST_VALU", "VALU active"), ("SQ_ACTIVE_INST_LDS", "LDS active"),
+            ("SQ_ACTIVE_INST_SCA", "SALU active"), ("SQ_WAIT_INST_ANY", "wait on deps"),
+            ("SQ_WAIT_ANY", "any wait")]
+    lines = ["| row | kernel | " + " | ".join(c[1] for c in cols) + " | " +
+             " | ".join(c[1] + " (of life)" for c in life) + " |",
+             "|---|---|" + "---:|" * (len(cols) + len(life))]
+    for row in ROW_KERNELS:
+        avg = {}
+        name = None
+        for cn, _, _ in cols + [(x[0], None, None) for x in life]:
+            hits = [r for r in recs if cn in r[2]]
+            k, h = _row_dispatches(hits, row)
+            if h:
+                name = k
+                avg[cn] = sum(x[2][cn] for x in h) / len(h)
+        if not name or not avg.get("SQ_WAVES"):
+            continue
+        w = avg["SQ_WAVES"]
+        cells = []
+        for cn, _, perwave in cols:
+            v = avg.get(cn)
+            cells.append("" if v is None else ("%.0f" % (v / w if perwave else v)))
+        wc = avg.get("SQ_WAVE_CYCLES")
+        for cn, _ in life:
+            v = avg.get(cn)
+            cells.append("" if v is None or not wc else "%.3f" % (v / wc))
+        lines.append("| %s | `%s` | %s |" % (row, name.split("(")[0].replace("void ", ""),
+                                               " | ".join(cells)))
+    return "\n".join(lines)
+
+
 def summarize_trace(db, nsteps):
     """Per-kernel table of a rocprofv3 --kernel-trace database: launches,
     total and average duration, share; per step = total / nsteps."""
@@ -683,6 +740,105 @@ def run_ensemble(a):
     return 0
 
 
+def run_poisson(a):
+    """Standalone Poisson solve at n^3 (cell-centred, homogeneous Neumann walls,
+    4 levels, the reference's standalone multigrid input: a seeded
+    standard-normal rhs, ring zeroed, interior demeaned): plain FP64 V(6,6)
+    cycles (300 coarsest sweeps) repeated until max|rhs - A phi| <= 1e-10
+    max|rhs|, against FP32 V-cycles inside FP64 defect correction stopped by
+    the same test; the per-kernel table of the level-0 and coarsest-level
+    work of both hierarchies (cold HIP-event timings)."""
+    from warpflow import _hip, build_topology
+    from warpflow.config import CaseConfig, validate
+    from warpflow.device import array, current_stream, synchronize, zeros
+    from warpflow.mixed import MixedMultigrid
+    from warpflow.multigrid import Multigrid
+    n, levels, rtol = a.n, 4, 1e-10
+    cfg = CaseConfig()
+    cfg.grid.nx = cfg.grid.ny = cfg.grid.nz = n
+    cfg.numerics.mg_levels = levels
+    validate(cfg)
+    topo = build_topology(cfg)
+    h = 1.0 / n
+    rhs = np.random.default_rng(0).standard_normal((n + 2,) * 3)
+    for ax in range(3):
+        sl = [slice(None)] * 3
+        sl[ax] = [0, -1]
+        rhs[tuple(sl)] = 0.0
+    rhs[1:-1, 1:-1, 1:-1] -= rhs[1:-1, 1:-1, 1:-1].mean()
+    bmax = float(np.abs(rhs[1:-1, 1:-1, 1:-1]).max())
+    mg = Multigrid(cfg, topo, "hip:0", h, h, h)
+    lev0 = mg.levels[0]
+    out = {}
+    for rep in range(2):  # an untimed solve first (first-touch, kernel loading)
+        lev0.rhs.assign(rhs)
+        lev0.phi.zero_()
+        mg._refill(lev0)
+        synchronize()
+        t0 = time.perf_counter()
+        hist = []
+        for _ in range(40):
+            mg._vcycle()
+            hist.append(mg.residual_max() / bmax)
+            if hist[-1] <= rtol:
+                break
+        out["fp64_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["fp64_cycles"], out["fp64_residual"] = len(hist), hist[-1]
+    phi64 = lev0.phi.numpy()[1:-1, 1:-1, 1:-1]
+    mm = MixedMultigrid(cfg, topo, "hip:0", h, h, h, rtol=rtol, max_cycles=40)
+    su, pp = array(rhs, np.float64), zeros(rhs.shape, np.float64)
+    for rep in range(2):
+        synchronize()
+        t0 = time.perf_counter()
+        mm.solve(su, pp, 1)
+        synchronize()
+        out["mixed_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["mixed_cycles"], out["mixed_residual"] = mm.last_cycles, mm.last_residual
+    pmx = pp.numpy()[1:-1, 1:-1, 1:-1]
+    d64, dmx = phi64 - phi64.mean(), pmx - pmx.mean()
+    err = float(np.sqrt(np.mean((d64 - dmx) ** 2)) / np.sqrt(np.mean(d64 ** 2)))
+    # per-kernel table: level 0 and the coarsest level of each hierarchy
+    T, flush, st = Timer(), Flush(), current_stream()
+    rows = []
+    for tag, m, eb in (("fp64", mg, 8), ("fp32", mm, 4)):
+        L = [lv.desc() for lv in m.levels]
+        cells0 = float(np.prod(m.levels[0].n3))
+        cellsc = float(np.prod(m.levels[-1].n3))
+        for name, fn, nbytes, launches in (
+                ("rbgs 12 sweeps L0 (%s)" % tag,
+                 lambda L=L: _hip.call("wf_mg_relax", C.byref(L[0]), 12, st),
+                 12 * 4 * eb * cells0, None),
+                ("restrict L0 (%s)" % tag,
+                 lambda L=L: _hip.call("wf_mg_restrict", C.byref(L[0]), C.byref(L[1]), st),
+                 17 * eb / 8 * cells0, 1),
+                ("prolong L0 (%s)" % tag,
+                 lambda L=L: _hip.call("wf_mg_prolong", C.byref(L[1]), C.byref(L[0]), st),
+                 17 * eb / 8 * cells0, 1),
+                ("rbgs 300 sweeps coarsest %d^3 (%s)" % (round(cellsc ** (1 / 3)), tag),
+                 lambda L=L: _hip.call("wf_mg_relax", C.byref(L[-1]), 300, st),
+                 300 * 4 * eb * cellsc, None)):
+            us = T.cold_us(fn, max(3, a.reps // 2), flush)
+            rows.append({"kernel": name, "us": round(us, 2), "bytes": int(nbytes),
+                         "GBps": round(nbytes / us / 1e3, 1),
+                         "hbm_frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 3)})
+    print("# poisson%d: fp64 %d cycles %.1f ms, mixed %d cycles %.1f ms, rms(p_mix - p_64) / "
+          "rms(p_64) = %.2e" % (n, out["fp64_cycles"], out["fp64_ms"], out["mixed_cycles"],
+                                out["mixed_ms"], err), file=sys.stderr, flush=True)
+    line = {"metric": "standalone Poisson %d^3 to 1e-10 relative residual: solve time" % n,
+            "value": round(out["mixed_ms"], 2), "unit": "ms", "n_gpus": 1,
+            "higher_is_better": False, "vs_baseline": None, "dtype": "f64 (V-cycle f32)",
+            "data": "synthetic (seeded standard-normal rhs, demeaned)",
+            "config": {"workload": "poisson%d: cell-centred Neumann, %d levels, V(6,6), 300 "
+                                   "coarsest sweeps" % (n, levels)},
+            "fp64_ms": round(out["fp64_ms"], 2), "fp64_cycles": out["fp64_cycles"],
+            "mixed_cycles": out["mixed_cycles"],
+            "speedup_mixed_vs_fp64": round(out["fp64_ms"] / out["mixed_ms"], 3),
+            "residual": {"fp64": out["fp64_residual"], "mixed": out["mixed_residual"]},
+            "rms_p_mixed_vs_fp64": err, "kernels": rows}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     a = parse(argv)
     if a.gpus != 1:
@@ -692,6 +848,9 @@ def main(argv=None):
     if a.summarize_trace:
         print(summarize_trace(a.summarize_trace, a.steps))
         return 0
+    if a.summarize_sq:
+        print(summarize_sq(a.summarize_sq))
+        return 0
     if a.summarize_pmc:
         print(json.dumps(summarize_pmc(*a.summarize_pmc)["kernels"], indent=1))
         return 0
@@ -699,6 +858,8 @@ def main(argv=None):
         return probe(a)
     if a.workload == "ensemble64":
         return run_ensemble(a)
+    if a.workload == "poisson256":
+        return run_poisson(a)
     forms = [f.strip() for f in a.forms.split(",") if f.strip()]
     res = {}
     for form in forms:

@@ -273,6 +273,15 @@ def test_ghost_fills_bitwise(gpu, n3):
     O.apply_p_mirror(f["q"], n3, 3)
     B.apply_p_mirror(d["q"], n3[0], 3)
     assert np.array_equal(d["q"].numpy(), f["q"])
+    # u, v, w in one call (one x launch and one y-then-z launch for the three)
+    h = rand_fields(n3, ["u", "v", "w"], seed=7)
+    dh = {k: up(v) for k, v in h.items()}
+    O.apply_u(h["u"], n3, 0.75)
+    O.apply_v(h["v"], n3)
+    O.apply_w(h["w"], n3)
+    B.apply_uvw(dh["u"], dh["v"], dh["w"], n3[0], 0.75)
+    for k in ("u", "v", "w"):
+        assert np.array_equal(dh[k].numpy(), h[k]), ("uvw", k)
     # single _mk_lin / _mk_const launches
     g = rand_fields(n3, ["a"], seed=6)["a"]
     da = up(g)

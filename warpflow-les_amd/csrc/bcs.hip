@@ -6,8 +6,10 @@
 // axis through one perpendicular position (a, b), so the two sides of one
 // axis can share a launch: a thread runs the low-side then the high-side
 // operations of its (a, b) line, which is exactly the reference's per-cell
-// order.  Different axes stay in separate launches because the y fill reads
-// ghosts written by the x fill at other (a, b) positions, and so on.
+// order.  The x fill has a launch of its own, because the y and z fills
+// read x ghosts it writes in other planes; the y and z fills of one i-plane
+// touch only that plane, so one workgroup per plane runs its y lines, a
+// barrier, then its z lines (bc_yz_kernel): two launches per fill.
 #include <algorithm>
 
 #include "batch.h"
@@ -136,6 +138,35 @@ __global__ void __launch_bounds__(256) bc_axis3_kernel(Bc3 A, wf_grid g, int axi
   const int aa = blockIdx.y * blockDim.y + threadIdx.y;
   if (aa >= A.na[z] || bb >= A.nb[z]) return;
   bc_line(A.f[z], g, axis, A.a0[z] + aa, A.b0[z] + bb, A.ops[z]);
+}
+
+// The y and z ops of a fill, one workgroup per array plane i (blockIdx.x =
+// i - i_base; blockIdx.y picks the field of a multi-field fill): the y ops of
+// every (i, k) line of the plane, a barrier, then the z ops of every (i, j)
+// line.  The workgroup has a thread per line up to 1024 lines, so each phase
+// is one round trip to memory.  A y or z op reads and writes only cells of
+// its own plane, and the z ops read the y ghosts the y ops have just
+// written, as in the reference's side order (bcs.py:201-311); the x ops,
+// which read other planes, run in the launch before.  Each cell sees the
+// op-by-op order of the three axis launches, so the bits are theirs.
+struct BcYZ {
+  double* f[3];
+  int32_t ay0[3], nay[3], by0[3], nby[3];  // y ops: planes [ay0, ay0+nay) x k lines
+  int32_t az0[3], naz[3], bz0[3], nbz[3];  // z ops: planes [az0, az0+naz) x j lines
+  BcOps oy[3], oz[3];
+  int32_t ibase;
+};
+
+__global__ void __launch_bounds__(1024) bc_yz_kernel(BcYZ A, wf_grid g) {
+  const int z = blockIdx.y;
+  const int i = A.ibase + (int)blockIdx.x;
+  if (i >= A.ay0[z] && i < A.ay0[z] + A.nay[z])
+    for (int b = threadIdx.x; b < A.nby[z]; b += blockDim.x)
+      bc_line(A.f[z], g, 1, i, A.by0[z] + b, A.oy[z]);
+  __syncthreads();
+  if (i >= A.az0[z] && i < A.az0[z] + A.naz[z])
+    for (int b = threadIdx.x; b < A.nbz[z]; b += blockDim.x)
+      bc_line(A.f[z], g, 2, i, A.bz0[z] + b, A.oz[z]);
 }
 
 }  // namespace wf
@@ -347,27 +378,67 @@ int wf_bc_const(double* fld, wf_grid g, int axis, int high, int g0, double val, 
   return launch_axis(fld, g, axis, a0, na, b0, nb, o, (hipStream_t)stream);
 }
 
-// Per-axis restatement of BoundaryKernels.apply_u/v/w/vis/pbound/p_mirror
-// (bcs.py:201-311): the x, y and z launches of bc_axis_ops, in that order.
-int wf_bc_fill(double* f, wf_grid g, int which, double param, void* stream) {
-  const hipStream_t s = (hipStream_t)stream;
-  if (g.npl < 1 || which < 0 || which > 5) return WF_EINVAL;
-  for (int axis = 0; axis < 3; ++axis) {
+// The y-then-z launch of fills `which[z]` of fields f[z] (z < nf): ranges and
+// ops of the y and z axis launches, checked against the array, one
+// workgroup per plane any of them touches.
+static int launch_yz(double* const* f, const int* which, const double* param, int nf,
+                     const wf_grid& g, hipStream_t s) {
+  BcYZ A{};
+  int lo = 1 << 30, hi = -1, lines = 64;
+  for (int z = 0; z < nf; ++z) {
+    A.f[z] = f[z];
     int a0, na, b0, nb;
-    BcOps o{};
-    if (!bc_axis_ops(which, axis, g.npl, g.ni, g.nj, g.nk, param, a0, na, b0, nb, o))
+    if (!bc_axis_ops(which[z], 1, g.npl, g.ni, g.nj, g.nk, param[z], a0, na, b0, nb, A.oy[z]) ||
+        !ops_ok(g, 1, a0, na, b0, nb, A.oy[z]))
       return WF_EINVAL;
-    WF_TRY((hipError_t)launch_axis(f, g, axis, a0, na, b0, nb, o, s));
+    A.ay0[z] = a0, A.nay[z] = na, A.by0[z] = b0, A.nby[z] = nb;
+    if (na > 0 && nb > 0 && A.oy[z].n > 0) {
+      lo = std::min(lo, a0), hi = std::max(hi, a0 + na);
+      lines = std::max(lines, nb);
+    } else {
+      A.nay[z] = 0;
+    }
+    if (!bc_axis_ops(which[z], 2, g.npl, g.ni, g.nj, g.nk, param[z], a0, na, b0, nb, A.oz[z]) ||
+        !ops_ok(g, 2, a0, na, b0, nb, A.oz[z]))
+      return WF_EINVAL;
+    A.az0[z] = a0, A.naz[z] = na, A.bz0[z] = b0, A.nbz[z] = nb;
+    if (na > 0 && nb > 0 && A.oz[z].n > 0) {
+      lo = std::min(lo, a0), hi = std::max(hi, a0 + na);
+      lines = std::max(lines, nb);
+    } else {
+      A.naz[z] = 0;
+    }
   }
+  if (hi <= lo) return 0;
+  A.ibase = lo;
+  const int threads = std::min(1024, (lines + 63) / 64 * 64);
+  hipLaunchKernelGGL(bc_yz_kernel, dim3(hi - lo, nf), dim3(threads), 0, s, A, g);
+  WF_LAUNCHED();
   return 0;
 }
 
+// Per-axis restatement of BoundaryKernels.apply_u/v/w/vis/pbound/p_mirror
+// (bcs.py:201-311): the x launch of bc_axis_ops, then its y and z ops in one
+// launch of a workgroup per plane (bc_yz_kernel).
+int wf_bc_fill(double* f, wf_grid g, int which, double param, void* stream) {
+  const hipStream_t s = (hipStream_t)stream;
+  if (g.npl < 1 || which < 0 || which > 5) return WF_EINVAL;
+  int a0, na, b0, nb;
+  BcOps o{};
+  if (!bc_axis_ops(which, 0, g.npl, g.ni, g.nj, g.nk, param, a0, na, b0, nb, o))
+    return WF_EINVAL;
+  WF_TRY((hipError_t)launch_axis(f, g, 0, a0, na, b0, nb, o, s));
+  return launch_yz(&f, &which, &param, 1, g, s);
+}
+
 // apply_u(u), apply_v(v), apply_w(w) (bcs.py:201-265) on three fields of
-// one geometry: per axis one launch covering the three fields.
+// one geometry: one x launch covering the three fields, then one y-then-z
+// launch covering them.
 int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream) {
   const hipStream_t s = (hipStream_t)stream;
   if (g.npl < 1 || !u || !v || !w) return WF_EINVAL;
-  for (int axis = 0; axis < 3; ++axis) {
+  {  // the x launch
+    const int axis = 0;
     Bc3 A{};
     A.f[0] = u;
     A.f[1] = v;
@@ -386,12 +457,16 @@ int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void
       namax = std::max(namax, na);
       nbmax = std::max(nbmax, nb);
     }
-    if (namax == 0 || nbmax == 0) continue;
-    hipLaunchKernelGGL(bc_axis3_kernel, dim3(div_up(nbmax, 64), div_up(namax, 4), 3),
-                       dim3(64, 4), 0, s, A, g, axis);
-    WF_LAUNCHED();
+    if (namax > 0 && nbmax > 0) {
+      hipLaunchKernelGGL(bc_axis3_kernel, dim3(div_up(nbmax, 64), div_up(namax, 4), 3),
+                         dim3(64, 4), 0, s, A, g, axis);
+      WF_LAUNCHED();
+    }
   }
-  return 0;
+  double* const f3[3] = {u, v, w};
+  const int which3[3] = {0, 1, 2};
+  const double param3[3] = {ulid, 0.0, 0.0};
+  return launch_yz(f3, which3, param3, 3, g, s);
 }
 
 }  // extern "C"
