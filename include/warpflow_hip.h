@@ -137,6 +137,22 @@ int wf_convdiff4th_save(int comp, const double* u, const double* v, const double
                         int d0, int d1, int d2, double dx, double dy, double dz, double dt,
                         double alpha, void* stream);
 
+/* wf_convdiff4th_save of u, v and w in one pass, each over its operator box
+ * (solver.py:98-109: (ni-1, nj, nk), (ni, nj-1, nk), (ni, nj, nk-1)): the
+ * first RK stage's fill/conv/diff sequence of solver.py:254-265 with the
+ * three fills done first.  The values are those of the three single passes
+ * bit for bit.  They are the reference's when u, v and w have been filled
+ * since their last change and filled once more just before (as at the first
+ * stage of every step: the projection ends with fill_bc of all three): a
+ * second fill then changes only deep edge ghosts that no other component's
+ * stencil reads.  It reads u, v, w and vis once, where the three single
+ * passes read them 3 + 3 + 3 + 3 times.  No single reference kernel
+ * corresponds to it. */
+int wf_convdiff3_save(const double* u, const double* v, const double* w, const double* vis,
+                      double* ustar, double* vstar, double* wstar, double* uoo, double* voo,
+                      double* woo, wf_grid g, double dx, double dy, double dz, double dt,
+                      double alpha, void* stream);
+
 /* calvel_u / _v / _w, kernels.py:232-264: c = cstar - dt*alpha*(p[+1]-p)/h
  * along the component's axis. */
 int wf_calvel(int comp, const double* cstar, const double* p, double* c,
@@ -499,6 +515,7 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  *   WF_TUNE_PROJ_ICH            i-planes per workgroup of wf_calvel3_calmas
  *                               (raised where needed to keep the workgroup
  *                               count within the reduction's partial slots)
+ *   WF_TUNE_CD3_ICH             i-planes per workgroup of wf_convdiff3_save
  * At most 128 entries.
  * ---------------------------------------------------------------------- */
 enum {
@@ -514,7 +531,8 @@ enum {
   WF_TUNE_SWEEP_PAIRS_F64 = 9,
   WF_TUNE_PROJ_ICH = 10,
   WF_TUNE_SWEEP_PAIRS_F32 = 11,
-  WF_TUNE_COUNT = 12
+  WF_TUNE_CD3_ICH = 12,
+  WF_TUNE_COUNT = 13
 };
 int wf_tune_set(int knob, int n0, int n1, int n2, int value);
 int wf_tune_get(int knob, int n0, int n1, int n2, int* value);

@@ -63,6 +63,42 @@ def test_conv_diff_convdiff(gpu, n3, comp):
     assert rel_err(fused.numpy() - f["coo"], got2 - f["coo"]) < TOL
 
 
+@pytest.mark.parametrize("n3", SHAPES + [(9, 17, 70), (41, 10, 33)])
+def test_convdiff3_save_equals_single_passes(gpu, n3):
+    """The first stage's three-component pass equals the three single-component
+    save passes bit for bit: ustar, vstar, wstar and the saved bases, each on
+    its operator box only (random ghosts, ragged tiles and chunks)."""
+    from warpflow import kernels as K
+    h = (0.031, 0.027, 0.019)
+    dt, alpha = 3e-4, 0.5
+    names = ["u", "v", "w", "vis", "us", "vs", "ws", "uo", "vo", "wo"]
+    f = rand_fields(n3, names, seed=11)
+    f["vis"] = 0.0025 + 0.001 * np.abs(f["vis"])
+    a = {k: up(v) for k, v in f.items()}
+    b = {k: up(v) for k, v in f.items()}
+    ops = {"u": (n3[0] - 1, n3[1], n3[2]), "v": (n3[0], n3[1] - 1, n3[2]),
+           "w": (n3[0], n3[1], n3[2] - 1)}
+    for c in "uvw":
+        getattr(K, "convdiff4th_save_" + c)(ops[c], a["u"], a["v"], a["w"], a[c + "o"],
+                                            a["vis"], a[c + "s"], *h, dt, alpha, N)
+    K.convdiff3_save(n3, b["u"], b["v"], b["w"], b["vis"], b["us"], b["vs"], b["ws"], b["uo"],
+                     b["vo"], b["wo"], *h, dt, alpha, N)
+    for k in names:
+        assert np.array_equal(a[k].numpy(), b[k].numpy()), k
+    # tuned chunk lengths change nothing
+    from warpflow import autotune
+    try:
+        for ich in (1, 7, 64):
+            autotune.set_shape(autotune.CD3_ICH, n3, ich)
+            c = {k: up(v) for k, v in f.items()}
+            K.convdiff3_save(n3, c["u"], c["v"], c["w"], c["vis"], c["us"], c["vs"], c["ws"],
+                             c["uo"], c["vo"], c["wo"], *h, dt, alpha, N)
+            for k in ("us", "vs", "ws", "uo", "vo", "wo"):
+                assert np.array_equal(c[k].numpy(), b[k].numpy()), (ich, k)
+    finally:
+        autotune.set_shape(autotune.CD3_ICH, n3, 0)
+
+
 @pytest.mark.parametrize("n3", SHAPES)
 def test_calvel_calvel3_calmas(gpu, n3):
     from warpflow import kernels as K
@@ -196,6 +232,10 @@ def test_wale_checkdt_one_pass(gpu, n3):
     assert np.array_equal(oa.numpy(), ob.numpy())
     assert np.array_equal(a["vis"].numpy(), b["vis"].numpy())
     assert ob.numpy()[3] < 1e10 and ob.numpy()[0] > 0.0
+    # the dt bound of the vis each cell had before, bit for bit: the kernels
+    # divide with one FMA correction and take 1 / max instead of min of 1 / x
+    exp = O.dtvisc_reduce(f["u"], f["v"], f["w"], f["vis"], *h, N, n3)
+    assert ob.numpy()[3] == exp
 
 
 @pytest.mark.parametrize("n3", SHAPES)
@@ -218,12 +258,15 @@ def test_wale_and_checkdt(gpu, n3):
     exp.append(O.dtvisc_reduce(f["u"], f["v"], f["w"], vis_ref, *h, N, n3))
     assert np.array_equal(got[:3], exp[:3])
     assert abs(got[3] - exp[3]) <= 1e-14 * exp[3]
+    # from the same vis the dt bound is the oracle's bit for bit
+    exact = O.dtvisc_reduce(f["u"], f["v"], f["w"], d["vis"].numpy(), *h, N, n3)
+    assert got[3] == exact
     b = zeros(1)
     K.maxabs_reduce(n3, d["w"], b, N)
     assert b.numpy()[0] == exp[2]
     b.fill_(1e10)
     K.dtvisc_reduce(n3, d["u"], d["v"], d["w"], d["vis"], b, *h, N)
-    assert abs(b.numpy()[0] - exp[3]) <= 1e-14 * exp[3]
+    assert b.numpy()[0] == exact
 
 
 @pytest.mark.parametrize("n3", SHAPES)

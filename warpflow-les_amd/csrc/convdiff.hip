@@ -32,7 +32,10 @@
 // expression term for term (summation order included), except that the
 // divisions by 24h and 12h^2 are multiplications by reciprocals formed once
 // on the host (a rounding difference of about one ulp) and /16 is *0.0625
-// (exact).
+// (exact).  Roundings are fixed by the source: contraction is off in this
+// file and the multiply-adds that are fused are written as fma (the
+// weighted terms of each stencil and the two RK updates), so every kernel
+// here -- one component, all three, batched -- gives the same bits.
 //
 // Workgroup order.  Consecutive workgroup ids are dispatched round-robin over
 // the 8 XCDs, each with its own L2; the ids are remapped so that each XCD
@@ -43,6 +46,8 @@
 
 #include "batch.h"
 #include "wf_common.h"
+
+#pragma clang fp contract(off)
 
 namespace wf {
 namespace cdv {
@@ -137,16 +142,25 @@ __device__ inline Pos halo_pos(const Geo& G, int tid) {
 
 // (-f[+1] + 9 f[0] + 9 f[-1] - f[-2]) / 16  (kernels.py:31-46)
 __device__ inline double i4(double p1, double c0, double m1, double m2) {
-  return (-p1 + 9.0 * c0 + 9.0 * m1 - m2) * 0.0625;
+  return (__builtin_fma(9.0, m1, __builtin_fma(9.0, c0, -p1)) - m2) * 0.0625;
 }
 // (-s[+] + 27 s[0] - 27 s[-] + s[--]) / (24 h)  (kernels.py:69)
 __device__ inline double d4(double sp, double sn, double sm, double smm, double r24h) {
-  return (-sp + 27.0 * sn - 27.0 * sm + smm) * r24h;
+  return (__builtin_fma(-27.0, sm, __builtin_fma(27.0, sn, -sp)) + smm) * r24h;
 }
 // (-f[+2] + 16 f[+1] - 30 f[0] + 16 f[-1] - f[-2]) / (12 h^2)  (kernels.py:177-186)
 __device__ inline double l4(double p2, double p1, double c, double m1, double m2,
                             double ir12) {
-  return (-p2 + 16.0 * p1 - 30.0 * c + 16.0 * m1 - m2) * ir12;
+  return (__builtin_fma(16.0, m1, __builtin_fma(-30.0, c, __builtin_fma(16.0, p1, -p2))) -
+          m2) *
+         ir12;
+}
+// The two roundings of the RK update (kernels.py:89, :199):
+// c_oo - dt alpha N, then + dt alpha (vis L4).
+__device__ inline double rk_update(double coo, double nsum, double vis, double lap,
+                                   double dta) {
+  const double t = __builtin_fma(-dta, nsum, coo);
+  return __builtin_fma(dta, vis * lap, t);
 }
 
 struct Coefs {
@@ -384,9 +398,7 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
         const double lap = l4(f[5], f[4], c0, f[2], f[1], K.ir12x) +
                            l4(uy2, uy1, c0, uym1, uym2, K.ir12y) +
                            l4(uz2, uz1, c0, uzm1, uzm2, K.ir12z);
-        double t = cur_coo - K.dta * nsum;
-        t = t + K.dta * (cur_vis * lap);
-        cstar[G.plane(i) + own.off] = t;
+        cstar[G.plane(i) + own.off] = rk_update(cur_coo, nsum, cur_vis, lap, K.dta);
         if constexpr (SAVE) coo_save[G.plane(i) + own.off] = cur_coo;
       }
     } else {
@@ -438,9 +450,7 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
         const double lap = l4(f[5], f[4], c0, f[2], f[1], K.ir12x) +
                            l4(y2, y1, c0, ym1, ym2, K.ir12y) +
                            l4(z2, z1, c0, zm1, zm2, K.ir12z);
-        double t = cur_coo - K.dta * nsum;
-        t = t + K.dta * (cur_vis * lap);
-        cstar[G.plane(i) + own.off] = t;
+        cstar[G.plane(i) + own.off] = rk_update(cur_coo, nsum, cur_vis, lap, K.dta);
         if constexpr (SAVE) coo_save[G.plane(i) + own.off] = cur_coo;
       }
     }
@@ -457,6 +467,374 @@ convdiff_kernel(const double* __restrict__ u, const double* __restrict__ v,
   const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
   convdiff_body<C, SAVE>(u, v, w, coo, vis, cstar, g, b, ich, L % kt, (L / kt) % jt,
                          L / (kt * jt), K, coo_save);
+}
+
+// ---------------------------------------------------------------------------
+// The first RK stage's conv+diff of u, v and w in one pass.
+//
+// At the first stage the three components' ghosts are as the end-of-step
+// fills (or the initial fill) left them, and a repeated fill changes none of
+// the cells one component's stencil reads of another (it changes only deep
+// edge cells no cross stencil reaches).  So once u, v and w are filled, the
+// three conv+diff operators can run together and give what the reference's
+// fill/conv sequence gives (solver.py:254-265).  The pass reads u, v, w and
+// vis once and writes ustar, vstar, wstar and the saved bases uoo, voo, woo:
+// 10 fields of traffic per cell instead of the 18 of three separate passes.
+// (At the second stage the cross stencils must see the other components'
+// ghosts as they were before their fills, so that stage keeps one pass per
+// component.)
+//
+// A TJ x TK tile of threads walks a chunk of i-planes over the union of the
+// three operator boxes; every output expression is convdiff_body's, term for
+// term, so each component's values are bit for bit those of its own pass.
+//   registers: planes i-3 .. i+3 of u, v and w in the thread's column; for
+//     the halo rows (-2, -1, TJ) of I4x(v) and the halo columns (-2, -1, TK)
+//     of I4x(w), four-plane windows of v and w at one position per thread;
+//   LDS: u at planes i and i+1 (four slots), v and w at plane i (parity) on
+//     a 14 x 38 rectangle of which the 3-cell cross and the corner cells of
+//     the other components' cross terms are staged, and I4x(v), I4x(w) at
+//     plane i+1 (parity);
+//   every global load for the next plane is issued, unconditionally (clamped
+//   planes), before the barrier of the current one.
+namespace c3 {
+constexpr int RW = TK + 6;                          // rectangle rows -3 .. TJ+2,
+__device__ inline int rix(int r, int c) {           // columns -3 .. TK+2
+  return (r + 3) * RW + (c + 3);
+}
+constexpr int RN = (TJ + 6) * RW;
+constexpr int IVW = TK, IVN = (TJ + 3) * IVW;       // I4x(v): rows -2 .. TJ
+__device__ inline int ivx(int r, int c) { return (r + 2) * IVW + c; }
+constexpr int IWW = TK + 3, IWN = TJ * IWW;         // I4x(w): columns -2 .. TK
+__device__ inline int iwx(int r, int c) { return r * IWW + (c + 2); }
+constexpr int NCROSS = 6 * TK + 6 * TJ;             // halo cells of the 3-cell cross
+static_assert(NCROSS + 9 <= NT, "one halo cell of each field per thread");
+// Halo cell h of a field's staged region: the cross arms, then (FIELD 1, v)
+// rows {-2, -1, TJ} x columns {-1, TK, TK+1} or (FIELD 2, w) rows
+// {-1, TJ, TJ+1} x columns {-2, -1, TK}; u (FIELD 0) stages the cross only.
+template <int FIELD>
+__device__ inline bool halo_cell(int h, int& r, int& c) {
+  r = c = 0;
+  if (h < 3 * TK) {
+    r = -3 + h / TK;
+    c = h % TK;
+  } else if (h < 6 * TK) {
+    h -= 3 * TK;
+    r = TJ + h / TK;
+    c = h % TK;
+  } else if (h < 6 * TK + 3 * TJ) {
+    h -= 6 * TK;
+    r = h / 3;
+    c = -3 + h % 3;
+  } else if (h < NCROSS) {
+    h -= 6 * TK + 3 * TJ;
+    r = h / 3;
+    c = TK + h % 3;
+  } else if (FIELD != 0 && h < NCROSS + 9) {
+    h -= NCROSS;
+    const int a = h / 3, b = h % 3;
+    if (FIELD == 1) {
+      r = a == 0 ? -2 : (a == 1 ? -1 : TJ);
+      c = b == 0 ? -1 : TK + b - 1;
+    } else {
+      r = a == 0 ? -1 : TJ + a - 1;
+      c = b == 2 ? TK : b - 2;
+    }
+  } else {
+    return false;
+  }
+  return true;
+}
+}  // namespace c3
+
+__device__ inline void convdiff3_save_body(
+    const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
+    const double* __restrict__ vis, double* __restrict__ us, double* __restrict__ vs,
+    double* __restrict__ ws, double* __restrict__ uoo, double* __restrict__ voo,
+    double* __restrict__ woo, const wf_grid& g, int ich, int ktile, int jtile, int chunk,
+    const Coefs& K) {
+  using namespace c3;
+  __shared__ double Us[4][RN];
+  __shared__ double Vs[2][RN];
+  __shared__ double Ws[2][RN];
+  __shared__ double IV[2][IVN];
+  __shared__ double IW[2][IWN];
+
+  const int tid = threadIdx.x;
+  const int tj = tid / TK, tk = tid % TK;
+  const int j0 = jtile * TJ, k0 = ktile * TK;
+  const int i0 = chunk * ich, i1 = min(i0 + ich, g.ni);
+  if (i0 >= i1) return;  // uniform per workgroup
+
+  Geo G;
+  G.sj = g.sj;
+  G.si = g.si;
+  G.nI = g.ni + 2 * g.npl;
+  G.nJ = g.nj + 2 * g.npl;
+  G.nK = g.nk + 2 * g.npl;
+  G.jb = g.npl + j0;
+  G.kb = g.npl + k0;
+  G.ib = g.npl;
+
+  // operator boxes (the reference's _op_dim): u (ni-1, nj, nk), v (ni,
+  // nj-1, nk), w (ni, nj, nk-1); the tile walks their union (ni, nj, nk)
+  const int jj = j0 + tj, kk = k0 + tk;
+  const bool in_jk = jj < g.nj && kk < g.nk;
+  const bool lu_jk = in_jk, lv_jk = in_jk && jj < g.nj - 1, lw_jk = in_jk && kk < g.nk - 1;
+
+  const int64_t own = G.inplane(tj, tk);
+  const int ownl = rix(tj, tk);
+  int hr, hc;
+  const bool hu = halo_cell<0>(tid, hr, hc);
+  const int64_t hu_off = G.inplane(hr, hc);
+  const int hu_l = rix(hr, hc);
+  const bool hv = halo_cell<1>(tid, hr, hc);
+  const int64_t hv_off = G.inplane(hr, hc);
+  const int hv_l = rix(hr, hc);
+  const bool hw = halo_cell<2>(tid, hr, hc);
+  const int64_t hw_off = G.inplane(hr, hc);
+  const int hw_l = rix(hr, hc);
+  // I4x(v) halo rows -2, -1, TJ on threads 0..95; I4x(w) halo columns -2,
+  // -1, TK on threads 128..151
+  const bool xv = tid < 3 * TK;
+  const int xvr = tid < 2 * TK ? -2 + tid / TK : TJ, xvc = tid % TK;
+  const int64_t xv_off = G.inplane(xv ? xvr : 0, xv ? xvc : 0);
+  const int xv_l = ivx(xv ? xvr : 0, xv ? xvc : 0);
+  const int tw = tid - 128;
+  const bool xw = tw >= 0 && tw < 3 * TJ;
+  const int xwc = tw < 2 * TJ ? -2 + tw / TJ : TK, xwr = tw % TJ;
+  const int64_t xw_off = G.inplane(xw ? xwr : 0, xw ? xwc : 0);
+  const int xw_l = iwx(xw ? xwr : 0, xw ? xwc : 0);
+  // vis at each component's face (kernels.py:198, :211, :224)
+  const int64_t vis_u = own, vis_v = G.inplane(tj + 1, tk), vis_w = G.inplane(tj, tk + 1);
+
+  // ---- prologue ----
+  double fu[7], fv[7], fw[7];  // planes i-3 .. i+3 after the shift at the loop top
+#pragma unroll
+  for (int q = 1; q < 7; ++q) {
+    const int64_t pb = G.plane(i0 - 4 + q);
+    fu[q] = u[pb + own];
+    fv[q] = v[pb + own];
+    fw[q] = w[pb + own];
+  }
+  fu[0] = fv[0] = fw[0] = 0.0;
+  double nu = u[G.plane(i0 + 3) + own], nv = v[G.plane(i0 + 3) + own],
+         nw = w[G.plane(i0 + 3) + own];
+  // u plane i0 into its slot (the loop stages plane i+1 in slot (i+1-i0) & 3)
+  Us[0][ownl] = fu[4];
+  if (hu) Us[0][hu_l] = u[G.plane(i0) + hu_off];
+  double hun = u[G.plane(i0 + 1) + hu_off];  // halo of u plane i0+1
+  double hvn = v[G.plane(i0) + hv_off];      // halo of v, w plane i0
+  double hwn = w[G.plane(i0) + hw_off];
+  // v, w windows at the I4x halo positions, planes i-1 .. i+2 after the shift
+  double xvw[4], xww[4];
+#pragma unroll
+  for (int q = 1; q < 4; ++q) {
+    const int64_t pb = G.plane(i0 - 2 + q);
+    xvw[q] = v[pb + xv_off];
+    xww[q] = w[pb + xw_off];
+  }
+  xvw[0] = xww[0] = 0.0;
+  double xvn = v[G.plane(i0 + 2) + xv_off], xwn = w[G.plane(i0 + 2) + xw_off];
+  double snu = vis[G.plane(i0) + vis_u], snv = vis[G.plane(i0) + vis_v],
+         snw = vis[G.plane(i0) + vis_w];
+  // rolled x-terms: squares of I4x(u) (u), Q = I4x(c) * I4(u) (v, w), at
+  // planes i+1, i, i-1 before the shift (f[1..6] = planes i0-3 .. i0+2)
+  double ur1, ur0, urm1, vr1, vr0, vrm1, wr1, wr0, wrm1;
+  {
+    const double a1 = i4(fu[6], fu[5], fu[4], fu[3]);
+    const double a0 = i4(fu[5], fu[4], fu[3], fu[2]);
+    const double am1 = i4(fu[4], fu[3], fu[2], fu[1]);
+    ur1 = a1 * a1;
+    ur0 = a0 * a0;
+    urm1 = am1 * am1;
+    auto iuy = [&](int p) {  // I4y(u) across rows +2 .. -1 at plane p
+      const int64_t pb = G.plane(p);
+      return i4(u[pb + G.inplane(tj + 2, tk)], u[pb + G.inplane(tj + 1, tk)],
+                u[pb + G.inplane(tj, tk)], u[pb + G.inplane(tj - 1, tk)]);
+    };
+    auto iuz = [&](int p) {  // I4z(u) across columns +2 .. -1 at plane p
+      const int64_t pb = G.plane(p);
+      return i4(u[pb + G.inplane(tj, tk + 2)], u[pb + G.inplane(tj, tk + 1)],
+                u[pb + G.inplane(tj, tk)], u[pb + G.inplane(tj, tk - 1)]);
+    };
+    vr1 = i4(fv[6], fv[5], fv[4], fv[3]) * iuy(i0);
+    vr0 = i4(fv[5], fv[4], fv[3], fv[2]) * iuy(i0 - 1);
+    vrm1 = i4(fv[4], fv[3], fv[2], fv[1]) * iuy(i0 - 2);
+    wr1 = i4(fw[6], fw[5], fw[4], fw[3]) * iuz(i0);
+    wr0 = i4(fw[5], fw[4], fw[3], fw[2]) * iuz(i0 - 1);
+    wrm1 = i4(fw[4], fw[3], fw[2], fw[1]) * iuz(i0 - 2);
+  }
+
+  for (int i = i0; i < i1; ++i) {
+    const int rel = i - i0, buf = rel & 1;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      fu[q] = fu[q + 1];
+      fv[q] = fv[q + 1];
+      fw[q] = fw[q + 1];
+    }
+    fu[6] = nu;
+    fv[6] = nv;
+    fw[6] = nw;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      xvw[q] = xvw[q + 1];
+      xww[q] = xww[q + 1];
+    }
+    xvw[3] = xvn;
+    xww[3] = xwn;
+    const double vis_cu = snu, vis_cv = snv, vis_cw = snw;
+    double* __restrict__ U0 = Us[rel & 3];        // u, plane i
+    double* __restrict__ U1 = Us[(rel + 1) & 3];  // u, plane i+1
+    double* __restrict__ V = Vs[buf];
+    double* __restrict__ W = Ws[buf];
+    double* __restrict__ AV = IV[buf];
+    double* __restrict__ AW = IW[buf];
+    U1[ownl] = fu[4];
+    if (hu) U1[hu_l] = hun;
+    V[ownl] = fv[3];
+    if (hv) V[hv_l] = hvn;
+    W[ownl] = fw[3];
+    if (hw) W[hw_l] = hwn;
+    // x-interpolants at plane i+1 (planes i+2, i+1, i, i-1)
+    AV[ivx(tj, tk)] = i4(fv[5], fv[4], fv[3], fv[2]);
+    if (xv) AV[xv_l] = i4(xvw[3], xvw[2], xvw[1], xvw[0]);
+    AW[iwx(tj, tk)] = i4(fw[5], fw[4], fw[3], fw[2]);
+    if (xw) AW[xw_l] = i4(xww[3], xww[2], xww[1], xww[0]);
+    // loads for plane i+1 (clamped past the chunk: a load under a branch
+    // would make the compiler wait for every load in flight where the paths
+    // join)
+    {
+      const int64_t p4 = G.plane(i + 4), p3 = G.plane(i + 3), p2 = G.plane(i + 2),
+                    p1 = G.plane(i + 1);
+      nu = (u + p4)[own];
+      nv = (v + p4)[own];
+      nw = (w + p4)[own];
+      hun = (u + p2)[hu_off];
+      hvn = (v + p1)[hv_off];
+      hwn = (w + p1)[hw_off];
+      xvn = (v + p3)[xv_off];
+      xwn = (w + p3)[xw_off];
+      snu = (vis + p1)[vis_u];
+      snv = (vis + p1)[vis_v];
+      snw = (vis + p1)[vis_w];
+    }
+    __syncthreads();
+
+    const int64_t pi = G.plane(i);
+    const bool lu = lu_jk && i < g.ni - 1;
+    // ---- u (convdiff_body<0, true>) ----
+    {
+      auto Fo = [&](int dj, int dk) { return U0[rix(tj + dj, tk + dk)]; };
+      const double c0 = fu[3];
+      const double a2 = i4(fu[6], fu[5], fu[4], fu[3]);
+      const double s2 = a2 * a2;
+      const double tx = d4(s2, ur1, ur0, urm1, K.r24x);
+      urm1 = ur0;
+      ur0 = ur1;
+      ur1 = s2;
+      const double uy3 = Fo(3, 0), uy2 = Fo(2, 0), uy1 = Fo(1, 0);
+      const double uym1 = Fo(-1, 0), uym2 = Fo(-2, 0), uym3 = Fo(-3, 0);
+      const double ty = d4(i4(uy3, uy2, uy1, c0) * AV[ivx(tj + 1, tk)],
+                           i4(uy2, uy1, c0, uym1) * AV[ivx(tj, tk)],
+                           i4(uy1, c0, uym1, uym2) * AV[ivx(tj - 1, tk)],
+                           i4(c0, uym1, uym2, uym3) * AV[ivx(tj - 2, tk)], K.r24y);
+      const double uz3 = Fo(0, 3), uz2 = Fo(0, 2), uz1 = Fo(0, 1);
+      const double uzm1 = Fo(0, -1), uzm2 = Fo(0, -2), uzm3 = Fo(0, -3);
+      const double tz = d4(i4(uz3, uz2, uz1, c0) * AW[iwx(tj, tk + 1)],
+                           i4(uz2, uz1, c0, uzm1) * AW[iwx(tj, tk)],
+                           i4(uz1, c0, uzm1, uzm2) * AW[iwx(tj, tk - 1)],
+                           i4(c0, uzm1, uzm2, uzm3) * AW[iwx(tj, tk - 2)], K.r24z);
+      const double nsum = tx + ty + tz;
+      if (lu) {
+        const double lap = l4(fu[5], fu[4], c0, fu[2], fu[1], K.ir12x) +
+                           l4(uy2, uy1, c0, uym1, uym2, K.ir12y) +
+                           l4(uz2, uz1, c0, uzm1, uzm2, K.ir12z);
+        (us + pi)[own] = rk_update(c0, nsum, vis_cu, lap, K.dta);
+        (uoo + pi)[own] = c0;
+      }
+    }
+    // ---- v (convdiff_body<1, true>) ----
+    {
+      auto Fo = [&](int dj, int dk) { return V[rix(tj + dj, tk + dk)]; };
+      const double c0 = fv[3];
+      const double iu = i4(U1[rix(tj + 2, tk)], U1[rix(tj + 1, tk)], U1[rix(tj, tk)],
+                           U1[rix(tj - 1, tk)]);
+      const double q1 = i4(fv[6], fv[5], fv[4], fv[3]) * iu;
+      const double tx = d4(q1, vr1, vr0, vrm1, K.r24x);
+      vrm1 = vr0;
+      vr0 = vr1;
+      vr1 = q1;
+      const double y3 = Fo(3, 0), y2 = Fo(2, 0), y1 = Fo(1, 0);
+      const double ym1 = Fo(-1, 0), ym2 = Fo(-2, 0), ym3 = Fo(-3, 0);
+      const double z3 = Fo(0, 3), z2 = Fo(0, 2), z1 = Fo(0, 1);
+      const double zm1 = Fo(0, -1), zm2 = Fo(0, -2), zm3 = Fo(0, -3);
+      const double p2 = i4(y3, y2, y1, c0), p1 = i4(y2, y1, c0, ym1);
+      const double p0 = i4(y1, c0, ym1, ym2), m1 = i4(c0, ym1, ym2, ym3);
+      const double ty = d4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, K.r24y);
+      auto wq = [&](int m) {
+        return i4(W[rix(tj + 2, tk + m)], W[rix(tj + 1, tk + m)], W[rix(tj, tk + m)],
+                  W[rix(tj - 1, tk + m)]);
+      };
+      const double tz = d4(i4(z3, z2, z1, c0) * wq(1), i4(z2, z1, c0, zm1) * wq(0),
+                           i4(z1, c0, zm1, zm2) * wq(-1), i4(c0, zm1, zm2, zm3) * wq(-2),
+                           K.r24z);
+      const double nsum = tx + ty + tz;
+      if (lv_jk) {
+        const double lap = l4(fv[5], fv[4], c0, fv[2], fv[1], K.ir12x) +
+                           l4(y2, y1, c0, ym1, ym2, K.ir12y) +
+                           l4(z2, z1, c0, zm1, zm2, K.ir12z);
+        (vs + pi)[own] = rk_update(c0, nsum, vis_cv, lap, K.dta);
+        (voo + pi)[own] = c0;
+      }
+    }
+    // ---- w (convdiff_body<2, true>) ----
+    {
+      auto Fo = [&](int dj, int dk) { return W[rix(tj + dj, tk + dk)]; };
+      const double c0 = fw[3];
+      const double iu = i4(U1[rix(tj, tk + 2)], U1[rix(tj, tk + 1)], U1[rix(tj, tk)],
+                           U1[rix(tj, tk - 1)]);
+      const double q1 = i4(fw[6], fw[5], fw[4], fw[3]) * iu;
+      const double tx = d4(q1, wr1, wr0, wrm1, K.r24x);
+      wrm1 = wr0;
+      wr0 = wr1;
+      wr1 = q1;
+      const double y3 = Fo(3, 0), y2 = Fo(2, 0), y1 = Fo(1, 0);
+      const double ym1 = Fo(-1, 0), ym2 = Fo(-2, 0), ym3 = Fo(-3, 0);
+      const double z3 = Fo(0, 3), z2 = Fo(0, 2), z1 = Fo(0, 1);
+      const double zm1 = Fo(0, -1), zm2 = Fo(0, -2), zm3 = Fo(0, -3);
+      auto vq = [&](int m) {
+        return i4(V[rix(tj + m, tk + 2)], V[rix(tj + m, tk + 1)], V[rix(tj + m, tk)],
+                  V[rix(tj + m, tk - 1)]);
+      };
+      const double ty = d4(i4(y3, y2, y1, c0) * vq(1), i4(y2, y1, c0, ym1) * vq(0),
+                           i4(y1, c0, ym1, ym2) * vq(-1), i4(c0, ym1, ym2, ym3) * vq(-2),
+                           K.r24y);
+      const double p2 = i4(z3, z2, z1, c0), p1 = i4(z2, z1, c0, zm1);
+      const double p0 = i4(z1, c0, zm1, zm2), m1 = i4(c0, zm1, zm2, zm3);
+      const double tz = d4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, K.r24z);
+      const double nsum = tx + ty + tz;
+      if (lw_jk) {
+        const double lap = l4(fw[5], fw[4], c0, fw[2], fw[1], K.ir12x) +
+                           l4(y2, y1, c0, ym1, ym2, K.ir12y) +
+                           l4(z2, z1, c0, zm1, zm2, K.ir12z);
+        (ws + pi)[own] = rk_update(c0, nsum, vis_cw, lap, K.dta);
+        (woo + pi)[own] = c0;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+convdiff3_save_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                      const double* __restrict__ w, const double* __restrict__ vis,
+                      double* __restrict__ us, double* __restrict__ vs, double* __restrict__ ws,
+                      double* __restrict__ uoo, double* __restrict__ voo,
+                      double* __restrict__ woo, wf_grid g, int ich, int kt, int jt, Coefs K) {
+  const int nb = kt * jt * gridDim.y;
+  const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
+  convdiff3_save_body(u, v, w, vis, us, vs, ws, uoo, voo, woo, g, ich, L % kt, (L / kt) % jt,
+                      L / (kt * jt), K);
 }
 
 // Ensemble form: a persistent grid over the (member, tile, chunk) items of
@@ -565,6 +943,33 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
       CD_LAUNCH(2, false);
   }
 #undef CD_LAUNCH
+  WF_LAUNCHED();
+  return 0;
+}
+
+int convdiff3_save_launch(const double* u, const double* v, const double* w,
+                          const double* vis, double* us, double* vs, double* ws, double* uoo,
+                          double* voo, double* woo, const wf_grid& g, double dx, double dy,
+                          double dz, double dta, int ich, hipStream_t s) {
+  using namespace cdv;
+  const int kt = div_up(g.nk, TK), jt = div_up(g.nj, TJ);
+  if (ich <= 0) ich = tune_get(WF_TUNE_CD3_ICH, g.ni, g.nj, g.nk);
+  if (ich <= 0) {  // about three workgroups per CU in one round, >= 16 planes each
+    const int64_t tiles = (int64_t)kt * jt;
+    ich = (int)std::max<int64_t>(16, (tiles * g.ni + 767) / 768);
+  }
+  const int nch = div_up(g.ni, ich);
+  if ((int64_t)kt * jt * nch >= (int64_t)1 << 31) return WF_EINVAL;
+  Coefs K;
+  K.r24x = 1.0 / (24.0 * dx);
+  K.r24y = 1.0 / (24.0 * dy);
+  K.r24z = 1.0 / (24.0 * dz);
+  K.ir12x = 1.0 / (12.0 * dx * dx);
+  K.ir12y = 1.0 / (12.0 * dy * dy);
+  K.ir12z = 1.0 / (12.0 * dz * dz);
+  K.dta = dta;
+  hipLaunchKernelGGL(convdiff3_save_kernel, dim3((unsigned)(kt * jt), (unsigned)nch), dim3(NT),
+                     0, s, u, v, w, vis, us, vs, ws, uoo, voo, woo, g, ich, kt, jt, K);
   WF_LAUNCHED();
   return 0;
 }

@@ -144,19 +144,40 @@ __device__ inline double div4(double tx, double ty, double tz, double r24x, doub
   return __builtin_fma(tz, r24z, __builtin_fma(ty, r24y, tx * r24x));
 }
 
-// The viscous/advective time-step bound of one cell (kernels.py:384-402) from
-// the velocities at the cell and its low neighbours and the cell's vis, with
-// fixed roundings (no contraction) so that checkdt and the fused WALE pass
-// agree bit for bit.
-__device__ inline double dtvisc_val(double u0, double um, double v0, double vm, double w0,
+// x / d rounded to nearest, from rd = 1/d rounded to nearest: q = x rd is
+// within an ulp of x/d, the FMA gives the remainder x - q d exactly, and one
+// correction step returns the correctly rounded quotient (Markstein's
+// theorem; it holds for finite x whose quotient is normal or zero).  Three
+// FP64 operations in place of the ~11 of a full division, same bits.
+__device__ inline double div_rn(double x, double d, double rd) {
+#pragma clang fp contract(off)
+  const double q = x * rd;
+  const double r = __builtin_fma(-q, d, x);
+  return __builtin_fma(r, rd, q);
+}
+
+// The viscous/advective time-step bound of one cell (kernels.py:384-402) is
+// 1 / X, X the denominator below, from the velocities at the cell and its
+// low neighbours and the cell's vis, with fixed roundings (no contraction)
+// so that checkdt and the fused WALE pass agree bit for bit; rdx, rdy, rdz
+// are 1/dx, 1/dy, 1/dz rounded to nearest.  X > 0 and 1/x rounded is
+// monotone in x, so the reductions take the min of the bounds as one
+// reciprocal of the max of X: the same bits, one division per workgroup
+// instead of one per cell.
+__device__ inline double dtvisc_den(double u0, double um, double v0, double vm, double w0,
                                     double wm, double vis, double dx, double dy, double dz,
-                                    double rsum) {
+                                    double rdx, double rdy, double rdz, double rsum) {
 #pragma clang fp contract(off)
   const double uc = 0.5 * (u0 + um);
   const double vc = 0.5 * (v0 + vm);
   const double wc = 0.5 * (w0 + wm);
-  return 1.0 / (fabs(uc) / dx + fabs(vc) / dy + fabs(wc) / dz + 2.0 * vis * rsum + 1.0e-30);
+  return div_rn(fabs(uc), dx, rdx) + div_rn(fabs(vc), dy, rdy) + div_rn(fabs(wc), dz, rdz) +
+         2.0 * vis * rsum + 1.0e-30;
 }
+
+// min(1e10, 1 / X over the cells) from xmax = the max of X (0 for no cells),
+// NaN if any X is.
+__device__ inline double dtvisc_of_max(double xmax) { return nan_min(1.0e10, 1.0 / xmax); }
 __global__ void __launch_bounds__(256)
 calvel_kernel(const double* __restrict__ cstar, const double* __restrict__ p,
               double* __restrict__ c, wf_grid g, Box3 b, int64_t s, double rh, double dta,
@@ -204,16 +225,6 @@ __device__ inline double block_max(double v) {
   __syncthreads();
   double r = part[0];
   for (int q = 1; q < 4; ++q) r = nan_max(r, part[q]);
-  return r;
-}
-__device__ inline double block_min(double v) {
-  __shared__ double part[4];
-  v = wave_min(v);
-  const int t = threadIdx.y * blockDim.x + threadIdx.x;
-  if ((t & 63) == 0) part[t >> 6] = v;
-  __syncthreads();
-  double r = part[0];
-  for (int q = 1; q < 4; ++q) r = nan_min(r, part[q]);
   return r;
 }
 __device__ inline bool block_leader() { return threadIdx.x == 0 && threadIdx.y == 0; }
@@ -539,7 +550,7 @@ __device__ inline double wale_value(const FU& U, const FV& V, const FW& W, doubl
   const double g2yz = dot3(dvdx, dudz, dvdy, dvdz, dvdz, dwdz);
   const double g2zy = dot3(dwdx, dudy, dwdy, dvdy, dwdz, dudy);
   // kernels.py:340-358
-  const double tr = (g2xx + g2yy + g2zz) / 3.0;
+  const double tr = div_rn(g2xx + g2yy + g2zz, 3.0, 1.0 / 3.0);
   const double sdxx = g2xx - tr, sdyy = g2yy - tr, sdzz = g2zz - tr;
   const double sdxy = 0.5 * (g2xy + g2yx);
   const double sdxz = 0.5 * (g2xz + g2zx);
@@ -575,7 +586,7 @@ constexpr int WS_NL = (WS_RN + 255) / 256;  // staged values per thread per fiel
 // |v|, |w| and min dtvisc with the vis read before it is overwritten), from
 // the values staged for WALE, into part[q * RED_MAX_BLOCKS + slot].
 struct ChkArgs {
-  double dx, dy, dz, rsum;
+  double dx, dy, dz, rsum;  // with the rdx, rdy, rdz of the WALE gradients
   double* part;
   int slot;
 };
@@ -600,7 +611,7 @@ __device__ inline void wale_stream_body(const double* __restrict__ u,
     }
     return;
   }
-  double mu = 0.0, mv = 0.0, mw = 0.0, dtv = 1.0e10;
+  double mu = 0.0, mv = 0.0, mw = 0.0, xmax = 0.0;
   const int nI = g.ni + 2 * g.npl, nJ = g.nj + 2 * g.npl, nK = g.nk + 2 * g.npl;
   int64_t off[WS_NL];
   int lds[WS_NL];
@@ -644,14 +655,27 @@ __device__ inline void wale_stream_body(const double* __restrict__ u,
   store(1);
   load(i0 + 1);
   const int64_t xrow = (int64_t)(b.lo1 + j0 + tj) * g.sj + (b.lo2 + k0 + tk);
+  // CHK reads each cell's vis before WALE overwrites it, one plane ahead and
+  // from every thread (clamped into the box), so the load never sits in a
+  // branch that the compiler closes with a wait for every load in flight --
+  // the u, v, w planes prefetched two planes ahead included
+  const int64_t xrow_ld = (int64_t)(b.lo1 + min(j0 + tj, b.d1 - 1)) * g.sj +
+                          (b.lo2 + min(k0 + tk, b.d2 - 1));
+  double vnext = 0.0;
+  if (CHK) vnext = vis[plane(i0) + xrow_ld];
   auto step = [&](auto Rc, int i) {
     constexpr int R = decltype(Rc)::value;  // (i - i0) mod 4
     constexpr int SPv = R, S0v = (R + 1) % 4, SNv = (R + 2) % 4;
     store(SNv);  // plane i+1
-    if (i + 2 <= i1) load(i + 2);
-    double vold = 0.0;  // this step's vis of the cell, before WALE overwrites it
-    if (CHK && live) vold = vis[plane(i) + xrow];
+    double vold = vnext;  // this step's vis of the cell, before WALE overwrites it
+    // unconditional (the last step re-reads plane i1, an L2 hit): a load
+    // under a branch makes the compiler wait for it where the paths join
+    load(min(i + 2, i1));
+    if (CHK) vnext = vis[plane(i + 1) + xrow_ld];
     __syncthreads();
+    // the vis value is first used here, after the barrier: the prefetched
+    // planes stay in flight through it
+    if (CHK) asm volatile("" : "+v"(vold));
     if (live) {
       auto F = [&](int f, int di, int dj, int dk) {
         const int sl = di < 0 ? SPv : (di > 0 ? SNv : S0v);
@@ -664,9 +688,9 @@ __device__ inline void wale_stream_body(const double* __restrict__ u,
         mu = nan_max(mu, fabs(U(0, 0, 0)));
         mv = nan_max(mv, fabs(V(0, 0, 0)));
         mw = nan_max(mw, fabs(W(0, 0, 0)));
-        dtv = nan_min(dtv, dtvisc_val(U(0, 0, 0), U(-1, 0, 0), V(0, 0, 0), V(0, -1, 0),
-                                      W(0, 0, 0), W(0, 0, -1), vold, ck.dx, ck.dy, ck.dz,
-                                      ck.rsum));
+        xmax = nan_max(xmax, dtvisc_den(U(0, 0, 0), U(-1, 0, 0), V(0, 0, 0), V(0, -1, 0),
+                                        W(0, 0, 0), W(0, 0, -1), vold, ck.dx, ck.dy, ck.dz,
+                                        rdx, rdy, rdz, ck.rsum));
       }
       vis[plane(i) + xrow] = wale_value(U, V, W, rdx, rdy, rdz, rrey, ls);
     }
@@ -685,12 +709,12 @@ __device__ inline void wale_stream_body(const double* __restrict__ u,
     __syncthreads();
     mw = block_max(mw);
     __syncthreads();
-    dtv = block_min(dtv);
+    xmax = block_max(xmax);
     if (tid == 0) {
       ck.part[0 * RED_MAX_BLOCKS + ck.slot] = mu;
       ck.part[1 * RED_MAX_BLOCKS + ck.slot] = mv;
       ck.part[2 * RED_MAX_BLOCKS + ck.slot] = mw;
-      ck.part[3 * RED_MAX_BLOCKS + ck.slot] = dtv;
+      ck.part[3 * RED_MAX_BLOCKS + ck.slot] = dtvisc_of_max(xmax);
     }
   }
 }
@@ -731,29 +755,30 @@ maxabs_kernel(const double* __restrict__ f, wf_grid g, Box3 b, int ich) {
   if (block_leader()) g_red_part[red_block()] = r;
 }
 
-// checkdt.for dt limit at one cell (kernels.py:394-401); rsum = 1/dx^2 +
-// 1/dy^2 + 1/dz^2 computed once by the host in the reference's order.
+// checkdt.for dt-limit denominator at one cell (kernels.py:394-401); rsum =
+// 1/dx^2 + 1/dy^2 + 1/dz^2 computed once by the host in the reference's order.
 __device__ inline double dtvisc_cell(const double* __restrict__ u,
                                      const double* __restrict__ v,
                                      const double* __restrict__ w,
                                      const double* __restrict__ vis, int64_t x,
                                      const wf_grid& g, double dx, double dy, double dz,
-                                     double rsum) {
-  return dtvisc_val(u[x], u[x - g.si], v[x], v[x - g.sj], w[x], w[x - 1], vis[x], dx, dy, dz,
-                    rsum);
+                                     double rdx, double rdy, double rdz, double rsum) {
+  return dtvisc_den(u[x], u[x - g.si], v[x], v[x - g.sj], w[x], w[x - 1], vis[x], dx, dy, dz,
+                    rdx, rdy, rdz, rsum);
 }
 
 __global__ void __launch_bounds__(256)
 dtvisc_kernel(const double* __restrict__ u, const double* __restrict__ v,
               const double* __restrict__ w, const double* __restrict__ vis, wf_grid g,
               Box3 b, int ich, double dx, double dy, double dz, double rsum) {
+  const double rdx = 1.0 / dx, rdy = 1.0 / dy, rdz = 1.0 / dz;
   int j, k, ibeg, iend;
-  double val = 1.0e10;
+  double xmax = 0.0;
   if (red_rows(b, ich, j, k, ibeg, iend))
     for (int ii = ibeg; ii < iend; ++ii)
-      val = nan_min(val, dtvisc_cell(u, v, w, vis, at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k),
-                                     g, dx, dy, dz, rsum));
-  const double r = block_min(val);
+      xmax = nan_max(xmax, dtvisc_cell(u, v, w, vis, at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k),
+                                       g, dx, dy, dz, rdx, rdy, rdz, rsum));
+  const double r = dtvisc_of_max(block_max(xmax));
   if (block_leader()) g_red_part[red_block()] = r;
 }
 
@@ -763,15 +788,16 @@ __device__ inline void checkdt_body(const double* __restrict__ u, const double* 
                                     const double* __restrict__ vis, const wf_grid& g,
                                     const Box3& b, Blk B, int ich, double dx, double dy,
                                     double dz, double rsum, double* part, int stride, int slot) {
+  const double rdx = 1.0 / dx, rdy = 1.0 / dy, rdz = 1.0 / dz;
   int j, k, ibeg, iend;
-  double mu = 0.0, mv = 0.0, mw = 0.0, dtv = 1.0e10;
+  double mu = 0.0, mv = 0.0, mw = 0.0, xmax = 0.0;
   if (red_rows(b, B, ich, j, k, ibeg, iend)) {
     for (int ii = ibeg; ii < iend; ++ii) {
       const int64_t x = at(g, b.lo0 + ii, b.lo1 + j, b.lo2 + k);
       mu = nan_max(mu, fabs(u[x]));
       mv = nan_max(mv, fabs(v[x]));
       mw = nan_max(mw, fabs(w[x]));
-      dtv = nan_min(dtv, dtvisc_cell(u, v, w, vis, x, g, dx, dy, dz, rsum));
+      xmax = nan_max(xmax, dtvisc_cell(u, v, w, vis, x, g, dx, dy, dz, rdx, rdy, rdz, rsum));
     }
   }
   mu = block_max(mu);
@@ -780,12 +806,12 @@ __device__ inline void checkdt_body(const double* __restrict__ u, const double* 
   __syncthreads();
   mw = block_max(mw);
   __syncthreads();
-  dtv = block_min(dtv);
+  xmax = block_max(xmax);
   if (block_leader()) {
     part[0 * stride + slot] = mu;
     part[1 * stride + slot] = mv;
     part[2 * stride + slot] = mw;
-    part[3 * stride + slot] = dtv;
+    part[3 * stride + slot] = dtvisc_of_max(xmax);
   }
 }
 
@@ -1208,6 +1234,18 @@ int wf_convdiff4th_save(int comp, const double* u, const double* v, const double
   if (empty(b)) return 0;
   return convdiff_launch(comp, u, v, w, nullptr, vis, cstar, g, b, dx, dy, dz, dt * alpha, 0,
                          S_(stream), coo_save);
+}
+
+int wf_convdiff3_save(const double* u, const double* v, const double* w, const double* vis,
+                      double* ustar, double* vstar, double* wstar, double* uoo, double* voo,
+                      double* woo, wf_grid g, double dx, double dy, double dz, double dt,
+                      double alpha, void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 3, 3) || !u || !v || !w || !vis || !ustar || !vstar || !wstar || !uoo ||
+      !voo || !woo)
+    return WF_EINVAL;
+  return convdiff3_save_launch(u, v, w, vis, ustar, vstar, wstar, uoo, voo, woo, g, dx, dy, dz,
+                               dt * alpha, 0, S_(stream));
 }
 
 int wf_calvel(int comp, const double* cstar, const double* p, double* c, wf_grid g, int d0,

@@ -141,5 +141,11 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
                     const double* coo, const double* vis, double* cstar, const wf_grid& g,
                     const Box3& b, double dx, double dy, double dz, double dta, int ich,
                     hipStream_t s, double* coo_save = nullptr);
+// The first RK stage's conv+diff of u, v and w in one pass over the union of
+// their operator boxes (convdiff.hip), saving u, v, w into uoo, voo, woo.
+int convdiff3_save_launch(const double* u, const double* v, const double* w,
+                          const double* vis, double* us, double* vs, double* ws, double* uoo,
+                          double* voo, double* woo, const wf_grid& g, double dx, double dy,
+                          double dz, double dta, int ich, hipStream_t s);
 
 }  // namespace wf

@@ -101,6 +101,7 @@ _SIGS = {
     "wf_diff4th": [_I, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _D, _P],
     "wf_convdiff4th": [_I, _P, _P, _P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _D, _P],
     "wf_convdiff4th_save": [_I, _P, _P, _P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _D, _P],
+    "wf_convdiff3_save": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _P],
     "wf_calvel": [_I, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _P],
     "wf_calvel3": [_P, _P, _P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _P],
     "wf_calmas": [_P, _P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _P],

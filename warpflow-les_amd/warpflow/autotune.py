@@ -1,19 +1,22 @@
 """Launch-shape autotuner for the hot kernels.
 
-The fused momentum kernel (``wf_convdiff4th``), the fused calvel/calmas pass
+The fused momentum kernels (``wf_convdiff4th`` and the first stage's
+three-component ``wf_convdiff3_save``), the fused calvel/calmas pass
 (``wf_calvel3_calmas``) and the RBGS sweep kernels have launch-shape
 parameters that change no result: the i-planes each workgroup streams
-through (``cd_ich``, ``proj_ich``, ``sweep_ich``, ``sweep2_ich``) and the
-(j, k) tile of the one-sweep kernel (``sweep_tile``: 1 = 8 x 64, 2 = 16 x 32,
-3 = 16 x 64, 4 = 32 x 32) and of the two-sweep kernel (``sweep2_tile``: 1 =
-8 x 64, 2 = 16 x 32, and in FP32 also 3 = 16 x 64, 4 = 32 x 32); ``sweep_pairs`` picks, per level and precision,
-one-sweep launches (1) or two-sweep launches (2) for pairs of sweeps.  The library keeps one setting per (parameter,
-launch shape); ``tune_solver`` times every candidate with HIP events on a
-solver's own arrays -- the momentum kernel on the solver's grid, the sweeps
-on each multigrid level that uses them -- checks that each candidate's
-output equals the built-in choice bit for bit (two-sweep launches against
-one-sweep launches), restores the arrays and records the fastest.  Results
-can be cached in a JSON file keyed by device and grid.
+through (``cd_ich``, ``cd3_ich``, ``proj_ich``, ``sweep_ich``,
+``sweep2_ich``) and the (j, k) tile of the one-sweep kernel (``sweep_tile``:
+1 = 8 x 64, 2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32) and of the two-sweep
+kernel (``sweep2_tile``: 1 = 8 x 64, 2 = 16 x 32, and in FP32 also
+3 = 16 x 64, 4 = 32 x 32); ``sweep_pairs`` picks, per level and precision,
+one-sweep launches (1) or two-sweep launches (2) for pairs of sweeps.  The
+library keeps one setting per (parameter, launch shape); ``tune_solver``
+times every candidate with HIP events on a solver's own arrays -- the
+momentum kernels on the solver's grid, the sweeps on each multigrid level
+that uses them -- checks that each candidate's output equals the built-in
+choice bit for bit (two-sweep launches against one-sweep launches), restores
+the arrays and records the fastest.  Results can be cached in a JSON file
+keyed by device and grid.
 """
 from __future__ import annotations
 
@@ -29,8 +32,9 @@ from .device import copy, current_stream, zeros, zeros_like
 # knob ids (include/warpflow_hip.h)
 (CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32, SWEEP2_TILE_F64,
  SWEEP2_TILE_F32, SWEEP2_ICH_F64, SWEEP2_ICH_F32, SWEEP_PAIRS_F64, PROJ_ICH,
- SWEEP_PAIRS_F32) = range(12)
+ SWEEP_PAIRS_F32, CD3_ICH) = range(13)
 CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
+CD3_ICH_CANDIDATES = (0, 32, 43, 64, 86, 128, 256)
 SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
 SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32)
 SWEEP2_TILE_CANDIDATES = {True: (1, 2), False: (1, 2, 3, 4)}  # by "is FP64"
@@ -119,6 +123,40 @@ def _tune_convdiff(sol, T, reps, out):
     set_shape(CD_ICH, n3, best[1])
     out["entries"].append((CD_ICH, tuple(n3), best[1]))
     copy(f["ustar"], saved)
+
+
+def _tune_convdiff3(sol, T, reps, out):
+    """The first stage's three-component pass: every output it writes is
+    restored afterwards."""
+    from . import kernels as K
+    from .topology import NPL
+    f = sol.f
+    h = (sol.dx, sol.dy, sol.dz)
+    n3 = sol.ncells
+    names = ("ustar", "vstar", "wstar", "uoo", "voo", "woo")
+    saved = {nm: zeros_like(f[nm]) for nm in names}
+    for nm in names:
+        copy(saved[nm], f[nm])
+
+    def cd3():
+        K.convdiff3_save(n3, f["u"], f["v"], f["w"], f["vis"], f["ustar"], f["vstar"],
+                         f["wstar"], f["uoo"], f["voo"], f["woo"], *h, sol.dt, 0.5, NPL)
+    set_shape(CD3_ICH, n3, 0)
+    cd3()
+    want = [f[nm].numpy() for nm in names[:3]]
+    best = None
+    for ich in CD3_ICH_CANDIDATES:
+        set_shape(CD3_ICH, n3, ich)
+        t = T.us(cd3, reps)
+        if not all(np.array_equal(f[nm].numpy(), a) for nm, a in zip(names, want)):
+            raise RuntimeError("convdiff3 result depends on cd3_ich=%d" % ich)
+        out["timings_us"]["convdiff3 %s ich=%d" % ("x".join(map(str, n3)), ich)] = round(t, 2)
+        if best is None or t < best[0]:
+            best = (t, ich)
+    set_shape(CD3_ICH, n3, best[1])
+    out["entries"].append((CD3_ICH, tuple(n3), best[1]))
+    for nm in names:
+        copy(f[nm], saved[nm])
 
 
 def _tune_proj(sol, T, reps, out):
@@ -251,6 +289,7 @@ def tune_solver(sol, reps=10, cache=None):
     T = _Timer()
     try:
         _tune_convdiff(sol, T, reps, out)
+        _tune_convdiff3(sol, T, reps, out)
         _tune_proj(sol, T, reps, out)
         for lev in sol.mg.levels:
             if lev.tmp is not None and int(np.prod(lev.n3)) > SWEEP_MIN_CELLS:

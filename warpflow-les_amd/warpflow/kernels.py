@@ -9,10 +9,11 @@ reference's call sites read the same::
 
 ``dim`` is the launch box; like the reference kernels, every handle offsets
 the thread index by ``npl`` (the last input).  Arrays are ``DeviceArray``s of
-the same shape.  Three fused handles have no single reference kernel:
-``convdiff4th_{u,v,w}`` (the conv/diff pair of one RK stage), ``calvel3``
-(the three calvel launches of ``FlowSolver._calvel``) and ``checkdt`` (the
-four reductions of ``FlowSolver.compute_dt``).
+the same shape.  The fused handles have no single reference kernel:
+``convdiff4th_{u,v,w}`` (the conv/diff pair of one RK stage),
+``convdiff3_save`` (the three pairs of the first stage in one pass),
+``calvel3`` (the three calvel launches of ``FlowSolver._calvel``) and
+``checkdt`` (the four reductions of ``FlowSolver.compute_dt``).
 """
 from __future__ import annotations
 
@@ -92,6 +93,20 @@ def _convdiff_save(comp):
                   cstar.ptr, grid_of(cstar, npl), *dim, dx, dy, dz, dt, alpha,
                   current_stream())
     return f
+
+
+def _convdiff3_save(dim, u, v, w, vis, ustar, vstar, wstar, uoo, voo, woo, dx, dy, dz, dt,
+                    alpha, npl):
+    """convdiff4th_save_u, _v and _w in one pass, each over its operator box,
+    launched on the real cells (dim = ncells): the first RK stage with the
+    three fills done first, valid when u, v, w were filled since their last
+    change."""
+    _same(u, v, w, vis, ustar, vstar, wstar, uoo, voo, woo)
+    if tuple(dim) != tuple(n - 2 * npl for n in u.shape):
+        raise ValueError("convdiff3_save: dim %s is not the real-cell extent" % (dim,))
+    _hip.call("wf_convdiff3_save", u.ptr, v.ptr, w.ptr, vis.ptr, ustar.ptr, vstar.ptr,
+              wstar.ptr, uoo.ptr, voo.ptr, woo.ptr, grid_of(u, npl), dx, dy, dz, dt, alpha,
+              current_stream())
 
 
 def _calvel(comp):
@@ -208,6 +223,7 @@ convdiff4th_w = Kernel("convdiff4th_w", _convdiff(2))
 convdiff4th_save_u = Kernel("convdiff4th_save_u", _convdiff_save(0))
 convdiff4th_save_v = Kernel("convdiff4th_save_v", _convdiff_save(1))
 convdiff4th_save_w = Kernel("convdiff4th_save_w", _convdiff_save(2))
+convdiff3_save = Kernel("convdiff3_save", _convdiff3_save)
 calvel_u = Kernel("calvel_u", _calvel(0))
 calvel_v = Kernel("calvel_v", _calvel(1))
 calvel_w = Kernel("calvel_w", _calvel(2))

@@ -85,6 +85,8 @@ class FlowSolver:
                          getattr(cfg.initial, "seed", 1), NPL)
         self.f["vis"].fill_(self.rey)
         self.bcs.apply(self.f, self.ncells, self.ulid, NPL)
+        # u, v, w are as a fill left them: see _rk_stage
+        self._fill_stable = True
 
     def _make_mg(self):
         if getattr(self.cfg.numerics, "mg_precision", "fp64") == "mixed":
@@ -275,6 +277,7 @@ class FlowSolver:
                       "iterations" % (self.rmax, maxcy), flush=True)
         self.outer_iters = iters
         self.fill_bc_uvw(0)
+        self._fill_stable = True
 
     # ------------------------------------------------------------------ #
     # one time step                                                      #
@@ -312,11 +315,28 @@ class FlowSolver:
         comps = (("u", K.conv4th_u, K.diff4th_u, K.convdiff4th_u),
                  ("v", K.conv4th_v, K.diff4th_v, K.convdiff4th_v),
                  ("w", K.conv4th_w, K.diff4th_w, K.convdiff4th_w))
+        if self.fused and stage == 0 and self._fill_stable:
+            # First stage, u, v, w as the end-of-step (or initial) fills left
+            # them: a second fill changes only deep edge ghosts that no other
+            # component's stencil reads, so the three fills can come first
+            # and the three conv+diff pairs run as one pass, with the same
+            # values as the fill/conv sequence.  The second fills follow for
+            # their effect on the ghosts.
+            self._fill_stable = False
+            self.fill_bc_uvw(alfabc)
+            K.convdiff3_save(self.ncells, f["u"], f["v"], f["w"], f["vis"], f["ustar"],
+                             f["vstar"], f["wstar"], f["uoo"], f["voo"], f["woo"], *d3, dt,
+                             alpha, NPL)
+            self.fill_bc_uvw(alfabc)
+            return
+        self._fill_stable = False
         if self.fused:
             # conv+diff of component c right after its first fill; the second
             # fills still run, in order, for their effect on the ghosts.  In
             # the first stage c_oo is c itself (step() skips the copies):
             # the kernel takes it from c and stores it into c_oo on its box.
+            # (Second stage: the cross stencils read the other components'
+            # ghosts as calvel left them, before their fills.)
             save = (K.convdiff4th_save_u, K.convdiff4th_save_v, K.convdiff4th_save_w)
             for q, (c, _, _, kcd) in enumerate(comps):
                 self.fill_bc(c, alfabc)
