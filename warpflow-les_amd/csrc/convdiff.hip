@@ -606,6 +606,16 @@ __device__ inline void convdiff3_save_body(
   const int xw_l = iwx(xw ? xwr : 0, xw ? xwc : 0);
   // vis at each component's face (kernels.py:198, :211, :224)
   const int64_t vis_u = own, vis_v = G.inplane(tj + 1, tk), vis_w = G.inplane(tj, tk + 1);
+  // in the plane loop every access is a raw buffer access: a per-plane
+  // descriptor (scalar) and a 32-bit byte offset per lane (the host keeps a
+  // plane below 2^30 bytes); a store outside the component's box goes to
+  // BUF_DEAD and is dropped
+  auto prs = [&](const double* a, int p) { return buf_rsrc(a + G.plane(p), 0x3fffffffu); };
+  const uint32_t b_own = (uint32_t)own * 8u, b_hu = (uint32_t)hu_off * 8u,
+                 b_hv = (uint32_t)hv_off * 8u, b_hw = (uint32_t)hw_off * 8u,
+                 b_xv = (uint32_t)xv_off * 8u, b_xw = (uint32_t)xw_off * 8u,
+                 b_vv = (uint32_t)vis_v * 8u, b_vw = (uint32_t)vis_w * 8u;
+  const uint32_t st_v = lv_jk ? b_own : BUF_DEAD, st_w = lw_jk ? b_own : BUF_DEAD;
 
   // ---- prologue ----
   double fu[7], fv[7], fw[7];  // planes i-3 .. i+3 after the shift at the loop top
@@ -705,24 +715,23 @@ __device__ inline void convdiff3_save_body(
     // would make the compiler wait for every load in flight where the paths
     // join)
     {
-      const int64_t p4 = G.plane(i + 4), p3 = G.plane(i + 3), p2 = G.plane(i + 2),
-                    p1 = G.plane(i + 1);
-      nu = (u + p4)[own];
-      nv = (v + p4)[own];
-      nw = (w + p4)[own];
-      hun = (u + p2)[hu_off];
-      hvn = (v + p1)[hv_off];
-      hwn = (w + p1)[hw_off];
-      xvn = (v + p3)[xv_off];
-      xwn = (w + p3)[xw_off];
-      snu = (vis + p1)[vis_u];
-      snv = (vis + p1)[vis_v];
-      snw = (vis + p1)[vis_w];
+      const auto u4 = prs(u, i + 4), v4 = prs(v, i + 4), w4 = prs(w, i + 4);
+      nu = buf_ld<double>(u4, b_own, 0);
+      nv = buf_ld<double>(v4, b_own, 0);
+      nw = buf_ld<double>(w4, b_own, 0);
+      hun = buf_ld<double>(prs(u, i + 2), b_hu, 0);
+      const auto v1 = prs(v, i + 1), w1 = prs(w, i + 1), s1 = prs(vis, i + 1);
+      hvn = buf_ld<double>(v1, b_hv, 0);
+      hwn = buf_ld<double>(w1, b_hw, 0);
+      xvn = buf_ld<double>(prs(v, i + 3), b_xv, 0);
+      xwn = buf_ld<double>(prs(w, i + 3), b_xw, 0);
+      snu = buf_ld<double>(s1, b_own, 0);
+      snv = buf_ld<double>(s1, b_vv, 0);
+      snw = buf_ld<double>(s1, b_vw, 0);
     }
     __syncthreads();
 
-    const int64_t pi = G.plane(i);
-    const bool lu = lu_jk && i < g.ni - 1;
+    const uint32_t st_u = lu_jk && i < g.ni - 1 ? b_own : BUF_DEAD;
     // ---- u (convdiff_body<0, true>) ----
     {
       auto Fo = [&](int dj, int dk) { return U0[rix(tj + dj, tk + dk)]; };
@@ -746,14 +755,15 @@ __device__ inline void convdiff3_save_body(
                            i4(uz1, c0, uzm1, uzm2) * AW[iwx(tj, tk - 1)],
                            i4(c0, uzm1, uzm2, uzm3) * AW[iwx(tj, tk - 2)], K.r24z);
       const double nsum = tx + ty + tz;
-      if (lu) {
-        const double lap = l4(fu[5], fu[4], c0, fu[2], fu[1], K.ir12x) +
-                           l4(uy2, uy1, c0, uym1, uym2, K.ir12y) +
-                           l4(uz2, uz1, c0, uzm1, uzm2, K.ir12z);
-        (us + pi)[own] = rk_update(c0, nsum, vis_cu, lap, K.dta);
-        (uoo + pi)[own] = c0;
-      }
+      const double lap = l4(fu[5], fu[4], c0, fu[2], fu[1], K.ir12x) +
+                         l4(uy2, uy1, c0, uym1, uym2, K.ir12y) +
+                         l4(uz2, uz1, c0, uzm1, uzm2, K.ir12z);
+      buf_st<double>(prs(us, i), st_u, 0, rk_update(c0, nsum, vis_cu, lap, K.dta));
+      buf_st<double>(prs(uoo, i), st_u, 0, c0);
     }
+    // the three components one after the other: the scheduler may not mix
+    // them, which would hold all three stencils' operands live at once
+    __builtin_amdgcn_sched_barrier(0);
     // ---- v (convdiff_body<1, true>) ----
     {
       auto Fo = [&](int dj, int dk) { return V[rix(tj + dj, tk + dk)]; };
@@ -780,14 +790,13 @@ __device__ inline void convdiff3_save_body(
                            i4(z1, c0, zm1, zm2) * wq(-1), i4(c0, zm1, zm2, zm3) * wq(-2),
                            K.r24z);
       const double nsum = tx + ty + tz;
-      if (lv_jk) {
-        const double lap = l4(fv[5], fv[4], c0, fv[2], fv[1], K.ir12x) +
-                           l4(y2, y1, c0, ym1, ym2, K.ir12y) +
-                           l4(z2, z1, c0, zm1, zm2, K.ir12z);
-        (vs + pi)[own] = rk_update(c0, nsum, vis_cv, lap, K.dta);
-        (voo + pi)[own] = c0;
-      }
+      const double lap = l4(fv[5], fv[4], c0, fv[2], fv[1], K.ir12x) +
+                         l4(y2, y1, c0, ym1, ym2, K.ir12y) +
+                         l4(z2, z1, c0, zm1, zm2, K.ir12z);
+      buf_st<double>(prs(vs, i), st_v, 0, rk_update(c0, nsum, vis_cv, lap, K.dta));
+      buf_st<double>(prs(voo, i), st_v, 0, c0);
     }
+    __builtin_amdgcn_sched_barrier(0);
     // ---- w (convdiff_body<2, true>) ----
     {
       auto Fo = [&](int dj, int dk) { return W[rix(tj + dj, tk + dk)]; };
@@ -814,18 +823,18 @@ __device__ inline void convdiff3_save_body(
       const double p0 = i4(z1, c0, zm1, zm2), m1 = i4(c0, zm1, zm2, zm3);
       const double tz = d4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, K.r24z);
       const double nsum = tx + ty + tz;
-      if (lw_jk) {
-        const double lap = l4(fw[5], fw[4], c0, fw[2], fw[1], K.ir12x) +
-                           l4(y2, y1, c0, ym1, ym2, K.ir12y) +
-                           l4(z2, z1, c0, zm1, zm2, K.ir12z);
-        (ws + pi)[own] = rk_update(c0, nsum, vis_cw, lap, K.dta);
-        (woo + pi)[own] = c0;
-      }
+      const double lap = l4(fw[5], fw[4], c0, fw[2], fw[1], K.ir12x) +
+                         l4(y2, y1, c0, ym1, ym2, K.ir12y) +
+                         l4(z2, z1, c0, zm1, zm2, K.ir12z);
+      buf_st<double>(prs(ws, i), st_w, 0, rk_update(c0, nsum, vis_cw, lap, K.dta));
+      buf_st<double>(prs(woo, i), st_w, 0, c0);
     }
   }
 }
 
-__global__ void __launch_bounds__(256)
+// Three workgroups per CU: 44 KB of LDS each, and the kernel fits 168 VGPRs
+// without spilling (with the components evaluated one after the other).
+__global__ void __launch_bounds__(256, 3)
 convdiff3_save_kernel(const double* __restrict__ u, const double* __restrict__ v,
                       const double* __restrict__ w, const double* __restrict__ vis,
                       double* __restrict__ us, double* __restrict__ vs, double* __restrict__ ws,
@@ -960,6 +969,7 @@ int convdiff3_save_launch(const double* u, const double* v, const double* w,
   }
   const int nch = div_up(g.ni, ich);
   if ((int64_t)kt * jt * nch >= (int64_t)1 << 31) return WF_EINVAL;
+  if (g.si * (int64_t)sizeof(double) >= ((int64_t)1 << 30)) return WF_EINVAL;  // buffer range
   Coefs K;
   K.r24x = 1.0 / (24.0 * dx);
   K.r24y = 1.0 / (24.0 * dy);

@@ -165,38 +165,15 @@ prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev 
   for_planes(f.n0, ich, [&](Blk B) { prolong_body(phic, c, phif, f, B); });
 }
 
-// Buffer access to a level array: the base, chosen per workgroup, sits in a
-// scalar buffer descriptor, the plane offset in the scalar soffset operand,
-// and each lane supplies a 32-bit byte offset within the plane, so a load or
-// store costs no vector address arithmetic.  A lane offset of SWEEP_DEAD
-// (or SWEEP_DEAD plus or minus one row) lies past every descriptor's range --
-// the host keeps a chunk's plane offsets below 2^30 -- so its load returns
-// zero and its store is dropped, which is how cells outside the domain are
-// masked.  Descriptor
-// word 3 = 0x00020000: untyped 32-bit data format, the value the gfx9 family
-// (gfx950 included) expects for raw buffer instructions.
-constexpr uint32_t SWEEP_DEAD = 0xc0000000u;
-__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
-                                           0x00020000);
-}
-template <class T>
-__device__ inline T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t boff, int soff) {
-  if constexpr (sizeof(T) == 8)
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, soff, 0));
-  else
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)boff, soff, 0));
-}
-template <class T>
-__device__ inline void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t boff, int soff, T v) {
-  if constexpr (sizeof(T) == 8) {
-    using U2 = __attribute__((ext_vector_type(2))) unsigned;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)boff, soff, 0);
-  } else {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)boff, soff,
-                                          0);
-  }
-}
+// Buffer access to a level array (buf_rsrc, buf_ld, buf_st in wf_common.h):
+// the base, chosen per workgroup, sits in a scalar buffer descriptor, the
+// plane offset in the scalar soffset operand, and each lane supplies a 32-bit
+// byte offset within the plane, so a load or store costs no vector address
+// arithmetic.  A lane offset of SWEEP_DEAD (or SWEEP_DEAD plus or minus one
+// row) lies past every descriptor's range -- the host keeps a chunk's plane
+// offsets below 2^30 -- so its load returns zero and its store is dropped,
+// which is how cells outside the domain are masked.
+constexpr uint32_t SWEEP_DEAD = BUF_DEAD;
 
 // One full RBGS sweep (colour 0, then colour 1, then the ring refill) out of
 // place: pout = S(pin).  Colour-0 (odd i+j+k) cells read only even cells of
@@ -688,7 +665,6 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
   // pin == nullptr: an iterate of zeros, read through an empty descriptor
   const auto dpin = pin ? chunk_rsrc(pin) : buf_rsrc(rhs, 0u);
   const auto drhs = chunk_rsrc(rhs), dout = chunk_rsrc(pout);
-  const auto dnone = buf_rsrc(pout, 0u);
   // soffset of real plane x, clamped to the planes [lo, hi] a stage needs
   // (values of other planes are never used) and to the array
   auto soff = [&](int x, int lo, int hi) {
@@ -787,7 +763,11 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
     }
     __syncthreads();
     constexpr int XA = U, XB = U & 1;
-    {  // A-odd at q: pin planes q-1, q, q+1 -> OA[q]
+    // Each stage runs only for the planes a later stage or the output needs
+    // (A-odd i0-3 .. i1+2, A-even i0-2 .. i1+1, B-odd i0-1 .. i1, B-even
+    // i0 .. i1-1); the pipeline's fill and drain iterations skip the rest.
+    // The conditions are uniform over the workgroup.
+    if (q <= i1 + 2) {  // A-odd at q: pin planes q-1, q, q+1 -> OA[q]
       T rp[N3];
 #pragma unroll
       for (int e = 0; e < N3; ++e) rp[e] = ao_rp[r][e];
@@ -796,7 +776,7 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
             std::integral_constant<int, C3>{}, P[Sm1], P[S0], P[Sp1], ao_a0, ao_aj[r],
             rao[XA], rp, OA[S0]);
     }
-    {  // A-even at q-2: OA planes q-3, q-2, q-1 -> EA[q-2]
+    if (q - 2 >= i0 - 2 && q - 2 <= i1 + 1) {  // A-even at q-2: OA planes q-3 .. q-1 -> EA[q-2]
       const int x = q - 2;
       T rp[N2];
 #pragma unroll
@@ -806,7 +786,7 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
             std::integral_constant<int, C2>{}, OA[(U + 1) & 3], OA[Sm2], OA[Sm1], ae_a0,
             ae_aj[r], rae[XA], rp, EA[Sm2]);
     }
-    {  // B-odd at q-4: EA planes q-5, q-4, q-3 -> OB[q-4]
+    if (q - 4 >= i0 - 1 && q - 4 <= i1) {  // B-odd at q-4: EA planes q-5 .. q-3 -> OB[q-4]
       const int x = q - 4;
       T rp[N1];
 #pragma unroll
@@ -816,9 +796,8 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
             std::integral_constant<int, C1>{}, EA[Sm1], EA[S0], EA[Sp1], bo_a0, bo_aj[r],
             rbo[XB], rp, OB[S0]);
     }
-    {  // B-even at q-6: OB planes q-7, q-6, q-5 -> pout plane q-6 (with B-odd)
+    if (q - 6 >= i0 && q - 6 < i1) {  // B-even at q-6: OB planes q-7 .. q-5 -> pout plane q-6
       const int x = q - 6;
-      const bool own = x >= i0 && x < i1;
       T rp[NPAIR];
 #pragma unroll
       for (int e = 0; e < NPAIR; ++e) rp[e] = be_rp[r][e];
@@ -826,8 +805,8 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
       const T* om = OB[Sp1];  // plane q-7
       const T* o0 = OB[Sm2];  // plane q-6
       const T* op = OB[Sm1];  // plane q-5
-      const auto dst = own ? dout : dnone;
-      const int so = own ? (x - xb) * pb : 0;
+      const auto dst = dout;
+      const int so = (x - xb) * pb;
 #pragma unroll
       for (int e = 0; e < NPAIR; ++e) {
         const int a = be_a0[e], c = be_aj[r][e];
@@ -838,7 +817,7 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
         const uint32_t oe = be_ro[r][e], oo = pofs[r][e];
         buf_st(dst, oe, so, ve);
         buf_st(dst, oo, so, vo);
-        if (own && (x == 0 || x == n0 - 1)) {  // i-face ring images
+        if (x == 0 || x == n0 - 1) {  // i-face ring images
           if (x == 0) {
             buf_st(dout, oe, (-1 - xb) * pb, ve);
             buf_st(dout, oo, (-1 - xb) * pb, vo);
@@ -848,7 +827,7 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
             buf_st(dout, oo, (n0 - xb) * pb, vo);
           }
         }
-        if (!wfast) {  // j and k face-ring images (dst drops them if !own)
+        if (!wfast) {  // j and k face-ring images
           const int f = fl[e];
           const bool xbf = ((f & FL_X) != 0) != (r != 0);  // the even cell is b
           const uint32_t oa = xbf ? oo : oe, ob = xbf ? oe : oo;

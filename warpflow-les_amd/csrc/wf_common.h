@@ -148,4 +148,33 @@ int convdiff3_save_launch(const double* u, const double* v, const double* w,
                           double* voo, double* woo, const wf_grid& g, double dx, double dy,
                           double dz, double dta, int ich, hipStream_t s);
 
+// Raw buffer access: a scalar descriptor holds the base (and a byte range),
+// each lane supplies a 32-bit byte offset, so a load or store needs no 64-bit
+// vector address.  A lane offset at or past the range (BUF_DEAD, with ranges
+// kept below 2^30) reads zero and drops its store.  Descriptor word 3 =
+// 0x00020000: untyped 32-bit data format, the value the gfx9 family (gfx950
+// included) expects for raw buffer instructions.
+constexpr uint32_t BUF_DEAD = 0xc0000000u;
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+template <class T>
+__device__ inline T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t boff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)boff, soff, 0));
+}
+template <class T>
+__device__ inline void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t boff, int soff, T v) {
+  if constexpr (sizeof(T) == 8) {
+    using U2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)boff, soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)boff, soff,
+                                          0);
+  }
+}
+
 }  // namespace wf
