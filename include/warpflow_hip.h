@@ -153,6 +153,23 @@ int wf_convdiff3_save(const double* u, const double* v, const double* w, const d
                       double* woo, wf_grid g, double dx, double dy, double dz, double dt,
                       double alpha, void* stream);
 
+/* The second RK stage's wf_convdiff4th of v and w in one pass, each over its
+ * operator box.  In solver.py:254-265 conv_v reads w before w's fill and
+ * conv_w after it; this pass runs after the fills of v and w, and v's z-term
+ * reads w_pre at the cells outside w's operator box it reaches (rows -1 and
+ * nj, columns -2, -1, nk-1, nk of every plane, real-cell coordinates), which
+ * wf_save_cross_ghosts copied from w just before w's fill; everywhere else
+ * it reads w.  The values are those of the two single passes in the
+ * reference's order bit for bit.  No single reference kernel corresponds. */
+int wf_convdiff_vw(const double* u, const double* v, const double* w, const double* voo,
+                   const double* woo, const double* vis, const double* w_pre, double* vstar,
+                   double* wstar, wf_grid g, double dx, double dy, double dz, double dt,
+                   double alpha, void* stream);
+
+/* w_pre <- w at the cells wf_convdiff_vw reads from w_pre (see there); no
+ * other cell of w_pre is written.  No reference counterpart. */
+int wf_save_cross_ghosts(const double* w, double* w_pre, wf_grid g, void* stream);
+
 /* calvel_u / _v / _w, kernels.py:232-264: c = cstar - dt*alpha*(p[+1]-p)/h
  * along the component's axis. */
 int wf_calvel(int comp, const double* cstar, const double* p, double* c,
@@ -516,6 +533,7 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  *                               (raised where needed to keep the workgroup
  *                               count within the reduction's partial slots)
  *   WF_TUNE_CD3_ICH             i-planes per workgroup of wf_convdiff3_save
+ *   WF_TUNE_CDVW_ICH            i-planes per workgroup of wf_convdiff_vw
  * At most 128 entries.
  * ---------------------------------------------------------------------- */
 enum {
@@ -532,7 +550,8 @@ enum {
   WF_TUNE_PROJ_ICH = 10,
   WF_TUNE_SWEEP_PAIRS_F32 = 11,
   WF_TUNE_CD3_ICH = 12,
-  WF_TUNE_COUNT = 13
+  WF_TUNE_CDVW_ICH = 13,
+  WF_TUNE_COUNT = 14
 };
 int wf_tune_set(int knob, int n0, int n1, int n2, int value);
 int wf_tune_get(int knob, int n0, int n1, int n2, int* value);

@@ -99,6 +99,46 @@ def test_convdiff3_save_equals_single_passes(gpu, n3):
         autotune.set_shape(autotune.CD3_ICH, n3, 0)
 
 
+@pytest.mark.parametrize("n3", SHAPES + [(9, 17, 70), (41, 10, 33)])
+def test_convdiff_vw_equals_single_passes(gpu, n3):
+    """The second stage's v/w pass equals the single v and w passes in the
+    reference's order bit for bit: v's pass reads w before w's fill (w0), w's
+    pass after it (w1 = w0 with every cell outside w's operator box changed,
+    as a fill may); the v/w pass reads w1 and, where v's z-term reaches those
+    cells, the copy save_cross_ghosts took from w0."""
+    from warpflow import kernels as K
+    from warpflow.device import zeros
+    h = (0.031, 0.027, 0.019)
+    dt, alpha = 3e-4, 1.0
+    names = ["u", "v", "w0", "vo", "wo", "vis", "vs", "ws"]
+    f = rand_fields(n3, names, seed=13)
+    f["vis"] = 0.0025 + 0.001 * np.abs(f["vis"])
+    ni, nj, nk = n3
+    box = np.zeros(f["w0"].shape, bool)
+    box[N:N + ni, N:N + nj, N:N + nk - 1] = True  # w's operator box
+    w1 = f["w0"].copy()
+    w1[~box] = np.random.default_rng(14).standard_normal(int((~box).sum()))
+    a = {k: up(v) for k, v in f.items()}
+    K.convdiff4th_v((ni, nj - 1, nk), a["u"], a["v"], a["w0"], a["vo"], a["vis"], a["vs"], *h,
+                    dt, alpha, N)
+    aw1 = up(w1)
+    K.convdiff4th_w((ni, nj, nk - 1), a["u"], a["v"], aw1, a["wo"], a["vis"], a["ws"], *h, dt,
+                    alpha, N)
+    b = {k: up(v) for k, v in f.items()}
+    w_pre = zeros(w1.shape)
+    K.save_cross_ghosts(n3, b["w0"], w_pre, N)
+    bw1 = up(w1)
+    K.convdiff_vw(n3, b["u"], b["v"], bw1, b["vo"], b["wo"], b["vis"], w_pre, b["vs"], b["ws"],
+                  *h, dt, alpha, N)
+    assert np.array_equal(a["vs"].numpy(), b["vs"].numpy())
+    assert np.array_equal(a["ws"].numpy(), b["ws"].numpy())
+    # without the copy v's z-term would see w1's cells: the test has teeth
+    c = {k: up(v) for k, v in f.items()}
+    K.convdiff_vw(n3, c["u"], c["v"], up(w1), c["vo"], c["wo"], c["vis"], up(w1), c["vs"],
+                  c["ws"], *h, dt, alpha, N)
+    assert not np.array_equal(c["vs"].numpy(), a["vs"].numpy())
+
+
 @pytest.mark.parametrize("n3", SHAPES)
 def test_calvel_calvel3_calmas(gpu, n3):
     from warpflow import kernels as K

@@ -846,6 +846,310 @@ convdiff3_save_kernel(const double* __restrict__ u, const double* __restrict__ v
                       L / (kt * jt), K);
 }
 
+// ---------------------------------------------------------------------------
+// The second RK stage's conv+diff of v and w in one pass.
+//
+// At the second stage calvel has changed u, v and w but not their ghosts, and
+// the reference fills and convects one component at a time (solver.py:254-
+// 265): conv_v reads u and v after their fills and w before its fill; conv_w
+// reads all three after their fills.  u's pass runs on its own right after
+// u's fill; this pass runs after the fills of v and w and reads, for v's
+// z-term, w's values from before w's fill where w's fill writes: the cells
+// outside w's operator box that the term reaches (rows -1 and nj, columns
+// -2, -1, nk-1, nk of each plane), which save_cross_ghosts copied to w_pre
+// just before that fill.  Every other read is the fresh field, as in the
+// reference.  It reads u, v, w, voo, woo and vis once (the two single passes
+// read u, v, w, vis twice) -- 8 fields of traffic instead of 12.  Output
+// expressions are convdiff_body<1>'s and <2>'s term for term.
+//   registers: planes i-3 .. i+3 of v and w in the thread's column;
+//   LDS (parity): u at plane i+1 (rows -1 .. TJ+1 and columns -1 .. TK+1
+//     around the tile), v and w at plane i (3-cell cross; v also the corner
+//     cells w's y-term reaches) and w at plane i as v's z-term sees it.
+namespace c2 {
+// halo cells of the u plane: rows -1, TJ, TJ+1 on tile columns, columns -1,
+// TK, TK+1 on tile rows
+constexpr int NUH = 3 * TK + 3 * TJ;
+__device__ inline bool u_halo(int h, int& r, int& c) {
+  r = c = 0;
+  if (h < 3 * TK) {
+    r = h < TK ? -1 : TJ + h / TK - 1;
+    c = h % TK;
+  } else if (h < NUH) {
+    h -= 3 * TK;
+    r = h % TJ;
+    const int q = h / TJ;
+    c = q == 0 ? -1 : TK + q - 1;
+  } else {
+    return false;
+  }
+  return true;
+}
+// halo cells of the w plane v's z-term reads (rows -1 .. TJ+1, columns -2 ..
+// TK): rows -1, TJ, TJ+1 on tile columns, columns -2, -1, TK on tile rows,
+// and the nine corner cells
+constexpr int NWH = 3 * TK + 3 * TJ + 9;
+static_assert(NUH <= NT && NWH <= NT, "one halo cell per thread");
+__device__ inline bool w1_halo(int h, int& r, int& c) {
+  r = c = 0;
+  auto row = [](int q) { return q == 0 ? -1 : TJ + q - 1; };  // -1, TJ, TJ+1
+  auto col = [](int q) { return q == 2 ? TK : q - 2; };       // -2, -1, TK
+  if (h < 3 * TK) {
+    r = row(h / TK);
+    c = h % TK;
+  } else if (h < 3 * TK + 3 * TJ) {
+    h -= 3 * TK;
+    r = h % TJ;
+    c = col(h / TJ);
+  } else if (h < NWH) {
+    h -= 3 * TK + 3 * TJ;
+    r = row(h / 3);
+    c = col(h % 3);
+  } else {
+    return false;
+  }
+  return true;
+}
+}  // namespace c2
+
+__device__ inline void convdiff_vw_body(
+    const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
+    const double* __restrict__ voo, const double* __restrict__ woo,
+    const double* __restrict__ vis, const double* __restrict__ wpre, double* __restrict__ vs,
+    double* __restrict__ ws, const wf_grid& g, int ich, int ktile, int jtile, int chunk,
+    const Coefs& K) {
+  using c3::rix;
+  using c3::RN;
+  __shared__ double Ux[2][RN];
+  __shared__ double Vs[2][RN];
+  __shared__ double Ws[2][RN];
+  __shared__ double W1[2][RN];
+
+  const int tid = threadIdx.x;
+  const int tj = tid / TK, tk = tid % TK;
+  const int j0 = jtile * TJ, k0 = ktile * TK;
+  const int i0 = chunk * ich, i1 = min(i0 + ich, g.ni);
+  if (i0 >= i1) return;  // uniform per workgroup
+
+  Geo G;
+  G.sj = g.sj;
+  G.si = g.si;
+  G.nI = g.ni + 2 * g.npl;
+  G.nJ = g.nj + 2 * g.npl;
+  G.nK = g.nk + 2 * g.npl;
+  G.jb = g.npl + j0;
+  G.kb = g.npl + k0;
+  G.ib = g.npl;
+
+  // operator boxes: v (ni, nj-1, nk), w (ni, nj, nk-1)
+  const int jj = j0 + tj, kk = k0 + tk;
+  const bool lv_jk = jj < g.nj - 1 && kk < g.nk, lw_jk = jj < g.nj && kk < g.nk - 1;
+  // outside w's operator box in (j, k): there v's z-term reads w_pre
+  auto w_out = [&](int r, int c) {
+    const int J = j0 + r, Kc = k0 + c;
+    return J < 0 || J >= g.nj || Kc < 0 || Kc >= g.nk - 1;
+  };
+
+  const int64_t own = G.inplane(tj, tk);
+  const int ownl = rix(tj, tk);
+  int hr, hc;
+  const bool hu = c2::u_halo(tid, hr, hc);
+  const int64_t hu_off = G.inplane(hr, hc);
+  const int hu_l = rix(hr, hc);
+  const bool hv = c3::halo_cell<1>(tid, hr, hc);
+  const int64_t hv_off = G.inplane(hr, hc);
+  const int hv_l = rix(hr, hc);
+  const bool hw = c3::halo_cell<0>(tid, hr, hc);
+  const int64_t hw_off = G.inplane(hr, hc);
+  const int hw_l = rix(hr, hc);
+  const bool h1 = c2::w1_halo(tid, hr, hc);
+  const int64_t h1_off = G.inplane(hr, hc);
+  const int h1_l = rix(hr, hc);
+  const double* __restrict__ w1h_src = h1 && w_out(hr, hc) ? wpre : w;
+  const double* __restrict__ w1t_src = w_out(tj, tk) ? wpre : w;
+  const int64_t vis_v = G.inplane(tj + 1, tk), vis_w = G.inplane(tj, tk + 1);
+  auto prs = [&](const double* a, int p) { return buf_rsrc(a + G.plane(p), 0x3fffffffu); };
+  const uint32_t b_own = (uint32_t)own * 8u, b_hu = (uint32_t)hu_off * 8u,
+                 b_hv = (uint32_t)hv_off * 8u, b_hw = (uint32_t)hw_off * 8u,
+                 b_vv = (uint32_t)vis_v * 8u, b_vw = (uint32_t)vis_w * 8u;
+  const uint32_t st_v = lv_jk ? b_own : BUF_DEAD, st_w = lw_jk ? b_own : BUF_DEAD;
+
+  // ---- prologue ----
+  double fv[7], fw[7];  // planes i-3 .. i+3 after the shift at the loop top
+#pragma unroll
+  for (int q = 1; q < 7; ++q) {
+    const int64_t pb = G.plane(i0 - 4 + q);
+    fv[q] = v[pb + own];
+    fw[q] = w[pb + own];
+  }
+  fv[0] = fw[0] = 0.0;
+  double nv = v[G.plane(i0 + 3) + own], nw = w[G.plane(i0 + 3) + own];
+  double utn = u[G.plane(i0 + 1) + own], uhn = u[G.plane(i0 + 1) + hu_off];
+  double hvn = v[G.plane(i0) + hv_off], hwn = w[G.plane(i0) + hw_off];
+  double w1t = w1t_src[G.plane(i0) + own], w1h = w1h_src[G.plane(i0) + h1_off];
+  double cvn = voo[G.plane(i0) + own], cwn = woo[G.plane(i0) + own];
+  double snv = vis[G.plane(i0) + vis_v], snw = vis[G.plane(i0) + vis_w];
+  double vr1, vr0, vrm1, wr1, wr0, wrm1;
+  {
+    auto iuy = [&](int p) {  // I4y(u) across rows +2 .. -1 at plane p
+      const int64_t pb = G.plane(p);
+      return i4(u[pb + G.inplane(tj + 2, tk)], u[pb + G.inplane(tj + 1, tk)],
+                u[pb + G.inplane(tj, tk)], u[pb + G.inplane(tj - 1, tk)]);
+    };
+    auto iuz = [&](int p) {  // I4z(u) across columns +2 .. -1 at plane p
+      const int64_t pb = G.plane(p);
+      return i4(u[pb + G.inplane(tj, tk + 2)], u[pb + G.inplane(tj, tk + 1)],
+                u[pb + G.inplane(tj, tk)], u[pb + G.inplane(tj, tk - 1)]);
+    };
+    vr1 = i4(fv[6], fv[5], fv[4], fv[3]) * iuy(i0);
+    vr0 = i4(fv[5], fv[4], fv[3], fv[2]) * iuy(i0 - 1);
+    vrm1 = i4(fv[4], fv[3], fv[2], fv[1]) * iuy(i0 - 2);
+    wr1 = i4(fw[6], fw[5], fw[4], fw[3]) * iuz(i0);
+    wr0 = i4(fw[5], fw[4], fw[3], fw[2]) * iuz(i0 - 1);
+    wrm1 = i4(fw[4], fw[3], fw[2], fw[1]) * iuz(i0 - 2);
+  }
+
+  for (int i = i0; i < i1; ++i) {
+    const int buf = (i - i0) & 1;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      fv[q] = fv[q + 1];
+      fw[q] = fw[q + 1];
+    }
+    fv[6] = nv;
+    fw[6] = nw;
+    const double coo_v = cvn, coo_w = cwn, vis_cv = snv, vis_cw = snw;
+    double* __restrict__ U1 = Ux[buf];  // u, plane i+1
+    double* __restrict__ V = Vs[buf];
+    double* __restrict__ W = Ws[buf];
+    double* __restrict__ Wv = W1[buf];  // w as v's z-term reads it
+    U1[ownl] = utn;
+    if (hu) U1[hu_l] = uhn;
+    V[ownl] = fv[3];
+    if (hv) V[hv_l] = hvn;
+    W[ownl] = fw[3];
+    if (hw) W[hw_l] = hwn;
+    Wv[ownl] = w1t;
+    if (h1) Wv[h1_l] = w1h;
+    // loads for plane i+1, unconditional (clamped planes)
+    {
+      const auto v4 = prs(v, i + 4), w4 = prs(w, i + 4), u2 = prs(u, i + 2);
+      nv = buf_ld<double>(v4, b_own, 0);
+      nw = buf_ld<double>(w4, b_own, 0);
+      utn = buf_ld<double>(u2, b_own, 0);
+      uhn = buf_ld<double>(u2, b_hu, 0);
+      const auto v1 = prs(v, i + 1), w1 = prs(w, i + 1), s1 = prs(vis, i + 1);
+      hvn = buf_ld<double>(v1, b_hv, 0);
+      hwn = buf_ld<double>(w1, b_hw, 0);
+      const int64_t p1 = G.plane(i + 1);
+      w1t = w1t_src[p1 + own];
+      w1h = w1h_src[p1 + h1_off];
+      cvn = buf_ld<double>(prs(voo, i + 1), b_own, 0);
+      cwn = buf_ld<double>(prs(woo, i + 1), b_own, 0);
+      snv = buf_ld<double>(s1, b_vv, 0);
+      snw = buf_ld<double>(s1, b_vw, 0);
+    }
+    __syncthreads();
+
+    // ---- v (convdiff_body<1, false>) ----
+    {
+      auto Fo = [&](int dj, int dk) { return V[rix(tj + dj, tk + dk)]; };
+      const double c0 = fv[3];
+      const double iu = i4(U1[rix(tj + 2, tk)], U1[rix(tj + 1, tk)], U1[rix(tj, tk)],
+                           U1[rix(tj - 1, tk)]);
+      const double q1 = i4(fv[6], fv[5], fv[4], fv[3]) * iu;
+      const double tx = d4(q1, vr1, vr0, vrm1, K.r24x);
+      vrm1 = vr0;
+      vr0 = vr1;
+      vr1 = q1;
+      const double y3 = Fo(3, 0), y2 = Fo(2, 0), y1 = Fo(1, 0);
+      const double ym1 = Fo(-1, 0), ym2 = Fo(-2, 0), ym3 = Fo(-3, 0);
+      const double z3 = Fo(0, 3), z2 = Fo(0, 2), z1 = Fo(0, 1);
+      const double zm1 = Fo(0, -1), zm2 = Fo(0, -2), zm3 = Fo(0, -3);
+      const double p2 = i4(y3, y2, y1, c0), p1 = i4(y2, y1, c0, ym1);
+      const double p0 = i4(y1, c0, ym1, ym2), m1 = i4(c0, ym1, ym2, ym3);
+      const double ty = d4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, K.r24y);
+      auto wq = [&](int m) {
+        return i4(Wv[rix(tj + 2, tk + m)], Wv[rix(tj + 1, tk + m)], Wv[rix(tj, tk + m)],
+                  Wv[rix(tj - 1, tk + m)]);
+      };
+      const double tz = d4(i4(z3, z2, z1, c0) * wq(1), i4(z2, z1, c0, zm1) * wq(0),
+                           i4(z1, c0, zm1, zm2) * wq(-1), i4(c0, zm1, zm2, zm3) * wq(-2),
+                           K.r24z);
+      const double nsum = tx + ty + tz;
+      const double lap = l4(fv[5], fv[4], c0, fv[2], fv[1], K.ir12x) +
+                         l4(y2, y1, c0, ym1, ym2, K.ir12y) +
+                         l4(z2, z1, c0, zm1, zm2, K.ir12z);
+      buf_st<double>(prs(vs, i), st_v, 0, rk_update(coo_v, nsum, vis_cv, lap, K.dta));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // ---- w (convdiff_body<2, false>) ----
+    {
+      auto Fo = [&](int dj, int dk) { return W[rix(tj + dj, tk + dk)]; };
+      const double c0 = fw[3];
+      const double iu = i4(U1[rix(tj, tk + 2)], U1[rix(tj, tk + 1)], U1[rix(tj, tk)],
+                           U1[rix(tj, tk - 1)]);
+      const double q1 = i4(fw[6], fw[5], fw[4], fw[3]) * iu;
+      const double tx = d4(q1, wr1, wr0, wrm1, K.r24x);
+      wrm1 = wr0;
+      wr0 = wr1;
+      wr1 = q1;
+      const double y3 = Fo(3, 0), y2 = Fo(2, 0), y1 = Fo(1, 0);
+      const double ym1 = Fo(-1, 0), ym2 = Fo(-2, 0), ym3 = Fo(-3, 0);
+      const double z3 = Fo(0, 3), z2 = Fo(0, 2), z1 = Fo(0, 1);
+      const double zm1 = Fo(0, -1), zm2 = Fo(0, -2), zm3 = Fo(0, -3);
+      auto vq = [&](int m) {
+        return i4(V[rix(tj + m, tk + 2)], V[rix(tj + m, tk + 1)], V[rix(tj + m, tk)],
+                  V[rix(tj + m, tk - 1)]);
+      };
+      const double ty = d4(i4(y3, y2, y1, c0) * vq(1), i4(y2, y1, c0, ym1) * vq(0),
+                           i4(y1, c0, ym1, ym2) * vq(-1), i4(c0, ym1, ym2, ym3) * vq(-2),
+                           K.r24y);
+      const double p2 = i4(z3, z2, z1, c0), p1 = i4(z2, z1, c0, zm1);
+      const double p0 = i4(z1, c0, zm1, zm2), m1 = i4(c0, zm1, zm2, zm3);
+      const double tz = d4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, K.r24z);
+      const double nsum = tx + ty + tz;
+      const double lap = l4(fw[5], fw[4], c0, fw[2], fw[1], K.ir12x) +
+                         l4(y2, y1, c0, ym1, ym2, K.ir12y) +
+                         l4(z2, z1, c0, zm1, zm2, K.ir12z);
+      buf_st<double>(prs(ws, i), st_w, 0, rk_update(coo_w, nsum, vis_cw, lap, K.dta));
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+convdiff_vw_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                   const double* __restrict__ w, const double* __restrict__ voo,
+                   const double* __restrict__ woo, const double* __restrict__ vis,
+                   const double* __restrict__ wpre, double* __restrict__ vs,
+                   double* __restrict__ ws, wf_grid g, int ich, int kt, int jt, Coefs K) {
+  const int nb = kt * jt * gridDim.y;
+  const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
+  convdiff_vw_body(u, v, w, voo, woo, vis, wpre, vs, ws, g, ich, L % kt, (L / kt) % jt,
+                   L / (kt * jt), K);
+}
+
+// w_pre <- w at the cells convdiff_vw reads from w_pre: on each real plane,
+// rows -1 and nj (columns -2 .. nk) and columns -2, -1, nk-1, nk (rows 0 ..
+// nj-1), in real-cell coordinates.
+__global__ void __launch_bounds__(256)
+save_cross_ghosts_kernel(const double* __restrict__ w, double* __restrict__ wpre, wf_grid g) {
+  const int nrow = g.nk + 3, n = 2 * nrow + 4 * g.nj;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int J, Kc;
+  if (t < 2 * nrow) {
+    J = t < nrow ? -1 : g.nj;
+    Kc = t % nrow - 2;
+  } else {
+    const int h = t - 2 * nrow, q = h / g.nj;
+    J = h % g.nj;
+    Kc = q < 2 ? q - 2 : g.nk - 1 + (q - 2);
+  }
+  const int64_t x = (int64_t)(g.npl + blockIdx.y) * g.si + (int64_t)(g.npl + J) * g.sj +
+                    (g.npl + Kc);
+  wpre[x] = w[x];
+}
+
 // Ensemble form: a persistent grid over the (member, tile, chunk) items of
 // component C, chunks of BATCH_CD_ICH planes.
 template <int C, bool SAVE>
@@ -952,6 +1256,43 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
       CD_LAUNCH(2, false);
   }
 #undef CD_LAUNCH
+  WF_LAUNCHED();
+  return 0;
+}
+
+int convdiff_vw_launch(const double* u, const double* v, const double* w, const double* voo,
+                       const double* woo, const double* vis, const double* wpre, double* vs,
+                       double* ws, const wf_grid& g, double dx, double dy, double dz,
+                       double dta, int ich, hipStream_t s) {
+  using namespace cdv;
+  const int kt = div_up(g.nk, TK), jt = div_up(g.nj, TJ);
+  if (ich <= 0) ich = tune_get(WF_TUNE_CDVW_ICH, g.ni, g.nj, g.nk);
+  if (ich <= 0) {  // aim for >= 2048 workgroups with at least 16 planes each
+    const int64_t tiles = (int64_t)kt * jt;
+    ich = (int)std::max<int64_t>(16, (tiles * g.ni + 2047) / 2048);
+  }
+  const int nch = div_up(g.ni, ich);
+  if ((int64_t)kt * jt * nch >= (int64_t)1 << 31) return WF_EINVAL;
+  if (g.si * (int64_t)sizeof(double) >= ((int64_t)1 << 30)) return WF_EINVAL;  // buffer range
+  Coefs K;
+  K.r24x = 1.0 / (24.0 * dx);
+  K.r24y = 1.0 / (24.0 * dy);
+  K.r24z = 1.0 / (24.0 * dz);
+  K.ir12x = 1.0 / (12.0 * dx * dx);
+  K.ir12y = 1.0 / (12.0 * dy * dy);
+  K.ir12z = 1.0 / (12.0 * dz * dz);
+  K.dta = dta;
+  hipLaunchKernelGGL(convdiff_vw_kernel, dim3((unsigned)(kt * jt), (unsigned)nch), dim3(NT), 0,
+                     s, u, v, w, voo, woo, vis, wpre, vs, ws, g, ich, kt, jt, K);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int save_cross_ghosts_launch(const double* w, double* wpre, const wf_grid& g, hipStream_t s) {
+  using namespace cdv;
+  const int n = 2 * (g.nk + 3) + 4 * g.nj;
+  hipLaunchKernelGGL(save_cross_ghosts_kernel, dim3((unsigned)div_up(n, 256), (unsigned)g.ni),
+                     dim3(256), 0, s, w, wpre, g);
   WF_LAUNCHED();
   return 0;
 }

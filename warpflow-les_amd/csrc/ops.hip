@@ -1248,6 +1248,24 @@ int wf_convdiff3_save(const double* u, const double* v, const double* w, const d
                                dt * alpha, 0, S_(stream));
 }
 
+int wf_convdiff_vw(const double* u, const double* v, const double* w, const double* voo,
+                   const double* woo, const double* vis, const double* w_pre, double* vstar,
+                   double* wstar, wf_grid g, double dx, double dy, double dz, double dt,
+                   double alpha, void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 3, 3) || !u || !v || !w || !voo || !woo || !vis || !w_pre || !vstar ||
+      !wstar)
+    return WF_EINVAL;
+  return convdiff_vw_launch(u, v, w, voo, woo, vis, w_pre, vstar, wstar, g, dx, dy, dz,
+                            dt * alpha, 0, S_(stream));
+}
+
+int wf_save_cross_ghosts(const double* w, double* w_pre, wf_grid g, void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 2, 2) || !w || !w_pre) return WF_EINVAL;
+  return save_cross_ghosts_launch(w, w_pre, g, S_(stream));
+}
+
 int wf_calvel(int comp, const double* cstar, const double* p, double* c, wf_grid g, int d0,
               int d1, int d2, double h, double dt, double alpha, void* stream) {
   const Box3 b = npl_box(g, d0, d1, d2);

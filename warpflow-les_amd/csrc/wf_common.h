@@ -143,6 +143,13 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
                     hipStream_t s, double* coo_save = nullptr);
 // The first RK stage's conv+diff of u, v and w in one pass over the union of
 // their operator boxes (convdiff.hip), saving u, v, w into uoo, voo, woo.
+// The second RK stage's conv+diff of v and w in one pass; v's z-term reads
+// w_pre where w's fill writes (convdiff.hip).
+int convdiff_vw_launch(const double* u, const double* v, const double* w, const double* voo,
+                       const double* woo, const double* vis, const double* wpre, double* vs,
+                       double* ws, const wf_grid& g, double dx, double dy, double dz,
+                       double dta, int ich, hipStream_t s);
+int save_cross_ghosts_launch(const double* w, double* wpre, const wf_grid& g, hipStream_t s);
 int convdiff3_save_launch(const double* u, const double* v, const double* w,
                           const double* vis, double* us, double* vs, double* ws, double* uoo,
                           double* voo, double* woo, const wf_grid& g, double dx, double dy,

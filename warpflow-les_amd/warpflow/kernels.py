@@ -12,6 +12,8 @@ the thread index by ``npl`` (the last input).  Arrays are ``DeviceArray``s of
 the same shape.  The fused handles have no single reference kernel:
 ``convdiff4th_{u,v,w}`` (the conv/diff pair of one RK stage),
 ``convdiff3_save`` (the three pairs of the first stage in one pass),
+``convdiff_vw`` with ``save_cross_ghosts`` (v's and w's pairs of the second
+stage in one pass),
 ``calvel3`` (the three calvel launches of ``FlowSolver._calvel``) and
 ``checkdt`` (the four reductions of ``FlowSolver.compute_dt``).
 """
@@ -107,6 +109,26 @@ def _convdiff3_save(dim, u, v, w, vis, ustar, vstar, wstar, uoo, voo, woo, dx, d
     _hip.call("wf_convdiff3_save", u.ptr, v.ptr, w.ptr, vis.ptr, ustar.ptr, vstar.ptr,
               wstar.ptr, uoo.ptr, voo.ptr, woo.ptr, grid_of(u, npl), dx, dy, dz, dt, alpha,
               current_stream())
+
+
+def _convdiff_vw(dim, u, v, w, voo, woo, vis, w_pre, vstar, wstar, dx, dy, dz, dt, alpha,
+                 npl):
+    """convdiff4th_v and _w of the second RK stage in one pass, each over its
+    operator box, launched on the real cells (dim = ncells) after the fills
+    of v and w; v's z-term reads w_pre where w's fill writes (filled by
+    save_cross_ghosts from w just before that fill)."""
+    _same(u, v, w, voo, woo, vis, w_pre, vstar, wstar)
+    if tuple(dim) != tuple(n - 2 * npl for n in u.shape):
+        raise ValueError("convdiff_vw: dim %s is not the real-cell extent" % (dim,))
+    _hip.call("wf_convdiff_vw", u.ptr, v.ptr, w.ptr, voo.ptr, woo.ptr, vis.ptr, w_pre.ptr,
+              vstar.ptr, wstar.ptr, grid_of(u, npl), dx, dy, dz, dt, alpha, current_stream())
+
+
+def _save_cross_ghosts(dim, w, w_pre, npl):
+    """w_pre <- w at the cells convdiff_vw's v z-term reads outside w's
+    operator box (rows -1 and nj, columns -2, -1, nk-1, nk of every plane)."""
+    _same(w, w_pre)
+    _hip.call("wf_save_cross_ghosts", w.ptr, w_pre.ptr, grid_of(w, npl), current_stream())
 
 
 def _calvel(comp):
@@ -224,6 +246,8 @@ convdiff4th_save_u = Kernel("convdiff4th_save_u", _convdiff_save(0))
 convdiff4th_save_v = Kernel("convdiff4th_save_v", _convdiff_save(1))
 convdiff4th_save_w = Kernel("convdiff4th_save_w", _convdiff_save(2))
 convdiff3_save = Kernel("convdiff3_save", _convdiff3_save)
+convdiff_vw = Kernel("convdiff_vw", _convdiff_vw)
+save_cross_ghosts = Kernel("save_cross_ghosts", _save_cross_ghosts)
 calvel_u = Kernel("calvel_u", _calvel(0))
 calvel_v = Kernel("calvel_v", _calvel(1))
 calvel_w = Kernel("calvel_w", _calvel(2))

@@ -54,6 +54,8 @@ class FlowSolver:
         shape = (ni + 2 * NPL, nj + 2 * NPL, nk + 2 * NPL)
         # rows padded so that real cell k = NPL of every row starts a 128-byte line
         self.f = {name: zeros(shape, np.float64, device, aligned=True) for name in FIELDS}
+        if fused:  # w's pre-fill cells for the second stage's v/w pass
+            self.f["w_pre"] = zeros(shape, np.float64, device, aligned=True)
 
         self.halo = HaloExchanger(topo, device, NPL)
         b = cfg.boundaries
@@ -87,6 +89,8 @@ class FlowSolver:
         self.bcs.apply(self.f, self.ncells, self.ulid, NPL)
         # u, v, w are as a fill left them: see _rk_stage
         self._fill_stable = True
+        # the second stage's v/w pass (off: one pass per component)
+        self.fused_vw = fused
 
     def _make_mg(self):
         if getattr(self.cfg.numerics, "mg_precision", "fp64") == "mixed":
@@ -330,6 +334,21 @@ class FlowSolver:
             self.fill_bc_uvw(alfabc)
             return
         self._fill_stable = False
+        if self.fused and stage > 0 and self.fused_vw:
+            # Second stage: u's pass right after u's fill (it reads v and w
+            # before their fills, as the reference's conv_u does), then v's
+            # and w's passes as one after both fills; v's z-term reads w's
+            # cells from before w's fill, which save_cross_ghosts keeps.
+            self.fill_bc("u", alfabc)
+            K.convdiff4th_u(self._op_dim("u"), f["u"], f["v"], f["w"], f["uoo"], f["vis"],
+                            f["ustar"], *d3, dt, alpha, NPL)
+            self.fill_bc("v", alfabc)
+            K.save_cross_ghosts(self.ncells, f["w"], f["w_pre"], NPL)
+            self.fill_bc("w", alfabc)
+            K.convdiff_vw(self.ncells, f["u"], f["v"], f["w"], f["voo"], f["woo"], f["vis"],
+                          f["w_pre"], f["vstar"], f["wstar"], *d3, dt, alpha, NPL)
+            self.fill_bc_uvw(alfabc)
+            return
         if self.fused:
             # conv+diff of component c right after its first fill; the second
             # fills still run, in order, for their effect on the ghosts.  In
