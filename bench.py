@@ -226,10 +226,7 @@ def hot_kernels(a, s64, smx):
     st = current_stream()
     opd = {c: s64._op_dim(c) for c in "uvw"}
 
-    def convdiff3():  # the three momentum launches of one RK stage, as the step issues them
-        for c, k in (("u", K.convdiff4th_u), ("v", K.convdiff4th_v), ("w", K.convdiff4th_w)):
-            k(opd[c], f["u"], f["v"], f["w"], f[c + "oo"], f["vis"], f[c + "star"], *h, dt, 0.5,
-              NPL)
+    n3 = s64.ncells
 
     def row(name, kernel, fn, bytes_per_call, per_step_fp64, per_step_mixed, launches=1):
         return {"name": name, "kernel": kernel, "fn": fn, "bytes": float(bytes_per_call),
@@ -237,10 +234,22 @@ def hot_kernels(a, s64, smx):
                 "per_step_mixed": per_step_mixed}
 
     out = [
-        # two stages per step, the first stage's storing form with the same
-        # bytes (its c_oo read becomes a c_oo write)
-        row("convdiff4th (u, v, w)", "convdiff_kernel<c, *> (conv4th + diff4th of one component)",
-            convdiff3, sum(48 * np.prod(opd[c]) for c in "uvw"), 2, 2, 3),
+        # the momentum launches of one step: the first stage's three-component
+        # pass (reads u, v, w, vis; writes ustar, vstar, wstar, uoo, voo, woo),
+        # then the second stage's u pass (reads u, v, w, uoo, vis; writes
+        # ustar) and v/w pass (reads u, v, w, voo, woo, vis; writes vstar, wstar)
+        row("convdiff3 stage 1 (u, v, w)", "convdiff3_save_kernel (conv4th + diff4th of u, v, w)",
+            lambda: K.convdiff3_save(n3, f["u"], f["v"], f["w"], f["vis"], f["ustar"], f["vstar"],
+                                     f["wstar"], f["uoo"], f["voo"], f["woo"], *h, dt, 0.5, NPL),
+            80 * cells, 1, 1),
+        row("convdiff u stage 2", "convdiff_kernel<0, false> (conv4th + diff4th of u)",
+            lambda: K.convdiff4th_u(opd["u"], f["u"], f["v"], f["w"], f["uoo"], f["vis"],
+                                    f["ustar"], *h, dt, 1.0, NPL),
+            48 * np.prod(opd["u"]), 1, 1),
+        row("convdiff_vw stage 2 (v, w)", "convdiff_vw_kernel (conv4th + diff4th of v and w)",
+            lambda: K.convdiff_vw(n3, f["u"], f["v"], f["w"], f["voo"], f["woo"], f["vis"],
+                                  f["w_pre"], f["vstar"], f["wstar"], *h, dt, 1.0, NPL),
+            64 * cells, 1, 1),
         row("calvel3", "calvel3_kernel",
             lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"], f["u"],
                               f["v"], f["w"], *h, dt, 1.0, NPL), 56 * cells, 1, 1),
@@ -299,22 +308,27 @@ def load_traffic():
         return json.load(fh).get("kernels", {})
 
 
+# table row of each momentum entry point the step calls
+INSTEP_ROWS = {"convdiff3_save": "convdiff3 stage 1 (u, v, w)",
+               "convdiff4th_u": "convdiff u stage 2",
+               "convdiff_vw": "convdiff_vw stage 2 (v, w)"}
+
+
 def instep_momentum_us(sol, nsteps=2):
-    """Average duration of the momentum (conv4th + diff4th) launches inside
-    `nsteps` real steps of `sol`: each launch the step makes is bracketed by
-    HIP events on the stream it is issued on (the kernels module's entry
-    points are wrapped for these steps only).  Inside a step these kernels
-    find part of their inputs in the Infinity Cache, left there by the
-    passes just before them, which neither a cold nor a back-to-back timing
-    reproduces."""
+    """Average duration per launch of each momentum (conv4th + diff4th)
+    entry point inside `nsteps` real steps of `sol`, keyed by table row:
+    each launch the step makes is bracketed by HIP events on the stream it
+    is issued on (the kernels module's entry points are wrapped for these
+    steps only).  Inside a step these kernels find part of their inputs in
+    the Infinity Cache, left there by the passes just before them, which
+    neither a cold nor a back-to-back timing reproduces."""
     from warpflow import _hip
     from warpflow import kernels as K
     from warpflow.device import current_stream, synchronize
-    names = [n for n in dir(K) if n.startswith("convdiff4th_")]
-    orig = {n: getattr(K, n) for n in names}
-    pairs = []
+    orig = {n: getattr(K, n) for n in INSTEP_ROWS}
+    pairs = {n: [] for n in INSTEP_ROWS}
 
-    def wrap(fn):
+    def wrap(name, fn):
         def timed(*args, **kw):
             e0, e1 = C.c_void_p(), C.c_void_p()
             _hip.call("wf_event_create", C.byref(e0))
@@ -322,25 +336,29 @@ def instep_momentum_us(sol, nsteps=2):
             _hip.call("wf_event_record", e0, current_stream())
             fn(*args, **kw)
             _hip.call("wf_event_record", e1, current_stream())
-            pairs.append((e0, e1))
+            pairs[name].append((e0, e1))
         return timed
     try:
-        for n in names:
-            setattr(K, n, wrap(orig[n]))
+        for n in INSTEP_ROWS:
+            setattr(K, n, wrap(n, orig[n]))
         for _ in range(nsteps):
             sol.step()
         synchronize()
     finally:
-        for n in names:
+        for n in INSTEP_ROWS:
             setattr(K, n, orig[n])
-    us = []
-    for e0, e1 in pairs:
-        ms = C.c_float(0.0)
-        _hip.call("wf_event_elapsed_ms", C.byref(ms), e0, e1)
-        us.append(1000.0 * ms.value)
-        _hip.call("wf_event_destroy", e0)
-        _hip.call("wf_event_destroy", e1)
-    return sum(us) / len(us), len(us)
+    out = {}
+    for n, lst in pairs.items():
+        us = []
+        for e0, e1 in lst:
+            ms = C.c_float(0.0)
+            _hip.call("wf_event_elapsed_ms", C.byref(ms), e0, e1)
+            us.append(1000.0 * ms.value)
+            _hip.call("wf_event_destroy", e0)
+            _hip.call("wf_event_destroy", e1)
+        if us:
+            out[INSTEP_ROWS[n]] = (sum(us) / len(us), len(us))
+    return out
 
 
 def kernel_table(a, s64, smx):
@@ -348,7 +366,7 @@ def kernel_table(a, s64, smx):
     flush = Flush()
     traffic = load_traffic()
     rows = []
-    instep = {"convdiff4th (u, v, w)": instep_momentum_us(smx)}
+    instep = instep_momentum_us(smx)
     for r in hot_kernels(a, s64, smx):
         fn = r.pop("fn")
         cold = T.cold_us(fn, a.reps, flush)
@@ -398,7 +416,9 @@ def probe(a):
 # Trace/counter rows: the kernel each table row launches, as rocprofv3 names
 # it (the level-0 launch is the one with the largest grid).
 ROW_KERNELS = {
-    "convdiff4th (u, v, w)": ("convdiff_kernel<",),
+    "convdiff3 stage 1 (u, v, w)": ("convdiff3_save_kernel(",),
+    "convdiff u stage 2": ("convdiff_kernel<0, false>",),
+    "convdiff_vw stage 2 (v, w)": ("convdiff_vw_kernel(",),
     "calvel3": ("calvel3_kernel(",),
     "calvel3+calmas": ("calvel3_calmas_kernel(",),
     "wale+checkdt": ("wale_checkdt_kernel(",),
