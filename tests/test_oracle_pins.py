@@ -623,3 +623,76 @@ def test_edge_and_corner_ghosts_compose_face_rules(name, n3):
         [tuple(int(x) for x in i) for i in np.argwhere(got.view(np.int64) != exp.view(np.int64))[:5]]
     # the fills do reach edges (two ghost axes) and corners (three)
     assert chains[2] > 0 and chains[3] > 0, chains
+
+
+# --------------------------------------------------------------------------
+# The fused momentum passes reorder the reference's fill/conv sequence
+# (solver.py:254-265).  What makes that exact, checked on the oracle:
+#   first stage: u, v, w were filled since their last change, and a second
+#   fill changes only deep edge ghosts that no other component's conv4th
+#   stencil reads;
+#   second stage: after calvel the other components' ghosts do matter (so
+#   conv_u must see v and w before their fills), and conv_v reads w's fill
+#   only at the cells outside w's operator box in rows -1, nj and columns
+#   -2, -1, nk-1, nk -- the cells the v/w pass takes from its saved copy.
+# --------------------------------------------------------------------------
+def _fill_all(f, n, ulid=0.75):
+    O.apply_u(f["u"], n, ulid)
+    O.apply_v(f["v"], n)
+    O.apply_w(f["w"], n)
+
+
+def _conv(comp, f, n, dims):
+    star = np.zeros_like(f["u"])
+    oo = np.ones_like(f["u"])
+    h = 1.0 / n
+    getattr(O, "conv4th_" + comp)(f["u"], f["v"], f["w"], oo, star, h, h, h, 1e-3, 0.5, O.NPL,
+                                  dims[comp])
+    return star
+
+
+@pytest.mark.parametrize("n", [8, 13])
+def test_refill_changes_no_cell_a_cross_stencil_reads(n):
+    N = O.NPL
+    rng = np.random.default_rng(31)
+    shape = (n + 2 * N,) * 3
+    dims = {"u": (n - 1, n, n), "v": (n, n - 1, n), "w": (n, n, n - 1)}
+    filled = {k: rng.standard_normal(shape) for k in "uvw"}  # random ghosts before
+    _fill_all(filled, n)
+    refilled = {k: a.copy() for k, a in filled.items()}
+    _fill_all(refilled, n)
+    # the refill does change cells (edges whose rule reads a later side's ghost)
+    assert any(not np.array_equal(filled[k], refilled[k]) for k in "uvw")
+    for comp in "uvw":
+        own = {k: (refilled[k] if k == comp else filled[k]) for k in "uvw"}
+        assert np.array_equal(_conv(comp, own, n, dims), _conv(comp, refilled, n, dims)), comp
+
+
+@pytest.mark.parametrize("n", [8, 13])
+def test_second_stage_cross_reads(n):
+    N = O.NPL
+    rng = np.random.default_rng(32)
+    shape = (n + 2 * N,) * 3
+    dims = {"u": (n - 1, n, n), "v": (n, n - 1, n), "w": (n, n, n - 1)}
+    f = {k: rng.standard_normal(shape) for k in "uvw"}
+    _fill_all(f, n)
+    for k in "uvw":  # calvel: new interiors, ghosts left as they were
+        f[k][N:N + dims[k][0], N:N + dims[k][1], N:N + dims[k][2]] += \
+            rng.standard_normal(dims[k])
+    fresh = {k: a.copy() for k, a in f.items()}
+    _fill_all(fresh, n)
+    # conv_u reads v's and w's fills (so it must run before them) ...
+    mixed_u = {"u": fresh["u"], "v": fresh["v"], "w": fresh["w"]}
+    stale_u = {"u": fresh["u"], "v": f["v"], "w": f["w"]}
+    assert not np.array_equal(_conv("u", mixed_u, n, dims), _conv("u", stale_u, n, dims))
+    # ... and conv_v reads w's fill exactly at the cells the v/w pass saves
+    w_view = fresh["w"].copy()
+    ni = nj = nk = n
+    for J in (-1, nj):
+        w_view[N:N + ni, N + J, N - 2:N + nk + 1] = f["w"][N:N + ni, N + J, N - 2:N + nk + 1]
+    for K in (-2, -1, nk - 1, nk):
+        w_view[N:N + ni, N:N + nj, N + K] = f["w"][N:N + ni, N:N + nj, N + K]
+    ref_v = _conv("v", {"u": fresh["u"], "v": fresh["v"], "w": f["w"]}, n, dims)
+    assert np.array_equal(_conv("v", {"u": fresh["u"], "v": fresh["v"], "w": w_view}, n, dims),
+                          ref_v)
+    assert not np.array_equal(_conv("v", fresh, n, dims), ref_v)
