@@ -253,10 +253,16 @@ def hot_kernels(a, s64, smx):
         row("calvel3", "calvel3_kernel",
             lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"], f["u"],
                               f["v"], f["w"], *h, dt, 1.0, NPL), 56 * cells, 1, 1),
-        row("calvel3+calmas", "calvel3_calmas_kernel",
+        # the second outer iteration's pass stores u, v, w; the first's cannot
+        # exit the loop and forms su and rmax only
+        row("calvel3+calmas", "calvel3_calmas_kernel<true>",
             lambda: K.calvel3_calmas(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
                                      f["u"], f["v"], f["w"], f["su"], s64._buf, *h, dt, 1.0,
-                                     NPL), 64 * cells, 2, 2),
+                                     NPL), 64 * cells, 1, 1),
+        row("calmas of calvel3 (no store)", "calvel3_calmas_kernel<false>",
+            lambda: K.calvel3_calmas(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"],
+                                     f["u"], f["v"], f["w"], f["su"], s64._buf, *h, dt, 1.0,
+                                     NPL, store_uvw=False), 40 * cells, 1, 1),
         row("wale+checkdt", "wale_checkdt_kernel",
             lambda: K.wale_checkdt(s64.ncells, f["u"], f["v"], f["w"], f["vis"], s64._buf, *h,
                                    s64.rey, 0.46, NPL), 40 * cells, 1, 1),
@@ -420,7 +426,8 @@ ROW_KERNELS = {
     "convdiff u stage 2": ("convdiff_kernel<0, false>",),
     "convdiff_vw stage 2 (v, w)": ("convdiff_vw_kernel(",),
     "calvel3": ("calvel3_kernel(",),
-    "calvel3+calmas": ("calvel3_calmas_kernel(",),
+    "calvel3+calmas": ("calvel3_calmas_kernel<true>",),
+    "calmas of calvel3 (no store)": ("calvel3_calmas_kernel<false>",),
     "wale+checkdt": ("wale_checkdt_kernel(",),
     "add_ring": ("add_kernel(",),
     "copy (u -> uoo)": ("copy16_kernel(",),

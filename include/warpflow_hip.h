@@ -206,6 +206,17 @@ int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* ws
                       double* rmax_dev, float* rhs32, wf_grid g, double dx, double dy,
                       double dz, double dt, double alpha, void* stream);
 
+/* wf_calvel3_calmas without writing u, v, w: su, rmax and rhs32 are the
+ * same, u, v, w are only read (where calvel does not write them, calmas reads
+ * them).  For the first outer iteration of the projection loop, which never
+ * exits the loop (solver.py:188: it > 1), so the velocities its calvel writes
+ * are overwritten by the next iteration's calvel before anything reads them.
+ * No single reference kernel corresponds. */
+int wf_calmas_of_calvel3(const double* ustar, const double* vstar, const double* wstar,
+                         const double* p, const double* u, const double* v, const double* w,
+                         double* su, double* rmax_dev, float* rhs32, wf_grid g, double dx,
+                         double dy, double dz, double dt, double alpha, void* stream);
+
 /* wale_viscosity, kernels.py:297-367: vis = rrey + nut (WALE, cw), incl. the
  * reference's g2zx/g2zy terms (kernels.py:336, :338).  delta is
  * (dx*dy*dz)^(1/3) (kernels.py:360), computed once by the caller. */

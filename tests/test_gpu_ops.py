@@ -214,6 +214,15 @@ def test_calvel3_calmas_one_pass(gpu, n3):
     for nm in ("u", "v", "w", "su"):
         assert np.array_equal(a[nm].numpy(), b[nm].numpy()), nm
     assert ra.numpy()[0] == rb.numpy()[0] > 0.0
+    # the first outer iteration's form: the same su and rmax, u, v, w untouched
+    c = {k: up(v) for k, v in f.items()}
+    rc = zeros(1)
+    K.calvel3_calmas(n3, c["us"], c["vs"], c["ws"], c["p"], c["u"], c["v"], c["w"], c["su"],
+                     rc, *h, dt, 1.0, N, store_uvw=False)
+    assert np.array_equal(c["su"].numpy(), b["su"].numpy())
+    assert rc.numpy()[0] == rb.numpy()[0]
+    for nm in ("u", "v", "w"):
+        assert np.array_equal(c[nm].numpy(), f[nm]), nm
     # oracle
     ref = {c: f[c].copy() for c in "uvw"}
     ops = {"u": (n3[0] - 1, n3[1], n3[2]), "v": (n3[0], n3[1] - 1, n3[2]),

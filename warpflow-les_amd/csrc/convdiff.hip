@@ -758,8 +758,8 @@ __device__ inline void convdiff3_save_body(
       const double lap = l4(fu[5], fu[4], c0, fu[2], fu[1], K.ir12x) +
                          l4(uy2, uy1, c0, uym1, uym2, K.ir12y) +
                          l4(uz2, uz1, c0, uzm1, uzm2, K.ir12z);
-      buf_st<double>(prs(us, i), st_u, 0, rk_update(c0, nsum, vis_cu, lap, K.dta));
-      buf_st<double>(prs(uoo, i), st_u, 0, c0);
+      buf_st<double, BUF_NT>(prs(us, i), st_u, 0, rk_update(c0, nsum, vis_cu, lap, K.dta));
+      buf_st<double, BUF_NT>(prs(uoo, i), st_u, 0, c0);
     }
     // the three components one after the other: the scheduler may not mix
     // them, which would hold all three stencils' operands live at once
@@ -793,8 +793,8 @@ __device__ inline void convdiff3_save_body(
       const double lap = l4(fv[5], fv[4], c0, fv[2], fv[1], K.ir12x) +
                          l4(y2, y1, c0, ym1, ym2, K.ir12y) +
                          l4(z2, z1, c0, zm1, zm2, K.ir12z);
-      buf_st<double>(prs(vs, i), st_v, 0, rk_update(c0, nsum, vis_cv, lap, K.dta));
-      buf_st<double>(prs(voo, i), st_v, 0, c0);
+      buf_st<double, BUF_NT>(prs(vs, i), st_v, 0, rk_update(c0, nsum, vis_cv, lap, K.dta));
+      buf_st<double, BUF_NT>(prs(voo, i), st_v, 0, c0);
     }
     __builtin_amdgcn_sched_barrier(0);
     // ---- w (convdiff_body<2, true>) ----
@@ -826,8 +826,8 @@ __device__ inline void convdiff3_save_body(
       const double lap = l4(fw[5], fw[4], c0, fw[2], fw[1], K.ir12x) +
                          l4(y2, y1, c0, ym1, ym2, K.ir12y) +
                          l4(z2, z1, c0, zm1, zm2, K.ir12z);
-      buf_st<double>(prs(ws, i), st_w, 0, rk_update(c0, nsum, vis_cw, lap, K.dta));
-      buf_st<double>(prs(woo, i), st_w, 0, c0);
+      buf_st<double, BUF_NT>(prs(ws, i), st_w, 0, rk_update(c0, nsum, vis_cw, lap, K.dta));
+      buf_st<double, BUF_NT>(prs(woo, i), st_w, 0, c0);
     }
   }
 }
@@ -1079,7 +1079,7 @@ __device__ inline void convdiff_vw_body(
       const double lap = l4(fv[5], fv[4], c0, fv[2], fv[1], K.ir12x) +
                          l4(y2, y1, c0, ym1, ym2, K.ir12y) +
                          l4(z2, z1, c0, zm1, zm2, K.ir12z);
-      buf_st<double>(prs(vs, i), st_v, 0, rk_update(coo_v, nsum, vis_cv, lap, K.dta));
+      buf_st<double, BUF_NT>(prs(vs, i), st_v, 0, rk_update(coo_v, nsum, vis_cv, lap, K.dta));
     }
     __builtin_amdgcn_sched_barrier(0);
     // ---- w (convdiff_body<2, false>) ----
@@ -1111,7 +1111,7 @@ __device__ inline void convdiff_vw_body(
       const double lap = l4(fw[5], fw[4], c0, fw[2], fw[1], K.ir12x) +
                          l4(y2, y1, c0, ym1, ym2, K.ir12y) +
                          l4(z2, z1, c0, zm1, zm2, K.ir12z);
-      buf_st<double>(prs(ws, i), st_w, 0, rk_update(coo_w, nsum, vis_cw, lap, K.dta));
+      buf_st<double, BUF_NT>(prs(ws, i), st_w, 0, rk_update(coo_w, nsum, vis_cw, lap, K.dta));
     }
   }
 }

@@ -321,8 +321,9 @@ struct Consts {
 
 // One (k tile, j tile, i chunk) work item of the fused pass on a TJ x TK
 // tile (TJ * TK threads, one per tile cell); returns the item's max|div dV|
-// to every thread.
-template <int TJ, int TK>
+// to every thread.  STORE = false: u, v, w are corrected and used but not
+// written (su, rmax and the FP32 rhs are the same).
+template <int TJ, int TK, bool STORE = true>
 __device__ inline double calvel3_calmas_body(
     const double* __restrict__ us, const double* __restrict__ vs,
     const double* __restrict__ ws, const double* __restrict__ p, double* __restrict__ u,
@@ -396,7 +397,7 @@ __device__ inline double calvel3_calmas_body(
       const int i = I0 - 2 + q;
       uw[2 - q] = u_at(i, u_src(i)[plane(i) + xo], pq[q + 1], pq[q]);
     }
-    if (live && u_in(I0)) u[plane(I0) + xo] = uw[0];
+    if (STORE && live && u_in(I0)) u[plane(I0) + xo] = uw[0];
     uw[3] = 0.0;
   }
   double p0 = p[plane(I0) + xo], p1 = p[plane(I0 + 1) + xo];
@@ -445,7 +446,7 @@ __device__ inline double calvel3_calmas_body(
     W[T::wix(tj, tk)] = wo;
     if (hv) V[T::vix(vr, vc)] = vh_in ? calvel_val(N.vh, N.vh1, N.vh0, dta, C.rdy) : N.vh;
     if (hw) W[T::wix(wr, wc)] = wh_in ? calvel_val(N.wh, N.wh1, N.wh0, dta, C.rdz) : N.wh;
-    if (live) {
+    if (STORE && live) {
       if (u_in(i + 1) && i + 1 < I1) u[plane(i + 1) + xo] = uw[0];
       if (vo_in) v[plane(i) + xo] = vo;
       if (wo_in) w[plane(i) + xo] = wo;
@@ -477,6 +478,7 @@ __device__ inline double calvel3_calmas_body(
 // Single domain: 8 x 32 tiles; the workgroup ids are remapped over the XCDs
 // (k tile fastest, then j tile, then i-chunk).
 constexpr int PJ_TJ = 8, PJ_TK = 32;
+template <bool STORE>
 __global__ void __launch_bounds__(256)
 calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ vs,
                       const double* __restrict__ ws, const double* __restrict__ p,
@@ -485,8 +487,8 @@ calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ 
                       float* __restrict__ r32) {
   const int nb = kt * jt * (int)gridDim.y;
   const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
-  const double r = calvel3_calmas_body<PJ_TJ, PJ_TK>(us, vs, ws, p, u, v, w, su, g, ich, L % kt,
-                                                      (L / kt) % jt, L / (kt * jt), K, r32);
+  const double r = calvel3_calmas_body<PJ_TJ, PJ_TK, STORE>(
+      us, vs, ws, p, u, v, w, su, g, ich, L % kt, (L / kt) % jt, L / (kt * jt), K, r32);
   if (threadIdx.x == 0) g_red_part[blockIdx.x + blockIdx.y * gridDim.x] = r;
 }
 
@@ -1303,10 +1305,11 @@ int wf_calmas(const double* u, const double* v, const double* w, double* su,
   return fold(rmax_dev, r, 1, 0, 1, S_(stream));
 }
 
-int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* wstar,
-                      const double* p, double* u, double* v, double* w, double* su,
-                      double* rmax_dev, float* rhs32, wf_grid g, double dx, double dy,
-                      double dz, double dt, double alpha, void* stream) {
+static int calvel3_calmas_launch(const double* ustar, const double* vstar, const double* wstar,
+                                 const double* p, double* u, double* v, double* w, double* su,
+                                 double* rmax_dev, float* rhs32, wf_grid g, double dx,
+                                 double dy, double dz, double dt, double alpha, void* stream,
+                                 bool store) {
   const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
   if (!box_ok(g, b, 2, 1) || !rmax_dev) return WF_EINVAL;
   if (empty(b)) return 0;
@@ -1330,10 +1333,32 @@ int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* ws
   const pj::Consts K{1.0 / dx, 1.0 / dy, 1.0 / dz, dt * alpha,
                      1.0 / (24.0 * dx), 1.0 / (24.0 * dy), 1.0 / (24.0 * dz),
                      1.0 / dt, dx, dy, dz};
-  hipLaunchKernelGGL(calvel3_calmas_kernel, r.grid, dim3(PJ_TJ * PJ_TK), 0, S_(stream), ustar,
-                     vstar, wstar, p, u, v, w, su, g, ich, kt, jt, K, rhs32);
+  if (store)
+    hipLaunchKernelGGL(calvel3_calmas_kernel<true>, r.grid, dim3(PJ_TJ * PJ_TK), 0, S_(stream),
+                       ustar, vstar, wstar, p, u, v, w, su, g, ich, kt, jt, K, rhs32);
+  else
+    hipLaunchKernelGGL(calvel3_calmas_kernel<false>, r.grid, dim3(PJ_TJ * PJ_TK), 0, S_(stream),
+                       ustar, vstar, wstar, p, u, v, w, su, g, ich, kt, jt, K, rhs32);
   WF_LAUNCHED();
   return fold(rmax_dev, r, 1, 0, 1, S_(stream));
+}
+
+int wf_calvel3_calmas(const double* ustar, const double* vstar, const double* wstar,
+                      const double* p, double* u, double* v, double* w, double* su,
+                      double* rmax_dev, float* rhs32, wf_grid g, double dx, double dy,
+                      double dz, double dt, double alpha, void* stream) {
+  return calvel3_calmas_launch(ustar, vstar, wstar, p, u, v, w, su, rmax_dev, rhs32, g, dx, dy,
+                               dz, dt, alpha, stream, true);
+}
+
+int wf_calmas_of_calvel3(const double* ustar, const double* vstar, const double* wstar,
+                         const double* p, const double* u, const double* v, const double* w,
+                         double* su, double* rmax_dev, float* rhs32, wf_grid g, double dx,
+                         double dy, double dz, double dt, double alpha, void* stream) {
+  // the kernel only reads u, v, w when STORE is false
+  return calvel3_calmas_launch(ustar, vstar, wstar, p, const_cast<double*>(u),
+                               const_cast<double*>(v), const_cast<double*>(w), su, rmax_dev,
+                               rhs32, g, dx, dy, dz, dt, alpha, stream, false);
 }
 
 int wf_wale(const double* u, const double* v, const double* w, double* vis, wf_grid g,

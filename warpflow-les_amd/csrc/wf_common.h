@@ -173,14 +173,18 @@ __device__ inline T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t boff, int soff) {
   else
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)boff, soff, 0));
 }
-template <class T>
+// AUX: the cache-policy operand; BUF_NT (the gfx940-family "nt" bit) marks a
+// store non-temporal, so streamed outputs do not displace lines in the L2 that
+// other workgroups are about to read.
+constexpr int BUF_NT = 2;
+template <class T, int AUX = 0>
 __device__ inline void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t boff, int soff, T v) {
   if constexpr (sizeof(T) == 8) {
     using U2 = __attribute__((ext_vector_type(2))) unsigned;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)boff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)boff, soff, AUX);
   } else {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)boff, soff,
-                                          0);
+                                          AUX);
   }
 }
 

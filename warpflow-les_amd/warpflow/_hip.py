@@ -108,6 +108,7 @@ _SIGS = {
     "wf_calvel3": [_P, _P, _P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _P],
     "wf_calmas": [_P, _P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _P],
     "wf_calvel3_calmas": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _P],
+    "wf_calmas_of_calvel3": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _P],
     "wf_wale": [_P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _D, _D, _P],
     "wf_wale_checkdt": [_P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _D, _P],
     "wf_maxabs": [_P, _P, _G, _I, _I, _I, _P],

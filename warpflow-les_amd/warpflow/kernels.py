@@ -34,11 +34,11 @@ class Kernel:
         self.name = name
         self._fn = fn
 
-    def __call__(self, dim, *inputs):
+    def __call__(self, dim, *inputs, **options):
         d = tuple(int(x) for x in dim)
         if len(d) != 3:
             raise ValueError("%s: dim must have 3 entries, got %s" % (self.name, dim))
-        return self._fn(d, *inputs)
+        return self._fn(d, *inputs, **options)
 
     def __repr__(self):
         return "<warpflow kernel %s>" % self.name
@@ -155,11 +155,13 @@ def _calmas(dim, u, v, w, su, rmax, dx, dy, dz, dt, npl):
 
 
 def _calvel3_calmas(dim, ustar, vstar, wstar, p, u, v, w, su, rmax, dx, dy, dz, dt, alpha,
-                    npl, rhs32=None):
+                    npl, rhs32=None, store_uvw=True):
     """calvel3 then calmas over the real cells in one pass (``_project`` runs
     them back to back, solver.py:186-197); same results as the two kernels.
     rhs32: an FP32 (n+2)^3 multigrid level array that also receives (float)su
-    on its real cells."""
+    on its real cells.  store_uvw=False: u, v, w are read but not written
+    (su, rmax, rhs32 the same) -- for an outer iteration that cannot exit the
+    loop, whose velocities the next iteration overwrites unread."""
     _same(ustar, vstar, wstar, p, u, v, w, su)
     g = grid_of(u, npl)
     if dim != (g.ni, g.nj, g.nk):
@@ -168,9 +170,10 @@ def _calvel3_calmas(dim, ustar, vstar, wstar, p, u, v, w, su, rmax, dx, dy, dz, 
                               or np.dtype(rhs32.dtype) != np.float32 or rhs32.pitch
                               != rhs32.shape[2] or rhs32.lead != 0):
         raise ValueError("rhs32 must be a dense FP32 (n+2)^3 level array")
-    _hip.call("wf_calvel3_calmas", ustar.ptr, vstar.ptr, wstar.ptr, p.ptr, u.ptr, v.ptr,
-              w.ptr, su.ptr, rmax.ptr, rhs32.ptr if rhs32 is not None else None, g, dx, dy,
-              dz, dt, alpha, current_stream())
+    _hip.call("wf_calvel3_calmas" if store_uvw else "wf_calmas_of_calvel3", ustar.ptr,
+              vstar.ptr, wstar.ptr, p.ptr, u.ptr, v.ptr, w.ptr, su.ptr, rmax.ptr,
+              rhs32.ptr if rhs32 is not None else None, g, dx, dy, dz, dt, alpha,
+              current_stream())
 
 
 def wale_delta(dx, dy, dz):

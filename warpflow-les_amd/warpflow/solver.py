@@ -220,11 +220,12 @@ class FlowSolver:
         loc = float(r[0])
         self.rmax = self.topo.comm.allreduce(loc)
 
-    def _calvel_calmas(self):
+    def _calvel_calmas(self, store_uvw=True):
         """``_calvel(1.0)`` then ``_calmas()`` in one pass (solver.py:155-176 as
         _project runs them back to back): calvel3_calmas writes u, v, w, su
         and rmax as the two kernels do; the halo exchange between them does
-        nothing on one domain and is issued afterwards."""
+        nothing on one domain and is issued afterwards.  store_uvw=False
+        leaves u, v, w unwritten (the first outer iteration: see _project)."""
         f = self.f
         self._rbuf.zero_()
         # mixed form: the pass also writes (float)su into the FP32 level-0 rhs,
@@ -232,7 +233,7 @@ class FlowSolver:
         rhs32 = self.mg.levels[0].rhs if hasattr(self.mg, "add_result") else None
         K.calvel3_calmas(self.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"], f["u"],
                          f["v"], f["w"], f["su"], self._rbuf, self.dx, self.dy, self.dz,
-                         self.dt, 1.0, NPL, rhs32)
+                         self.dt, 1.0, NPL, rhs32, store_uvw=store_uvw)
         for name in ("u", "v", "w"):
             self.halo.exchange(f[name])
         self._read_calmas()
@@ -252,13 +253,19 @@ class FlowSolver:
         for it in range(1, maxcy + 1):
             iters = it
             if self.fused:
-                self._calvel_calmas()
+                # the first iteration cannot exit the loop (it > 1 below), so
+                # the velocities its calvel writes would be overwritten unread
+                # by the next iteration's calvel (or the closing one): its pass
+                # forms su and rmax without storing them
+                self._calvel_calmas(store_uvw=it > 1)
             else:
                 self._calmas()
             self.rmax_history.append(self.rmax)
             if self.rmax < eps and it > 1:
                 break
             if self.rmax > 10.0:
+                if self.fused and it == 1:
+                    self._calvel(1.0)  # the velocities the reference leaves behind
                 raise RuntimeError("projection diverged: rmax = %.6e at itime %d"
                                    % (self.rmax, self.itime))
             if self.fused and hasattr(self.mg, "add_result"):
