@@ -139,15 +139,15 @@ int wf_convdiff4th_save(int comp, const double* u, const double* v, const double
 
 /* wf_convdiff4th_save of u, v and w in one pass, each over its operator box
  * (solver.py:98-109: (ni-1, nj, nk), (ni, nj-1, nk), (ni, nj, nk-1)): the
- * first RK stage's fill/conv/diff sequence of solver.py:254-265 with the
- * three fills done first.  The values are those of the three single passes
- * bit for bit.  They are the reference's when u, v and w have been filled
- * since their last change and filled once more just before (as at the first
- * stage of every step: the projection ends with fill_bc of all three): a
- * second fill then changes only deep edge ghosts that no other component's
- * stencil reads.  It reads u, v, w and vis once, where the three single
- * passes read them 3 + 3 + 3 + 3 times.  No single reference kernel
- * corresponds to it. */
+ * first RK stage's fill/conv/diff sequence of solver.py:254-265.  The values
+ * are those of the three single passes bit for bit, and the reference's when
+ * u, v and w have been filled since their last change (as at the first stage
+ * of every step: the projection ends with fill_bc of all three): a further
+ * fill then changes only deep edge ghosts that no conv4th or diff4th stencil
+ * reads, and a third fill changes nothing, so one fill of the three after
+ * this call leaves the ghosts as the reference's sequence does.  It reads u,
+ * v, w and vis once, where the three single passes read them 3 + 3 + 3 + 3
+ * times.  No single reference kernel corresponds to it. */
 int wf_convdiff3_save(const double* u, const double* v, const double* w, const double* vis,
                       double* ustar, double* vstar, double* wstar, double* uoo, double* voo,
                       double* woo, wf_grid g, double dx, double dy, double dz, double dt,

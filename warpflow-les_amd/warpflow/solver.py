@@ -328,13 +328,13 @@ class FlowSolver:
                  ("w", K.conv4th_w, K.diff4th_w, K.convdiff4th_w))
         if self.fused and stage == 0 and self._fill_stable:
             # First stage, u, v, w as the end-of-step (or initial) fills left
-            # them: a second fill changes only deep edge ghosts that no other
-            # component's stencil reads, so the three fills can come first
-            # and the three conv+diff pairs run as one pass, with the same
-            # values as the fill/conv sequence.  The second fills follow for
-            # their effect on the ghosts.
+            # them: a second fill changes only deep edge ghosts that no
+            # conv4th/diff4th stencil reads, and a third fill changes nothing
+            # more.  So the three conv+diff pairs run as one pass on the fields
+            # as they are, with the values of the reference's fill/conv/fill/
+            # diff sequence, and one fill after it leaves the ghosts as that
+            # sequence's two fills do.
             self._fill_stable = False
-            self.fill_bc_uvw(alfabc)
             K.convdiff3_save(self.ncells, f["u"], f["v"], f["w"], f["vis"], f["ustar"],
                              f["vstar"], f["wstar"], f["uoo"], f["voo"], f["woo"], *d3, dt,
                              alpha, NPL)
