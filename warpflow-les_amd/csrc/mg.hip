@@ -738,31 +738,116 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
 
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
+  // j and k face-ring images of an output pair (cells a = k_a, b = k_a + 1)
+  auto jk_images = [&](int e, int r, int so, T ve, T vo, uint32_t oe, uint32_t oo) {
+    const int f = fl[e];
+    const bool xbf = ((f & FL_X) != 0) != (r != 0);  // the even cell is b
+    const uint32_t oa = xbf ? oo : oe, ob = xbf ? oe : oo;
+    const T va = xbf ? vo : ve, vb = xbf ? ve : vo;
+    const uint32_t sj = (uint32_t)(s1 * (int)sizeof(T)), sk = (uint32_t)sizeof(T);
+    const uint32_t jd = (f & FL_JLO) ? 0u - sj : sj;
+    const bool jf = (f & (FL_JLO | FL_JHI)) != 0;
+    buf_st(dout, jf && (f & FL_AL) ? oa + jd : SWEEP_DEAD, so, va);
+    buf_st(dout, jf && (f & FL_BL) ? ob + jd : SWEEP_DEAD, so, vb);
+    buf_st(dout, (f & FL_ALO) ? oa - sk : (f & FL_AHI) ? oa + sk : SWEEP_DEAD, so, va);
+    buf_st(dout, (f & FL_BHI) ? ob + sk : SWEEP_DEAD, so, vb);
+    if (n1 == 1) {  // both j faces: the high images too
+      buf_st(dout, (f & FL_AL) ? oa + sj : SWEEP_DEAD, so, va);
+      buf_st(dout, (f & FL_BL) ? ob + sj : SWEEP_DEAD, so, vb);
+    }
+    if (n2 == 1) buf_st(dout, (f & FL_AL) ? oa + sk : SWEEP_DEAD, so, va);
+  };
+  // The four stages of a steady iteration.  Each reads LDS slots that no
+  // stage of the same iteration writes; the restrict qualifiers say so, which
+  // lets the compiler issue a later stage's reads before an earlier stage's
+  // writes.
+  auto steady_stages = [&](auto Rc, auto XAc, auto XBc, const T* __restrict__ pm,
+                           const T* __restrict__ p0, const T* __restrict__ pp, T* __restrict__ oa,
+                           const T* __restrict__ oam, const T* __restrict__ oa0,
+                           const T* __restrict__ oap, T* __restrict__ ea,
+                           const T* __restrict__ eam, const T* __restrict__ ea0,
+                           const T* __restrict__ eap, T* __restrict__ ob,
+                           const T* __restrict__ obm, const T* __restrict__ ob0,
+                           const T* __restrict__ obp, int so) {
+    constexpr int r = decltype(Rc)::value, XA = decltype(XAc)::value, XB = decltype(XBc)::value;
+    relax(std::integral_constant<int, N3>{}, std::integral_constant<int, W4>{},
+          std::integral_constant<int, C3>{}, pm, p0, pp, ao_a0, ao_aj[r], rao[XA], ao_rp[r], oa);
+    relax(std::integral_constant<int, N2>{}, std::integral_constant<int, W3>{},
+          std::integral_constant<int, C2>{}, oam, oa0, oap, ae_a0, ae_aj[r], rae[XA], ae_rp[r],
+          ea);
+    relax(std::integral_constant<int, N1>{}, std::integral_constant<int, W2>{},
+          std::integral_constant<int, C1>{}, eam, ea0, eap, bo_a0, bo_aj[r], rbo[XB], bo_rp[r],
+          ob);
+#pragma unroll
+    for (int e = 0; e < NPAIR; ++e) {
+      const int a = be_a0[e], c = be_aj[r][e];
+      const T sum = c0 * obm[c] + c0 * obp[c] + c1 * ob0[c - W1] + c1 * ob0[c + W1] +
+                    c2 * ob0[a] + c2 * ob0[a + 1];
+      const T ve = (rbe[XB][e] - sum) * be_rp[r][e];
+      const T vo = ob0[pcol[r][e]];
+      const uint32_t oe = be_ro[r][e], oo = pofs[r][e];
+      buf_st(dout, oe, so, ve);
+      buf_st(dout, oo, so, vo);
+      if (!wfast) jk_images(e, r, so, ve, vo, oe, oo);
+    }
+  };
+  // Steady iterations q in [qs, qe]: every stage runs and no stage's plane is
+  // an i face or ring plane (A-odd's q <= n0 - 2, B-even's q - 6 >= 1), so
+  // they skip the stage tests, the face diagonals and the i-face images, the
+  // pin plane they store (q + 1 < n0) enters as loaded, and their load planes
+  // need only the upper clamps (the plane offsets are those soff gives).
+  const int qs = max(i0 + 6, 7), qe = min(i1 + 2, n0 - 2);
+  const int hx_ao = min(i1 + 2, n0), hx_ae = min(i1 + 1, n0);
   // Iteration t handles q = i0-3+t; plane x sits in LDS slot (x - i0 + 3) & 3
   // and has class index (x - i0 + 3) & 1.
-  auto step = [&](auto Uc, int q) {
+  auto step = [&](auto Uc, auto Sc, int q) {
     constexpr int U = decltype(Uc)::value;
+    constexpr bool ST = decltype(Sc)::value;
     constexpr int r = U & 1, rn = r ^ 1;
     constexpr int Sm1 = (U + 3) & 3, S0 = U, Sp1 = (U + 1) & 3, Sm2 = (U + 2) & 3;
     using Cr = std::integral_constant<int, r>;
     using Crn = std::integral_constant<int, rn>;
-    store_pin(Crn{}, P[Sp1], q + 1);
-    load_pin(Crn{}, q + 3);
-    {  // rhs: A stages of iteration t+2 (class r), B stages of t+1 (class rn)
-      constexpr int XA = (U + 2) & 3, XB = (U + 1) & 1;
-      const int sao = soff(q + 2, i0 - 3, i1 + 2), sae = soff(q, i0 - 2, i1 + 1);
-      const int sbo = soff(q - 3, i0 - 1, i1), sbe = soff(q - 5, i0, i1 - 1);
-#pragma unroll
-      for (int e = 0; e < N3; ++e) rao[XA][e] = buf_ld<T>(drhs, ao_ro[r][e], sao);
-#pragma unroll
-      for (int e = 0; e < N2; ++e) rae[XA][e] = buf_ld<T>(drhs, ae_ro[r][e], sae);
+    // Loads, oldest first: the B stages' rhs (used next iteration), pin (two
+    // iterations on), the A stages' rhs (two iterations on).  The memory
+    // counter completes in order, so a wait for this iteration's B rhs next
+    // iteration leaves every later load in flight.
+    {  // rhs of the B stages of iteration t+1 (class rn)
+      constexpr int XB = (U + 1) & 1;
+      const int sbo = ST ? (q - 3 - xb) * pb : soff(q - 3, i0 - 1, i1);
+      const int sbe = ST ? (q - 5 - xb) * pb : soff(q - 5, i0, i1 - 1);
 #pragma unroll
       for (int e = 0; e < N1; ++e) rbo[XB][e] = buf_ld<T>(drhs, bo_ro[rn][e], sbo);
 #pragma unroll
       for (int e = 0; e < NPAIR; ++e) rbe[XB][e] = buf_ld<T>(drhs, be_ro[rn][e], sbe);
     }
+    if constexpr (ST) {
+#pragma unroll
+      for (int e = 0; e < N4; ++e) P[Sp1][tid + 256 * e < C4 ? tid + 256 * e : C4] = pre[rn][e];
+      const int so = (min(q + 3, pin_hi) - xb) * pb;
+#pragma unroll
+      for (int e = 0; e < N4; ++e) pre[rn][e] = buf_ld<T>(dpin, gofs[rn][e], so);
+    } else {
+      store_pin(Crn{}, P[Sp1], q + 1);
+      load_pin(Crn{}, q + 3);
+    }
+    {  // rhs of the A stages of iteration t+2 (class r)
+      constexpr int XA = (U + 2) & 3;
+      const int sao = ST ? (min(q + 2, hx_ao) - xb) * pb : soff(q + 2, i0 - 3, i1 + 2);
+      const int sae = ST ? (min(q, hx_ae) - xb) * pb : soff(q, i0 - 2, i1 + 1);
+#pragma unroll
+      for (int e = 0; e < N3; ++e) rao[XA][e] = buf_ld<T>(drhs, ao_ro[r][e], sao);
+#pragma unroll
+      for (int e = 0; e < N2; ++e) rae[XA][e] = buf_ld<T>(drhs, ae_ro[r][e], sae);
+    }
     __syncthreads();
     constexpr int XA = U, XB = U & 1;
+    if constexpr (ST) {
+      steady_stages(Cr{}, std::integral_constant<int, XA>{}, std::integral_constant<int, XB>{},
+                    P[Sm1], P[S0], P[Sp1], OA[S0], OA[(U + 1) & 3], OA[Sm2], OA[Sm1], EA[Sm2],
+                    EA[Sm1], EA[S0], EA[Sp1], OB[S0], OB[Sp1], OB[Sm2], OB[Sm1],
+                    (q - 6 - xb) * pb);
+      return;
+    }
     // Each stage runs only for the planes a later stage or the output needs
     // (A-odd i0-3 .. i1+2, A-even i0-2 .. i1+1, B-odd i0-1 .. i1, B-even
     // i0 .. i1-1); the pipeline's fill and drain iterations skip the rest.
@@ -827,24 +912,7 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
             buf_st(dout, oo, (n0 - xb) * pb, vo);
           }
         }
-        if (!wfast) {  // j and k face-ring images
-          const int f = fl[e];
-          const bool xbf = ((f & FL_X) != 0) != (r != 0);  // the even cell is b
-          const uint32_t oa = xbf ? oo : oe, ob = xbf ? oe : oo;
-          const T va = xbf ? vo : ve, vb = xbf ? ve : vo;
-          const uint32_t sj = (uint32_t)(s1 * (int)sizeof(T)), sk = (uint32_t)sizeof(T);
-          const uint32_t jd = (f & FL_JLO) ? 0u - sj : sj;
-          const bool jf = (f & (FL_JLO | FL_JHI)) != 0;
-          buf_st(dst, jf && (f & FL_AL) ? oa + jd : SWEEP_DEAD, so, va);
-          buf_st(dst, jf && (f & FL_BL) ? ob + jd : SWEEP_DEAD, so, vb);
-          buf_st(dst, (f & FL_ALO) ? oa - sk : (f & FL_AHI) ? oa + sk : SWEEP_DEAD, so, va);
-          buf_st(dst, (f & FL_BHI) ? ob + sk : SWEEP_DEAD, so, vb);
-          if (n1 == 1) {
-            buf_st(dst, (f & FL_AL) ? oa + sj : SWEEP_DEAD, so, va);
-            buf_st(dst, (f & FL_BL) ? ob + sj : SWEEP_DEAD, so, vb);
-          }
-          if (n2 == 1) buf_st(dst, (f & FL_AL) ? oa + sk : SWEEP_DEAD, so, va);
-        }
+        if (!wfast) jk_images(e, r, so, ve, vo, oe, oo);  // j and k face-ring images
       }
     }
   };
@@ -865,17 +933,26 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
       rao[1][e] = buf_ld<T>(drhs, ao_ro[1][e], s1x);
     }
   }
+  // Let the prologue's loads land before the loop, so that the compiler's
+  // memory-counter waits inside the loop count only the loop's own loads.
+  __builtin_amdgcn_s_waitcnt(0);
   const int cnt = i1 - i0 + 9;  // q = i0-3 .. i1+5
   using I2 = std::integral_constant<int, 2>;
   using I3 = std::integral_constant<int, 3>;
+  auto iter = [&](auto Uc, int q) {
+    if (q >= qs && q <= qe)
+      step(Uc, std::true_type{}, q);
+    else
+      step(Uc, std::false_type{}, q);
+  };
   for (int t = 0;; t += 4) {
-    step(I0{}, i0 - 3 + t);
+    iter(I0{}, i0 - 3 + t);
     if (t + 1 >= cnt) break;
-    step(I1{}, i0 - 2 + t);
+    iter(I1{}, i0 - 2 + t);
     if (t + 2 >= cnt) break;
-    step(I2{}, i0 - 1 + t);
+    iter(I2{}, i0 - 1 + t);
     if (t + 3 >= cnt) break;
-    step(I3{}, i0 + t);
+    iter(I3{}, i0 + t);
     if (t + 4 >= cnt) break;
   }
 }
