@@ -531,7 +531,9 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  *   WF_TUNE_CD_ICH              i-planes per workgroup of wf_convdiff4th
  *   WF_TUNE_SWEEP_TILE_F64/F32  fused RBGS sweep tile (j x k): 1 = 8 x 64,
  *                               2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32
- *   WF_TUNE_SWEEP_ICH_F64/F32   i-planes per workgroup of the fused sweep
+ *   WF_TUNE_SWEEP_ICH_F64/F32   i-planes per workgroup of the fused sweep;
+ *                               -2 / -4: chunks of 2 / 4 planes, each in one
+ *                               pass without the plane pipeline (small levels)
  *   WF_TUNE_SWEEP2_TILE_F64/F32 tile of the two-sweep kernel (pairs of sweeps
  *                               in a V-cycle): 1 = 8 x 64, 2 = 16 x 32; FP32
  *                               also 3 = 16 x 64, 4 = 32 x 32

@@ -190,8 +190,10 @@ def test_coarse_operator_equals_sweeps(gpu, n3, prec):
 @pytest.mark.parametrize("tile", [1, 2, 3, 4])
 def test_sweep_tiles_agree_bitwise(gpu, prec, tile):
     """Every fused-sweep tile shape (8x64, 16x32, 16x64, 32x32; one or two
-    (even, odd) pairs per thread) and i-chunk gives the built-in result bit
-    for bit, on a level whose extents are multiples of none of them."""
+    (even, odd) pairs per thread) and i-chunk -- also the one-pass chunks of
+    2 and 4 planes (-2, -4), whose last chunk here holds one plane -- gives
+    the built-in result bit for bit, on a level whose extents are multiples
+    of none of them."""
     import ctypes as C
     from warpflow import _hip, autotune
     from warpflow.multigrid import _Level
@@ -212,7 +214,7 @@ def test_sweep_tiles_agree_bitwise(gpu, prec, tile):
         lev.phi.assign(phi0)
         _hip.call("wf_mg_relax", C.byref(d), 4, None)
         want = lev.phi.numpy()
-        for ich in (1, 3, 16):
+        for ich in (1, 3, 16, -2, -4):
             autotune.set_shape(knob_t, n3, tile)
             autotune.set_shape(knob_i, n3, ich)
             lev.phi.assign(phi0)

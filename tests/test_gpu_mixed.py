@@ -143,6 +143,42 @@ def test_solve_takes_the_zero_start_unread(gpu, mixed):
     assert np.array_equal(pp2.numpy(), want)
 
 
+@pytest.mark.parametrize("nb", [2, 4])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_one_pass_sweep_chunks_change_no_result(gpu, mixed, nb):
+    """Solves whose every level relaxes with the one-sweep kernel's one-pass
+    chunks (nb planes per workgroup, all loads at once) give the built-in
+    shapes' pp bit for bit -- including each V-cycle's first sweep from the
+    unread zero iterate."""
+    from warpflow import autotune
+    from warpflow.device import array
+    from warpflow.mixed import MixedMultigrid
+    from warpflow.multigrid import Multigrid
+    n3 = (48, 40, 64)
+    h = tuple(1.0 / n for n in n3)
+    shape = tuple(n + 6 for n in n3)
+    su = np.random.default_rng(6).standard_normal(shape)
+    if mixed:
+        mg = MixedMultigrid(_Cfg(3), topo_for(n3), None, *h, max_cycles=2)
+    else:
+        mg = Multigrid(_Cfg(3), topo_for(n3), None, *h)
+    kp = autotune.SWEEP_PAIRS_F32 if mixed else autotune.SWEEP_PAIRS_F64
+    ki = autotune.SWEEP_ICH_F32 if mixed else autotune.SWEEP_ICH_F64
+    autotune.reset()
+    try:
+        pp = array(np.zeros(shape))
+        mg.solve(array(su), pp, 3)
+        want = pp.numpy()
+        for lev in mg.levels:
+            autotune.set_shape(kp, lev.n3, 1)
+            autotune.set_shape(ki, lev.n3, -nb)
+        pp2 = array(np.zeros(shape))
+        mg.solve(array(su), pp2, 3)
+    finally:
+        autotune.reset()
+    assert np.array_equal(pp2.numpy(), want)
+
+
 def test_deferred_gauge_mean_gives_the_same_p(gpu):
     """solve(defer_mean=True) + add_result (the shift of pp applied inside
     p += pp) leaves p bit for bit as solve + add_inplace of the shifted pp."""

@@ -231,8 +231,9 @@ int wf_graph_destroy(void* exec) {
 }
 
 int wf_tune_set(int knob, int n0, int n1, int n2, int value) {
-  if (knob < 0 || knob >= WF_TUNE_COUNT || value < 0 || n0 <= 0 || n1 <= 0 || n2 <= 0)
-    return WF_EINVAL;
+  if (knob < 0 || knob >= WF_TUNE_COUNT || n0 <= 0 || n1 <= 0 || n2 <= 0) return WF_EINVAL;
+  const bool sweep_ich = knob == WF_TUNE_SWEEP_ICH_F64 || knob == WF_TUNE_SWEEP_ICH_F32;
+  if (value < 0 && !(sweep_ich && (value == -2 || value == -4))) return WF_EINVAL;
   if ((knob == WF_TUNE_SWEEP_TILE_F64 || knob == WF_TUNE_SWEEP_TILE_F32) && value > 4)
     return WF_EINVAL;
   if ((knob == WF_TUNE_SWEEP2_TILE_F64 && value > 2) ||

@@ -203,7 +203,7 @@ constexpr uint32_t SWEEP_DEAD = BUF_DEAD;
 // needed read a valid plane and are ignored), so the waits on the prefetched
 // planes stay exact.  HBM traffic per sweep: one read of pin and rhs, one
 // write of pout.
-template <class T, int TJ, int TK>
+template <class T, int TJ, int TK, int NB = 0>
 __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pout,
                                   const T* __restrict__ rhs, const MgDev& d, T c0, T c1, T c2,
                                   int ich, Blk B) {
@@ -215,8 +215,11 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
   constexpr int HK = TK / 2;                     // pairs per tile row
   constexpr int NPE = (PJ * PH + 255) / 256;     // pin cells per thread per plane
   constexpr int NO = (OJ * OH + 255) / 256;      // odd cells per thread per plane
-  __shared__ T P[3][NPE * 256];
-  __shared__ T O[3][NO * 256];
+  // plane slots: three (a ring) when pipelined, every plane of the chunk in
+  // the one-pass chunks (NB > 0)
+  constexpr int QP = NB ? NB + 4 : 3, QO = NB ? NB + 2 : 3;
+  __shared__ T P[QP][NPE * 256];
+  __shared__ T O[QO][NO * 256];
   const int tid = threadIdx.x;
   const int j0 = B.y * TJ, k0 = B.x * TK;
   const int i0 = B.z * ich, i1 = min(i0 + ich, d.n0);
@@ -333,6 +336,154 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
     asm volatile("" : "+v"(t));
     return t;
   };
+  // j and k face-ring images of a pair's cells a and b (class rn)
+  auto jk_images = [&](const auto& dst, int e, int rn, int so, T ve, T vo, uint32_t oe,
+                       uint32_t oo) {
+    const int f = fl[e];
+    const bool xbf = ((f & FL_X) != 0) != (rn != 0);  // the even cell is b
+    const uint32_t oa = xbf ? oo : oe, ob = xbf ? oe : oo;
+    const T va = xbf ? vo : ve, vb = xbf ? ve : vo;
+    const uint32_t sj = (uint32_t)(s1 * (int)sizeof(T)), sk = (uint32_t)sizeof(T);
+    const uint32_t jd = (f & FL_JLO) ? 0u - sj : sj;
+    const bool jf = (f & (FL_JLO | FL_JHI)) != 0;
+    buf_st(dst, jf && (f & FL_AL) ? oa + jd : SWEEP_DEAD, so, va);
+    buf_st(dst, jf && (f & FL_BL) ? ob + jd : SWEEP_DEAD, so, vb);
+    buf_st(dst, (f & FL_ALO) ? oa - sk : (f & FL_AHI) ? oa + sk : SWEEP_DEAD, so, va);
+    buf_st(dst, (f & FL_BHI) ? ob + sk : SWEEP_DEAD, so, vb);
+    if (n1 == 1) {  // both j faces: the high images too
+      buf_st(dst, (f & FL_AL) ? oa + sj : SWEEP_DEAD, so, va);
+      buf_st(dst, (f & FL_BL) ? ob + sj : SWEEP_DEAD, so, vb);
+    }
+    if (n2 == 1) buf_st(dst, (f & FL_AL) ? oa + sk : SWEEP_DEAD, so, va);
+  };
+  // One-pass chunks (NB > 0), for the small levels, whose pipelined
+  // launches are a few dependent plane steps long: every load the chunk
+  // needs is issued at once -- pin's even cells of planes i0-2 .. i1+1 (tile
+  // + 2-cell halo), the odd cells' rhs of planes i0-1 .. i1 and the even
+  // cells' rhs of planes i0 .. i1-1.  Then the odd values of planes
+  // i0-1 .. i1 (tile + 1-cell halo; the neighbouring chunks compute the two
+  // outer planes too), a barrier, and the even values of planes i0 .. i1-1
+  // with the write.  Cells, terms and their order are the pipelined path's,
+  // so the result is its bit for bit; a launch waits for one memory round
+  // trip instead of one per plane step.
+  if constexpr (NB > 0) {
+    // Descriptors start at the array plane of real plane xb = max(i0-2, -1);
+    // a plane x is read at soffset (clamp(x, -1, n0) - xb) * pb (planes past
+    // the array are read clamped and their values ignored or zeroed).
+    const int pb = (int)(d.s0 * (int64_t)sizeof(T));
+    const int xb = max(i0 - 2, -1);
+    auto chunk_rsrc = [&](const T* a) {
+      return buf_rsrc(reinterpret_cast<const char*>(a) + (int64_t)(xb + 1) * pb, 0x7fffffffu);
+    };
+    const auto dpin = pin ? chunk_rsrc(pin) : buf_rsrc(rhs, 0u);
+    const auto drhs = chunk_rsrc(rhs), dout = chunk_rsrc(pout);
+    auto soff = [&](int x) { return (clampi(x, -1, n0) - xb) * pb; };
+
+    // every load of the chunk
+    T pre[QP][NPE], rodd[QO][NO], reven[NB][NPAIR];
+#pragma unroll
+    for (int p = 0; p < QP; ++p) {  // pin plane x = i0 - 2 + p, class (p + 1) & 1
+      const int so = soff(i0 - 2 + p);
+#pragma unroll
+      for (int e = 0; e < NPE; ++e) pre[p][e] = buf_ld<T>(dpin, gofs[(p + 1) & 1][e], so);
+    }
+#pragma unroll
+    for (int p = 0; p < QO; ++p) {  // odd plane x = i0 - 1 + p, class p & 1
+      const int so = soff(i0 - 1 + p);
+#pragma unroll
+      for (int e = 0; e < NO; ++e) rodd[p][e] = buf_ld<T>(drhs, oofs[p & 1][e], so);
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {  // even plane x = i0 + p, class (p + 1) & 1
+      const int so = soff(i0 + p);
+#pragma unroll
+      for (int e = 0; e < NPAIR; ++e) reven[p][e] = buf_ld<T>(drhs, eofs[(p + 1) & 1][e], so);
+    }
+    // pin to LDS; the ring planes -1 and n0 (and beyond) enter as zero
+#pragma unroll
+    for (int p = 0; p < QP; ++p) {
+      const int x = i0 - 2 + p;
+      const bool z = x < 0 || x >= n0;
+#pragma unroll
+      for (int e = 0; e < NPE; ++e) P[p][tid + 256 * e] = z ? T(0) : pre[p][e];
+    }
+    __syncthreads();
+    // odd values of planes i0-1 .. i0+NB (those past i1 are not read)
+#pragma unroll
+    for (int p = 0; p < QO; ++p) {
+      const int q = i0 - 1 + p;
+      const int r = p & 1;
+      T rp[NO];
+#pragma unroll
+      for (int e = 0; e < NO; ++e) rp[e] = rapo[r][e];
+      if (q <= 0 || q >= n0 - 1) {  // i faces (and the ring planes: all dead)
+        const int t = opaque_tid();
+        const bool qin = q >= 0 && q < n0;
+#pragma unroll
+        for (int e = 0; e < NO; ++e) {
+          int j, k;
+          const bool live = odd_cell(t, e, r, j, k) && qin;
+          rp[e] = live ? rdiag(qin ? q : 0, j, k) : T(0);
+        }
+      }
+      const T* SM = P[p];      // pin plane q-1
+      const T* S0 = P[p + 1];  // pin plane q
+      const T* SP = P[p + 2];  // pin plane q+1
+#pragma unroll
+      for (int e = 0; e < NO; ++e) {
+        const int a = a0[e], c = aj[r][e];
+        const T sum = c0 * SM[c] + c0 * SP[c] + c1 * S0[c - PH] + c1 * S0[c + PH] + c2 * S0[a] +
+                      c2 * S0[a + 1];
+        O[p][tid + 256 * e] = (rodd[p][e] - sum) * rp[e];
+      }
+    }
+    __syncthreads();
+    // even values of planes i0 .. i1-1 and their write
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int pl = i0 + p;
+      if (pl >= i1) break;
+      const int rn = (p + 1) & 1;  // class of plane pl
+      T re[NPAIR];
+#pragma unroll
+      for (int e = 0; e < NPAIR; ++e) re[e] = rape[rn][e];
+      if (pl == 0 || pl == n0 - 1) {
+        const int t = opaque_tid();
+#pragma unroll
+        for (int e = 0; e < NPAIR; ++e) {
+          int j, ke, ko;
+          pair_cell(t, e, rn, j, ke, ko);
+          re[e] = rdiag(pl, j, min(ke, n2 - 1));
+        }
+      }
+      const T* Om = O[p];      // odd values of plane pl-1
+      const T* O0 = O[p + 1];  // plane pl
+      const T* Op = O[p + 2];  // plane pl+1
+      const int so = soff(pl);
+#pragma unroll
+      for (int e = 0; e < NPAIR; ++e) {
+        const int b = b0[e], c = bj[rn][e];
+        const T sum = c0 * Om[c] + c0 * Op[c] + c1 * O0[c - OH] + c1 * O0[c + OH] + c2 * O0[b] +
+                      c2 * O0[b + 1];
+        const T ve = (reven[p][e] - sum) * re[e];
+        const T vo = O0[bo[rn][e]];
+        const uint32_t oe = eofs[rn][e], oo = pofs[rn][e];
+        buf_st(dout, oe, so, ve);
+        buf_st(dout, oo, so, vo);
+        if (pl == 0) {  // i-face ring images
+          buf_st(dout, oe, soff(-1), ve);
+          buf_st(dout, oo, soff(-1), vo);
+        }
+        if (pl == n0 - 1) {
+          buf_st(dout, oe, soff(n0), ve);
+          buf_st(dout, oo, soff(n0), vo);
+        }
+        if (!wfast) jk_images(dout, e, rn, so, ve, vo, oe, oo);
+      }
+    }
+    return;
+  }
+
   // Descriptors start at array plane i0 (real plane i0-1); real plane x >= i0-1
   // sits at soffset (x + 1 - i0) * pb.
   const int pb = (int)(d.s0 * (int64_t)sizeof(T));
@@ -461,24 +612,7 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
           buf_st(dout, oo, soff(n0), vo);
         }
       }
-      if (!wfast) {  // j and k face-ring images of cells a and b (dst: dropped if !own)
-        const int f = fl[e];
-        const bool xb = ((f & FL_X) != 0) != (rn != 0);  // the even cell is b
-        const uint32_t oa = xb ? oo : oe, ob = xb ? oe : oo;
-        const T va = xb ? vo : ve, vb = xb ? ve : vo;
-        const uint32_t sj = (uint32_t)(s1 * (int)sizeof(T)), sk = (uint32_t)sizeof(T);
-        const uint32_t jd = (f & FL_JLO) ? 0u - sj : sj;
-        const bool jf = (f & (FL_JLO | FL_JHI)) != 0;
-        buf_st(dst, jf && (f & FL_AL) ? oa + jd : SWEEP_DEAD, so, va);
-        buf_st(dst, jf && (f & FL_BL) ? ob + jd : SWEEP_DEAD, so, vb);
-        buf_st(dst, (f & FL_ALO) ? oa - sk : (f & FL_AHI) ? oa + sk : SWEEP_DEAD, so, va);
-        buf_st(dst, (f & FL_BHI) ? ob + sk : SWEEP_DEAD, so, vb);
-        if (n1 == 1) {  // both j faces: the high images too
-          buf_st(dst, (f & FL_AL) ? oa + sj : SWEEP_DEAD, so, va);
-          buf_st(dst, (f & FL_BL) ? ob + sj : SWEEP_DEAD, so, vb);
-        }
-        if (n2 == 1) buf_st(dst, (f & FL_AL) ? oa + sk : SWEEP_DEAD, so, va);
-      }
+      if (!wfast) jk_images(dst, e, rn, so, ve, vo, oe, oo);  // (dst: dropped if !own)
     }
   };
 
@@ -522,11 +656,11 @@ __device__ inline void sweep_body(const T* __restrict__ pin, T* __restrict__ pou
     if (t + 6 >= cnt) break;
   }
 }
-template <class T, int TJ, int TK>
+template <class T, int TJ, int TK, int NB = 0>
 __global__ void __launch_bounds__(256)
 sweep_kernel(const T* __restrict__ pin, T* __restrict__ pout, const T* __restrict__ rhs,
              MgDev d, T c0, T c1, T c2, int ich) {
-  sweep_body<T, TJ, TK>(pin, pout, rhs, d, c0, c1, c2, ich, this_blk());
+  sweep_body<T, TJ, TK, NB>(pin, pout, rhs, d, c0, c1, c2, ich, this_blk());
 }
 
 // Two full RBGS sweeps fused in one pass: pout = S(S(pin)).  The workgroup
@@ -1643,10 +1777,24 @@ int launch_sweep(const wf_mg_level& L, const void* pin, void* pout, hipStream_t 
   const int64_t tiles = (int64_t)kt * jt;
   const int tuned = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP_ICH_F64 : WF_TUNE_SWEEP_ICH_F32,
                              d.n0, d.n1, d.n2);
+  const int64_t pb = d.s0 * (int64_t)sizeof(T);
+  if (tuned == -2 || tuned == -4) {  // chunks of 2 or 4 planes in one pass each
+    const int nb = -tuned;
+    // the kernel's plane offsets within a chunk (nb + 4 planes) stay below 2^30
+    if ((int64_t)(nb + 4) * pb >= ((int64_t)1 << 30)) return WF_EINVAL;
+    const dim3 grid(kt, jt, div_up(d.n0, nb));
+    if (nb == 2)
+      hipLaunchKernelGGL((sweep_kernel<T, TJ, TK, 2>), grid, dim3(256), 0, s, (const T*)pin,
+                         (T*)pout, (const T*)L.rhs, d, c0, c1, c2, nb);
+    else
+      hipLaunchKernelGGL((sweep_kernel<T, TJ, TK, 4>), grid, dim3(256), 0, s, (const T*)pin,
+                         (T*)pout, (const T*)L.rhs, d, c0, c1, c2, nb);
+    WF_LAUNCHED();
+    return 0;
+  }
   int ich = tuned > 0 ? tuned
                       : (int)std::max<int64_t>(4, std::min<int64_t>(16, tiles * d.n0 / 2048));
   // the kernel's plane offsets within a chunk ((ich + 3) planes) stay below 2^30
-  const int64_t pb = d.s0 * (int64_t)sizeof(T);
   if ((int64_t)4 * pb >= ((int64_t)1 << 30)) return WF_EINVAL;
   ich = (int)std::min<int64_t>(ich, ((int64_t)1 << 30) / pb - 3);
   const dim3 grid(kt, jt, div_up(d.n0, ich));
@@ -1684,11 +1832,11 @@ int launch_sweep2(const wf_mg_level& L, const void* pin, void* pout, hipStream_t
   const int64_t tiles = (int64_t)kt * jt;
   const int tuned = tune_get(sizeof(T) == 8 ? WF_TUNE_SWEEP2_ICH_F64 : WF_TUNE_SWEEP2_ICH_F32,
                              d.n0, d.n1, d.n2);
-  int ich = tuned > 0 ? tuned
-                      : (int)std::max<int64_t>(4, std::min<int64_t>(32, tiles * d.n0 / 1024));
   // the kernel's plane offsets within a chunk ((ich + 8) planes) stay below 2^30
   const int64_t pb = d.s0 * (int64_t)sizeof(T);
   if ((int64_t)9 * pb >= ((int64_t)1 << 30)) return WF_EINVAL;
+  int ich = tuned > 0 ? tuned
+                      : (int)std::max<int64_t>(4, std::min<int64_t>(32, tiles * d.n0 / 1024));
   ich = (int)std::min<int64_t>(ich, ((int64_t)1 << 30) / pb - 8);
   const dim3 grid(kt, jt, div_up(d.n0, ich));
   hipLaunchKernelGGL((sweep2_kernel<T, TJ, TK>), grid, dim3(256), 0, s, (const T*)pin, (T*)pout,

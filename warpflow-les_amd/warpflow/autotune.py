@@ -9,7 +9,9 @@ through (``cd_ich``, ``cd3_ich``, ``cdvw_ich``, ``proj_ich``, ``sweep_ich``,
 ``sweep2_ich``) and the (j, k) tile of the one-sweep kernel (``sweep_tile``:
 1 = 8 x 64, 2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32) and of the two-sweep
 kernel (``sweep2_tile``: 1 = 8 x 64, 2 = 16 x 32, and in FP32 also
-3 = 16 x 64, 4 = 32 x 32); ``sweep_pairs`` picks, per level and precision,
+3 = 16 x 64, 4 = 32 x 32); a negative ``sweep_ich`` (-2, -4) runs the one-sweep
+kernel's one-pass variant, chunks of 2 or 4 planes whose loads are all issued
+at once (for small levels); ``sweep_pairs`` picks, per level and precision,
 one-sweep launches (1) or two-sweep launches (2) for pairs of sweeps.  The
 library keeps one setting per (parameter, launch shape); ``tune_solver``
 times every candidate with HIP events on a solver's own arrays -- the
@@ -38,7 +40,7 @@ CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
 CD3_ICH_CANDIDATES = (0, 32, 43, 64, 86, 128, 256)
 CDVW_ICH_CANDIDATES = (0, 16, 32, 43, 64, 86, 128)
 SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
-SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32)
+SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32, -2, -4)  # negative: one-pass chunks
 SWEEP2_TILE_CANDIDATES = {True: (1, 2), False: (1, 2, 3, 4)}  # by "is FP64"
 SWEEP2_ICH_CANDIDATES = (0, 2, 4, 8, 16, 32, 64)
 PROJ_ICH_CANDIDATES = (0, 32, 64, 86, 128, 172, 256)
