@@ -165,6 +165,74 @@ prolong_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev 
   for_planes(f.n0, ich, [&](Blk B) { prolong_body(phic, c, phif, f, B); });
 }
 
+// The same prolongation by 2 x 2 x 2 fine blocks (fine levels of even
+// extents, each twice the coarse one): a thread owns the fine cells
+// (2a+ei, 2b+ej, 2c+ek) of coarse cell (a, b, c), whose eight interpolations
+// read the 27 coarse cells a-1..a+1 x b-1..b+1 x c-1..c+1 (loaded once
+// instead of 64 times), and the weights of each fine cell are constants.
+// Every fine value is prolong_body's: the same products, summed in the same
+// order, each term one FMA.
+template <class T>
+__device__ inline void prolong8_body(const T* __restrict__ phic, const MgDev& c,
+                                     T* __restrict__ phif, const MgDev& f, Blk B) {
+  const int ck = B.x * 64 + threadIdx.x;
+  const int cj = B.y * 4 + threadIdx.y;
+  const int ci = B.z;
+  if (ck >= c.n2 || cj >= c.n1 || ci >= c.n0) return;
+  // coarse real cell (ci-1+di, cj-1+dj, ck-1+dk) sits at array index
+  // (ci+di, cj+dj, ck+dk): the ring at -1
+  T cc[3][3][3];
+#pragma unroll
+  for (int di = 0; di < 3; ++di)
+#pragma unroll
+    for (int dj = 0; dj < 3; ++dj)
+#pragma unroll
+      for (int dk = 0; dk < 3; ++dk)
+        cc[di][dj][dk] = phic[(int64_t)(ci + di) * c.s0 + (int64_t)(cj + dj) * c.s1 + (ck + dk)];
+#pragma unroll
+  for (int ei = 0; ei < 2; ++ei) {
+    const int i = 2 * ci + ei;
+    const T wx0 = ei == 0 ? T(0.25) : T(0.75);
+#pragma unroll
+    for (int ej = 0; ej < 2; ++ej) {
+      const int j = 2 * cj + ej;
+      const T wy0 = ej == 0 ? T(0.25) : T(0.75);
+#pragma unroll
+      for (int ek = 0; ek < 2; ++ek) {
+        const int k = 2 * ck + ek;
+        const T wz0 = ek == 0 ? T(0.25) : T(0.75);
+        // fine parity e: the first coarse cell of the pair is a-1+e
+        T acc = T(0);
+#pragma unroll
+        for (int di = 0; di < 2; ++di) {
+          const T wxi = di == 0 ? wx0 : T(1) - wx0;
+#pragma unroll
+          for (int dj = 0; dj < 2; ++dj) {
+            const T wyj = dj == 0 ? wy0 : T(1) - wy0;
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+              const T wzk = dk == 0 ? wz0 : T(1) - wz0;
+              // explicit FMA: prolong_body's terms contract to this, and a
+              // product shared by several fine cells must not be rounded
+              // once and reused instead
+              acc = fma(wxi * wyj * wzk, cc[ei + di][ej + dj][ek + dk], acc);
+            }
+          }
+        }
+        const int64_t x = mg_at(f, i, j, k);
+        const T v = phif[x] + acc;
+        phif[x] = v;
+        ring_images(phif, f, i, j, k, x, v);
+      }
+    }
+  }
+}
+template <class T>
+__global__ void __launch_bounds__(256)
+prolong8_kernel(const T* __restrict__ phic, MgDev c, T* __restrict__ phif, MgDev f, int ich) {
+  for_planes(c.n0, ich, [&](Blk B) { prolong8_body(phic, c, phif, f, B); });
+}
+
 // Buffer access to a level array (buf_rsrc, buf_ld, buf_st in wf_common.h):
 // the base, chosen per workgroup, sits in a scalar buffer descriptor, the
 // plane offset in the scalar soffset operand, and each lane supplies a 32-bit
@@ -1355,14 +1423,17 @@ coarse_op_apply_rows_kernel(const T* __restrict__ M, T* __restrict__ phi,
   __syncthreads();
   const int row = blockIdx.x * 64 + tid;
   if (row >= ncell) return;
+  // CB columns' loads in flight before their products: the few workgroups
+  // of a coarse level wait for one memory round trip per CB columns
+  constexpr int CB = sizeof(T) == 4 ? 64 : 32;
   T acc = T(0);
   int col = 0;
-  for (; col + 16 <= ncell; col += 16) {
-    T m[16];
+  for (; col + CB <= ncell; col += CB) {
+    T m[CB];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) m[q] = M[(size_t)(col + q) * ncell + row];
+    for (int q = 0; q < CB; ++q) m[q] = M[(size_t)(col + q) * ncell + row];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc = cop_fma(m[q], r[col + q], acc);
+    for (int q = 0; q < CB; ++q) acc = cop_fma(m[q], r[col + q], acc);
   }
   for (; col < ncell; ++col) acc = cop_fma(M[(size_t)col * ncell + row], r[col], acc);
   const int k = row % n2, j = (row / n2) % n1, i = row / (n1 * n2);
@@ -1958,6 +2029,14 @@ int prolong_t(const wf_mg_level& C, const wf_mg_level& F, hipStream_t s) {
   // hierarchy, 8 planes at the 256^3 fine level (as plane_chunk), 4 instead
   // of 1 at 128^3 (8.9 vs 13.1 us), 1 below; each cell's value does not
   // depend on the chunking
+  if (f.n0 == 2 * c.n0 && f.n1 == 2 * c.n1 && f.n2 == 2 * c.n2) {  // 2 x 2 x 2 blocks
+    const int64_t tiles = (int64_t)div_up(c.n2, BX) * div_up(c.n1, BY);
+    const int ich = (int)std::min<int64_t>(8, std::max<int64_t>(1, tiles * c.n0 / 2048));
+    hipLaunchKernelGGL(prolong8_kernel<T>, chunk_grid(c.n0, c.n1, c.n2, ich), dim3(64, 4), 0,
+                       s, (const T*)C.phi, c, (T*)F.phi, f, ich);
+    WF_LAUNCHED();
+    return 0;
+  }
   const int64_t tiles = (int64_t)div_up(f.n2, BX) * div_up(f.n1, BY);
   const int ich = (int)std::min<int64_t>(8, std::max<int64_t>(1, tiles * f.n0 / 2048));
   hipLaunchKernelGGL(prolong_kernel<T>, chunk_grid(f.n0, f.n1, f.n2, ich), dim3(64, 4), 0, s,
