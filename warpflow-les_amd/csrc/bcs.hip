@@ -149,13 +149,57 @@ __global__ void __launch_bounds__(256) bc_axis3_kernel(Bc3 A, wf_grid g, int axi
 // written, as in the reference's side order (bcs.py:201-311); the x ops,
 // which read other planes, run in the launch before.  Each cell sees the
 // op-by-op order of the three axis launches, so the bits are theirs.
+// The ghost cells (k indices) a field's z ops write, when there are at most
+// ZLANES of them and no op reads a cell the list writes (n = 0 otherwise),
+// each with the op that writes it last: ghost g[c] = val[c] * (cell src[c])
+// or, for src[c] < 0, val[c].
+constexpr int ZLANES = 8;
+struct BcCells {
+  int32_t g[ZLANES], src[ZLANES];
+  double val[ZLANES];
+  int32_t n;
+};
 struct BcYZ {
   double* f[3];
   int32_t ay0[3], nay[3], by0[3], nby[3];  // y ops: planes [ay0, ay0+nay) x k lines
   int32_t az0[3], naz[3], bz0[3], nbz[3];  // z ops: planes [az0, az0+naz) x j lines
   BcOps oy[3], oz[3];
+  BcCells zc[3];
   int32_t ibase;
 };
+
+// The z ops of ZLANES-lane groups, a lane per ghost cell of an (i, j) line:
+// the lane stores the value of the last op that writes its cell, which is
+// what the cell holds after the op-by-op walk (no op reads a cell the list
+// writes, so the earlier writes are never read).  A wave then touches the
+// cache lines of 64 / ZLANES lines per access, where a thread per line
+// touches 64 (the ghosts and sources of a line share its first and last
+// lines).  A thread issues the loads of ZU groups before their stores.
+__device__ inline void bc_zcells(double* __restrict__ f, const wf_grid& g, int i, int b0, int nb,
+                                 const BcCells& zc) {
+  constexpr int ZU = 4;
+  const int total = nb * ZLANES, nt = blockDim.x;
+  for (int t0 = threadIdx.x; t0 < total; t0 += ZU * nt) {
+    double v[ZU];
+    int64_t dst[ZU];
+    bool on[ZU];
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      const int t = t0 + u * nt, c = t % ZLANES;
+      on[u] = t < total && c < zc.n;
+      const int cc = on[u] ? c : 0;
+      const int64_t base = at(g, i, b0 + (on[u] ? t / ZLANES : 0), 0);
+      dst[u] = base + zc.g[cc];
+      const int sc = zc.src[cc];
+      // unconditional load (cell 0 of the line for a constant ghost)
+      const double x = f[base + (sc >= 0 ? sc : 0)];
+      v[u] = sc >= 0 ? zc.val[cc] * x : zc.val[cc];
+    }
+#pragma unroll
+    for (int u = 0; u < ZU; ++u)
+      if (on[u]) f[dst[u]] = v[u];
+  }
+}
 
 __global__ void __launch_bounds__(1024) bc_yz_kernel(BcYZ A, wf_grid g) {
   const int z = blockIdx.y;
@@ -164,9 +208,13 @@ __global__ void __launch_bounds__(1024) bc_yz_kernel(BcYZ A, wf_grid g) {
     for (int b = threadIdx.x; b < A.nby[z]; b += blockDim.x)
       bc_line(A.f[z], g, 1, i, A.by0[z] + b, A.oy[z]);
   __syncthreads();
-  if (i >= A.az0[z] && i < A.az0[z] + A.naz[z])
-    for (int b = threadIdx.x; b < A.nbz[z]; b += blockDim.x)
-      bc_line(A.f[z], g, 2, i, A.bz0[z] + b, A.oz[z]);
+  if (i >= A.az0[z] && i < A.az0[z] + A.naz[z]) {
+    if (A.zc[z].n > 0)
+      bc_zcells(A.f[z], g, i, A.bz0[z], A.nbz[z], A.zc[z]);
+    else
+      for (int b = threadIdx.x; b < A.nbz[z]; b += blockDim.x)
+        bc_line(A.f[z], g, 2, i, A.bz0[z] + b, A.oz[z]);
+  }
 }
 
 }  // namespace wf
@@ -381,6 +429,41 @@ int wf_bc_const(double* fld, wf_grid g, int axis, int high, int g0, double val, 
 // The y-then-z launch of fills `which[z]` of fields f[z] (z < nf): ranges and
 // ops of the y and z axis launches, checked against the array, one
 // workgroup per plane any of them touches.
+// The distinct ghost cells of a z op list, for bc_zcells (n = 0: the list
+// is not hoistable, some op reads a cell some op writes -- the lanes of a
+// line load and store concurrently -- or it writes more than ZLANES cells).
+static BcCells z_cells(const BcOps& ops) {
+  BcCells c{};
+  if (!bc_ops_hoistable(ops)) return c;
+  for (int q = 0; q < ops.n; ++q) {
+    const BcOp& o = ops.op[q];
+    if (o.kind == 0) continue;
+    for (int m = 0; m < o.nlay; ++m) {
+      const int si = o.high ? o.s0 - m : o.s0 + m;
+      for (int p = 0; p < ops.n; ++p)
+        for (int u = 0; u < ops.op[p].nlay; ++u)
+          if ((ops.op[p].high ? ops.op[p].g0 + u : ops.op[p].g0 - u) == si) return c;
+    }
+  }
+  int n = 0;
+  for (int q = 0; q < ops.n; ++q)
+    for (int m = 0; m < ops.op[q].nlay; ++m) {
+      const BcOp& o = ops.op[q];
+      const int gi = o.high ? o.g0 + m : o.g0 - m;
+      int u = 0;
+      while (u < n && c.g[u] != gi) ++u;
+      if (u == n) {
+        if (n == ZLANES) return BcCells{};
+        c.g[n++] = gi;
+      }
+      // a later op overwrites an earlier one's entry: the last writer stays
+      c.src[u] = o.kind != 0 ? (o.high ? o.s0 - m : o.s0 + m) : -1;
+      c.val[u] = o.val;
+    }
+  c.n = n;
+  return c;
+}
+
 static int launch_yz(double* const* f, const int* which, const double* param, int nf,
                      const wf_grid& g, hipStream_t s) {
   BcYZ A{};
@@ -402,6 +485,7 @@ static int launch_yz(double* const* f, const int* which, const double* param, in
         !ops_ok(g, 2, a0, na, b0, nb, A.oz[z]))
       return WF_EINVAL;
     A.az0[z] = a0, A.naz[z] = na, A.bz0[z] = b0, A.nbz[z] = nb;
+    A.zc[z] = z_cells(A.oz[z]);
     if (na > 0 && nb > 0 && A.oz[z].n > 0) {
       lo = std::min(lo, a0), hi = std::max(hi, a0 + na);
       lines = std::max(lines, nb);
