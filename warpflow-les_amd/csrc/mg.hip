@@ -768,6 +768,10 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
   __shared__ T OA[4][Z3];  // first sweep, odd cells, halo 3
   __shared__ T EA[4][Z2];  // first sweep, even cells, halo 2
   __shared__ T OB[4][Z1];  // second sweep, odd cells, halo 1
+  // rhs of the first sweep's stages, kept for the second sweep's (planes x
+  // in slot (x - i0 + 3) % 5: A-odd plane x at iteration x, B-odd at x + 4)
+  __shared__ T RAO[5][Z3];  // odd cells, halo 3
+  __shared__ T RAE[5][Z2];  // even cells, halo 2
   const int tid = threadIdx.x;
   const int j0 = B.y * TJ, k0 = B.x * TK;
   const int i0 = B.z * ich, i1 = min(i0 + ich, d.n0);
@@ -828,6 +832,21 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
   WF_STAGE_TABLES(bo, N1, 1, W1, C1, W2, true)
   WF_STAGE_TABLES(be, NPAIR, 0, W0, TJ * W0, W1, false)
 #undef WF_STAGE_TABLES
+  // LDS index, in the first sweep's layouts (odd halo 3, even halo 2), of
+  // the second sweep's cells (odd halo 1, even halo 0): row + 2, column + 1
+  // (the colour phase of a row is the same two rows on)
+  int bo_rix[N1], be_rix[NPAIR];
+#pragma unroll
+  for (int e = 0; e < N1; ++e) {
+    const int l = min(tid + 256 * e, C1 - 1);
+    bo_rix[e] = (l / W1 + 2) * W3 + l % W1 + 1;
+  }
+#pragma unroll
+  for (int e = 0; e < NPAIR; ++e) {
+    const int l = tid + 256 * e;
+    be_rix[e] = (l / W0 + 2) * W2 + l % W0 + 1;
+  }
+  auto rslot = [&](int x) { return (x - i0 + 3) % 5; };
   // output: the even cell's offset is be_ro; the odd partner's offset and
   // OB column, and face flags of the pair's cells a = k_a, b = k_a + 1
   constexpr int FL_JLO = 1, FL_JHI = 2, FL_ALO = 4, FL_AHI = 8, FL_BHI = 16, FL_AL = 32,
@@ -883,20 +902,13 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
   // Prefetch: pin two planes ahead (pre[c]: the next plane of class c); the
   // first sweep's rhs two planes ahead (4 sets), the second's one plane
   // ahead (2 sets; those planes were read a few planes earlier).
-  T pre[2][N4], rao[4][N3], rae[4][N2], rbo[2][N1], rbe[2][NPAIR];
+  T pre[2][N4], rao[4][N3], rae[4][N2];
 #pragma unroll
   for (int x = 0; x < 4; ++x) {
 #pragma unroll
     for (int e = 0; e < N3; ++e) rao[x][e] = T(0);
 #pragma unroll
     for (int e = 0; e < N2; ++e) rae[x][e] = T(0);
-  }
-#pragma unroll
-  for (int x = 0; x < 2; ++x) {
-#pragma unroll
-    for (int e = 0; e < N1; ++e) rbo[x][e] = T(0);
-#pragma unroll
-    for (int e = 0; e < NPAIR; ++e) rbe[x][e] = T(0);
   }
   const int pin_hi = min(i1 + 3, n0);  // last pin plane needed (n0: ring, zero)
   auto load_pin = [&](auto Rc, int x) {
@@ -923,6 +935,15 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
                     c2 * s0[a] + c2 * s0[a + 1];
       const int l = tid + 256 * e;
       dst[(e + 1) * 256 <= C ? l : (l < C ? l : C)] = (rv[e] - sum) * rp[e];
+    }
+  };
+  // a first-sweep stage's rhs into its ring slot (dead lanes: spare word)
+  auto keep_rhs = [&](auto Nc, auto Cc, const T* rv, T* dst) {
+    constexpr int N = decltype(Nc)::value, C = decltype(Cc)::value;
+#pragma unroll
+    for (int e = 0; e < N; ++e) {
+      const int l = tid + 256 * e;
+      dst[(e + 1) * 256 <= C ? l : (l < C ? l : C)] = rv[e];
     }
   };
   // 1/ap of a stage's cells on an i-face (or ring) plane x
@@ -963,29 +984,35 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
   // stage of the same iteration writes; the restrict qualifiers say so, which
   // lets the compiler issue a later stage's reads before an earlier stage's
   // writes.
-  auto steady_stages = [&](auto Rc, auto XAc, auto XBc, const T* __restrict__ pm,
+  auto steady_stages = [&](auto Rc, auto XAc, const T* __restrict__ pm,
                            const T* __restrict__ p0, const T* __restrict__ pp, T* __restrict__ oa,
                            const T* __restrict__ oam, const T* __restrict__ oa0,
                            const T* __restrict__ oap, T* __restrict__ ea,
                            const T* __restrict__ eam, const T* __restrict__ ea0,
                            const T* __restrict__ eap, T* __restrict__ ob,
                            const T* __restrict__ obm, const T* __restrict__ ob0,
-                           const T* __restrict__ obp, int so) {
-    constexpr int r = decltype(Rc)::value, XA = decltype(XAc)::value, XB = decltype(XBc)::value;
+                           const T* __restrict__ obp, T* __restrict__ kao,
+                           T* __restrict__ kae, const T* __restrict__ rbo,
+                           const T* __restrict__ rbe, int so) {
+    constexpr int r = decltype(Rc)::value, XA = decltype(XAc)::value;
     relax(std::integral_constant<int, N3>{}, std::integral_constant<int, W4>{},
           std::integral_constant<int, C3>{}, pm, p0, pp, ao_a0, ao_aj[r], rao[XA], ao_rp[r], oa);
+    keep_rhs(std::integral_constant<int, N3>{}, std::integral_constant<int, C3>{}, rao[XA], kao);
     relax(std::integral_constant<int, N2>{}, std::integral_constant<int, W3>{},
           std::integral_constant<int, C2>{}, oam, oa0, oap, ae_a0, ae_aj[r], rae[XA], ae_rp[r],
           ea);
+    keep_rhs(std::integral_constant<int, N2>{}, std::integral_constant<int, C2>{}, rae[XA], kae);
+    T rv[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) rv[e] = rbo[bo_rix[e]];
     relax(std::integral_constant<int, N1>{}, std::integral_constant<int, W2>{},
-          std::integral_constant<int, C1>{}, eam, ea0, eap, bo_a0, bo_aj[r], rbo[XB], bo_rp[r],
-          ob);
+          std::integral_constant<int, C1>{}, eam, ea0, eap, bo_a0, bo_aj[r], rv, bo_rp[r], ob);
 #pragma unroll
     for (int e = 0; e < NPAIR; ++e) {
       const int a = be_a0[e], c = be_aj[r][e];
       const T sum = c0 * obm[c] + c0 * obp[c] + c1 * ob0[c - W1] + c1 * ob0[c + W1] +
                     c2 * ob0[a] + c2 * ob0[a + 1];
-      const T ve = (rbe[XB][e] - sum) * be_rp[r][e];
+      const T ve = (rbe[be_rix[e]] - sum) * be_rp[r][e];
       const T vo = ob0[pcol[r][e]];
       const uint32_t oe = be_ro[r][e], oo = pofs[r][e];
       buf_st(dout, oe, so, ve);
@@ -1009,19 +1036,8 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
     constexpr int Sm1 = (U + 3) & 3, S0 = U, Sp1 = (U + 1) & 3, Sm2 = (U + 2) & 3;
     using Cr = std::integral_constant<int, r>;
     using Crn = std::integral_constant<int, rn>;
-    // Loads, oldest first: the B stages' rhs (used next iteration), pin (two
-    // iterations on), the A stages' rhs (two iterations on).  The memory
-    // counter completes in order, so a wait for this iteration's B rhs next
-    // iteration leaves every later load in flight.
-    {  // rhs of the B stages of iteration t+1 (class rn)
-      constexpr int XB = (U + 1) & 1;
-      const int sbo = ST ? (q - 3 - xb) * pb : soff(q - 3, i0 - 1, i1);
-      const int sbe = ST ? (q - 5 - xb) * pb : soff(q - 5, i0, i1 - 1);
-#pragma unroll
-      for (int e = 0; e < N1; ++e) rbo[XB][e] = buf_ld<T>(drhs, bo_ro[rn][e], sbo);
-#pragma unroll
-      for (int e = 0; e < NPAIR; ++e) rbe[XB][e] = buf_ld<T>(drhs, be_ro[rn][e], sbe);
-    }
+    // Loads, two iterations ahead: pin and the A stages' rhs (the B stages
+    // read theirs from the ring the A stages fill).
     if constexpr (ST) {
 #pragma unroll
       for (int e = 0; e < N4; ++e) P[Sp1][tid + 256 * e < C4 ? tid + 256 * e : C4] = pre[rn][e];
@@ -1042,12 +1058,12 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
       for (int e = 0; e < N2; ++e) rae[XA][e] = buf_ld<T>(drhs, ae_ro[r][e], sae);
     }
     __syncthreads();
-    constexpr int XA = U, XB = U & 1;
+    constexpr int XA = U;
     if constexpr (ST) {
-      steady_stages(Cr{}, std::integral_constant<int, XA>{}, std::integral_constant<int, XB>{},
-                    P[Sm1], P[S0], P[Sp1], OA[S0], OA[(U + 1) & 3], OA[Sm2], OA[Sm1], EA[Sm2],
-                    EA[Sm1], EA[S0], EA[Sp1], OB[S0], OB[Sp1], OB[Sm2], OB[Sm1],
-                    (q - 6 - xb) * pb);
+      steady_stages(Cr{}, std::integral_constant<int, XA>{}, P[Sm1], P[S0], P[Sp1], OA[S0],
+                    OA[(U + 1) & 3], OA[Sm2], OA[Sm1], EA[Sm2], EA[Sm1], EA[S0], EA[Sp1],
+                    OB[S0], OB[Sp1], OB[Sm2], OB[Sm1], RAO[rslot(q)], RAE[rslot(q - 2)],
+                    RAO[rslot(q - 4)], RAE[rslot(q - 6)], (q - 6 - xb) * pb);
       return;
     }
     // Each stage runs only for the planes a later stage or the output needs
@@ -1062,6 +1078,8 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
       relax(std::integral_constant<int, N3>{}, std::integral_constant<int, W4>{},
             std::integral_constant<int, C3>{}, P[Sm1], P[S0], P[Sp1], ao_a0, ao_aj[r],
             rao[XA], rp, OA[S0]);
+      keep_rhs(std::integral_constant<int, N3>{}, std::integral_constant<int, C3>{}, rao[XA],
+               RAO[rslot(q)]);
     }
     if (q - 2 >= i0 - 2 && q - 2 <= i1 + 1) {  // A-even at q-2: OA planes q-3 .. q-1 -> EA[q-2]
       const int x = q - 2;
@@ -1072,6 +1090,8 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
       relax(std::integral_constant<int, N2>{}, std::integral_constant<int, W3>{},
             std::integral_constant<int, C2>{}, OA[(U + 1) & 3], OA[Sm2], OA[Sm1], ae_a0,
             ae_aj[r], rae[XA], rp, EA[Sm2]);
+      keep_rhs(std::integral_constant<int, N2>{}, std::integral_constant<int, C2>{}, rae[XA],
+               RAE[rslot(x)]);
     }
     if (q - 4 >= i0 - 1 && q - 4 <= i1) {  // B-odd at q-4: EA planes q-5 .. q-3 -> OB[q-4]
       const int x = q - 4;
@@ -1079,9 +1099,12 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
 #pragma unroll
       for (int e = 0; e < N1; ++e) rp[e] = bo_rp[r][e];
       if (x <= 0 || x >= n0 - 1) face_rp(std::integral_constant<int, N1>{}, x, r, 1, W1, C1, true, rp);
+      T rv[N1];
+#pragma unroll
+      for (int e = 0; e < N1; ++e) rv[e] = RAO[rslot(x)][bo_rix[e]];
       relax(std::integral_constant<int, N1>{}, std::integral_constant<int, W2>{},
-            std::integral_constant<int, C1>{}, EA[Sm1], EA[S0], EA[Sp1], bo_a0, bo_aj[r],
-            rbo[XB], rp, OB[S0]);
+            std::integral_constant<int, C1>{}, EA[Sm1], EA[S0], EA[Sp1], bo_a0, bo_aj[r], rv,
+            rp, OB[S0]);
     }
     if (q - 6 >= i0 && q - 6 < i1) {  // B-even at q-6: OB planes q-7 .. q-5 -> pout plane q-6
       const int x = q - 6;
@@ -1099,7 +1122,7 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
         const int a = be_a0[e], c = be_aj[r][e];
         const T sum = c0 * om[c] + c0 * op[c] + c1 * o0[c - W1] + c1 * o0[c + W1] +
                       c2 * o0[a] + c2 * o0[a + 1];
-        const T ve = (rbe[XB][e] - sum) * rp[e];
+        const T ve = (RAE[rslot(x)][be_rix[e]] - sum) * rp[e];
         const T vo = o0[pcol[r][e]];
         const uint32_t oe = be_ro[r][e], oo = pofs[r][e];
         buf_st(dst, oe, so, ve);
@@ -1165,7 +1188,7 @@ __device__ inline void sweep2_body(const T* __restrict__ pin, T* __restrict__ po
 // stage tables and are held to 256 VGPRs (two workgroups per CU).
 template <class T, int TJ, int TK>
 constexpr int sweep2_min_blocks() {
-  return sizeof(T) == 4 ? (TJ * TK == 512 ? 4 : 2) : 1;
+  return sizeof(T) == 4 ? (TJ * TK == 512 ? 3 : 2) : 1;
 }
 template <class T, int TJ, int TK>
 __global__ void __launch_bounds__(256, (sweep2_min_blocks<T, TJ, TK>()))
