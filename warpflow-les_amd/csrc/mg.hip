@@ -1619,6 +1619,17 @@ batch_prolong_kernel(BView V, Plan P, int lf, const float* phic, float* phif,
                         cells_blk(q, f.n1, f.n2));
   });
 }
+// the same by 2 x 2 x 2 fine blocks (every member's level lf twice level
+// lf+1), items over the coarse cells
+__global__ void __launch_bounds__(256)
+batch_prolong8_kernel(BView V, Plan P, int lf, const float* phic, float* phif,
+                      const int* active) {
+  mg_items<false>(P, V, active, [&](int q, const MemDev& M) {
+    const MgDev c = level_dev(M, lf + 1), f = level_dev(M, lf);
+    prolong8_body<float>(phic + M.loff[lf + 1], c, phif + M.loff[lf], f,
+                         cells_blk(q, c.n1, c.n2));
+  });
+}
 
 // one fused full sweep pout = S(pin) on level l, tile shape per member
 __global__ void __launch_bounds__(256)
@@ -1734,9 +1745,19 @@ int batch_vcycle(const Batch& B, float* const* phi, float* const* rhs, float* co
     if (r) return r;
   }
   for (int l = nlev - 2; l >= 0; --l) {
-    const Plan& P = B.plan[PK_LCELLS0 + l];
-    hipLaunchKernelGGL(batch_prolong_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, l,
-                       (const float*)phi[l + 1], phi[l], active);
+    bool halves = true;  // every member's level l twice its level l+1
+    for (int m = 0; m < B.nmem; ++m)
+      for (int a = 0; a < 3; ++a)
+        halves = halves && B.mem_host[m].ln[l][a] == 2 * B.mem_host[m].ln[l + 1][a];
+    if (halves) {
+      const Plan& P = B.plan[PK_LCELLS0 + l + 1];
+      hipLaunchKernelGGL(batch_prolong8_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, l,
+                         (const float*)phi[l + 1], phi[l], active);
+    } else {
+      const Plan& P = B.plan[PK_LCELLS0 + l];
+      hipLaunchKernelGGL(batch_prolong_kernel, dim3(P.grid), dim3(64, 4), 0, s, V, P, l,
+                         (const float*)phi[l + 1], phi[l], active);
+    }
     WF_LAUNCHED();
     const int r = relax(l, npost);
     if (r) return r;
