@@ -42,7 +42,9 @@ CDVW_ICH_CANDIDATES = (0, 16, 32, 43, 64, 86, 128)
 SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
 SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32, -2, -4)  # negative: one-pass chunks
 SWEEP2_TILE_CANDIDATES = {True: (1, 2), False: (1, 2, 3, 4)}  # by "is FP64"
-SWEEP2_ICH_CANDIDATES = (0, 2, 4, 8, 16, 32, 64)
+# 43: six chunks of 256 planes, one round of three workgroups per CU for the
+# FP32 tiles of one pair per thread at 256^3
+SWEEP2_ICH_CANDIDATES = (0, 2, 4, 8, 16, 32, 43, 64)
 PROJ_ICH_CANDIDATES = (0, 32, 64, 86, 128, 172, 256)
 SWEEP_MIN_CELLS = 4096  # smaller levels relax in one LDS-resident workgroup
 
