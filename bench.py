@@ -297,7 +297,7 @@ def hot_kernels(a, s64, smx):
                        lambda L0=L0, L1=L1: _hip.call("wf_mg_restrict", C.byref(L0),
                                                       C.byref(L1), st),
                        (17 * eb / 8) * cells, cyc64, cycmx))
-        out.append(row("prolong L0 (%s)" % tag, "prolong_kernel<%s>" % ctype,
+        out.append(row("prolong L0 (%s)" % tag, "prolong8_kernel<%s>" % ctype,
                        lambda L0=L0, L1=L1: _hip.call("wf_mg_prolong", C.byref(L1), C.byref(L0),
                                                       st),
                        (17 * eb / 8) * cells, cyc64, cycmx))
@@ -434,10 +434,10 @@ ROW_KERNELS = {
     "bc_fill (u)": ("bc_axis_kernel(",),
     "rbgs 12 sweeps L0 (fp64)": ("sweep2_kernel<double", "sweep_kernel<double"),
     "restrict L0 (fp64)": ("restrict_kernel<double>",),
-    "prolong L0 (fp64)": ("prolong_kernel<double>",),
+    "prolong L0 (fp64)": ("prolong8_kernel<double>", "prolong_kernel<double>"),
     "rbgs 12 sweeps L0 (fp32)": ("sweep2_kernel<float", "sweep_kernel<float"),
     "restrict L0 (fp32)": ("restrict_kernel<float>",),
-    "prolong L0 (fp32)": ("prolong_kernel<float>",),
+    "prolong L0 (fp32)": ("prolong8_kernel<float>", "prolong_kernel<float>"),
 }
 
 
