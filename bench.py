@@ -378,8 +378,9 @@ def kernel_table(a, s64, smx):
         cold = T.cold_us(fn, a.reps, flush)
         warm = T.warm_us(fn, a.reps)
         # the time the ranking and roofline use: in-step where measured, else
-        # the cold call (a cycle's level-0 smoothing started cold matches its
-        # in-step duration)
+        # the cold call (a cycle's level-0 smoothing started cold takes a few
+        # per cent longer than inside a step, so the cold figure is the lower
+        # bound on the rate)
         t = cold
         if r["name"] in instep:
             per_launch, n = instep[r["name"]]
@@ -941,7 +942,7 @@ def main(argv=None):
                             "launches_per_step": dom["launches"] * dom[
                                 "per_step_mixed" if head == "mixed" else "per_step_fp64"],
                             "us_per_step": dom[key]}
-        nxt = ranked[1]  # the two largest rows take about the same time per step
+        nxt = ranked[1]
         line["roofline"]["runner_up"] = {
             "row": nxt["name"], "kernel": nxt["kernel"], "us_per_step": nxt[key],
             "frac": nxt["hbm_frac"], "traffic": nxt.get("traffic_per_launch")}
