@@ -236,20 +236,23 @@ def hot_kernels(a, s64, smx):
     out = [
         # the momentum launches of one step: the first stage's three-component
         # pass (reads u, v, w, vis; writes ustar, vstar, wstar, uoo, voo, woo),
-        # then the second stage's u pass (reads u, v, w, uoo, vis; writes
-        # ustar) and v/w pass (reads u, v, w, voo, woo, vis; writes vstar, wstar)
+        # then the second stage's pass of u's planes 0 and ni-2 (two launches;
+        # reads u, v, w, uoo, vis, writes ustar on those planes) and its
+        # three-component pass (reads u, v, w, uoo, voo, woo, vis; writes
+        # ustar, vstar, wstar)
         row("convdiff3 stage 1 (u, v, w)", "convdiff3_save_kernel (conv4th + diff4th of u, v, w)",
             lambda: K.convdiff3_save(n3, f["u"], f["v"], f["w"], f["vis"], f["ustar"], f["vstar"],
                                      f["wstar"], f["uoo"], f["voo"], f["woo"], *h, dt, 0.5, NPL),
             80 * cells, 1, 1),
-        row("convdiff u stage 2", "convdiff_kernel<0, false> (conv4th + diff4th of u)",
-            lambda: K.convdiff4th_u(opd["u"], f["u"], f["v"], f["w"], f["uoo"], f["vis"],
-                                    f["ustar"], *h, dt, 1.0, NPL),
-            48 * np.prod(opd["u"]), 1, 1),
-        row("convdiff_vw stage 2 (v, w)", "convdiff_vw_kernel (conv4th + diff4th of v and w)",
-            lambda: K.convdiff_vw(n3, f["u"], f["v"], f["w"], f["voo"], f["woo"], f["vis"],
-                                  f["w_pre"], f["vstar"], f["wstar"], *h, dt, 1.0, NPL),
-            64 * cells, 1, 1),
+        row("convdiff u edges stage 2", "convdiff_kernel<0, false> (u's planes 0 and ni-2)",
+            lambda: K.convdiff4th_u_edges(n3, f["u"], f["v"], f["w"], f["uoo"], f["vis"],
+                                          f["ustar"], *h, dt, 1.0, NPL),
+            48 * 2 * opd["u"][1] * opd["u"][2], 1, 1, 2),
+        row("convdiff3 stage 2 (u, v, w)", "convdiff3_kernel (conv4th + diff4th of u, v, w)",
+            lambda: K.convdiff3(n3, f["u"], f["v"], f["w"], f["uoo"], f["voo"], f["woo"],
+                                f["vis"], f["w_pre"], f["ustar"], f["vstar"], f["wstar"], *h,
+                                dt, 1.0, NPL),
+            80 * cells, 1, 1),
         row("calvel3", "calvel3_kernel",
             lambda: K.calvel3(s64.ncells, f["ustar"], f["vstar"], f["wstar"], f["p"], f["u"],
                               f["v"], f["w"], *h, dt, 1.0, NPL), 56 * cells, 1, 1),
@@ -316,14 +319,14 @@ def load_traffic():
 
 # table row of each momentum entry point the step calls
 INSTEP_ROWS = {"convdiff3_save": "convdiff3 stage 1 (u, v, w)",
-               "convdiff4th_u": "convdiff u stage 2",
-               "convdiff_vw": "convdiff_vw stage 2 (v, w)"}
+               "convdiff4th_u_edges": "convdiff u edges stage 2",
+               "convdiff3": "convdiff3 stage 2 (u, v, w)"}
 
 
 def instep_momentum_us(sol, nsteps=2):
-    """Average duration per launch of each momentum (conv4th + diff4th)
+    """Average duration per call of each momentum (conv4th + diff4th)
     entry point inside `nsteps` real steps of `sol`, keyed by table row:
-    each launch the step makes is bracketed by HIP events on the stream it
+    each call the step makes is bracketed by HIP events on the stream it
     is issued on (the kernels module's entry points are wrapped for these
     steps only).  Inside a step these kernels find part of their inputs in
     the Infinity Cache, left there by the passes just before them, which
@@ -383,10 +386,10 @@ def kernel_table(a, s64, smx):
         # bound on the rate)
         t = cold
         if r["name"] in instep:
-            per_launch, n = instep[r["name"]]
-            t = per_launch * r["launches"]
-            r["us_instep_per_launch"] = round(per_launch, 2)
-            r["instep_launches_timed"] = n
+            per_call, n = instep[r["name"]]  # one call = the row's launches
+            t = per_call
+            r["us_instep_per_launch"] = round(per_call / r["launches"], 2)
+            r["instep_calls_timed"] = n
         r.update({"us_cold": round(cold, 2), "us_warm": round(warm, 2), "us": round(t, 2),
                   "us_per_launch": round(t / r["launches"], 2),
                   "GBps": round(r["bytes"] / t / 1e3, 1) if r["bytes"] else None,
@@ -424,8 +427,8 @@ def probe(a):
 # it (the level-0 launch is the one with the largest grid).
 ROW_KERNELS = {
     "convdiff3 stage 1 (u, v, w)": ("convdiff3_save_kernel(",),
-    "convdiff u stage 2": ("convdiff_kernel<0, false>",),
-    "convdiff_vw stage 2 (v, w)": ("convdiff_vw_kernel(",),
+    "convdiff u edges stage 2": ("convdiff_kernel<0, false>",),
+    "convdiff3 stage 2 (u, v, w)": ("convdiff3_kernel(",),
     "calvel3": ("calvel3_kernel(",),
     "calvel3+calmas": ("calvel3_calmas_kernel<true>",),
     "calmas of calvel3 (no store)": ("calvel3_calmas_kernel<false>",),

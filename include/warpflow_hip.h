@@ -166,8 +166,31 @@ int wf_convdiff_vw(const double* u, const double* v, const double* w, const doub
                    double* wstar, wf_grid g, double dx, double dy, double dz, double dt,
                    double alpha, void* stream);
 
-/* w_pre <- w at the cells wf_convdiff_vw reads from w_pre (see there); no
- * other cell of w_pre is written.  No reference counterpart. */
+/* The second RK stage's wf_convdiff4th of u, v and w in one pass, each over
+ * its operator box, after the fills of all three (solver.py:254-265 fills and
+ * convects one component at a time).  v's z-term reads w_pre as in
+ * wf_convdiff_vw.  conv_u reads v and w before their fills; of the cells it
+ * reads, those fills change only the x-ghost planes -1 and ni, which u's
+ * planes 0 and ni-2 reach: this pass leaves those two planes of ustar
+ * unwritten, and wf_convdiff4th_u_edges, called after u's fill and before the
+ * fills of v and w, writes them.  The values are those of the single passes
+ * in the reference's order bit for bit.  It reads u, v, w, uoo, voo, woo and
+ * vis once.  No single reference kernel corresponds. */
+int wf_convdiff3(const double* u, const double* v, const double* w, const double* uoo,
+                 const double* voo, const double* woo, const double* vis, const double* w_pre,
+                 double* ustar, double* vstar, double* wstar, wf_grid g, double dx, double dy,
+                 double dz, double dt, double alpha, void* stream);
+
+/* wf_convdiff4th of u on planes 0 and ni-2 of its operator box only (the
+ * planes wf_convdiff3 leaves to it).  No single reference kernel corresponds. */
+int wf_convdiff4th_u_edges(const double* u, const double* v, const double* w,
+                           const double* uoo, const double* vis, double* ustar, wf_grid g,
+                           double dx, double dy, double dz, double dt, double alpha,
+                           void* stream);
+
+/* w_pre <- w at the cells wf_convdiff_vw and wf_convdiff3 read from w_pre (see
+ * wf_convdiff_vw); no other cell of w_pre is written.  No reference
+ * counterpart. */
 int wf_save_cross_ghosts(const double* w, double* w_pre, wf_grid g, void* stream);
 
 /* calvel_u / _v / _w, kernels.py:232-264: c = cstar - dt*alpha*(p[+1]-p)/h
@@ -547,6 +570,7 @@ int wf_batch_set_int(int* p, int n, int value, void* stream);
  *                               count within the reduction's partial slots)
  *   WF_TUNE_CD3_ICH             i-planes per workgroup of wf_convdiff3_save
  *   WF_TUNE_CDVW_ICH            i-planes per workgroup of wf_convdiff_vw
+ *   WF_TUNE_CD3S2_ICH           i-planes per workgroup of wf_convdiff3
  * At most 128 entries.
  * ---------------------------------------------------------------------- */
 enum {
@@ -564,7 +588,8 @@ enum {
   WF_TUNE_SWEEP_PAIRS_F32 = 11,
   WF_TUNE_CD3_ICH = 12,
   WF_TUNE_CDVW_ICH = 13,
-  WF_TUNE_COUNT = 14
+  WF_TUNE_CD3S2_ICH = 14,
+  WF_TUNE_COUNT = 15
 };
 int wf_tune_set(int knob, int n0, int n1, int n2, int value);
 int wf_tune_get(int knob, int n0, int n1, int n2, int* value);

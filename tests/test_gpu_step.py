@@ -105,6 +105,7 @@ def test_second_stage_vw_pass(gpu, precision):
     sequence bit for bit over whole steps."""
     cfg = make_cfg(n3=(32, 32, 32), precision=precision)
     a, b = gpu_solver(cfg), gpu_solver(cfg)
+    a.fused_uvw = b.fused_uvw = False
     b.fused_vw = False
     for _ in range(5):
         ia, ib = a.step(), b.step()
@@ -112,6 +113,36 @@ def test_second_stage_vw_pass(gpu, precision):
         assert ia["dt"] == ib["dt"]
     for nm in ("u", "v", "w", "p", "vis", "ustar", "vstar", "wstar", "uoo", "voo", "woo"):
         assert np.array_equal(a.f[nm].numpy(), b.f[nm].numpy()), nm
+
+
+@pytest.mark.parametrize("n3, levels", [((32, 32, 32), 3), ((40, 20, 44), 3), ((4, 8, 36), 2)])
+def test_second_stage_uvw_pass(gpu, n3, levels):
+    """The second RK stage's u/v/w pass (after the three fills; u's planes 0
+    and ni-2 from the edge pass run after u's fill, v's z-term reading w's
+    saved pre-fill cells) equals the per-component fill/conv sequence and the
+    v/w pass bit for bit over whole steps, also on tiles the grid cuts
+    (20 rows, 44 and 36 columns against 8 x 32 tiles), and with ni = 4, where
+    u's two edge planes are two planes apart."""
+    from warpflow import autotune
+    cfg = make_cfg(n3=n3, levels=levels)
+    a, b, c = gpu_solver(cfg), gpu_solver(cfg), gpu_solver(cfg)
+    b.fused_uvw = c.fused_uvw = False
+    c.fused_vw = False
+    for q in range(4):
+        # a's pass in chunks of 7 planes for two of the steps (tuning changes
+        # no result)
+        autotune.set_shape(autotune.CD3S2_ICH, n3, 7 if q % 2 else 0)
+        try:
+            ia = a.step()
+        finally:
+            autotune.set_shape(autotune.CD3S2_ICH, n3, 0)
+        ib, ic = b.step(), c.step()
+        assert ia["iters"] == ib["iters"] == ic["iters"]
+        assert ia["rmax"] == ib["rmax"] == ic["rmax"] and ia["dt"] == ib["dt"] == ic["dt"]
+    for nm in ("u", "v", "w", "p", "vis", "ustar", "vstar", "wstar", "uoo", "voo", "woo"):
+        ref = c.f[nm].numpy()
+        assert np.array_equal(a.f[nm].numpy(), ref), nm
+        assert np.array_equal(b.f[nm].numpy(), ref), nm
 
 
 def test_step1_rmax_matches_report(gpu):

@@ -712,3 +712,16 @@ def test_second_stage_cross_reads(n):
     assert np.array_equal(_conv("v", {"u": fresh["u"], "v": fresh["v"], "w": w_view}, n, dims),
                           ref_v)
     assert not np.array_equal(_conv("v", fresh, n, dims), ref_v)
+    # ... while of v's and w's fills conv_u reads only the x-ghost planes -1
+    # and ni: with those two planes from before the fills the fresh fields
+    # give conv_u's values, and with none of them only u's planes 0 and ni-2
+    # (the ones the edge pass runs before the fills) differ
+    ref_u = _conv("u", stale_u, n, dims)
+    view = {"u": fresh["u"], "v": fresh["v"].copy(), "w": fresh["w"].copy()}
+    for k in "vw":
+        for I in (-1, ni):
+            view[k][N + I] = f[k][N + I]
+    assert np.array_equal(_conv("u", view, n, dims), ref_u)
+    got = _conv("u", mixed_u, n, dims)
+    diff_planes = sorted(set((np.argwhere(got != ref_u)[:, 0] - N).tolist()))
+    assert diff_planes == [0, ni - 2], diff_planes

@@ -457,16 +457,18 @@ __device__ inline void convdiff_body(const double* __restrict__ u, const double*
   }
 }
 
+// Chunk q of the launch is chunk q * cstep of the box (cstep 1: every chunk;
+// the edge pass of u runs chunks 0 and d0 - 1 of one plane each).
 template <int C, bool SAVE>
 __global__ void __launch_bounds__(256)
 convdiff_kernel(const double* __restrict__ u, const double* __restrict__ v,
                 const double* __restrict__ w, const double* __restrict__ coo,
                 const double* __restrict__ vis, double* __restrict__ cstar, wf_grid g, Box3 b,
-                int ich, int kt, int jt, Coefs K, double* __restrict__ coo_save) {
+                int ich, int kt, int jt, Coefs K, double* __restrict__ coo_save, int cstep) {
   const int nb = kt * jt * gridDim.y;
   const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
   convdiff_body<C, SAVE>(u, v, w, coo, vis, cstar, g, b, ich, L % kt, (L / kt) % jt,
-                         L / (kt * jt), K, coo_save);
+                         L / (kt * jt) * cstep, K, coo_save);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,9 +482,10 @@ convdiff_kernel(const double* __restrict__ u, const double* __restrict__ v,
 // fill/conv sequence gives (solver.py:254-265).  The pass reads u, v, w and
 // vis once and writes ustar, vstar, wstar and the saved bases uoo, voo, woo:
 // 10 fields of traffic per cell instead of the 18 of three separate passes.
-// (At the second stage the cross stencils must see the other components'
-// ghosts as they were before their fills, so that stage keeps one pass per
-// component.)
+// At the second stage the cross stencils must see some of the other
+// components' ghosts as they were before their fills; the same pass serves it
+// (convdiff3_body<true>) with those cells taken from a saved copy or left to a
+// pass that runs before the fills.
 //
 // A TJ x TK tile of threads walks a chunk of i-planes over the union of the
 // three operator boxes; every output expression is convdiff_body's, term for
@@ -546,18 +549,77 @@ __device__ inline bool halo_cell(int h, int& r, int& c) {
 }
 }  // namespace c3
 
-__device__ inline void convdiff3_save_body(
+// Halo cells of the u plane and of w as v's z-term sees it, for the second
+// stage's passes (convdiff3_body<true>, convdiff_vw_body).
+namespace c2 {
+// halo cells of the u plane: rows -1, TJ, TJ+1 on tile columns, columns -1,
+// TK, TK+1 on tile rows
+constexpr int NUH = 3 * TK + 3 * TJ;
+__device__ inline bool u_halo(int h, int& r, int& c) {
+  r = c = 0;
+  if (h < 3 * TK) {
+    r = h < TK ? -1 : TJ + h / TK - 1;
+    c = h % TK;
+  } else if (h < NUH) {
+    h -= 3 * TK;
+    r = h % TJ;
+    const int q = h / TJ;
+    c = q == 0 ? -1 : TK + q - 1;
+  } else {
+    return false;
+  }
+  return true;
+}
+// halo cells of the w plane v's z-term reads (rows -1 .. TJ+1, columns -2 ..
+// TK): rows -1, TJ, TJ+1 on tile columns, columns -2, -1, TK on tile rows,
+// and the nine corner cells
+constexpr int NWH = 3 * TK + 3 * TJ + 9;
+static_assert(NUH <= NT && NWH <= NT, "one halo cell per thread");
+__device__ inline bool w1_halo(int h, int& r, int& c) {
+  r = c = 0;
+  auto row = [](int q) { return q == 0 ? -1 : TJ + q - 1; };  // -1, TJ, TJ+1
+  auto col = [](int q) { return q == 2 ? TK : q - 2; };       // -2, -1, TK
+  if (h < 3 * TK) {
+    r = row(h / TK);
+    c = h % TK;
+  } else if (h < 3 * TK + 3 * TJ) {
+    h -= 3 * TK;
+    r = h % TJ;
+    c = col(h / TJ);
+  } else if (h < NWH) {
+    h -= 3 * TK + 3 * TJ;
+    r = row(h / 3);
+    c = col(h % 3);
+  } else {
+    return false;
+  }
+  return true;
+}
+}  // namespace c2
+
+// S2 = false: the first stage (convdiff3_save_kernel); uoo, voo, woo are
+// written with the components' values.  S2 = true: the second stage
+// (convdiff3_kernel), after the fills of u, v and w; uoo, voo, woo are read as
+// the base values, v's z-term reads w as it was before w's fill (wpre) at the
+// cells outside w's operator box, as convdiff_vw_body does, and u's planes 0
+// and ni-2 are left to the caller's edge pass: their x-interpolants of v and
+// w reach the x-ghost planes -1 and ni, which the fills of v and w change and
+// which conv_u reads before those fills.  u's other planes read no cell either
+// fill changes.
+template <bool S2>
+__device__ inline void convdiff3_body(
     const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
-    const double* __restrict__ vis, double* __restrict__ us, double* __restrict__ vs,
-    double* __restrict__ ws, double* __restrict__ uoo, double* __restrict__ voo,
-    double* __restrict__ woo, const wf_grid& g, int ich, int ktile, int jtile, int chunk,
-    const Coefs& K) {
+    const double* __restrict__ vis, const double* __restrict__ wpre, double* __restrict__ us,
+    double* __restrict__ vs, double* __restrict__ ws, double* __restrict__ uoo,
+    double* __restrict__ voo, double* __restrict__ woo, const wf_grid& g, int ich, int ktile,
+    int jtile, int chunk, const Coefs& K) {
   using namespace c3;
   __shared__ double Us[4][RN];
   __shared__ double Vs[2][RN];
   __shared__ double Ws[2][RN];
   __shared__ double IV[2][IVN];
   __shared__ double IW[2][IWN];
+  __shared__ double W1s[S2 ? 2 : 1][S2 ? RN : 1];  // w as v's z-term reads it (S2)
 
   const int tid = threadIdx.x;
   const int tj = tid / TK, tk = tid % TK;
@@ -616,6 +678,17 @@ __device__ inline void convdiff3_save_body(
                  b_xv = (uint32_t)xv_off * 8u, b_xw = (uint32_t)xw_off * 8u,
                  b_vv = (uint32_t)vis_v * 8u, b_vw = (uint32_t)vis_w * 8u;
   const uint32_t st_v = lv_jk ? b_own : BUF_DEAD, st_w = lw_jk ? b_own : BUF_DEAD;
+  // S2: w as v's z-term reads it, from wpre outside w's operator box in (j, k)
+  auto w_out = [&](int r, int c) {
+    const int J = j0 + r, Kc = k0 + c;
+    return J < 0 || J >= g.nj || Kc < 0 || Kc >= g.nk - 1;
+  };
+  int h1r = 0, h1c = 0;
+  const bool h1 = S2 && c2::w1_halo(tid, h1r, h1c);
+  const int64_t h1_off = G.inplane(h1r, h1c);
+  const int h1_l = rix(h1r, h1c);
+  const double* __restrict__ w1h_src = S2 && h1 && w_out(h1r, h1c) ? wpre : w;
+  const double* __restrict__ w1t_src = S2 && w_out(tj, tk) ? wpre : w;
 
   // ---- prologue ----
   double fu[7], fv[7], fw[7];  // planes i-3 .. i+3 after the shift at the loop top
@@ -647,6 +720,15 @@ __device__ inline void convdiff3_save_body(
   double xvn = v[G.plane(i0 + 2) + xv_off], xwn = w[G.plane(i0 + 2) + xw_off];
   double snu = vis[G.plane(i0) + vis_u], snv = vis[G.plane(i0) + vis_v],
          snw = vis[G.plane(i0) + vis_w];
+  double w1t = 0.0, w1h = 0.0, cun = 0.0, cvn = 0.0, cwn = 0.0;  // S2
+  if constexpr (S2) {
+    const int64_t p0 = G.plane(i0);
+    w1t = w1t_src[p0 + own];
+    w1h = w1h_src[p0 + h1_off];
+    cun = uoo[p0 + own];
+    cvn = voo[p0 + own];
+    cwn = woo[p0 + own];
+  }
   // rolled x-terms: squares of I4x(u) (u), Q = I4x(c) * I4(u) (v, w), at
   // planes i+1, i, i-1 before the shift (f[1..6] = planes i0-3 .. i0+2)
   double ur1, ur0, urm1, vr1, vr0, vrm1, wr1, wr0, wrm1;
@@ -694,6 +776,7 @@ __device__ inline void convdiff3_save_body(
     xvw[3] = xvn;
     xww[3] = xwn;
     const double vis_cu = snu, vis_cv = snv, vis_cw = snw;
+    const double coo_u = cun, coo_v = cvn, coo_w = cwn;  // S2
     double* __restrict__ U0 = Us[rel & 3];        // u, plane i
     double* __restrict__ U1 = Us[(rel + 1) & 3];  // u, plane i+1
     double* __restrict__ V = Vs[buf];
@@ -711,6 +794,12 @@ __device__ inline void convdiff3_save_body(
     if (xv) AV[xv_l] = i4(xvw[3], xvw[2], xvw[1], xvw[0]);
     AW[iwx(tj, tk)] = i4(fw[5], fw[4], fw[3], fw[2]);
     if (xw) AW[xw_l] = i4(xww[3], xww[2], xww[1], xww[0]);
+    double* __restrict__ Wv = W;  // w as v's z-term reads it
+    if constexpr (S2) {
+      Wv = W1s[buf];
+      Wv[ownl] = w1t;
+      if (h1) Wv[h1_l] = w1h;
+    }
     // loads for plane i+1 (clamped past the chunk: a load under a branch
     // would make the compiler wait for every load in flight where the paths
     // join)
@@ -728,10 +817,19 @@ __device__ inline void convdiff3_save_body(
       snu = buf_ld<double>(s1, b_own, 0);
       snv = buf_ld<double>(s1, b_vv, 0);
       snw = buf_ld<double>(s1, b_vw, 0);
+      if constexpr (S2) {
+        const int64_t p1 = G.plane(i + 1);
+        w1t = w1t_src[p1 + own];
+        w1h = w1h_src[p1 + h1_off];
+        cun = buf_ld<double>(prs(uoo, i + 1), b_own, 0);
+        cvn = buf_ld<double>(prs(voo, i + 1), b_own, 0);
+        cwn = buf_ld<double>(prs(woo, i + 1), b_own, 0);
+      }
     }
     __syncthreads();
 
-    const uint32_t st_u = lu_jk && i < g.ni - 1 ? b_own : BUF_DEAD;
+    const uint32_t st_u =
+        lu_jk && i < g.ni - 1 && !(S2 && (i == 0 || i == g.ni - 2)) ? b_own : BUF_DEAD;
     // ---- u (convdiff_body<0, true>) ----
     {
       auto Fo = [&](int dj, int dk) { return U0[rix(tj + dj, tk + dk)]; };
@@ -758,8 +856,9 @@ __device__ inline void convdiff3_save_body(
       const double lap = l4(fu[5], fu[4], c0, fu[2], fu[1], K.ir12x) +
                          l4(uy2, uy1, c0, uym1, uym2, K.ir12y) +
                          l4(uz2, uz1, c0, uzm1, uzm2, K.ir12z);
-      buf_st<double, BUF_NT>(prs(us, i), st_u, 0, rk_update(c0, nsum, vis_cu, lap, K.dta));
-      buf_st<double, BUF_NT>(prs(uoo, i), st_u, 0, c0);
+      buf_st<double, BUF_NT>(prs(us, i), st_u, 0,
+                             rk_update(S2 ? coo_u : c0, nsum, vis_cu, lap, K.dta));
+      if constexpr (!S2) buf_st<double, BUF_NT>(prs(uoo, i), st_u, 0, c0);
     }
     // the three components one after the other: the scheduler may not mix
     // them, which would hold all three stencils' operands live at once
@@ -783,8 +882,8 @@ __device__ inline void convdiff3_save_body(
       const double p0 = i4(y1, c0, ym1, ym2), m1 = i4(c0, ym1, ym2, ym3);
       const double ty = d4(p2 * p2, p1 * p1, p0 * p0, m1 * m1, K.r24y);
       auto wq = [&](int m) {
-        return i4(W[rix(tj + 2, tk + m)], W[rix(tj + 1, tk + m)], W[rix(tj, tk + m)],
-                  W[rix(tj - 1, tk + m)]);
+        return i4(Wv[rix(tj + 2, tk + m)], Wv[rix(tj + 1, tk + m)], Wv[rix(tj, tk + m)],
+                  Wv[rix(tj - 1, tk + m)]);
       };
       const double tz = d4(i4(z3, z2, z1, c0) * wq(1), i4(z2, z1, c0, zm1) * wq(0),
                            i4(z1, c0, zm1, zm2) * wq(-1), i4(c0, zm1, zm2, zm3) * wq(-2),
@@ -793,8 +892,9 @@ __device__ inline void convdiff3_save_body(
       const double lap = l4(fv[5], fv[4], c0, fv[2], fv[1], K.ir12x) +
                          l4(y2, y1, c0, ym1, ym2, K.ir12y) +
                          l4(z2, z1, c0, zm1, zm2, K.ir12z);
-      buf_st<double, BUF_NT>(prs(vs, i), st_v, 0, rk_update(c0, nsum, vis_cv, lap, K.dta));
-      buf_st<double, BUF_NT>(prs(voo, i), st_v, 0, c0);
+      buf_st<double, BUF_NT>(prs(vs, i), st_v, 0,
+                             rk_update(S2 ? coo_v : c0, nsum, vis_cv, lap, K.dta));
+      if constexpr (!S2) buf_st<double, BUF_NT>(prs(voo, i), st_v, 0, c0);
     }
     __builtin_amdgcn_sched_barrier(0);
     // ---- w (convdiff_body<2, true>) ----
@@ -826,8 +926,9 @@ __device__ inline void convdiff3_save_body(
       const double lap = l4(fw[5], fw[4], c0, fw[2], fw[1], K.ir12x) +
                          l4(y2, y1, c0, ym1, ym2, K.ir12y) +
                          l4(z2, z1, c0, zm1, zm2, K.ir12z);
-      buf_st<double, BUF_NT>(prs(ws, i), st_w, 0, rk_update(c0, nsum, vis_cw, lap, K.dta));
-      buf_st<double, BUF_NT>(prs(woo, i), st_w, 0, c0);
+      buf_st<double, BUF_NT>(prs(ws, i), st_w, 0,
+                             rk_update(S2 ? coo_w : c0, nsum, vis_cw, lap, K.dta));
+      if constexpr (!S2) buf_st<double, BUF_NT>(prs(woo, i), st_w, 0, c0);
     }
   }
 }
@@ -842,8 +943,25 @@ convdiff3_save_kernel(const double* __restrict__ u, const double* __restrict__ v
                       double* __restrict__ woo, wf_grid g, int ich, int kt, int jt, Coefs K) {
   const int nb = kt * jt * gridDim.y;
   const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
-  convdiff3_save_body(u, v, w, vis, us, vs, ws, uoo, voo, woo, g, ich, L % kt, (L / kt) % jt,
-                      L / (kt * jt), K);
+  convdiff3_body<false>(u, v, w, vis, nullptr, us, vs, ws, uoo, voo, woo, g, ich, L % kt,
+                        (L / kt) % jt, L / (kt * jt), K);
+}
+
+// The second stage's form (convdiff3_body<true>): two workgroups per CU.  At
+// three (53 KB of LDS each would fit) the base values and w's pre-fill plane
+// push it past 168 VGPRs, and its spills cost more than the third workgroup
+// gains.
+__global__ void __launch_bounds__(256, 2)
+convdiff3_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                 const double* __restrict__ w, const double* __restrict__ vis,
+                 const double* __restrict__ wpre, double* __restrict__ us,
+                 double* __restrict__ vs, double* __restrict__ ws, double* __restrict__ uoo,
+                 double* __restrict__ voo, double* __restrict__ woo, wf_grid g, int ich, int kt,
+                 int jt, Coefs K) {
+  const int nb = kt * jt * gridDim.y;
+  const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
+  convdiff3_body<true>(u, v, w, vis, wpre, us, vs, ws, uoo, voo, woo, g, ich, L % kt,
+                       (L / kt) % jt, L / (kt * jt), K);
 }
 
 // ---------------------------------------------------------------------------
@@ -853,7 +971,8 @@ convdiff3_save_kernel(const double* __restrict__ u, const double* __restrict__ v
 // the reference fills and convects one component at a time (solver.py:254-
 // 265): conv_v reads u and v after their fills and w before its fill; conv_w
 // reads all three after their fills.  u's pass runs on its own right after
-// u's fill; this pass runs after the fills of v and w and reads, for v's
+// u's fill (where this pass is used instead of convdiff3_kernel, which also
+// takes u); this pass runs after the fills of v and w and reads, for v's
 // z-term, w's values from before w's fill where w's fill writes: the cells
 // outside w's operator box that the term reaches (rows -1 and nj, columns
 // -2, -1, nk-1, nk of each plane), which save_cross_ghosts copied to w_pre
@@ -865,51 +984,6 @@ convdiff3_save_kernel(const double* __restrict__ u, const double* __restrict__ v
 //   LDS (parity): u at plane i+1 (rows -1 .. TJ+1 and columns -1 .. TK+1
 //     around the tile), v and w at plane i (3-cell cross; v also the corner
 //     cells w's y-term reaches) and w at plane i as v's z-term sees it.
-namespace c2 {
-// halo cells of the u plane: rows -1, TJ, TJ+1 on tile columns, columns -1,
-// TK, TK+1 on tile rows
-constexpr int NUH = 3 * TK + 3 * TJ;
-__device__ inline bool u_halo(int h, int& r, int& c) {
-  r = c = 0;
-  if (h < 3 * TK) {
-    r = h < TK ? -1 : TJ + h / TK - 1;
-    c = h % TK;
-  } else if (h < NUH) {
-    h -= 3 * TK;
-    r = h % TJ;
-    const int q = h / TJ;
-    c = q == 0 ? -1 : TK + q - 1;
-  } else {
-    return false;
-  }
-  return true;
-}
-// halo cells of the w plane v's z-term reads (rows -1 .. TJ+1, columns -2 ..
-// TK): rows -1, TJ, TJ+1 on tile columns, columns -2, -1, TK on tile rows,
-// and the nine corner cells
-constexpr int NWH = 3 * TK + 3 * TJ + 9;
-static_assert(NUH <= NT && NWH <= NT, "one halo cell per thread");
-__device__ inline bool w1_halo(int h, int& r, int& c) {
-  r = c = 0;
-  auto row = [](int q) { return q == 0 ? -1 : TJ + q - 1; };  // -1, TJ, TJ+1
-  auto col = [](int q) { return q == 2 ? TK : q - 2; };       // -2, -1, TK
-  if (h < 3 * TK) {
-    r = row(h / TK);
-    c = h % TK;
-  } else if (h < 3 * TK + 3 * TJ) {
-    h -= 3 * TK;
-    r = h % TJ;
-    c = col(h / TJ);
-  } else if (h < NWH) {
-    h -= 3 * TK + 3 * TJ;
-    r = row(h / 3);
-    c = col(h % 3);
-  } else {
-    return false;
-  }
-  return true;
-}
-}  // namespace c2
 
 __device__ inline void convdiff_vw_body(
     const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
@@ -1216,7 +1290,7 @@ int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,
 int convdiff_launch(int comp, const double* u, const double* v, const double* w,
                     const double* coo, const double* vis, double* cstar, const wf_grid& g,
                     const Box3& b, double dx, double dy, double dz, double dta, int ich,
-                    hipStream_t s, double* coo_save) {
+                    hipStream_t s, double* coo_save, int nch, int cstep) {
   using namespace cdv;
   const int kt = div_up(b.d2, TK), jt = div_up(b.d1, TJ);
   if (ich <= 0) ich = tune_get(WF_TUNE_CD_ICH, g.ni, g.nj, g.nk);
@@ -1224,7 +1298,8 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
     const int64_t tiles = (int64_t)kt * jt;
     ich = (int)std::max<int64_t>(16, (tiles * b.d0 + 2047) / 2048);
   }
-  const int nch = div_up(b.d0, ich);
+  if (nch <= 0) nch = div_up(b.d0, ich);
+  if (cstep < 1 || (int64_t)(nch - 1) * cstep * ich >= b.d0) return WF_EINVAL;
   const int64_t nb = (int64_t)kt * jt * nch;
   if (nb >= (int64_t)1 << 31) return WF_EINVAL;
   Coefs K;
@@ -1239,7 +1314,7 @@ int convdiff_launch(int comp, const double* u, const double* v, const double* w,
   const dim3 grid((unsigned)(kt * jt), (unsigned)nch);
 #define CD_LAUNCH(C_, S_)                                                                   \
   hipLaunchKernelGGL((convdiff_kernel<C_, S_>), grid, dim3(NT), 0, s, u, v, w, coo, vis, cstar, \
-                     g, b, ich, kt, jt, K, coo_save)
+                     g, b, ich, kt, jt, K, coo_save, cstep)
   if (coo_save) {
     if (comp == 0)
       CD_LAUNCH(0, true);
@@ -1297,13 +1372,17 @@ int save_cross_ghosts_launch(const double* w, double* wpre, const wf_grid& g, hi
   return 0;
 }
 
-int convdiff3_save_launch(const double* u, const double* v, const double* w,
-                          const double* vis, double* us, double* vs, double* ws, double* uoo,
-                          double* voo, double* woo, const wf_grid& g, double dx, double dy,
-                          double dz, double dta, int ich, hipStream_t s) {
+// The three-component pass: the first stage's form (wpre == nullptr, uoo, voo,
+// woo written) or the second stage's (uoo, voo, woo read, wpre as in
+// convdiff_vw_launch).
+static int convdiff3_any_launch(const double* u, const double* v, const double* w,
+                                const double* vis, const double* wpre, double* us, double* vs,
+                                double* ws, double* uoo, double* voo, double* woo,
+                                const wf_grid& g, double dx, double dy, double dz, double dta,
+                                int ich, hipStream_t s) {
   using namespace cdv;
   const int kt = div_up(g.nk, TK), jt = div_up(g.nj, TJ);
-  if (ich <= 0) ich = tune_get(WF_TUNE_CD3_ICH, g.ni, g.nj, g.nk);
+  if (ich <= 0) ich = tune_get(wpre ? WF_TUNE_CD3S2_ICH : WF_TUNE_CD3_ICH, g.ni, g.nj, g.nk);
   if (ich <= 0) {  // about three workgroups per CU in one round, >= 16 planes each
     const int64_t tiles = (int64_t)kt * jt;
     ich = (int)std::max<int64_t>(16, (tiles * g.ni + 767) / 768);
@@ -1319,10 +1398,47 @@ int convdiff3_save_launch(const double* u, const double* v, const double* w,
   K.ir12y = 1.0 / (12.0 * dy * dy);
   K.ir12z = 1.0 / (12.0 * dz * dz);
   K.dta = dta;
-  hipLaunchKernelGGL(convdiff3_save_kernel, dim3((unsigned)(kt * jt), (unsigned)nch), dim3(NT),
-                     0, s, u, v, w, vis, us, vs, ws, uoo, voo, woo, g, ich, kt, jt, K);
+  const dim3 grid((unsigned)(kt * jt), (unsigned)nch);
+  if (wpre)
+    hipLaunchKernelGGL(convdiff3_kernel, grid, dim3(NT), 0, s, u, v, w, vis, wpre, us, vs, ws,
+                       uoo, voo, woo, g, ich, kt, jt, K);
+  else
+    hipLaunchKernelGGL(convdiff3_save_kernel, grid, dim3(NT), 0, s, u, v, w, vis, us, vs, ws,
+                       uoo, voo, woo, g, ich, kt, jt, K);
   WF_LAUNCHED();
   return 0;
+}
+
+int convdiff3_save_launch(const double* u, const double* v, const double* w,
+                          const double* vis, double* us, double* vs, double* ws, double* uoo,
+                          double* voo, double* woo, const wf_grid& g, double dx, double dy,
+                          double dz, double dta, int ich, hipStream_t s) {
+  return convdiff3_any_launch(u, v, w, vis, nullptr, us, vs, ws, uoo, voo, woo, g, dx, dy, dz,
+                              dta, ich, s);
+}
+
+int convdiff3_launch(const double* u, const double* v, const double* w, const double* uoo,
+                     const double* voo, const double* woo, const double* vis,
+                     const double* wpre, double* us, double* vs, double* ws, const wf_grid& g,
+                     double dx, double dy, double dz, double dta, int ich, hipStream_t s) {
+  if (!wpre) return WF_EINVAL;
+  // read-only in the second stage's form
+  return convdiff3_any_launch(u, v, w, vis, wpre, us, vs, ws, const_cast<double*>(uoo),
+                              const_cast<double*>(voo), const_cast<double*>(woo), g, dx, dy, dz,
+                              dta, ich, s);
+}
+
+// u's conv+diff on planes 0 and ni-2 of its operator box (the planes
+// convdiff3_kernel leaves to this pass): one launch of one-plane chunks 0 and
+// d0 - 1.
+int convdiff_u_edges_launch(const double* u, const double* v, const double* w,
+                            const double* uoo, const double* vis, double* us, const wf_grid& g,
+                            double dx, double dy, double dz, double dta, hipStream_t s) {
+  const int d0 = g.ni - 1;
+  if (d0 <= 0) return 0;
+  const Box3 b{g.npl, g.npl, g.npl, d0, g.nj, g.nk};
+  return convdiff_launch(0, u, v, w, uoo, vis, us, g, b, dx, dy, dz, dta, 1, s, nullptr,
+                         d0 > 1 ? 2 : 1, d0 > 1 ? d0 - 1 : 1);
 }
 
 }  // namespace wf

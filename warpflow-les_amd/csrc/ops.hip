@@ -1262,6 +1262,28 @@ int wf_convdiff_vw(const double* u, const double* v, const double* w, const doub
                             dt * alpha, 0, S_(stream));
 }
 
+int wf_convdiff3(const double* u, const double* v, const double* w, const double* uoo,
+                 const double* voo, const double* woo, const double* vis, const double* w_pre,
+                 double* ustar, double* vstar, double* wstar, wf_grid g, double dx, double dy,
+                 double dz, double dt, double alpha, void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 3, 3) || !u || !v || !w || !uoo || !voo || !woo || !vis || !w_pre ||
+      !ustar || !vstar || !wstar)
+    return WF_EINVAL;
+  return convdiff3_launch(u, v, w, uoo, voo, woo, vis, w_pre, ustar, vstar, wstar, g, dx, dy,
+                          dz, dt * alpha, 0, S_(stream));
+}
+
+int wf_convdiff4th_u_edges(const double* u, const double* v, const double* w,
+                           const double* uoo, const double* vis, double* ustar, wf_grid g,
+                           double dx, double dy, double dz, double dt, double alpha,
+                           void* stream) {
+  const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
+  if (!box_ok(g, b, 3, 3) || !u || !v || !w || !uoo || !vis || !ustar) return WF_EINVAL;
+  return convdiff_u_edges_launch(u, v, w, uoo, vis, ustar, g, dx, dy, dz, dt * alpha,
+                                 S_(stream));
+}
+
 int wf_save_cross_ghosts(const double* w, double* w_pre, wf_grid g, void* stream) {
   const Box3 b = npl_box(g, g.ni, g.nj, g.nk);
   if (!box_ok(g, b, 2, 2) || !w || !w_pre) return WF_EINVAL;

@@ -140,7 +140,7 @@ int tune_get(int knob, int n0, int n1, int n2);
 int convdiff_launch(int comp, const double* u, const double* v, const double* w,
                     const double* coo, const double* vis, double* cstar, const wf_grid& g,
                     const Box3& b, double dx, double dy, double dz, double dta, int ich,
-                    hipStream_t s, double* coo_save = nullptr);
+                    hipStream_t s, double* coo_save = nullptr, int nch = 0, int cstep = 1);
 // The first RK stage's conv+diff of u, v and w in one pass over the union of
 // their operator boxes (convdiff.hip), saving u, v, w into uoo, voo, woo.
 // The second RK stage's conv+diff of v and w in one pass; v's z-term reads
@@ -154,6 +154,17 @@ int convdiff3_save_launch(const double* u, const double* v, const double* w,
                           const double* vis, double* us, double* vs, double* ws, double* uoo,
                           double* voo, double* woo, const wf_grid& g, double dx, double dy,
                           double dz, double dta, int ich, hipStream_t s);
+// The second RK stage's conv+diff of u, v and w in one pass after the three
+// fills: v's z-term reads w_pre as convdiff_vw_launch's does; u's planes 0 and
+// ni-2 are not written (convdiff_u_edges_launch writes them before the fills
+// of v and w).
+int convdiff3_launch(const double* u, const double* v, const double* w, const double* uoo,
+                     const double* voo, const double* woo, const double* vis,
+                     const double* wpre, double* us, double* vs, double* ws, const wf_grid& g,
+                     double dx, double dy, double dz, double dta, int ich, hipStream_t s);
+int convdiff_u_edges_launch(const double* u, const double* v, const double* w,
+                            const double* uoo, const double* vis, double* us, const wf_grid& g,
+                            double dx, double dy, double dz, double dta, hipStream_t s);
 
 // Raw buffer access: a scalar descriptor holds the base (and a byte range),
 // each lane supplies a 32-bit byte offset, so a load or store needs no 64-bit

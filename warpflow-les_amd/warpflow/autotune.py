@@ -1,11 +1,12 @@
 """Launch-shape autotuner for the hot kernels.
 
 The fused momentum kernels (``wf_convdiff4th``, the first stage's
-three-component ``wf_convdiff3_save`` and the second stage's ``wf_convdiff_vw``),
+three-component ``wf_convdiff3_save`` and the second stage's ``wf_convdiff3``
+or ``wf_convdiff_vw``),
 the fused calvel/calmas pass
 (``wf_calvel3_calmas``) and the RBGS sweep kernels have launch-shape
 parameters that change no result: the i-planes each workgroup streams
-through (``cd_ich``, ``cd3_ich``, ``cdvw_ich``, ``proj_ich``, ``sweep_ich``,
+through (``cd_ich``, ``cd3_ich``, ``cdvw_ich``, ``cd3s2_ich``, ``proj_ich``, ``sweep_ich``,
 ``sweep2_ich``) and the (j, k) tile of the one-sweep kernel (``sweep_tile``:
 1 = 8 x 64, 2 = 16 x 32, 3 = 16 x 64, 4 = 32 x 32) and of the two-sweep
 kernel (``sweep2_tile``: 1 = 8 x 64, 2 = 16 x 32, and in FP32 also
@@ -35,10 +36,11 @@ from .device import copy, current_stream, zeros, zeros_like
 # knob ids (include/warpflow_hip.h)
 (CD_ICH, SWEEP_TILE_F64, SWEEP_TILE_F32, SWEEP_ICH_F64, SWEEP_ICH_F32, SWEEP2_TILE_F64,
  SWEEP2_TILE_F32, SWEEP2_ICH_F64, SWEEP2_ICH_F32, SWEEP_PAIRS_F64, PROJ_ICH,
- SWEEP_PAIRS_F32, CD3_ICH, CDVW_ICH) = range(14)
+ SWEEP_PAIRS_F32, CD3_ICH, CDVW_ICH, CD3S2_ICH) = range(15)
 CD_ICH_CANDIDATES = (0, 8, 16, 24, 32, 48, 64)
 CD3_ICH_CANDIDATES = (0, 32, 43, 64, 86, 128, 256)
 CDVW_ICH_CANDIDATES = (0, 16, 32, 43, 64, 86, 128)
+CD3S2_ICH_CANDIDATES = (0, 32, 43, 64, 86, 128, 256)
 SWEEP_TILE_CANDIDATES = (1, 2, 3, 4)
 SWEEP_ICH_CANDIDATES = (0, 1, 2, 4, 8, 16, 32, -2, -4)  # negative: one-pass chunks
 SWEEP2_TILE_CANDIDATES = {True: (1, 2), False: (1, 2, 3, 4)}  # by "is FP64"
@@ -201,6 +203,42 @@ def _tune_convdiff_vw(sol, T, reps, out):
         copy(f[nm], saved[nm])
 
 
+def _tune_convdiff3s2(sol, T, reps, out):
+    """The second stage's three-component pass (w_pre as the solver holds
+    it): ustar, vstar and wstar are restored afterwards."""
+    from . import kernels as K
+    from .topology import NPL
+    f = sol.f
+    if "w_pre" not in f:
+        return
+    h = (sol.dx, sol.dy, sol.dz)
+    n3 = sol.ncells
+    names = ("ustar", "vstar", "wstar")
+    saved = {nm: zeros_like(f[nm]) for nm in names}
+    for nm in names:
+        copy(saved[nm], f[nm])
+
+    def uvw():
+        K.convdiff3(n3, f["u"], f["v"], f["w"], f["uoo"], f["voo"], f["woo"], f["vis"],
+                    f["w_pre"], f["ustar"], f["vstar"], f["wstar"], *h, sol.dt, 1.0, NPL)
+    set_shape(CD3S2_ICH, n3, 0)
+    uvw()
+    want = [f[nm].numpy() for nm in names]
+    best = None
+    for ich in CD3S2_ICH_CANDIDATES:
+        set_shape(CD3S2_ICH, n3, ich)
+        t = T.us(uvw, reps)
+        if not all(np.array_equal(f[nm].numpy(), a) for nm, a in zip(names, want)):
+            raise RuntimeError("convdiff3 (second stage) result depends on cd3s2_ich=%d" % ich)
+        out["timings_us"]["convdiff3s2 %s ich=%d" % ("x".join(map(str, n3)), ich)] = round(t, 2)
+        if best is None or t < best[0]:
+            best = (t, ich)
+    set_shape(CD3S2_ICH, n3, best[1])
+    out["entries"].append((CD3S2_ICH, tuple(n3), best[1]))
+    for nm in names:
+        copy(f[nm], saved[nm])
+
+
 def _tune_proj(sol, T, reps, out):
     from . import kernels as K
     from .topology import NPL
@@ -332,7 +370,10 @@ def tune_solver(sol, reps=10, cache=None):
     try:
         _tune_convdiff(sol, T, reps, out)
         _tune_convdiff3(sol, T, reps, out)
-        _tune_convdiff_vw(sol, T, reps, out)
+        if getattr(sol, "fused_uvw", False):
+            _tune_convdiff3s2(sol, T, reps, out)
+        else:
+            _tune_convdiff_vw(sol, T, reps, out)
         _tune_proj(sol, T, reps, out)
         for lev in sol.mg.levels:
             if lev.tmp is not None and int(np.prod(lev.n3)) > SWEEP_MIN_CELLS:

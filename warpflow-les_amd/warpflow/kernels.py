@@ -13,7 +13,8 @@ the same shape.  The fused handles have no single reference kernel:
 ``convdiff4th_{u,v,w}`` (the conv/diff pair of one RK stage),
 ``convdiff3_save`` (the three pairs of the first stage in one pass),
 ``convdiff_vw`` with ``save_cross_ghosts`` (v's and w's pairs of the second
-stage in one pass),
+stage in one pass), ``convdiff3`` with ``convdiff4th_u_edges`` (the three
+pairs of the second stage in one pass, u's two x-edge planes apart),
 ``calvel3`` (the three calvel launches of ``FlowSolver._calvel``) and
 ``checkdt`` (the four reductions of ``FlowSolver.compute_dt``).
 """
@@ -122,6 +123,30 @@ def _convdiff_vw(dim, u, v, w, voo, woo, vis, w_pre, vstar, wstar, dx, dy, dz, d
         raise ValueError("convdiff_vw: dim %s is not the real-cell extent" % (dim,))
     _hip.call("wf_convdiff_vw", u.ptr, v.ptr, w.ptr, voo.ptr, woo.ptr, vis.ptr, w_pre.ptr,
               vstar.ptr, wstar.ptr, grid_of(u, npl), dx, dy, dz, dt, alpha, current_stream())
+
+
+def _convdiff3(dim, u, v, w, uoo, voo, woo, vis, w_pre, ustar, vstar, wstar, dx, dy, dz, dt,
+               alpha, npl):
+    """convdiff4th_u, _v and _w of the second RK stage in one pass, each over
+    its operator box, launched on the real cells (dim = ncells) after the
+    fills of u, v and w; v's z-term reads w_pre as in convdiff_vw.  u's planes
+    0 and ni-2 are not written: convdiff4th_u_edges writes them after u's fill
+    and before the fills of v and w, when conv_u reads v and w."""
+    _same(u, v, w, uoo, voo, woo, vis, w_pre, ustar, vstar, wstar)
+    if tuple(dim) != tuple(n - 2 * npl for n in u.shape):
+        raise ValueError("convdiff3: dim %s is not the real-cell extent" % (dim,))
+    _hip.call("wf_convdiff3", u.ptr, v.ptr, w.ptr, uoo.ptr, voo.ptr, woo.ptr, vis.ptr,
+              w_pre.ptr, ustar.ptr, vstar.ptr, wstar.ptr, grid_of(u, npl), dx, dy, dz, dt,
+              alpha, current_stream())
+
+
+def _convdiff_u_edges(dim, u, v, w, uoo, vis, ustar, dx, dy, dz, dt, alpha, npl):
+    """convdiff4th_u on planes 0 and ni-2 of u's operator box (dim = ncells)."""
+    _same(u, v, w, uoo, vis, ustar)
+    if tuple(dim) != tuple(n - 2 * npl for n in u.shape):
+        raise ValueError("convdiff4th_u_edges: dim %s is not the real-cell extent" % (dim,))
+    _hip.call("wf_convdiff4th_u_edges", u.ptr, v.ptr, w.ptr, uoo.ptr, vis.ptr, ustar.ptr,
+              grid_of(u, npl), dx, dy, dz, dt, alpha, current_stream())
 
 
 def _save_cross_ghosts(dim, w, w_pre, npl):
@@ -251,6 +276,8 @@ convdiff4th_save_w = Kernel("convdiff4th_save_w", _convdiff_save(2))
 convdiff3_save = Kernel("convdiff3_save", _convdiff3_save)
 convdiff_vw = Kernel("convdiff_vw", _convdiff_vw)
 save_cross_ghosts = Kernel("save_cross_ghosts", _save_cross_ghosts)
+convdiff3 = Kernel("convdiff3", _convdiff3)
+convdiff4th_u_edges = Kernel("convdiff4th_u_edges", _convdiff_u_edges)
 calvel_u = Kernel("calvel_u", _calvel(0))
 calvel_v = Kernel("calvel_v", _calvel(1))
 calvel_w = Kernel("calvel_w", _calvel(2))

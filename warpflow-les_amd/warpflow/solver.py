@@ -89,6 +89,8 @@ class FlowSolver:
         self.bcs.apply(self.f, self.ncells, self.ulid, NPL)
         # u, v, w are as a fill left them: see _rk_stage
         self._fill_stable = True
+        # the second stage's u/v/w pass (off: the v/w pass, with fused_vw)
+        self.fused_uvw = fused
         # the second stage's v/w pass (off: one pass per component)
         self.fused_vw = fused
 
@@ -341,6 +343,24 @@ class FlowSolver:
             self.fill_bc_uvw(alfabc)
             return
         self._fill_stable = False
+        if self.fused and stage > 0 and self.fused_uvw:
+            # Second stage, the three pairs in one pass after the three fills.
+            # conv_u reads v and w before their fills, which change only the
+            # x-ghost planes -1 and ni of what conv_u reads; u's planes 0 and
+            # ni-2 reach them, so those two run right after u's fill, and the
+            # pass leaves them alone.  v's z-term reads w's cells from before
+            # w's fill, which save_cross_ghosts keeps.
+            self.fill_bc("u", alfabc)
+            K.convdiff4th_u_edges(self.ncells, f["u"], f["v"], f["w"], f["uoo"], f["vis"],
+                                  f["ustar"], *d3, dt, alpha, NPL)
+            self.fill_bc("v", alfabc)
+            K.save_cross_ghosts(self.ncells, f["w"], f["w_pre"], NPL)
+            self.fill_bc("w", alfabc)
+            K.convdiff3(self.ncells, f["u"], f["v"], f["w"], f["uoo"], f["voo"], f["woo"],
+                        f["vis"], f["w_pre"], f["ustar"], f["vstar"], f["wstar"], *d3, dt,
+                        alpha, NPL)
+            self.fill_bc_uvw(alfabc)
+            return
         if self.fused and stage > 0 and self.fused_vw:
             # Second stage: u's pass right after u's fill (it reads v and w
             # before their fills, as the reference's conv_u does), then v's
