@@ -322,8 +322,10 @@ struct Consts {
 // One (k tile, j tile, i chunk) work item of the fused pass on a TJ x TK
 // tile (TJ * TK threads, one per tile cell); returns the item's max|div dV|
 // to every thread.  STORE = false: u, v, w are corrected and used but not
-// written (su, rmax and the FP32 rhs are the same).
-template <int TJ, int TK, bool STORE = true>
+// written (su, rmax and the FP32 rhs are the same).  DEPTH: a plane's loads
+// are issued DEPTH planes ahead (2: two register banks, the loop takes two
+// planes per trip).
+template <int TJ, int TK, bool STORE = true, int DEPTH = 1>
 __device__ inline double calvel3_calmas_body(
     const double* __restrict__ us, const double* __restrict__ vs,
     const double* __restrict__ ws, const double* __restrict__ p, double* __restrict__ u,
@@ -426,12 +428,16 @@ __device__ inline double calvel3_calmas_body(
       G.wh1 = p[pi + xw1];
     }
   };
-  Grp N;
-  N.vh = N.vh0 = N.vh1 = N.wh = N.wh0 = N.wh1 = 0.0;
-  issue(N, I0);
+  Grp A, B;
+  A.vh = A.vh0 = A.vh1 = A.wh = A.wh0 = A.wh1 = 0.0;
+  B.vh = B.vh0 = B.vh1 = B.wh = B.wh0 = B.wh1 = 0.0;
+  issue(A, I0);
+  if constexpr (DEPTH == 2) issue(B, I0 + 1);
 
   double val = 0.0;
-  for (int i = I0; i < I1; ++i) {
+  // plane i on the loads N holds; then N takes those of plane i + DEPTH
+  auto step = [&](const int i, Grp& N) {
+    const bool on = DEPTH == 1 || i < I1;  // the second bank's plane past I1
     const int buf = (i - I0) & 1;
     // u of plane i+1 into the window
     uw[3] = uw[2];
@@ -446,16 +452,16 @@ __device__ inline double calvel3_calmas_body(
     W[T::wix(tj, tk)] = wo;
     if (hv) V[T::vix(vr, vc)] = vh_in ? calvel_val(N.vh, N.vh1, N.vh0, dta, C.rdy) : N.vh;
     if (hw) W[T::wix(wr, wc)] = wh_in ? calvel_val(N.wh, N.wh1, N.wh0, dta, C.rdz) : N.wh;
-    if (STORE && live) {
+    if (STORE && live && on) {
       if (u_in(i + 1) && i + 1 < I1) u[plane(i + 1) + xo] = uw[0];
       if (vo_in) v[plane(i) + xo] = vo;
       if (wo_in) w[plane(i) + xo] = wo;
     }
     p0 = p1;
     p1 = N.p2;
-    issue(N, i + 1);
+    issue(N, i + DEPTH);
     __syncthreads();
-    if (live) {
+    if (live && on) {
       // dudx + dvdy + dwdz at (i, J, K) (kernels.py:281-287)
       const double fact =
           div4(d4_bracket(uw[0], uw[1], uw[2], uw[3]),
@@ -470,6 +476,14 @@ __device__ inline double calvel3_calmas_body(
         r32[((int64_t)(i - npl + 1) * (g.nj + 2) + (J - npl + 1)) * (g.nk + 2) + (K - npl + 1)] =
             (float)sv;
       val = nan_max(val, fabs(fact * C.dx * C.dy * C.dz));
+    }
+  };
+  if constexpr (DEPTH == 1) {
+    for (int i = I0; i < I1; ++i) step(i, A);
+  } else {
+    for (int i = I0; i < I1; i += 2) {
+      step(i, A);
+      step(i + 1, B);
     }
   }
   return block_max(val);
@@ -487,7 +501,7 @@ calvel3_calmas_kernel(const double* __restrict__ us, const double* __restrict__ 
                       float* __restrict__ r32) {
   const int nb = kt * jt * (int)gridDim.y;
   const int L = xcd_linear(blockIdx.x + blockIdx.y * gridDim.x, nb);
-  const double r = calvel3_calmas_body<PJ_TJ, PJ_TK, STORE>(
+  const double r = calvel3_calmas_body<PJ_TJ, PJ_TK, STORE, 2>(
       us, vs, ws, p, u, v, w, su, g, ich, L % kt, (L / kt) % jt, L / (kt * jt), K, r32);
   if (threadIdx.x == 0) g_red_part[blockIdx.x + blockIdx.y * gridDim.x] = r;
 }
