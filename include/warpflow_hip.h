@@ -322,9 +322,10 @@ int wf_bc_fill(double* fld, wf_grid g, int which, double param, void* stream);
 
 /* apply_u(u, lid velocity ulid), apply_v(v), apply_w(w) (bcs.py:201-265) --
  * the three fills solver.py issues back to back (boundu, boundv, boundw) --
- * on three fields of one geometry, as three launches (one per axis) that
- * each cover the three fields.  The fields are independent, so the result is
- * that of three wf_bc_fill calls. */
+ * on three fields of one geometry, as two launches (the x ops, then the y and
+ * z ops) that each cover the three fields.  The fields are independent, so
+ * the result is that of three wf_bc_fill calls.  u may be NULL: then v's and
+ * w's fills only. */
 int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream);
 
 /* ------------------------------------------------------------------------

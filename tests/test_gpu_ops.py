@@ -374,6 +374,14 @@ def test_ghost_fills_bitwise(gpu, n3):
     B.apply_uvw(dh["u"], dh["v"], dh["w"], n3[0], 0.75)
     for k in ("u", "v", "w"):
         assert np.array_equal(dh[k].numpy(), h[k]), ("uvw", k)
+    # v, w in one call (u absent)
+    h = rand_fields(n3, ["v", "w"], seed=8)
+    dh = {k: up(v) for k, v in h.items()}
+    O.apply_v(h["v"], n3)
+    O.apply_w(h["w"], n3)
+    B.apply_vw(dh["v"], dh["w"], n3[0])
+    for k in ("v", "w"):
+        assert np.array_equal(dh[k].numpy(), h[k]), ("vw", k)
     # single _mk_lin / _mk_const launches
     g = rand_fields(n3, ["a"], seed=6)["a"]
     da = up(g)

@@ -520,18 +520,27 @@ int wf_bc_fill(double* f, wf_grid g, int which, double param, void* stream) {
 // launch covering them.
 int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream) {
   const hipStream_t s = (hipStream_t)stream;
-  if (g.npl < 1 || !u || !v || !w) return WF_EINVAL;
+  if (g.npl < 1 || !v || !w) return WF_EINVAL;
+  // the fields filled, in the reference's order (u may be absent)
+  double* fs[3];
+  int which[3], nf = 0;
+  if (u) {
+    fs[nf] = u;
+    which[nf++] = 0;
+  }
+  fs[nf] = v;
+  which[nf++] = 1;
+  fs[nf] = w;
+  which[nf++] = 2;
   {  // the x launch
     const int axis = 0;
     Bc3 A{};
-    A.f[0] = u;
-    A.f[1] = v;
-    A.f[2] = w;
     int namax = 0, nbmax = 0;
-    for (int z = 0; z < 3; ++z) {
+    for (int z = 0; z < nf; ++z) {
       int a0, na, b0, nb;
-      if (!bc_axis_ops(z, axis, g.npl, g.ni, g.nj, g.nk, z == 0 ? ulid : 0.0, a0, na, b0, nb,
-                       A.ops[z]) ||
+      A.f[z] = fs[z];
+      if (!bc_axis_ops(which[z], axis, g.npl, g.ni, g.nj, g.nk, which[z] == 0 ? ulid : 0.0, a0,
+                       na, b0, nb, A.ops[z]) ||
           !ops_ok(g, axis, a0, na, b0, nb, A.ops[z]))
         return WF_EINVAL;
       A.a0[z] = a0;
@@ -542,15 +551,14 @@ int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void
       nbmax = std::max(nbmax, nb);
     }
     if (namax > 0 && nbmax > 0) {
-      hipLaunchKernelGGL(bc_axis3_kernel, dim3(div_up(nbmax, 64), div_up(namax, 4), 3),
+      hipLaunchKernelGGL(bc_axis3_kernel, dim3(div_up(nbmax, 64), div_up(namax, 4), nf),
                          dim3(64, 4), 0, s, A, g, axis);
       WF_LAUNCHED();
     }
   }
-  double* const f3[3] = {u, v, w};
-  const int which3[3] = {0, 1, 2};
-  const double param3[3] = {ulid, 0.0, 0.0};
-  return launch_yz(f3, which3, param3, 3, g, s);
+  double param[3];
+  for (int z = 0; z < nf; ++z) param[z] = which[z] == 0 ? ulid : 0.0;
+  return launch_yz(fs, which, param, nf, g, s);
 }
 
 }  // extern "C"

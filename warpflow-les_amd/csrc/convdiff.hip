@@ -1383,9 +1383,10 @@ static int convdiff3_any_launch(const double* u, const double* v, const double* 
   using namespace cdv;
   const int kt = div_up(g.nk, TK), jt = div_up(g.nj, TJ);
   if (ich <= 0) ich = tune_get(wpre ? WF_TUNE_CD3S2_ICH : WF_TUNE_CD3_ICH, g.ni, g.nj, g.nk);
-  if (ich <= 0) {  // about three workgroups per CU in one round, >= 16 planes each
-    const int64_t tiles = (int64_t)kt * jt;
-    ich = (int)std::max<int64_t>(16, (tiles * g.ni + 767) / 768);
+  if (ich <= 0) {  // one round of workgroups (three per CU in the first stage's
+                   // form, two in the second's), >= 16 planes each
+    const int64_t tiles = (int64_t)kt * jt, slots = wpre ? 512 : 768;
+    ich = (int)std::max<int64_t>(16, (tiles * g.ni + slots - 1) / slots);
   }
   const int nch = div_up(g.ni, ich);
   if ((int64_t)kt * jt * nch >= (int64_t)1 << 31) return WF_EINVAL;

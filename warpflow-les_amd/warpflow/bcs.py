@@ -81,6 +81,15 @@ class BoundaryKernels:
                 raise ValueError("u, v and w must share one geometry")
         _hip.call("wf_bc_fill_uvw", u.ptr, v.ptr, w.ptr, g, float(ulid), current_stream())
 
+    def apply_vw(self, v, w, n):
+        """apply_v, apply_w (bcs.py:225-265) back to back, as one launch per
+        axis covering the two fields."""
+        g = _check_n(v, n)
+        key = lambda x: (x.ni, x.nj, x.nk, x.npl, x.sj, x.si)  # noqa: E731
+        if key(grid_of(w, NPL)) != key(g) or w.lead != v.lead:
+            raise ValueError("v and w must share one geometry")
+        _hip.call("wf_bc_fill_uvw", None, v.ptr, w.ptr, g, 0.0, current_stream())
+
     def apply_vis(self, fld, n, rrey):
         """bcs.py:267-293"""
         self._fill(fld, "vis", rrey, n)

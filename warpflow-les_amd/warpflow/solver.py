@@ -112,6 +112,13 @@ class FlowSolver:
             self.halo.exchange(self.f[name])
         self.bcs.apply_uvw(self.f["u"], self.f["v"], self.f["w"], self.ncells[0], self.ulid)
 
+    def fill_bc_vw(self, alfabc=1):
+        """fill_bc of v, then w (solver.py:88-97 twice), as fill_bc_uvw does
+        it for three fields."""
+        for name in ("v", "w"):
+            self.halo.exchange(self.f[name])
+        self.bcs.apply_vw(self.f["v"], self.f["w"], self.ncells[0])
+
     def fill_bc(self, name, alfabc=1):
         """solver.py:88-97: one boundu/boundv/boundw call for one field."""
         self.halo.exchange(self.f[name])
@@ -353,9 +360,11 @@ class FlowSolver:
             self.fill_bc("u", alfabc)
             K.convdiff4th_u_edges(self.ncells, f["u"], f["v"], f["w"], f["uoo"], f["vis"],
                                   f["ustar"], *d3, dt, alpha, NPL)
-            self.fill_bc("v", alfabc)
+            # w's cells before its fill, then v's and w's fills in one pair
+            # of launches (independent fields, the reference's order within
+            # each)
             K.save_cross_ghosts(self.ncells, f["w"], f["w_pre"], NPL)
-            self.fill_bc("w", alfabc)
+            self.fill_bc_vw(alfabc)
             K.convdiff3(self.ncells, f["u"], f["v"], f["w"], f["uoo"], f["voo"], f["woo"],
                         f["vis"], f["w_pre"], f["ustar"], f["vstar"], f["wstar"], *d3, dt,
                         alpha, NPL)
