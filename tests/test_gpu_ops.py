@@ -340,6 +340,77 @@ def test_noise_add_fill_bitwise(gpu, n3):
     assert np.array_equal(p.numpy(), p_ref)
 
 
+@pytest.mark.parametrize("n3", SHAPES + [(9, 17, 70), (41, 10, 33)])
+def test_convdiff3_stage2_equals_single_passes(gpu, n3):
+    """The second stage's three-component pass with u's edge-plane pass equals
+    the single u, v and w passes in the reference's fill/conv order bit for
+    bit, with the cavity's own fills: u, v, w start filled (the previous
+    step's fills), get new interiors (as calvel writes them), then conv_u
+    reads u after its fill and v, w before theirs, conv_v reads v after its
+    fill and w before, conv_w reads all three filled.  Ragged tiles, and the
+    pass's chunk length changes nothing."""
+    from warpflow import autotune
+    from warpflow import kernels as K
+    from warpflow.bcs import BoundaryKernels
+    from warpflow.device import zeros
+    bnd = {"west": "wall", "east": "wall", "south": "slip", "north": "slip",
+           "bottom": "wall", "top": "moving_lid"}
+    B = BoundaryKernels(bnd)
+    h = (0.031, 0.027, 0.019)
+    dt, alpha, ulid = 3e-4, 1.0, 1.0
+    ni, nj, nk = n3
+    ops = {"u": (ni - 1, nj, nk), "v": (ni, nj - 1, nk), "w": (ni, nj, nk - 1)}
+    f = rand_fields(n3, ["u", "v", "w", "uo", "vo", "wo", "vis", "us", "vs", "ws"], seed=17)
+    f["vis"] = 0.0025 + 0.001 * np.abs(f["vis"])
+    d = {k: up(v) for k, v in f.items()}
+    B.apply_uvw(d["u"], d["v"], d["w"], ni, ulid)  # the previous fills
+    rng = np.random.default_rng(18)
+    pre = {}
+    for c in "uvw":  # new interiors, ghosts as the fills left them
+        a = d[c].numpy()
+        box = tuple(slice(N, N + m) for m in ops[c])
+        a[box] += rng.standard_normal(ops[c])
+        pre[c] = a
+    fill = {"u": lambda x: B.apply_u(x, ni, ulid), "v": lambda x: B.apply_v(x, ni),
+            "w": lambda x: B.apply_w(x, ni)}
+    post = {}
+    for c in "uvw":
+        x = up(pre[c])
+        fill[c](x)
+        post[c] = x.numpy()
+    # the reference's order
+    a = {k: up(f[k]) for k in ("uo", "vo", "wo", "vis", "us", "vs", "ws")}
+    K.convdiff4th_u(ops["u"], up(post["u"]), up(pre["v"]), up(pre["w"]), a["uo"], a["vis"],
+                    a["us"], *h, dt, alpha, N)
+    K.convdiff4th_v(ops["v"], up(post["u"]), up(post["v"]), up(pre["w"]), a["vo"], a["vis"],
+                    a["vs"], *h, dt, alpha, N)
+    K.convdiff4th_w(ops["w"], up(post["u"]), up(post["v"]), up(post["w"]), a["wo"], a["vis"],
+                    a["ws"], *h, dt, alpha, N)
+    want = {k: a[k].numpy() for k in ("us", "vs", "ws")}
+    try:
+        for ich in (0, 7, 64):
+            autotune.set_shape(autotune.CD3S2_ICH, n3, ich)
+            b = {k: up(f[k]) for k in ("uo", "vo", "wo", "vis", "us", "vs", "ws")}
+            u1, v0, w0 = up(post["u"]), up(pre["v"]), up(pre["w"])
+            K.convdiff4th_u_edges(n3, u1, v0, w0, b["uo"], b["vis"], b["us"], *h, dt, alpha, N)
+            w_pre = zeros(w0.shape)
+            K.save_cross_ghosts(n3, w0, w_pre, N)
+            B.apply_vw(v0, w0, ni)  # v0, w0 now hold v's and w's fills
+            K.convdiff3(n3, u1, v0, w0, b["uo"], b["vo"], b["wo"], b["vis"], w_pre, b["us"],
+                        b["vs"], b["ws"], *h, dt, alpha, N)
+            for k in ("us", "vs", "ws"):
+                assert np.array_equal(b[k].numpy(), want[k]), (ich, k)
+    finally:
+        autotune.set_shape(autotune.CD3S2_ICH, n3, 0)
+    # the edge pass has teeth: conv_u on the filled v and w differs from the
+    # reference's on u's planes 0 and ni-2 and nowhere else
+    c = {k: up(f[k]) for k in ("uo", "vis", "us")}
+    K.convdiff4th_u(ops["u"], up(post["u"]), up(post["v"]), up(post["w"]), c["uo"], c["vis"],
+                    c["us"], *h, dt, alpha, N)
+    planes = sorted(set((np.argwhere(c["us"].numpy() != want["us"])[:, 0] - N).tolist()))
+    assert planes == [0, ni - 2], planes
+
+
 # (2, 2, 2): the high side's sources reach the low side's ghosts, so those
 # op lists keep the op-by-op walk; the others load every source first.
 @pytest.mark.parametrize("n3", SHAPES + [(2, 2, 2), (3, 5, 4)])
