@@ -516,6 +516,23 @@ int wf_batch_convdiff(wf_batch b, int comp, const double* u, const double* v, co
 int wf_batch_convdiff_save(wf_batch b, int comp, const double* u, const double* v,
                            const double* w, double* coo_save, const double* vis, double* cstar,
                            double alpha, void* stream);
+/* wf_convdiff3_save: the first RK stage's three pairs in one pass (valid on
+ * fields filled since their last change). */
+int wf_batch_convdiff3_save(wf_batch b, const double* u, const double* v, const double* w,
+                            const double* vis, double* ustar, double* vstar, double* wstar,
+                            double* uoo, double* voo, double* woo, double alpha, void* stream);
+/* wf_convdiff3: the second RK stage's three pairs in one pass after the three
+ * fills (u's planes 0 and ni-2 from wf_batch_convdiff4th_u_edges). */
+int wf_batch_convdiff3(wf_batch b, const double* u, const double* v, const double* w,
+                       const double* uoo, const double* voo, const double* woo,
+                       const double* vis, const double* w_pre, double* ustar, double* vstar,
+                       double* wstar, double alpha, void* stream);
+/* wf_convdiff4th_u_edges. */
+int wf_batch_convdiff4th_u_edges(wf_batch b, const double* u, const double* v, const double* w,
+                                 const double* uoo, const double* vis, double* ustar,
+                                 double alpha, void* stream);
+/* wf_save_cross_ghosts. */
+int wf_batch_save_cross_ghosts(wf_batch b, const double* w, double* w_pre, void* stream);
 /* wf_calvel3. */
 int wf_batch_calvel3(wf_batch b, const double* us, const double* vs, const double* ws,
                      const double* p, double* u, double* v, double* w, double alpha,

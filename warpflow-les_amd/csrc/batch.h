@@ -86,7 +86,10 @@ enum PlanKind : int {
   PK_BC0,           // ghost-fill lines of the x, y, z axis launches
   PK_BC1,
   PK_BC2,
-  PK_WALE,          // WALE tiles (8 x 32 (j, k) x BATCH_CD_ICH planes) over the real cells
+  PK_WALE,          // WALE tiles (8 x 32 (j, k) x BATCH_CD_ICH planes) over the real cells;
+                    // also the three-component conv+diff passes
+  PK_CDE,           // u's edge planes 0 and ni-2: 8 x 32 (j, k) tiles x 2
+  PK_XG,            // w's cross-ghost cells of each real plane, 256 per item
   PK_MEMBER,        // one item per member
   PK_LCELLS0,       // 64x4 blocks over level-l real cells, l = 0 .. 7
   PK_SWEEP0 = PK_LCELLS0 + BATCH_MAX_LEVELS,  // fused-sweep tiles of level l
@@ -154,6 +157,16 @@ int batch_add_ring(const Batch& B, double* p, const double* pp, const int* activ
 int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,
                    const double* w, const double* coo, const double* vis, double* cstar,
                    double alpha, hipStream_t s, double* coo_save = nullptr);
+// The three-component conv+diff passes of the two RK stages (convdiff3_save_
+// kernel, convdiff3_kernel), u's edge-plane pass and w's cross-ghost copy, for
+// every member (convdiff.hip).
+int batch_convdiff3(const Batch& B, const double* u, const double* v, const double* w,
+                    const double* vis, const double* wpre, double* us, double* vs, double* ws,
+                    double* uoo, double* voo, double* woo, double alpha, hipStream_t s);
+int batch_convdiff_u_edges(const Batch& B, const double* u, const double* v, const double* w,
+                           const double* uoo, const double* vis, double* us, double alpha,
+                           hipStream_t s);
+int batch_save_cross_ghosts(const Batch& B, const double* w, double* wpre, hipStream_t s);
 int batch_bc_fill(const Batch& B, double* f, int which, const int* active, hipStream_t s);
 int batch_vcycle(const Batch& B, float* const* phi, float* const* rhs, float* const* tmp,
                  const float* cop, int npre, int npost, int ncoarse, const int* active,

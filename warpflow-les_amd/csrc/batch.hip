@@ -69,6 +69,10 @@ int member_count(int kind, const MemDev& M) {
       return bc_blocks(kind - PK_BC0, n0, n1, n2);
     case PK_WALE:
       return cd_tiles(n0, n1, n2);
+    case PK_CDE:
+      return ((n2 + 31) / 32) * ((n1 + 7) / 8) * (n0 - 1 > 1 ? 2 : (n0 - 1 > 0 ? 1 : 0));
+    case PK_XG:
+      return ((2 * (n2 + 3) + 4 * n1 + 255) / 256) * n0;
     case PK_MEMBER:
       return 1;
     default:
@@ -289,6 +293,39 @@ int wf_batch_convdiff_save(wf_batch b, int comp, const double* u, const double* 
     return WF_EINVAL;
   return batch_convdiff(B_(b), comp, u, v, w, nullptr, vis, cstar, alpha, (hipStream_t)stream,
                         coo_save);
+}
+
+int wf_batch_convdiff3_save(wf_batch b, const double* u, const double* v, const double* w,
+                            const double* vis, double* ustar, double* vstar, double* wstar,
+                            double* uoo, double* voo, double* woo, double alpha, void* stream) {
+  if (!ok(b) || !u || !v || !w || !vis || !ustar || !vstar || !wstar || !uoo || !voo || !woo)
+    return WF_EINVAL;
+  return batch_convdiff3(B_(b), u, v, w, vis, nullptr, ustar, vstar, wstar, uoo, voo, woo,
+                         alpha, (hipStream_t)stream);
+}
+
+int wf_batch_convdiff3(wf_batch b, const double* u, const double* v, const double* w,
+                       const double* uoo, const double* voo, const double* woo,
+                       const double* vis, const double* w_pre, double* ustar, double* vstar,
+                       double* wstar, double alpha, void* stream) {
+  if (!ok(b) || !u || !v || !w || !uoo || !voo || !woo || !vis || !w_pre || !ustar || !vstar ||
+      !wstar)
+    return WF_EINVAL;
+  return batch_convdiff3(B_(b), u, v, w, vis, w_pre, ustar, vstar, wstar,
+                         const_cast<double*>(uoo), const_cast<double*>(voo),
+                         const_cast<double*>(woo), alpha, (hipStream_t)stream);
+}
+
+int wf_batch_convdiff4th_u_edges(wf_batch b, const double* u, const double* v, const double* w,
+                                 const double* uoo, const double* vis, double* ustar,
+                                 double alpha, void* stream) {
+  if (!ok(b) || !u || !v || !w || !uoo || !vis || !ustar) return WF_EINVAL;
+  return batch_convdiff_u_edges(B_(b), u, v, w, uoo, vis, ustar, alpha, (hipStream_t)stream);
+}
+
+int wf_batch_save_cross_ghosts(wf_batch b, const double* w, double* w_pre, void* stream) {
+  if (!ok(b) || !w || !w_pre) return WF_EINVAL;
+  return batch_save_cross_ghosts(B_(b), w, w_pre, (hipStream_t)stream);
 }
 
 int wf_batch_calvel3(wf_batch b, const double* us, const double* vs, const double* ws,

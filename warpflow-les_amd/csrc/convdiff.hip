@@ -1205,10 +1205,9 @@ convdiff_vw_kernel(const double* __restrict__ u, const double* __restrict__ v,
 // w_pre <- w at the cells convdiff_vw reads from w_pre: on each real plane,
 // rows -1 and nj (columns -2 .. nk) and columns -2, -1, nk-1, nk (rows 0 ..
 // nj-1), in real-cell coordinates.
-__global__ void __launch_bounds__(256)
-save_cross_ghosts_kernel(const double* __restrict__ w, double* __restrict__ wpre, wf_grid g) {
+__device__ inline void save_cross_body(const double* __restrict__ w, double* __restrict__ wpre,
+                                       const wf_grid& g, int t, int plane) {
   const int nrow = g.nk + 3, n = 2 * nrow + 4 * g.nj;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int J, Kc;
   if (t < 2 * nrow) {
@@ -1219,9 +1218,13 @@ save_cross_ghosts_kernel(const double* __restrict__ w, double* __restrict__ wpre
     J = h % g.nj;
     Kc = q < 2 ? q - 2 : g.nk - 1 + (q - 2);
   }
-  const int64_t x = (int64_t)(g.npl + blockIdx.y) * g.si + (int64_t)(g.npl + J) * g.sj +
+  const int64_t x = (int64_t)(g.npl + plane) * g.si + (int64_t)(g.npl + J) * g.sj +
                     (g.npl + Kc);
   wpre[x] = w[x];
+}
+__global__ void __launch_bounds__(256)
+save_cross_ghosts_kernel(const double* __restrict__ w, double* __restrict__ wpre, wf_grid g) {
+  save_cross_body(w, wpre, g, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 // Ensemble form: a persistent grid over the (member, tile, chunk) items of
@@ -1256,7 +1259,128 @@ batch_convdiff_kernel(BView V, Plan P, const double* __restrict__ u,
   }
 }
 
+// Coefficients of member M (as convdiff_launch forms them on the host).
+__device__ inline Coefs member_coefs(const MemDev& M, double alpha) {
+  Coefs K;
+  K.r24x = 1.0 / (24.0 * M.dx);
+  K.r24y = 1.0 / (24.0 * M.dy);
+  K.r24z = 1.0 / (24.0 * M.dz);
+  K.ir12x = 1.0 / (12.0 * M.dx * M.dx);
+  K.ir12y = 1.0 / (12.0 * M.dy * M.dy);
+  K.ir12z = 1.0 / (12.0 * M.dz * M.dz);
+  K.dta = M.dt * alpha;
+  return K;
+}
+
+// Ensemble forms of the three-component passes: a persistent grid over the
+// (member, tile, chunk) items of PK_WALE (8 x 32 tiles over the real cells,
+// chunks of BATCH_CD_ICH planes), two workgroups per CU (the item loop's
+// member lookup pushes the first stage's form past 168 VGPRs as well).
+template <bool S2>
+__device__ inline void batch_convdiff3_items(BView V, Plan P, const double* u, const double* v,
+                                             const double* w, const double* vis,
+                                             const double* wpre, double* us, double* vs,
+                                             double* ws, double* uoo, double* voo, double* woo,
+                                             double alpha) {
+  for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
+    const int m = member_of(P.start, V.nmem, it);
+    const int q = it - P.start[m];
+    const MemDev& M = V.mem[m];
+    const wf_grid g = M.g;
+    const int kt = (g.nk + TK - 1) / TK, jt = (g.nj + TJ - 1) / TJ;
+    convdiff3_body<S2>(u + M.off, v + M.off, w + M.off, vis + M.off, S2 ? wpre + M.off : nullptr,
+                       us + M.off, vs + M.off, ws + M.off, uoo + M.off, voo + M.off, woo + M.off,
+                       g, BATCH_CD_ICH, q % kt, (q / kt) % jt, q / (kt * jt),
+                       member_coefs(M, alpha));
+    __syncthreads();  // LDS of this item is free for the next
+  }
+}
+__global__ void __launch_bounds__(256, 2)
+batch_convdiff3_save_kernel(BView V, Plan P, const double* u, const double* v, const double* w,
+                            const double* vis, double* us, double* vs, double* ws, double* uoo,
+                            double* voo, double* woo, double alpha) {
+  batch_convdiff3_items<false>(V, P, u, v, w, vis, nullptr, us, vs, ws, uoo, voo, woo, alpha);
+}
+__global__ void __launch_bounds__(256, 2)
+batch_convdiff3_kernel(BView V, Plan P, const double* u, const double* v, const double* w,
+                       const double* vis, const double* wpre, double* us, double* vs,
+                       double* ws, double* uoo, double* voo, double* woo, double alpha) {
+  batch_convdiff3_items<true>(V, P, u, v, w, vis, wpre, us, vs, ws, uoo, voo, woo, alpha);
+}
+
+// u's planes 0 and ni-2 of every member (PK_CDE items: a tile and one of the
+// two planes, one-plane chunks).
+__global__ void __launch_bounds__(256)
+batch_convdiff_u_edges_kernel(BView V, Plan P, const double* __restrict__ u,
+                              const double* __restrict__ v, const double* __restrict__ w,
+                              const double* __restrict__ uoo, const double* __restrict__ vis,
+                              double* __restrict__ us, double alpha) {
+  for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
+    const int m = member_of(P.start, V.nmem, it);
+    const int q = it - P.start[m];
+    const MemDev& M = V.mem[m];
+    const wf_grid g = M.g;
+    const int d0 = g.ni - 1;
+    const Box3 b{g.npl, g.npl, g.npl, d0, g.nj, g.nk};
+    const int kt = (g.nk + TK - 1) / TK, jt = (g.nj + TJ - 1) / TJ;
+    const int tile = q % (kt * jt), which = q / (kt * jt);
+    convdiff_body<0, false>(u + M.off, v + M.off, w + M.off, uoo + M.off, vis + M.off,
+                            us + M.off, g, b, 1, tile % kt, tile / kt, which ? d0 - 1 : 0,
+                            member_coefs(M, alpha));
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256)
+batch_save_cross_ghosts_kernel(BView V, Plan P, const double* __restrict__ w,
+                               double* __restrict__ wpre) {
+  for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
+    const int m = member_of(P.start, V.nmem, it);
+    const int q = it - P.start[m];
+    const MemDev& M = V.mem[m];
+    const wf_grid g = M.g;
+    const int nb = (2 * (g.nk + 3) + 4 * g.nj + 255) / 256;
+    save_cross_body(w + M.off, wpre + M.off, g, (q % nb) * 256 + (int)threadIdx.x, q / nb);
+  }
+}
+
 }  // namespace cdv
+
+int batch_convdiff3(const Batch& B, const double* u, const double* v, const double* w,
+                    const double* vis, const double* wpre, double* us, double* vs, double* ws,
+                    double* uoo, double* voo, double* woo, double alpha, hipStream_t s) {
+  using namespace cdv;
+  const Plan& P = B.plan[PK_WALE];
+  if (wpre)
+    hipLaunchKernelGGL(batch_convdiff3_kernel, dim3(P.grid), dim3(NT), 0, s, B.view(), P, u, v,
+                       w, vis, wpre, us, vs, ws, uoo, voo, woo, alpha);
+  else
+    hipLaunchKernelGGL(batch_convdiff3_save_kernel, dim3(P.grid), dim3(NT), 0, s, B.view(), P, u,
+                       v, w, vis, us, vs, ws, uoo, voo, woo, alpha);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_convdiff_u_edges(const Batch& B, const double* u, const double* v, const double* w,
+                           const double* uoo, const double* vis, double* us, double alpha,
+                           hipStream_t s) {
+  using namespace cdv;
+  const Plan& P = B.plan[PK_CDE];
+  if (P.total == 0) return 0;
+  hipLaunchKernelGGL(batch_convdiff_u_edges_kernel, dim3(P.grid), dim3(NT), 0, s, B.view(), P,
+                     u, v, w, uoo, vis, us, alpha);
+  WF_LAUNCHED();
+  return 0;
+}
+
+int batch_save_cross_ghosts(const Batch& B, const double* w, double* wpre, hipStream_t s) {
+  using namespace cdv;
+  const Plan& P = B.plan[PK_XG];
+  hipLaunchKernelGGL(batch_save_cross_ghosts_kernel, dim3(P.grid), dim3(256), 0, s, B.view(), P,
+                     w, wpre);
+  WF_LAUNCHED();
+  return 0;
+}
 
 int batch_convdiff(const Batch& B, int comp, const double* u, const double* v,
                    const double* w, const double* coo, const double* vis, double* cstar,

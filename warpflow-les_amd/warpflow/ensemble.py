@@ -119,7 +119,7 @@ class Ensemble:
         _hip.call("wf_batch_layout", self.handle, foff, C.byref(fel), loff, lel, coff,
                   C.byref(cel))
         self.field_off = list(foff)
-        self.pool = {nm: zeros(fel.value, np.float64, device) for nm in FIELDS}
+        self.pool = {nm: zeros(fel.value, np.float64, device) for nm in FIELDS + ("w_pre",)}
         self.level_n3 = [[tuple(x >> l for x in sp.n3) for sp in self.specs]
                          for l in range(self.nlev)]
         self.level_off = [[loff[l * self.nmem + m] for m in range(self.nmem)]
@@ -216,11 +216,25 @@ class Ensemble:
         else:
             self._call("wf_batch_fill_vis", P["vis"].ptr, s)
         for stage, alpha in enumerate(ALFARK):
-            for comp, c in enumerate("uvw"):
-                self._fill(c)
-                self._call("wf_batch_convdiff_save" if stage == 0 else "wf_batch_convdiff",
-                           comp, P["u"].ptr, P["v"].ptr, P["w"].ptr, P[c + "oo"].ptr,
-                           P["vis"].ptr, P[c + "star"].ptr, float(alpha), s)
+            if stage == 0:
+                # u, v, w as the initial or end-of-step fills left them: the
+                # three conv+diff pairs in one pass (FlowSolver._rk_stage)
+                self._call("wf_batch_convdiff3_save", P["u"].ptr, P["v"].ptr, P["w"].ptr,
+                           P["vis"].ptr, P["ustar"].ptr, P["vstar"].ptr, P["wstar"].ptr,
+                           P["uoo"].ptr, P["voo"].ptr, P["woo"].ptr, float(alpha), s)
+            else:
+                # u's fill, u's planes 0 and ni-2, w's cells before its fill,
+                # the fills of v and w, then the three pairs in one pass
+                self._fill("u")
+                self._call("wf_batch_convdiff4th_u_edges", P["u"].ptr, P["v"].ptr, P["w"].ptr,
+                           P["uoo"].ptr, P["vis"].ptr, P["ustar"].ptr, float(alpha), s)
+                self._call("wf_batch_save_cross_ghosts", P["w"].ptr, P["w_pre"].ptr, s)
+                self._fill("v")
+                self._fill("w")
+                self._call("wf_batch_convdiff3", P["u"].ptr, P["v"].ptr, P["w"].ptr,
+                           P["uoo"].ptr, P["voo"].ptr, P["woo"].ptr, P["vis"].ptr,
+                           P["w_pre"].ptr, P["ustar"].ptr, P["vstar"].ptr, P["wstar"].ptr,
+                           float(alpha), s)
             for c in "uvw":
                 self._fill(c)
             if stage + 1 < len(ALFARK):
