@@ -507,6 +507,10 @@ int wf_batch_checkdt(wf_batch b, const double* u, const double* v, const double*
                      const double* vis, double* out4, void* stream);
 /* wf_bc_fill(which) with param = lid velocity (u), 1/Re (vis), 3 (p mirror). */
 int wf_batch_bc_fill(wf_batch b, double* f, int which, const int* active, void* stream);
+/* wf_bc_fill_uvw: the fills of u, v and w (u may be NULL: v and w only), each
+ * axis launch covering the fields. */
+int wf_batch_bc_fill_uvw(wf_batch b, double* u, double* v, double* w, const int* active,
+                         void* stream);
 /* wf_convdiff4th over the operator box of component comp. */
 int wf_batch_convdiff(wf_batch b, int comp, const double* u, const double* v, const double* w,
                       const double* coo, const double* vis, double* cstar, double alpha,

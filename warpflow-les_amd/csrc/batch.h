@@ -168,6 +168,8 @@ int batch_convdiff_u_edges(const Batch& B, const double* u, const double* v, con
                            hipStream_t s);
 int batch_save_cross_ghosts(const Batch& B, const double* w, double* wpre, hipStream_t s);
 int batch_bc_fill(const Batch& B, double* f, int which, const int* active, hipStream_t s);
+int batch_bc_fill_n(const Batch& B, double* const* f, const int* which, int nf,
+                    const int* active, hipStream_t s);
 int batch_vcycle(const Batch& B, float* const* phi, float* const* rhs, float* const* tmp,
                  const float* cop, int npre, int npost, int ncoarse, const int* active,
                  hipStream_t s);

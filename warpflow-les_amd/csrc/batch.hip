@@ -278,6 +278,22 @@ int wf_batch_bc_fill(wf_batch b, double* f, int which, const int* active, void* 
   return batch_bc_fill(B_(b), f, which, active, (hipStream_t)stream);
 }
 
+int wf_batch_bc_fill_uvw(wf_batch b, double* u, double* v, double* w, const int* active,
+                         void* stream) {
+  if (!ok(b) || !v || !w) return WF_EINVAL;
+  double* f[3];
+  int which[3], nf = 0;
+  if (u) {
+    f[nf] = u;
+    which[nf++] = 0;
+  }
+  f[nf] = v;
+  which[nf++] = 1;
+  f[nf] = w;
+  which[nf++] = 2;
+  return batch_bc_fill_n(B_(b), f, which, nf, active, (hipStream_t)stream);
+}
+
 int wf_batch_convdiff(wf_batch b, int comp, const double* u, const double* v, const double* w,
                       const double* coo, const double* vis, double* cstar, double alpha,
                       void* stream) {

@@ -373,9 +373,16 @@ int bc_blocks(int axis, int ni, int nj, int nk) {
   return div_up(nb, 64) * div_up(na, 4);
 }
 
+// Fields of a multi-field batched fill: blockIdx.y picks field z.
+struct BcFields {
+  double* f[3];
+  int which[3];
+};
+
 __global__ void __launch_bounds__(256)
-batch_bc_kernel(BView V, Plan P, double* __restrict__ f, int which, int axis,
-                const int* active) {
+batch_bc_kernel(BView V, Plan P, BcFields F, int axis, const int* active) {
+  double* __restrict__ f = F.f[blockIdx.y];
+  const int which = F.which[blockIdx.y];
   for (int it = blockIdx.x; it < P.total; it += gridDim.x) {
     const int m = member_of(P.start, V.nmem, it);
     if (active && !active[m]) continue;
@@ -394,16 +401,29 @@ batch_bc_kernel(BView V, Plan P, double* __restrict__ f, int which, int axis,
   }
 }
 
-// The three axis launches of one fill over every (active) member.
-int batch_bc_fill(const Batch& B, double* f, int which, const int* active, hipStream_t s) {
-  if (which < 0 || which > 5) return WF_EINVAL;
+// The three axis launches of the fills which[z] of fields f[z] (z < nf,
+// independent fields) over every (active) member: each launch covers the nf
+// fields, one per grid row, so the axes keep their order within each field.
+int batch_bc_fill_n(const Batch& B, double* const* f, const int* which, int nf,
+                    const int* active, hipStream_t s) {
+  if (nf < 1 || nf > 3) return WF_EINVAL;
+  BcFields F{};
+  for (int z = 0; z < nf; ++z) {
+    if (!f[z] || which[z] < 0 || which[z] > 5) return WF_EINVAL;
+    F.f[z] = f[z];
+    F.which[z] = which[z];
+  }
   for (int axis = 0; axis < 3; ++axis) {
     const Plan& P = B.plan[PK_BC0 + axis];
-    hipLaunchKernelGGL(batch_bc_kernel, dim3(P.grid), dim3(64, 4), 0, s, B.view(), P, f, which,
+    hipLaunchKernelGGL(batch_bc_kernel, dim3(P.grid, nf), dim3(64, 4), 0, s, B.view(), P, F,
                        axis, active);
     WF_LAUNCHED();
   }
   return 0;
+}
+
+int batch_bc_fill(const Batch& B, double* f, int which, const int* active, hipStream_t s) {
+  return batch_bc_fill_n(B, &f, &which, 1, active, s);
 }
 
 }  // namespace wf

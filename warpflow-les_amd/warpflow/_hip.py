@@ -110,6 +110,7 @@ _SIGS = {
     "wf_batch_convdiff3": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P],
     "wf_batch_convdiff4th_u_edges": [_P, _P, _P, _P, _P, _P, _P, _D, _P],
     "wf_batch_save_cross_ghosts": [_P, _P, _P, _P],
+    "wf_batch_bc_fill_uvw": [_P, _P, _P, _P, _P, _P],
     "wf_calvel": [_I, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _P],
     "wf_calvel3": [_P, _P, _P, _P, _P, _P, _P, _G, _D, _D, _D, _D, _D, _P],
     "wf_calmas": [_P, _P, _P, _P, _P, _G, _I, _I, _I, _D, _D, _D, _D, _P],

@@ -197,6 +197,13 @@ class Ensemble:
         self._call("wf_batch_bc_fill", self.pool[field].ptr, _BC[which or field],
                    active.ptr if active is not None else None, self.stream)
 
+    def _fill_uvw(self, with_u=True):
+        """The fills of u (unless with_u is false), v and w for every member,
+        one launch per axis for the fields."""
+        P = self.pool
+        self._call("wf_batch_bc_fill_uvw", P["u"].ptr if with_u else None, P["v"].ptr,
+                   P["w"].ptr, None, self.stream)
+
     def _calvel(self, alpha, active=None):
         P = self.pool
         self._call("wf_batch_calvel3", P["ustar"].ptr, P["vstar"].ptr, P["wstar"].ptr,
@@ -229,14 +236,12 @@ class Ensemble:
                 self._call("wf_batch_convdiff4th_u_edges", P["u"].ptr, P["v"].ptr, P["w"].ptr,
                            P["uoo"].ptr, P["vis"].ptr, P["ustar"].ptr, float(alpha), s)
                 self._call("wf_batch_save_cross_ghosts", P["w"].ptr, P["w_pre"].ptr, s)
-                self._fill("v")
-                self._fill("w")
+                self._fill_uvw(with_u=False)
                 self._call("wf_batch_convdiff3", P["u"].ptr, P["v"].ptr, P["w"].ptr,
                            P["uoo"].ptr, P["voo"].ptr, P["woo"].ptr, P["vis"].ptr,
                            P["w_pre"].ptr, P["ustar"].ptr, P["vstar"].ptr, P["wstar"].ptr,
                            float(alpha), s)
-            for c in "uvw":
-                self._fill(c)
+            self._fill_uvw()
             if stage + 1 < len(ALFARK):
                 self._calvel(alpha)
         # projection: reset the per-member loop state; every outer test opens
@@ -280,8 +285,7 @@ class Ensemble:
             self._outer_test(self.graph_iters + 1)
 
     def _part_b(self):
-        for c in "uvw":
-            self._fill(c)
+        self._fill_uvw()
 
     def _capture(self, fn):
         _hip.call("wf_graph_begin", self.stream)
