@@ -614,12 +614,15 @@ __device__ inline void convdiff3_body(
     double* __restrict__ voo, double* __restrict__ woo, const wf_grid& g, int ich, int ktile,
     int jtile, int chunk, const Coefs& K) {
   using namespace c3;
-  __shared__ double Us[4][RN];
+  __shared__ double Us[3][RN];  // u at planes i, i+1 (a third slot for the next)
   __shared__ double Vs[2][RN];
   __shared__ double Ws[2][RN];
   __shared__ double IV[2][IVN];
   __shared__ double IW[2][IWN];
   __shared__ double W1s[S2 ? 2 : 1][S2 ? RN : 1];  // w as v's z-term reads it (S2)
+  // v and w at the I4x halo positions, planes i-1 .. i+2 (slot (p - i0 + 4) & 3),
+  // each position written and read by its own thread only
+  __shared__ double XR[4][3 * TK + 3 * TJ];
 
   const int tid = threadIdx.x;
   const int tj = tid / TK, tk = tid % TK;
@@ -666,6 +669,7 @@ __device__ inline void convdiff3_body(
   const int xwc = tw < 2 * TJ ? -2 + tw / TJ : TK, xwr = tw % TJ;
   const int64_t xw_off = G.inplane(xw ? xwr : 0, xw ? xwc : 0);
   const int xw_l = iwx(xw ? xwr : 0, xw ? xwc : 0);
+  const int xr = xv ? tid : 3 * TK + (xw ? tw : 0);  // ring position
   // vis at each component's face (kernels.py:198, :211, :224)
   const int64_t vis_u = own, vis_v = G.inplane(tj + 1, tk), vis_w = G.inplane(tj, tk + 1);
   // in the plane loop every access is a raw buffer access: a per-plane
@@ -708,26 +712,21 @@ __device__ inline void convdiff3_body(
   double hun = u[G.plane(i0 + 1) + hu_off];  // halo of u plane i0+1
   double hvn = v[G.plane(i0) + hv_off];      // halo of v, w plane i0
   double hwn = w[G.plane(i0) + hw_off];
-  // v, w windows at the I4x halo positions, planes i-1 .. i+2 after the shift
-  double xvw[4], xww[4];
+  // v, w at the I4x halo positions, planes i0-1 .. i0+1, into the ring
 #pragma unroll
   for (int q = 1; q < 4; ++q) {
     const int64_t pb = G.plane(i0 - 2 + q);
-    xvw[q] = v[pb + xv_off];
-    xww[q] = w[pb + xw_off];
+    if (xv) XR[(q + 2) & 3][xr] = v[pb + xv_off];
+    if (xw) XR[(q + 2) & 3][xr] = w[pb + xw_off];
   }
-  xvw[0] = xww[0] = 0.0;
   double xvn = v[G.plane(i0 + 2) + xv_off], xwn = w[G.plane(i0 + 2) + xw_off];
   double snu = vis[G.plane(i0) + vis_u], snv = vis[G.plane(i0) + vis_v],
          snw = vis[G.plane(i0) + vis_w];
-  double w1t = 0.0, w1h = 0.0, cun = 0.0, cvn = 0.0, cwn = 0.0;  // S2
+  double w1t = 0.0, w1h = 0.0;  // S2
   if constexpr (S2) {
     const int64_t p0 = G.plane(i0);
     w1t = w1t_src[p0 + own];
     w1h = w1h_src[p0 + h1_off];
-    cun = uoo[p0 + own];
-    cvn = voo[p0 + own];
-    cwn = woo[p0 + own];
   }
   // rolled x-terms: squares of I4x(u) (u), Q = I4x(c) * I4(u) (v, w), at
   // planes i+1, i, i-1 before the shift (f[1..6] = planes i0-3 .. i0+2)
@@ -768,17 +767,9 @@ __device__ inline void convdiff3_body(
     fu[6] = nu;
     fv[6] = nv;
     fw[6] = nw;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      xvw[q] = xvw[q + 1];
-      xww[q] = xww[q + 1];
-    }
-    xvw[3] = xvn;
-    xww[3] = xwn;
-    const double vis_cu = snu, vis_cv = snv, vis_cw = snw;
-    const double coo_u = cun, coo_v = cvn, coo_w = cwn;  // S2
-    double* __restrict__ U0 = Us[rel & 3];        // u, plane i
-    double* __restrict__ U1 = Us[(rel + 1) & 3];  // u, plane i+1
+    const double vis_pu = snu, vis_pv = snv, vis_pw = snw;  // first stage's form
+    double* __restrict__ U0 = Us[rel % 3];        // u, plane i
+    double* __restrict__ U1 = Us[(rel + 1) % 3];  // u, plane i+1
     double* __restrict__ V = Vs[buf];
     double* __restrict__ W = Ws[buf];
     double* __restrict__ AV = IV[buf];
@@ -791,9 +782,16 @@ __device__ inline void convdiff3_body(
     if (hw) W[hw_l] = hwn;
     // x-interpolants at plane i+1 (planes i+2, i+1, i, i-1)
     AV[ivx(tj, tk)] = i4(fv[5], fv[4], fv[3], fv[2]);
-    if (xv) AV[xv_l] = i4(xvw[3], xvw[2], xvw[1], xvw[0]);
+    const int r2 = (rel + 2) & 3, r1 = (rel + 1) & 3, r0 = rel & 3, rm = (rel + 3) & 3;
+    if (xv) {
+      XR[r2][xr] = xvn;
+      AV[xv_l] = i4(xvn, XR[r1][xr], XR[r0][xr], XR[rm][xr]);
+    }
     AW[iwx(tj, tk)] = i4(fw[5], fw[4], fw[3], fw[2]);
-    if (xw) AW[xw_l] = i4(xww[3], xww[2], xww[1], xww[0]);
+    if (xw) {
+      XR[r2][xr] = xwn;
+      AW[xw_l] = i4(xwn, XR[r1][xr], XR[r0][xr], XR[rm][xr]);
+    }
     double* __restrict__ Wv = W;  // w as v's z-term reads it
     if constexpr (S2) {
       Wv = W1s[buf];
@@ -814,16 +812,15 @@ __device__ inline void convdiff3_body(
       hwn = buf_ld<double>(w1, b_hw, 0);
       xvn = buf_ld<double>(prs(v, i + 3), b_xv, 0);
       xwn = buf_ld<double>(prs(w, i + 3), b_xw, 0);
-      snu = buf_ld<double>(s1, b_own, 0);
-      snv = buf_ld<double>(s1, b_vv, 0);
-      snw = buf_ld<double>(s1, b_vw, 0);
+      if constexpr (!S2) {  // (the second stage's form loads them per component)
+        snu = buf_ld<double>(s1, b_own, 0);
+        snv = buf_ld<double>(s1, b_vv, 0);
+        snw = buf_ld<double>(s1, b_vw, 0);
+      }
       if constexpr (S2) {
         const int64_t p1 = G.plane(i + 1);
         w1t = w1t_src[p1 + own];
         w1h = w1h_src[p1 + h1_off];
-        cun = buf_ld<double>(prs(uoo, i + 1), b_own, 0);
-        cvn = buf_ld<double>(prs(voo, i + 1), b_own, 0);
-        cwn = buf_ld<double>(prs(woo, i + 1), b_own, 0);
       }
     }
     __syncthreads();
@@ -832,6 +829,11 @@ __device__ inline void convdiff3_body(
         lu_jk && i < g.ni - 1 && !(S2 && (i == 0 || i == g.ni - 2)) ? b_own : BUF_DEAD;
     // ---- u (convdiff_body<0, true>) ----
     {
+      double coo_u = 0.0, vis_cu = vis_pu;
+      if constexpr (S2) {
+        coo_u = buf_ld<double>(prs(uoo, i), b_own, 0);
+        vis_cu = buf_ld<double>(prs(vis, i), b_own, 0);
+      }
       auto Fo = [&](int dj, int dk) { return U0[rix(tj + dj, tk + dk)]; };
       const double c0 = fu[3];
       const double a2 = i4(fu[6], fu[5], fu[4], fu[3]);
@@ -865,6 +867,11 @@ __device__ inline void convdiff3_body(
     __builtin_amdgcn_sched_barrier(0);
     // ---- v (convdiff_body<1, true>) ----
     {
+      double coo_v = 0.0, vis_cv = vis_pv;
+      if constexpr (S2) {
+        coo_v = buf_ld<double>(prs(voo, i), b_own, 0);
+        vis_cv = buf_ld<double>(prs(vis, i), b_vv, 0);
+      }
       auto Fo = [&](int dj, int dk) { return V[rix(tj + dj, tk + dk)]; };
       const double c0 = fv[3];
       const double iu = i4(U1[rix(tj + 2, tk)], U1[rix(tj + 1, tk)], U1[rix(tj, tk)],
@@ -899,6 +906,11 @@ __device__ inline void convdiff3_body(
     __builtin_amdgcn_sched_barrier(0);
     // ---- w (convdiff_body<2, true>) ----
     {
+      double coo_w = 0.0, vis_cw = vis_pw;
+      if constexpr (S2) {
+        coo_w = buf_ld<double>(prs(woo, i), b_own, 0);
+        vis_cw = buf_ld<double>(prs(vis, i), b_vw, 0);
+      }
       auto Fo = [&](int dj, int dk) { return W[rix(tj + dj, tk + dk)]; };
       const double c0 = fw[3];
       const double iu = i4(U1[rix(tj, tk + 2)], U1[rix(tj, tk + 1)], U1[rix(tj, tk)],
@@ -947,11 +959,11 @@ convdiff3_save_kernel(const double* __restrict__ u, const double* __restrict__ v
                         (L / kt) % jt, L / (kt * jt), K);
 }
 
-// The second stage's form (convdiff3_body<true>): two workgroups per CU.  At
-// three (53 KB of LDS each would fit) the base values and w's pre-fill plane
-// push it past 168 VGPRs, and its spills cost more than the third workgroup
-// gains.
-__global__ void __launch_bounds__(256, 2)
+// The second stage's form (convdiff3_body<true>): three workgroups per CU
+// (52 KB of LDS each), which its 168 VGPRs allow because it loads each
+// component's base value and vis at the start of that component instead of
+// one plane ahead.
+__global__ void __launch_bounds__(256, 3)
 convdiff3_kernel(const double* __restrict__ u, const double* __restrict__ v,
                  const double* __restrict__ w, const double* __restrict__ vis,
                  const double* __restrict__ wpre, double* __restrict__ us,
@@ -1507,9 +1519,8 @@ static int convdiff3_any_launch(const double* u, const double* v, const double* 
   using namespace cdv;
   const int kt = div_up(g.nk, TK), jt = div_up(g.nj, TJ);
   if (ich <= 0) ich = tune_get(wpre ? WF_TUNE_CD3S2_ICH : WF_TUNE_CD3_ICH, g.ni, g.nj, g.nk);
-  if (ich <= 0) {  // one round of workgroups (three per CU in the first stage's
-                   // form, two in the second's), >= 16 planes each
-    const int64_t tiles = (int64_t)kt * jt, slots = wpre ? 512 : 768;
+  if (ich <= 0) {  // one round of workgroups (three per CU), >= 16 planes each
+    const int64_t tiles = (int64_t)kt * jt, slots = 768;
     ich = (int)std::max<int64_t>(16, (tiles * g.ni + slots - 1) / slots);
   }
   const int nch = div_up(g.ni, ich);
