@@ -328,6 +328,12 @@ int wf_bc_fill(double* fld, wf_grid g, int which, double param, void* stream);
  * w's fills only. */
 int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream);
 
+/* wf_bc_fill_uvw again on fields filled since their last change (as the RK
+ * stages fill u, v, w a second time): the same result, running only the lines
+ * that lie in another axis's ghost band -- the only lines whose sources a later
+ * op of the first fill can have changed.  No reference counterpart. */
+int wf_bc_refill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream);
+
 /* ------------------------------------------------------------------------
  * Multigrid (warpflow/multigrid.py).  Level arrays follow wf_mg_level.
  * ---------------------------------------------------------------------- */

@@ -445,6 +445,14 @@ def test_ghost_fills_bitwise(gpu, n3):
     B.apply_uvw(dh["u"], dh["v"], dh["w"], n3[0], 0.75)
     for k in ("u", "v", "w"):
         assert np.array_equal(dh[k].numpy(), h[k]), ("uvw", k)
+    # a refill (fields filled since their last change): only the band lines
+    # run, and every cell equals a second full fill
+    full = {k: up(dh[k].numpy()) for k in ("u", "v", "w")}
+    part = {k: up(dh[k].numpy()) for k in ("u", "v", "w")}
+    B.apply_uvw(full["u"], full["v"], full["w"], n3[0], 0.75)
+    B.refill_uvw(part["u"], part["v"], part["w"], n3[0], 0.75)
+    for k in ("u", "v", "w"):
+        assert np.array_equal(part[k].numpy(), full[k].numpy()), ("refill", k)
     # v, w in one call (u absent)
     h = rand_fields(n3, ["v", "w"], seed=8)
     dh = {k: up(v) for k, v in h.items()}

@@ -130,14 +130,26 @@ struct Bc3 {
   double* f[3];
   int32_t a0[3], na[3], b0[3], nb[3];
   BcOps ops[3];
+  int32_t edges;  // refill: only the lines in another axis's written band
+  int32_t yl[3], yh[3], zl[3], zh[3];  // field z's y and z bands (see in_band)
 };
+
+// Refill of fields filled since their last change: an op rewrites the values
+// it wrote before unless one of its sources is a cell another axis's ops
+// wrote, and a line along one axis has such a source only if it lies in
+// another axis's written band: array index <= lo (the highest index the low
+// side's ops write) or >= hi (the lowest the high side's write), ghosts and
+// wall faces.  The other lines are skipped.
+__device__ inline bool in_band(int x, int lo, int hi) { return x <= lo || x >= hi; }
 
 __global__ void __launch_bounds__(256) bc_axis3_kernel(Bc3 A, wf_grid g, int axis) {
   const int z = blockIdx.z;
   const int bb = blockIdx.x * blockDim.x + threadIdx.x;
   const int aa = blockIdx.y * blockDim.y + threadIdx.y;
   if (aa >= A.na[z] || bb >= A.nb[z]) return;
-  bc_line(A.f[z], g, axis, A.a0[z] + aa, A.b0[z] + bb, A.ops[z]);
+  const int a = A.a0[z] + aa, b = A.b0[z] + bb;  // (j, k) lines of the x ops
+  if (A.edges && !in_band(a, A.yl[z], A.yh[z]) && !in_band(b, A.zl[z], A.zh[z])) return;
+  bc_line(A.f[z], g, axis, a, b, A.ops[z]);
 }
 
 // The y and z ops of a fill, one workgroup per array plane i (blockIdx.x =
@@ -166,6 +178,8 @@ struct BcYZ {
   BcOps oy[3], oz[3];
   BcCells zc[3];
   int32_t ibase;
+  int32_t edges;  // refill (see in_band)
+  int32_t xl[3], xh[3], yl[3], yh[3], zl[3], zh[3];  // field z's written bands
 };
 
 // The z ops of ZLANES-lane groups, a lane per ghost cell of an (i, j) line:
@@ -204,12 +218,18 @@ __device__ inline void bc_zcells(double* __restrict__ f, const wf_grid& g, int i
 __global__ void __launch_bounds__(1024) bc_yz_kernel(BcYZ A, wf_grid g) {
   const int z = blockIdx.y;
   const int i = A.ibase + (int)blockIdx.x;
+  // refill: a plane outside the x ghost band keeps only its band lines
+  const bool part = A.edges && !in_band(i, A.xl[z], A.xh[z]);
   if (i >= A.ay0[z] && i < A.ay0[z] + A.nay[z])
     for (int b = threadIdx.x; b < A.nby[z]; b += blockDim.x)
-      bc_line(A.f[z], g, 1, i, A.by0[z] + b, A.oy[z]);
+      if (!part || in_band(A.by0[z] + b, A.zl[z], A.zh[z]))
+        bc_line(A.f[z], g, 1, i, A.by0[z] + b, A.oy[z]);
   __syncthreads();
   if (i >= A.az0[z] && i < A.az0[z] + A.naz[z]) {
-    if (A.zc[z].n > 0)
+    if (part) {
+      for (int b = threadIdx.x; b < A.nbz[z]; b += blockDim.x)
+        if (in_band(A.bz0[z] + b, A.yl[z], A.yh[z])) bc_line(A.f[z], g, 2, i, A.bz0[z] + b, A.oz[z]);
+    } else if (A.zc[z].n > 0)
       bc_zcells(A.f[z], g, i, A.bz0[z], A.nbz[z], A.zc[z]);
     else
       for (int b = threadIdx.x; b < A.nbz[z]; b += blockDim.x)
@@ -484,9 +504,53 @@ static BcCells z_cells(const BcOps& ops) {
   return c;
 }
 
+// The band an op list writes along its axis: lo = the highest index a low-side
+// op writes, hi = the lowest a high-side op writes.
+static void op_band(const BcOps& o, int& lo, int& hi) {
+  lo = -1;
+  hi = 1 << 30;
+  for (int q = 0; q < o.n; ++q) {
+    if (o.op[q].nlay < 1) continue;
+    if (o.op[q].high)
+      hi = std::min(hi, (int)o.op[q].g0);
+    else
+      lo = std::max(lo, (int)o.op[q].g0);
+  }
+}
+
+// The written bands of fill `which` (param) along each axis.  *self_dep is
+// set when an op reads a cell of its own axis's band (tiny grids, where one
+// side's sources reach the other side's ghosts): a line can then change on a
+// refill without lying in another axis's band, and the refill runs in full.
+static bool fill_bands(int which, double param, const wf_grid& g, int lo[3], int hi[3],
+                       bool* self_dep = nullptr) {
+  for (int axis = 0; axis < 3; ++axis) {
+    int a0, na, b0, nb;
+    BcOps o{};
+    if (!bc_axis_ops(which, axis, g.npl, g.ni, g.nj, g.nk, param, a0, na, b0, nb, o))
+      return false;
+    op_band(o, lo[axis], hi[axis]);
+    if (self_dep)
+      for (int q = 0; q < o.n; ++q)
+        if (o.op[q].kind == 1)
+          for (int m = 0; m < o.op[q].nlay; ++m) {
+            const int si = o.op[q].high ? o.op[q].s0 - m : o.op[q].s0 + m;
+            if (si <= lo[axis] || si >= hi[axis]) *self_dep = true;
+          }
+  }
+  return true;
+}
+
 static int launch_yz(double* const* f, const int* which, const double* param, int nf,
-                     const wf_grid& g, hipStream_t s) {
+                     const wf_grid& g, hipStream_t s, bool edges = false) {
   BcYZ A{};
+  A.edges = edges;
+  for (int z = 0; z < nf; ++z) {
+    int lo[3], hi[3];
+    if (!fill_bands(which[z], param[z], g, lo, hi)) return WF_EINVAL;
+    A.xl[z] = lo[0], A.xh[z] = hi[0], A.yl[z] = lo[1], A.yh[z] = hi[1];
+    A.zl[z] = lo[2], A.zh[z] = hi[2];
+  }
   int lo = 1 << 30, hi = -1, lines = 64;
   for (int z = 0; z < nf; ++z) {
     A.f[z] = f[z];
@@ -538,8 +602,15 @@ int wf_bc_fill(double* f, wf_grid g, int which, double param, void* stream) {
 // apply_u(u), apply_v(v), apply_w(w) (bcs.py:201-265) on three fields of
 // one geometry: one x launch covering the three fields, then one y-then-z
 // launch covering them.
-int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream) {
-  const hipStream_t s = (hipStream_t)stream;
+static int fill_uvw(double* u, double* v, double* w, const wf_grid& g, double ulid, hipStream_t s,
+                    bool edges) {
+  if (edges) {  // the band rule needs sources outside every band
+    bool dep = false;
+    int lo[3], hi[3];
+    for (int which = 0; which < 3; ++which)
+      if (!fill_bands(which, which == 0 ? ulid : 0.0, g, lo, hi, &dep)) return WF_EINVAL;
+    if (dep) edges = false;
+  }
   if (g.npl < 1 || !v || !w) return WF_EINVAL;
   // the fields filled, in the reference's order (u may be absent)
   double* fs[3];
@@ -555,6 +626,12 @@ int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void
   {  // the x launch
     const int axis = 0;
     Bc3 A{};
+    A.edges = edges;
+    for (int z = 0; z < nf; ++z) {
+      int lo[3], hi[3];
+      if (!fill_bands(which[z], which[z] == 0 ? ulid : 0.0, g, lo, hi)) return WF_EINVAL;
+      A.yl[z] = lo[1], A.yh[z] = hi[1], A.zl[z] = lo[2], A.zh[z] = hi[2];
+    }
     int namax = 0, nbmax = 0;
     for (int z = 0; z < nf; ++z) {
       int a0, na, b0, nb;
@@ -578,7 +655,15 @@ int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void
   }
   double param[3];
   for (int z = 0; z < nf; ++z) param[z] = which[z] == 0 ? ulid : 0.0;
-  return launch_yz(fs, which, param, nf, g, s);
+  return launch_yz(fs, which, param, nf, g, s, edges);
+}
+
+int wf_bc_fill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream) {
+  return fill_uvw(u, v, w, g, ulid, (hipStream_t)stream, false);
+}
+
+int wf_bc_refill_uvw(double* u, double* v, double* w, wf_grid g, double ulid, void* stream) {
+  return fill_uvw(u, v, w, g, ulid, (hipStream_t)stream, true);
 }
 
 }  // extern "C"

@@ -130,6 +130,7 @@ _SIGS = {
     "wf_bc_const": [_P, _G, _I, _I, _I, _D, _I, _I, _I, _I, _I, _P],
     "wf_bc_fill": [_P, _G, _I, _D, _P],
     "wf_bc_fill_uvw": [_P, _P, _P, _G, _D, _P],
+    "wf_bc_refill_uvw": [_P, _P, _P, _G, _D, _P],
     # multigrid.py
     "wf_mg_coeff": [_L, _P, _P, _P, _P, _P, _P, _P, _P],
     "wf_mg_rbgs": [_L, _I, _P],

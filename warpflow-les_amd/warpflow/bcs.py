@@ -81,6 +81,16 @@ class BoundaryKernels:
                 raise ValueError("u, v and w must share one geometry")
         _hip.call("wf_bc_fill_uvw", u.ptr, v.ptr, w.ptr, g, float(ulid), current_stream())
 
+    def refill_uvw(self, u, v, w, n, ulid):
+        """apply_uvw on fields filled since their last change: the same result,
+        computed on the lines of the ghost bands only."""
+        g = _check_n(u, n)
+        key = lambda x: (x.ni, x.nj, x.nk, x.npl, x.sj, x.si)  # noqa: E731
+        for f in (v, w):
+            if key(grid_of(f, NPL)) != key(g) or f.lead != u.lead:
+                raise ValueError("u, v and w must share one geometry")
+        _hip.call("wf_bc_refill_uvw", u.ptr, v.ptr, w.ptr, g, float(ulid), current_stream())
+
     def apply_vw(self, v, w, n):
         """apply_v, apply_w (bcs.py:225-265) back to back, as one launch per
         axis covering the two fields."""

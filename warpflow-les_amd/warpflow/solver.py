@@ -112,6 +112,14 @@ class FlowSolver:
             self.halo.exchange(self.f[name])
         self.bcs.apply_uvw(self.f["u"], self.f["v"], self.f["w"], self.ncells[0], self.ulid)
 
+    def refill_bc_uvw(self, alfabc=1):
+        """fill_bc_uvw where u, v, w were filled since their last change (the
+        second fills of an RK stage): the same ghosts, computed on the edge
+        lines only."""
+        for name in ("u", "v", "w"):
+            self.halo.exchange(self.f[name])
+        self.bcs.refill_uvw(self.f["u"], self.f["v"], self.f["w"], self.ncells[0], self.ulid)
+
     def fill_bc_vw(self, alfabc=1):
         """fill_bc of v, then w (solver.py:88-97 twice), as fill_bc_uvw does
         it for three fields."""
@@ -347,7 +355,7 @@ class FlowSolver:
             K.convdiff3_save(self.ncells, f["u"], f["v"], f["w"], f["vis"], f["ustar"],
                              f["vstar"], f["wstar"], f["uoo"], f["voo"], f["woo"], *d3, dt,
                              alpha, NPL)
-            self.fill_bc_uvw(alfabc)
+            self.refill_bc_uvw(alfabc)
             return
         self._fill_stable = False
         if self.fused and stage > 0 and self.fused_uvw:
@@ -368,7 +376,7 @@ class FlowSolver:
             K.convdiff3(self.ncells, f["u"], f["v"], f["w"], f["uoo"], f["voo"], f["woo"],
                         f["vis"], f["w_pre"], f["ustar"], f["vstar"], f["wstar"], *d3, dt,
                         alpha, NPL)
-            self.fill_bc_uvw(alfabc)
+            self.refill_bc_uvw(alfabc)
             return
         if self.fused and stage > 0 and self.fused_vw:
             # Second stage: u's pass right after u's fill (it reads v and w
